@@ -1,0 +1,333 @@
+/*
+ * er_device.h - device data layout and the per-path estimator functions of the gfx950 tracer.
+ *
+ * Data layout in HBM (per volume):
+ *   cells   : one 16-byte record per lattice cell (vx, vy, vz) in [-1, R-1]^3 holding the eight
+ *             clamped uint16 corner voxels a trilinear sample at that cell reads
+ *             (Source/buffer3d.h:210-242). A sample is one 128-bit load instead of eight
+ *             2-byte gathers; the clamp-to-edge of the reference is folded into the records.
+ *             (Rx+1)(Ry+1)(Rz+1) x 16 B (8x the voxel bytes; 1.9 GB for 512x512x460).
+ *   mc_*    : 8^3-cell macrocells. For each, the min/max voxel over the voxels its samples can
+ *             read, and (per tracer) the max of DensityScale * Opacity over the intensities a
+ *             truncating trilinear sample can produce from that range ([min-3, max]). A sample
+ *             whose macrocell maximum is 0 contributes exactly 0 to the optical depth, so the
+ *             march skips its fetch and keeps the exact sequence of MinT additions.
+ * Per tracer: XYZA float4 frame / running estimates, RGBA8 display (+ bilateral temporaries),
+ * two uint32 seed planes (SoA), all row-major y * W + x.
+ *
+ * Operation order follows the reference expression by expression (compiled with
+ * -ffp-contract=off and correctly rounded division / square root), so decisions and sample
+ * values match the CPU restatement bit for bit.
+ */
+#ifndef ER_DEVICE_H
+#define ER_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/er_capi.h"
+#include "er_fmath.h"
+
+#define ER_MC_SHIFT 3              /* macrocell edge = 8 cells */
+#define ER_NEGLOG_BITS 23          /* -log(u) table over u = m * 2^-23 */
+
+namespace erd {
+
+/* reference constants (Source/defines.h:58-77) */
+#define ER_PI_F          3.141592654f
+#define ER_INV_PI_F      0.31830988618379067154f
+#define ER_INV_TWO_PI_F  0.15915494309189533577f
+#define ER_RAY_EPS       0.0001f
+
+struct DevLight {
+    float tm[3][4];
+    float inv_tm[3][4];
+    int   type, one_sided, visible, unit;
+    float size[3];
+    float area, inner_radius, outer_radius, multiplier;
+    int   texture_id;
+    int   valid;
+};
+
+struct DevTexture {
+    int   type, proc_type, bitmap_id, valid;
+    float output_level;
+    float offset[2], repeat[2];
+    int   flip[2];
+    float uniform[3], checker1[3], checker2[3];
+    int   gradient_index;            /* index into DevParams::tex_gradients, -1 if none */
+};
+
+struct DevClip {
+    float inv_tm[3][4];
+    int   type, invert;
+    float size[3];
+    float outer_radius;
+};
+
+/* Everything a pass reads besides the frame buffers; lives in device memory, read through a
+ * uniform pointer (scalar loads). */
+struct DevParams {
+    /* camera (Source/camera.h:61-96) */
+    float cam_pos[3], cam_N[3], cam_U[3], cam_V[3];
+    float screen00, screen10, invscreen0, invscreen1;
+    float focal, aperture, clip_near, clip_far, exposure;
+    int   film_w, film_h;
+    /* settings */
+    float step_primary, step_shadow, max_shadow, density_scale;
+    float grad_threshold, grad_factor, gm_inv;
+    int   shadows, estimator, shading_type, opacity_modulated, gradient_mode, sampler;
+    float woodcock_majorant;         /* DS^2 * max opacity (global majorant) */
+    /* volume (Source/volume.h:100-138) */
+    int   res[3];
+    float minp[3], maxp[3], size[3], invsize[3], resf[3], spacing[3], minstep;
+    const uint4* cells;
+    int   cstride_y, cstride_z;      /* (Rx+1), (Rx+1)(Ry+1) */
+    const float* sigma_lut;          /* DensityScale * Opacity(i), i in [0, lut_max] */
+    int   lut_max;
+    const float* mc_sigma;           /* per macrocell max sigma */
+    int   mc_dim[3];
+    const float* neglog;             /* -log(m 2^-23), m in [0, 2^23) */
+    /* lights, textures, clipping objects; indexed by ID (< ER_LIST_CAPACITY) */
+    int   light_count;
+    int   light_ids[ER_MAX_INDICES];
+    DevLight   lights[ER_LIST_CAPACITY];
+    DevTexture textures[ER_LIST_CAPACITY];
+    int   clip_count;
+    DevClip    clips[ER_LIST_CAPACITY];
+    /* transfer functions (Source/ertracer.h:105-109) */
+    er_plf      opacity;
+    er_plf      glossiness;
+    er_color_tf diffuse, specular, emission;
+    er_color_tf tex_gradients[4];
+};
+
+/* ---- small vector type; component-wise, reference operand order ---------------------- */
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 mul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ f3 scl(float s, f3 a) { return mk(a.x * s, a.y * s, a.z * s); }   /* float * Vec3f */
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 nrm(f3 a) { const float L = len(a); return mk(a.x / L, a.y / L, a.z / L); }
+__device__ __forceinline__ f3 crs(f3 a, f3 b)
+{
+    return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+__device__ __forceinline__ float absdot(f3 a, f3 b) { return fabsf(dot(a, b)); }
+__device__ __forceinline__ bool black(f3 c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
+__device__ __forceinline__ f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+__device__ __forceinline__ float clampf_(float v, float lo, float hi) { return fmaxf(lo, fminf(v, hi)); }
+
+__device__ __forceinline__ f3 xform_point(const float m[3][4], f3 p)
+{
+    return mk(m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3],
+              m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3],
+              m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3]);
+}
+__device__ __forceinline__ f3 xform_vector(const float m[3][4], f3 v)
+{
+    return mk(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z,
+              m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+              m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z);
+}
+
+struct Ray { f3 O, D; float MinT, MaxT; };
+__device__ __forceinline__ f3 ray_at(const Ray& r, float t) { return add(r.O, scl(t, nrm(r.D))); }   /* ray.h:47-50 */
+
+/* TransformRay (geometry.h:54-68) */
+__device__ __forceinline__ Ray xform_ray(const float m[3][4], const Ray& r)
+{
+    const f3 P = xform_point(m, r.O);
+    const f3 MinP = xform_point(m, ray_at(r, r.MinT));
+    const f3 MaxP = xform_point(m, ray_at(r, r.MaxT));
+    Ray t;
+    t.O = P;
+    t.D = nrm(sub(MaxP, P));
+    t.MinT = len(sub(MinP, P));
+    t.MaxT = len(sub(MaxP, P));
+    return t;
+}
+
+/* ---- RNG: register-resident pair of 16-bit multiply-with-carry generators (rng.h:35-51) */
+struct Rng {
+    uint32_t s0, s1;
+    __device__ __forceinline__ uint32_t next_bits()
+    {
+        s0 = 36969u * (s0 & 65535u) + (s0 >> 16);
+        s1 = 18000u * (s1 & 65535u) + (s1 >> 16);
+        return ((s0 << 16) + s1) & 0x007fffffu;
+    }
+    /* u = ((r & 0x7fffff) | 0x40000000 as float - 2) / 2 = (r & 0x7fffff) * 2^-23 exactly */
+    __device__ __forceinline__ float u() { return (float)next_bits() * 1.1920928955078125e-07f; }
+};
+
+/* ---- transfer functions (Source/plf.h:68-91) ----------------------------------------- */
+__device__ __forceinline__ float plf_eval(const er_plf& p, float x)
+{
+    const int n = p.count;
+    if (n <= 0) return 0.0f;
+    if (x < p.node_range[0]) return p.value[0];
+    if (x > p.node_range[1]) return p.value[n - 1];
+    for (int i = 1; i < n; ++i) {
+        const float P1 = p.position[i - 1];
+        const float P2 = p.position[i];
+        const float LerpT = (x - P1) / (P2 - P1);
+        if (x >= P1 && x < P2) return p.value[i - 1] + LerpT * (p.value[i] - p.value[i - 1]);
+    }
+    return 0.0f;
+}
+__device__ __forceinline__ f3 ctf_eval(const er_color_tf& c, float x)
+{
+    return mk(plf_eval(c.plf[0], x), plf_eval(c.plf[1], x), plf_eval(c.plf[2], x));
+}
+
+/* ---- volume sampling ----------------------------------------------------------------- */
+struct Lattice { float Lx, Ly, Lz; };
+
+/* Volume::operator() (volume.h:131-138): lattice coordinates of a world point */
+__device__ __forceinline__ Lattice to_lattice(const DevParams& p, f3 P)
+{
+    Lattice l;
+    l.Lx = ((P.x - p.minp[0]) * p.invsize[0]) * p.resf[0];
+    l.Ly = ((P.y - p.minp[1]) * p.invsize[1]) * p.resf[1];
+    l.Lz = ((P.z - p.minp[2]) * p.invsize[2]) * p.resf[2];
+    return l;
+}
+
+__device__ __forceinline__ float trunc16(float v) { return (float)(unsigned short)v; }
+
+/* Buffer3D<ushort>::operator()(Vec3f) (buffer3d.h:222-242) from one cell record: 7 lerps,
+ * each truncated to uint16 (geometry.h:87-90). */
+__device__ __forceinline__ float trilinear_cell(const DevParams& p, const Lattice& l)
+{
+    const float fx = floorf(l.Lx), fy = floorf(l.Ly), fz = floorf(l.Lz);
+    const int vx = (int)fx, vy = (int)fy, vz = (int)fz;
+    const float dx = l.Lx - (float)vx, dy = l.Ly - (float)vy, dz = l.Lz - (float)vz;
+    const int cx = min(max(vx, -1), p.res[0] - 1) + 1;
+    const int cy = min(max(vy, -1), p.res[1] - 1) + 1;
+    const int cz = min(max(vz, -1), p.res[2] - 1) + 1;
+    const uint4 c = p.cells[(size_t)cz * (size_t)p.cstride_z + (size_t)(cy * p.cstride_y + cx)];
+    const float v000 = (float)(c.x & 0xffffu), v100 = (float)(c.x >> 16);
+    const float v010 = (float)(c.y & 0xffffu), v110 = (float)(c.y >> 16);
+    const float v001 = (float)(c.z & 0xffffu), v101 = (float)(c.z >> 16);
+    const float v011 = (float)(c.w & 0xffffu), v111 = (float)(c.w >> 16);
+    const float ox = 1.0f - dx, oy = 1.0f - dy, oz = 1.0f - dz;
+    const float d00 = trunc16(ox * v000 + dx * v100);
+    const float d10 = trunc16(ox * v010 + dx * v110);
+    const float d01 = trunc16(ox * v001 + dx * v101);
+    const float d11 = trunc16(ox * v011 + dx * v111);
+    const float d0 = trunc16(oy * d00 + dy * d10);
+    const float d1 = trunc16(oy * d01 + dy * d11);
+    return trunc16(oz * d0 + dz * d1);
+}
+
+__device__ __forceinline__ float intensity(const DevParams& p, f3 P) { return trilinear_cell(p, to_lattice(p, P)); }
+
+/* macrocell of a sample: clamp(floor(L) >> 3) per axis */
+__device__ __forceinline__ int macrocell(const DevParams& p, const Lattice& l)
+{
+    const int mx = min(max(((int)floorf(l.Lx)) >> ER_MC_SHIFT, 0), p.mc_dim[0] - 1);
+    const int my = min(max(((int)floorf(l.Ly)) >> ER_MC_SHIFT, 0), p.mc_dim[1] - 1);
+    const int mz = min(max(((int)floorf(l.Lz)) >> ER_MC_SHIFT, 0), p.mc_dim[2] - 1);
+    return (mz * p.mc_dim[1] + my) * p.mc_dim[0] + mx;
+}
+
+/* sigma of one sample: DensityScale * Opacity1D(I) through the exact LUT */
+__device__ __forceinline__ float sigma_lut(const DevParams& p, float I)
+{
+    return p.sigma_lut[min((int)I, p.lut_max)];
+}
+
+/* GradientCD / FD / Filtered (volumes.h:31-74); Normalize (Q20: zero gradient -> NaN) */
+__device__ __forceinline__ f3 grad_cd(const DevParams& p, f3 P)
+{
+    const float d = p.minstep;
+    const f3 dX = mk(d, 0.0f, 0.0f), dY = mk(0.0f, d, 0.0f), dZ = mk(0.0f, 0.0f, d);
+    const float i00 = intensity(p, add(P, dX)), i01 = intensity(p, sub(P, dX));
+    const float i10 = intensity(p, add(P, dY)), i11 = intensity(p, sub(P, dY));
+    const float i20 = intensity(p, add(P, dZ)), i21 = intensity(p, sub(P, dZ));
+    return mk(i01 - i00, i11 - i10, i21 - i20);
+}
+__device__ __forceinline__ f3 grad_fd(const DevParams& p, f3 P)
+{
+    const float d = p.minstep;
+    const float i0 = intensity(p, P);
+    const float i1 = intensity(p, add(P, mk(d, 0.0f, 0.0f)));
+    const float i2 = intensity(p, add(P, mk(0.0f, d, 0.0f)));
+    const float i3 = intensity(p, add(P, mk(0.0f, 0.0f, d)));
+    return mk(i0 - i1, i0 - i2, i0 - i3);
+}
+__device__ __forceinline__ f3 vlerp(f3 a, f3 b, float c) { return add(a, scl(c, sub(b, a))); }
+__device__ __noinline__ f3 grad_filtered(const DevParams& p, f3 P)
+{
+    const float o = p.minstep;
+    const f3 G0 = grad_cd(p, P);
+    const f3 G1 = grad_cd(p, add(P, mk(-o, -o, -o)));
+    const f3 G2 = grad_cd(p, add(P, mk(o, o, o)));
+    const f3 G3 = grad_cd(p, add(P, mk(-o, o, -o)));
+    const f3 G4 = grad_cd(p, add(P, mk(o, -o, o)));
+    const f3 G5 = grad_cd(p, add(P, mk(-o, -o, o)));
+    const f3 G6 = grad_cd(p, add(P, mk(o, o, -o)));
+    const f3 G7 = grad_cd(p, add(P, mk(-o, o, o)));
+    const f3 G8 = grad_cd(p, add(P, mk(o, -o, -o)));
+    const f3 L0 = vlerp(vlerp(G1, G2, 0.5f), vlerp(G3, G4, 0.5f), 0.5f);
+    const f3 L1 = vlerp(vlerp(G5, G6, 0.5f), vlerp(G7, G8, 0.5f), 0.5f);
+    return vlerp(G0, vlerp(L0, L1, 0.5f), 0.75f);
+}
+__device__ __forceinline__ int gradient_samples(const DevParams& p)
+{
+    return p.gradient_mode == ER_GRADIENT_CENTRAL ? 6 : (p.gradient_mode == ER_GRADIENT_FILTERED ? 54 : 4);
+}
+__device__ __forceinline__ f3 normalized_gradient(const DevParams& p, f3 P)
+{
+    if (p.gradient_mode == ER_GRADIENT_CENTRAL) return nrm(grad_cd(p, P));
+    if (p.gradient_mode == ER_GRADIENT_FILTERED) return nrm(grad_filtered(p, P));
+    return nrm(grad_fd(p, P));
+}
+/* GradientMagnitude (volumes.h:93-114) */
+__device__ __forceinline__ float gradient_magnitude(const DevParams& p, f3 P)
+{
+    const float d = p.minstep;
+    float Sum = 0.0f;
+    {
+        float D = intensity(p, sub(P, mk(d, 0.0f, 0.0f))) - intensity(p, add(P, mk(d, 0.0f, 0.0f)));
+        D *= 0.5f / p.spacing[0];
+        Sum += D * D;
+    }
+    {
+        float D = intensity(p, sub(P, mk(0.0f, d, 0.0f))) - intensity(p, add(P, mk(0.0f, d, 0.0f)));
+        D *= 0.5f / p.spacing[1];
+        Sum += D * D;
+    }
+    {
+        float D = intensity(p, sub(P, mk(0.0f, 0.0f, d))) - intensity(p, add(P, mk(0.0f, 0.0f, d)));
+        D *= 0.5f / p.spacing[2];
+        Sum += D * D;
+    }
+    return sqrtf(Sum);
+}
+
+/* IntersectBox (box.h:51-83) */
+__device__ __forceinline__ bool intersect_box(const Ray& r, f3 mn, f3 mx, float& nearT, float& farT)
+{
+    const f3 InvR = mk(1.0f / r.D.x, 1.0f / r.D.y, 1.0f / r.D.z);
+    const f3 B = mul(InvR, sub(mn, r.O));
+    const f3 T = mul(InvR, sub(mx, r.O));
+    const float mn0 = fminf(T.x, B.x), mn1 = fminf(T.y, B.y), mn2 = fminf(T.z, B.z);
+    const float mx0 = fmaxf(T.x, B.x), mx1 = fmaxf(T.y, B.y), mx2 = fmaxf(T.z, B.z);
+    const float LargestMinT = fmaxf(fmaxf(mn0, mn1), fmaxf(mn0, mn2));
+    const float LargestMaxT = fminf(fminf(mx0, mx1), fminf(mx0, mx2));
+    if (LargestMaxT < LargestMinT) return false;
+    nearT = LargestMinT > 0.0f ? LargestMinT : 0.0f;
+    farT = LargestMaxT;
+    if (nearT < r.MinT || nearT > r.MaxT) return false;
+    return true;
+}
+
+} /* namespace erd */
+
+#endif /* ER_DEVICE_H */

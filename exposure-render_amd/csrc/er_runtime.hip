@@ -1,0 +1,979 @@
+/*
+ * er_runtime.hip - the C-ABI (include/er_capi.h) over HIP: resource registries, device memory,
+ * per-tracer frame state and pass sequencing.
+ *
+ * Registry semantics follow Cuda::List (Source/list.cuh:27-173): one map per kind, IDs from a
+ * per-kind counter, a rebind with a known ID updates in place, an unbind of an unknown ID is
+ * ignored, a 32nd live object is ignored. Device copies of the scene are built lazily: every
+ * bind bumps a scene version, and the next pass of a tracer rebuilds its DevParams (the
+ * reference instead re-uploads the ~21 KB tracer on every pass, core.cu:128). Lights and
+ * textures are indexed by ID on the device, which equals the reference's list position while
+ * nothing has been unbound.
+ *
+ * Every entry point is synchronous on return, as the reference's, and serialised by a mutex.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/er_capi.h"
+#include "er_device.h"
+
+using erd::DevParams;
+
+extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2,
+                                       unsigned long long* counters, int W, int H, int full, int count, hipStream_t st);
+extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
+                                      int no_iterations, int W, int H, hipStream_t st);
+extern "C" hipError_t erd_launch_bilateral(const void* bp, const uchar4* in, uchar4* tmp, uchar4* out, int W, int H,
+                                           hipStream_t st);
+extern "C" size_t erd_bilateral_params_size(void);
+extern "C" hipError_t erd_launch_build_cells(const uint16_t* vox, uint4* cells, int rx, int ry, int rz, hipStream_t st);
+extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm, int rx, int ry, int rz, int mx, int my,
+                                              int mz, hipStream_t st);
+extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lut, int lut_max, float* out, int n,
+                                             hipStream_t st);
+extern "C" hipError_t erd_launch_gm_max(const DevParams* p, unsigned int* out, hipStream_t st);
+
+namespace {
+
+std::recursive_mutex g_mutex;
+std::string g_last_error;
+
+int set_error(int status, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_error(ER_ERROR, "%s (%s)", hipGetErrorString(e_), #expr);             \
+    } while (0)
+
+template<class T> struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    int alloc(size_t count)
+    {
+        if (count == n && ptr) return ER_OK;
+        release();
+        if (count == 0) return ER_OK;
+        HIP_TRY(hipMalloc((void**)&ptr, count * sizeof(T)));
+        n = count;
+        return ER_OK;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+};
+
+/* ---- host-side restatements used to build device data (same op order as the kernels) -- */
+float plf_eval_host(const er_plf& p, float x)   /* Source/plf.h:68-91 */
+{
+    const int n = p.count;
+    if (n <= 0) return 0.0f;
+    if (x < p.node_range[0]) return p.value[0];
+    if (x > p.node_range[1]) return p.value[n - 1];
+    for (int i = 1; i < n; ++i) {
+        const float P1 = p.position[i - 1], P2 = p.position[i];
+        const float LerpT = (x - P1) / (P2 - P1);
+        if (x >= P1 && x < P2) return p.value[i - 1] + LerpT * (p.value[i] - p.value[i - 1]);
+    }
+    return 0.0f;
+}
+
+float shape_area(const er_shape& s)   /* Shape::Update (Source/shape.h:91-103) */
+{
+    const float PI_F = 3.141592654f;
+    switch (s.type) {
+        case ER_SHAPE_PLANE:    return s.size[0] * s.size[1];
+        case ER_SHAPE_DISK:     return PI_F * (s.outer_radius * s.outer_radius);
+        case ER_SHAPE_RING:     return PI_F * (s.outer_radius * s.outer_radius) - PI_F * (s.inner_radius * s.inner_radius);
+        case ER_SHAPE_BOX:      return (2.0f * s.size[0] * s.size[1]) + (2.0f * s.size[0] * s.size[2]) + (2.0f * s.size[1] * s.size[2]);
+        case ER_SHAPE_SPHERE:   return 4.0f * PI_F * (s.outer_radius * s.outer_radius);
+        case ER_SHAPE_CYLINDER: return (2.0f * PI_F * (s.outer_radius * s.outer_radius)) + (2.0f * PI_F * s.outer_radius * s.size[2]);
+    }
+    return s.area;
+}
+
+/* ---- the -log(u) table: u = m 2^-23, value = -log(u) rounded once to float ------------- */
+std::vector<float> g_neglog_host;
+float* g_neglog_dev = nullptr;
+
+void build_neglog_host()
+{
+    if (!g_neglog_host.empty()) return;
+    const size_t n = (size_t)1 << ER_NEGLOG_BITS;
+    g_neglog_host.resize(n);
+    g_neglog_host[0] = INFINITY;
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([t, nt]() {
+            for (size_t m = 1 + t; m < n; m += nt)
+                g_neglog_host[m] = (float)(-logl(ldexpl((long double)m, -ER_NEGLOG_BITS)));
+        });
+    for (auto& x : th) x.join();
+}
+
+int ensure_neglog_device()
+{
+    if (g_neglog_dev) return ER_OK;
+    build_neglog_host();
+    const size_t bytes = g_neglog_host.size() * sizeof(float);
+    HIP_TRY(hipMalloc((void**)&g_neglog_dev, bytes));
+    HIP_TRY(hipMemcpy(g_neglog_dev, g_neglog_host.data(), bytes, hipMemcpyHostToDevice));
+    return ER_OK;
+}
+
+/* ---- resources --------------------------------------------------------------------------- */
+uint64_t g_scene_version = 1;
+
+struct Volume {
+    er_volume_desc desc{};
+    DevBuf<uint16_t> vox;
+    DevBuf<uint4> cells;
+    DevBuf<uint32_t> mm;
+    int mc[3] = {0, 0, 0};
+    uint16_t vmax = 0;
+};
+
+struct Tracer {
+    er_tracer_desc desc{};
+    int W = 0, H = 0;
+    DevBuf<float4> frame, running;
+    DevBuf<uchar4> display, disp_tmp, disp_filt;
+    DevBuf<uint32_t> seeds1, seeds2;
+    DevBuf<float> lut, mc_sigma;
+    DevBuf<DevParams> params;
+    DevBuf<unsigned long long> counters;
+    DevBuf<unsigned int> gm_tmp;
+    DevBuf<char> bilateral;
+    DevParams host_params{};
+    uint64_t built_version = 0;
+    int built_volume = -1;
+    float gm_max = 0.0f;
+    int counting = 0;
+    hipStream_t stream = nullptr;
+    ~Tracer()
+    {
+        frame.release(); running.release(); display.release(); disp_tmp.release(); disp_filt.release();
+        seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); params.release();
+        counters.release(); gm_tmp.release(); bilateral.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+template<class T> struct Registry {
+    std::map<int, T*> map;
+    int counter = 0;
+    T* find(int id)
+    {
+        if (id < 0) return nullptr;
+        auto it = map.find(id);
+        return it == map.end() ? nullptr : it->second;
+    }
+    void clear()
+    {
+        for (auto& kv : map) delete kv.second;
+        map.clear();
+        counter = 0;
+    }
+};
+
+struct LightObj { er_light_desc desc; };
+struct TextureObj { er_texture_desc desc; };
+struct ClipObj { er_clipping_desc desc; };
+struct ObjectObj { er_object_desc desc; };
+struct BitmapObj { er_bitmap_desc desc; std::vector<uint8_t> pixels; };
+
+Registry<Tracer> g_tracers;
+Registry<Volume> g_volumes;
+Registry<LightObj> g_lights;
+Registry<TextureObj> g_textures;
+Registry<ClipObj> g_clips;
+Registry<ObjectObj> g_objects;
+Registry<BitmapObj> g_bitmaps;
+bool g_device_ready = false;
+
+int ensure_device()
+{
+    if (g_device_ready) return ER_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return set_error(ER_ERROR, "no HIP device available (%s)", hipGetErrorString(e));
+    g_device_ready = true;
+    return ER_OK;
+}
+
+/* SplitMix64 of (seed ^ pixel) split in two non-zero 32-bit halves */
+void seed_pixels(uint64_t seed, size_t n, std::vector<uint32_t>& s1, std::vector<uint32_t>& s2)
+{
+    s1.resize(n); s2.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t z = (seed ^ (uint64_t)i) + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        uint32_t a = (uint32_t)z, b = (uint32_t)(z >> 32);
+        s1[i] = a ? a : 1u;
+        s2[i] = b ? b : 1u;
+    }
+}
+
+int tracer_seed(Tracer& t, uint64_t seed)
+{
+    std::vector<uint32_t> s1, s2;
+    const size_t n = (size_t)t.W * t.H;
+    seed_pixels(seed, n, s1, s2);
+    HIP_TRY(hipMemcpy(t.seeds1.ptr, s1.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.seeds2.ptr, s2.data(), n * 4, hipMemcpyHostToDevice));
+    return ER_OK;
+}
+
+/* FrameBuffer::Resize (Source/framebuffer.h:45-68): buffers, zeroed estimates, new seeds */
+int tracer_resize(Tracer& t, int W, int H, int id)
+{
+    if (W == t.W && H == t.H && t.frame.ptr) return ER_OK;
+    if (W < 0 || H < 0) return set_error(ER_ERROR, "invalid film size %d x %d", W, H);
+    const size_t n = (size_t)W * H;
+    int st;
+    if ((st = t.frame.alloc(n)) || (st = t.running.alloc(n)) || (st = t.display.alloc(n)) ||
+        (st = t.disp_tmp.alloc(n)) || (st = t.disp_filt.alloc(n)) || (st = t.seeds1.alloc(n)) ||
+        (st = t.seeds2.alloc(n)))
+        return st;
+    t.W = W; t.H = H;
+    if (n) {
+        HIP_TRY(hipMemset(t.frame.ptr, 0, n * sizeof(float4)));
+        HIP_TRY(hipMemset(t.running.ptr, 0, n * sizeof(float4)));
+        HIP_TRY(hipMemset(t.display.ptr, 0, n * sizeof(uchar4)));
+        HIP_TRY(hipMemset(t.disp_filt.ptr, 0, n * sizeof(uchar4)));
+        return tracer_seed(t, 0x45787052ull + (uint64_t)id);
+    }
+    return ER_OK;
+}
+
+void mat34(float dst[3][4], const float src[4][4])
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j) dst[i][j] = src[i][j];
+}
+
+float norm_len(const float v[3]) { return sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* Camera::Update (Source/camera.h:61-96) */
+void camera_update(DevParams& P, const er_camera& c)
+{
+    float N[3], U[3], V[3], d[3];
+    for (int i = 0; i < 3; ++i) d[i] = c.target[i] - c.pos[i];
+    const float Ln = norm_len(d);
+    for (int i = 0; i < 3; ++i) N[i] = d[i] / Ln;
+    float cu[3] = {(N[1] * c.up[2]) - (N[2] * c.up[1]), (N[2] * c.up[0]) - (N[0] * c.up[2]), (N[0] * c.up[1]) - (N[1] * c.up[0])};
+    const float Lu = norm_len(cu);
+    for (int i = 0; i < 3; ++i) U[i] = cu[i] / Lu;
+    float cv[3] = {(N[1] * U[2]) - (N[2] * U[1]), (N[2] * U[0]) - (N[0] * U[2]), (N[0] * U[1]) - (N[1] * U[0])};
+    const float Lv = norm_len(cv);
+    for (int i = 0; i < 3; ++i) V[i] = cv[i] / Lv;
+    for (int i = 0; i < 3; ++i) { P.cam_pos[i] = c.pos[i]; P.cam_N[i] = N[i]; P.cam_U[i] = U[i]; P.cam_V[i] = V[i]; }
+    P.focal = c.focal_distance == -1.0f ? Ln : c.focal_distance;
+    const float Scale = tanf((0.5f * c.fov / 57.29577951308232f));
+    const float Aspect = (float)c.film_size[1] / (float)c.film_size[0];
+    float s00, s01, s10, s11;
+    if (Aspect > 1.0f) { s00 = -Scale; s01 = Scale; s10 = -Scale * Aspect; s11 = Scale * Aspect; }
+    else { s00 = -Scale / Aspect; s01 = Scale / Aspect; s10 = -Scale; s11 = Scale; }
+    P.screen00 = s00;
+    P.screen10 = s10;
+    P.invscreen0 = (s01 - s00) / (float)c.film_size[0];
+    P.invscreen1 = (s11 - s10) / (float)c.film_size[1];
+    P.aperture = c.aperture_size;
+    P.clip_near = c.clip_near;
+    P.clip_far = c.clip_far;
+    P.exposure = c.exposure;
+    P.film_w = c.film_size[0];
+    P.film_h = c.film_size[1];
+}
+
+int upload_volume(Volume& v)
+{
+    const er_volume_desc& d = v.desc;
+    const int rx = d.resolution[0], ry = d.resolution[1], rz = d.resolution[2];
+    const size_t n = (size_t)rx * ry * rz;
+    if (rx <= 0 || ry <= 0 || rz <= 0) return set_error(ER_ERROR, "invalid volume resolution %d x %d x %d", rx, ry, rz);
+    if (!d.voxels) return set_error(ER_ERROR, "volume voxels pointer is NULL");
+    int st;
+    if ((st = v.vox.alloc(n))) return st;
+    HIP_TRY(hipMemcpy(v.vox.ptr, d.voxels, n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    uint16_t vmax = 0;
+    for (size_t i = 0; i < n; ++i) vmax = std::max(vmax, d.voxels[i]);
+    v.vmax = vmax;
+    const size_t nc = (size_t)(rx + 1) * (ry + 1) * (rz + 1);
+    if ((st = v.cells.alloc(nc))) return st;
+    HIP_TRY(erd_launch_build_cells(v.vox.ptr, v.cells.ptr, rx, ry, rz, 0));
+    v.mc[0] = (rx + 7) / 8; v.mc[1] = (ry + 7) / 8; v.mc[2] = (rz + 7) / 8;
+    if ((st = v.mm.alloc((size_t)v.mc[0] * v.mc[1] * v.mc[2]))) return st;
+    HIP_TRY(erd_launch_macro_minmax(v.vox.ptr, v.mm.ptr, rx, ry, rz, v.mc[0], v.mc[1], v.mc[2], 0));
+    HIP_TRY(hipDeviceSynchronize());
+    return ER_OK;
+}
+
+bool needs_gm(const er_tracer_desc& d)
+{
+    const int t = d.settings.shading_type;
+    return d.settings.estimator == ER_ESTIMATOR_FULL && t != ER_SHADING_BRDF_ONLY && t != ER_SHADING_PHASE_ONLY;
+}
+
+/* Rebuild the tracer's DevParams (and the LUT / macrocell maxima / gm_max that depend on it). */
+int build_params(Tracer& t)
+{
+    if (t.built_version == g_scene_version) return ER_OK;
+    int st;
+    if ((st = ensure_neglog_device())) return st;
+    const er_tracer_desc& d = t.desc;
+    Volume* vol = g_volumes.find(d.volume_id);
+    if (!vol) return set_error(ER_WARNING, "volume with ID:%d does not exist", d.volume_id);
+    DevParams& P = t.host_params;
+    memset(&P, 0, sizeof(P));
+    camera_update(P, d.camera);
+    const er_render_settings& s = d.settings;
+    P.step_primary = s.step_factor_primary;
+    P.step_shadow = s.step_factor_shadow;
+    P.max_shadow = s.max_shadow_distance;
+    P.density_scale = s.density_scale;
+    P.grad_threshold = s.gradient_threshold;
+    P.grad_factor = s.gradient_factor;
+    P.shadows = s.shadows;
+    P.estimator = s.estimator;
+    P.shading_type = s.shading_type;
+    P.opacity_modulated = s.opacity_modulated;
+    P.gradient_mode = s.gradient_computation;
+    P.sampler = s.sampler;
+
+    /* Volume::operator=(ErVolume) (Source/volume.h:100-129) */
+    const er_volume_desc& vd = vol->desc;
+    float scale = 0.0f;
+    if (vd.normalize_size) {
+        const float ps[3] = {(float)vd.resolution[0] * vd.spacing[0], (float)vd.resolution[1] * vd.spacing[1],
+                             (float)vd.resolution[2] * vd.spacing[2]};
+        scale = 1.0f / std::max(ps[0], std::max(ps[1], ps[2]));
+    }
+    for (int i = 0; i < 3; ++i) {
+        P.res[i] = vd.resolution[i];
+        P.resf[i] = (float)vd.resolution[i];
+        P.spacing[i] = vd.spacing[i] * scale;
+        P.size[i] = (float)vd.resolution[i] * P.spacing[i];
+        P.invsize[i] = 1.0f / P.size[i];
+        P.minp[i] = P.size[i] * -0.5f;
+        P.maxp[i] = P.size[i] * 0.5f;
+    }
+    P.minstep = std::min(P.spacing[0], std::min(P.spacing[1], P.spacing[2]));
+    P.cells = vol->cells.ptr;
+    P.cstride_y = vd.resolution[0] + 1;
+    P.cstride_z = (vd.resolution[0] + 1) * (vd.resolution[1] + 1);
+    P.neglog = g_neglog_dev;
+    for (int i = 0; i < 3; ++i) P.mc_dim[i] = vol->mc[i];
+
+    /* lights, textures, clipping objects by ID */
+    P.light_count = std::min(d.light_count, ER_MAX_INDICES);
+    for (int i = 0; i < ER_MAX_INDICES; ++i) P.light_ids[i] = d.light_ids[i];
+    for (auto& kv : g_lights.map) {
+        if (kv.first >= ER_LIST_CAPACITY) continue;
+        const er_light_desc& L = kv.second->desc;
+        erd::DevLight& D = P.lights[kv.first];
+        mat34(D.tm, L.shape.tm);
+        mat34(D.inv_tm, L.shape.inv_tm);
+        D.type = L.shape.type; D.one_sided = L.shape.one_sided; D.visible = L.visible; D.unit = L.unit;
+        for (int i = 0; i < 3; ++i) D.size[i] = L.shape.size[i];
+        D.area = shape_area(L.shape);
+        D.inner_radius = L.shape.inner_radius; D.outer_radius = L.shape.outer_radius;
+        D.multiplier = L.multiplier; D.texture_id = L.texture_id; D.valid = 1;
+    }
+    int ngrad = 0;
+    for (auto& kv : g_textures.map) {
+        if (kv.first >= ER_LIST_CAPACITY) continue;
+        const er_texture_desc& T = kv.second->desc;
+        erd::DevTexture& D = P.textures[kv.first];
+        D.type = T.type; D.proc_type = T.procedural.type; D.bitmap_id = T.bitmap_id; D.valid = 1;
+        D.output_level = T.output_level;
+        for (int i = 0; i < 2; ++i) { D.offset[i] = T.offset[i]; D.repeat[i] = T.repeat[i]; D.flip[i] = T.flip[i]; }
+        for (int i = 0; i < 3; ++i) {
+            D.uniform[i] = T.procedural.uniform_color[i];
+            D.checker1[i] = T.procedural.checker_color1[i];
+            D.checker2[i] = T.procedural.checker_color2[i];
+        }
+        D.gradient_index = -1;
+        if (T.type == ER_TEXTURE_PROCEDURAL && T.procedural.type == ER_PROCEDURAL_GRADIENT) {
+            if (ngrad >= 4) return set_error(ER_ERROR, "at most 4 gradient textures are supported");
+            P.tex_gradients[ngrad] = T.procedural.gradient;
+            D.gradient_index = ngrad++;
+        }
+        if (T.type == ER_TEXTURE_BITMAP) return set_error(ER_ERROR, "bitmap textures are not supported by this build");
+    }
+    P.clip_count = 0;
+    if (s.estimator == ER_ESTIMATOR_FULL) {
+        for (int k = 0; k < d.clipping_count && k < ER_MAX_INDICES; ++k) {
+            ClipObj* c = g_clips.find(d.clipping_ids[k]);
+            if (!c || P.clip_count >= ER_LIST_CAPACITY) continue;
+            erd::DevClip& D = P.clips[P.clip_count++];
+            mat34(D.inv_tm, c->desc.shape.inv_tm);
+            D.type = c->desc.shape.type; D.invert = c->desc.invert;
+            for (int i = 0; i < 3; ++i) D.size[i] = c->desc.shape.size[i];
+            D.outer_radius = c->desc.shape.outer_radius;
+        }
+    }
+    if (d.object_count > 0) return set_error(ER_ERROR, "surface objects are not supported by this build");
+
+    P.opacity = d.opacity; P.glossiness = d.glossiness;
+    P.diffuse = d.diffuse; P.specular = d.specular; P.emission = d.emission;
+
+    /* exact opacity LUT over the intensities the volume can produce */
+    const int lut_n = (int)vol->vmax + 1;
+    std::vector<float> lut(lut_n);
+    float omax = 0.0f;
+    for (int i = 0; i < lut_n; ++i) {
+        const float o = plf_eval_host(d.opacity, (float)i);
+        lut[i] = s.density_scale * o;
+        omax = std::max(omax, o);
+    }
+    P.woodcock_majorant = s.density_scale * s.density_scale * omax;
+    if ((st = t.lut.alloc(lut_n))) return st;
+    HIP_TRY(hipMemcpy(t.lut.ptr, lut.data(), lut_n * sizeof(float), hipMemcpyHostToDevice));
+    P.sigma_lut = t.lut.ptr;
+    P.lut_max = lut_n - 1;
+    const int nmc = vol->mc[0] * vol->mc[1] * vol->mc[2];
+    if ((st = t.mc_sigma.alloc(nmc))) return st;
+    HIP_TRY(erd_launch_macro_sigma(vol->mm.ptr, t.lut.ptr, P.lut_max, t.mc_sigma.ptr, nmc, t.stream));
+    P.mc_sigma = t.mc_sigma.ptr;
+
+    if ((st = t.params.alloc(1))) return st;
+    P.gm_inv = 0.0f;
+    t.gm_max = 0.0f;
+    HIP_TRY(hipMemcpyAsync(t.params.ptr, &P, sizeof(P), hipMemcpyHostToDevice, t.stream));
+    if (needs_gm(d)) {
+        if ((st = t.gm_tmp.alloc(1))) return st;
+        HIP_TRY(hipMemsetAsync(t.gm_tmp.ptr, 0, sizeof(unsigned int), t.stream));
+        HIP_TRY(erd_launch_gm_max(t.params.ptr, t.gm_tmp.ptr, t.stream));
+        unsigned int bits = 0;
+        HIP_TRY(hipMemcpyAsync(&bits, t.gm_tmp.ptr, sizeof(bits), hipMemcpyDeviceToHost, t.stream));
+        HIP_TRY(hipStreamSynchronize(t.stream));
+        memcpy(&t.gm_max, &bits, 4);
+        P.gm_inv = t.gm_max > 0.0f ? 1.0f / t.gm_max : 0.0f;
+        HIP_TRY(hipMemcpyAsync(t.params.ptr, &P, sizeof(P), hipMemcpyHostToDevice, t.stream));
+    }
+    /* bilateral tables (Source/filter.h:34-40) */
+    if (s.estimator == ER_ESTIMATOR_FULL && s.bilateral) {
+        const int R = s.bilateral_radius;
+        if (R < 0 || R > 32) return set_error(ER_ERROR, "bilateral radius must be in [0, 32]");
+        std::vector<char> bp(erd_bilateral_params_size(), 0);
+        int* radius = (int*)bp.data();
+        float* kd = (float*)(bp.data() + sizeof(int));
+        float* gs = kd + 65;
+        *radius = R;
+        const float ss = s.bilateral_sigma_spatial, sr = s.bilateral_sigma_range;
+        for (int k = -R; k <= R; ++k) kd[k + R] = erm::expf_(-((float)(k * k)) / (2.0f * ss * ss));
+        for (int q = 0; q < 256; ++q) gs[q] = erm::expf_(-((float)(q * q)) / (2.0f * sr * sr));
+        if ((st = t.bilateral.alloc(bp.size()))) return st;
+        HIP_TRY(hipMemcpyAsync(t.bilateral.ptr, bp.data(), bp.size(), hipMemcpyHostToDevice, t.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(t.stream));
+    t.built_version = g_scene_version;
+    return ER_OK;
+}
+
+bool use_bilateral(const Tracer& t)
+{
+    return t.desc.settings.estimator == ER_ESTIMATOR_FULL && t.desc.settings.bilateral;
+}
+
+/* one pass on the tracer's stream (no host synchronisation) */
+int enqueue_pass(Tracer& t, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
+{
+    if (t.W == 0 || t.H == 0) return ER_OK;
+    const int full = t.desc.settings.estimator == ER_ESTIMATOR_FULL;
+    if (e0) HIP_TRY(hipEventRecord(e0, t.stream));
+    HIP_TRY(erd_launch_trace(t.params.ptr, t.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, t.counters.ptr, t.W, t.H, full,
+                             t.counting, t.stream));
+    if (e1) HIP_TRY(hipEventRecord(e1, t.stream));
+    HIP_TRY(erd_launch_post(t.params.ptr, t.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations, t.W, t.H,
+                            t.stream));
+    if (use_bilateral(t))
+        HIP_TRY(erd_launch_bilateral(t.bilateral.ptr, t.display.ptr, t.disp_tmp.ptr, t.disp_filt.ptr, t.W, t.H, t.stream));
+    if (e2) HIP_TRY(hipEventRecord(e2, t.stream));
+    t.desc.no_iterations++;   /* core.cu:135 */
+    return ER_OK;
+}
+
+Tracer* get_tracer(int id, int& st)
+{
+    Tracer* t = g_tracers.find(id);
+    st = t ? ER_OK : set_error(ER_WARNING, "resource item with ID:%d does not exist", id);
+    return t;
+}
+
+int copy_out(void* dst, const void* src, size_t bytes, size_t size, hipStream_t s)
+{
+    if (!dst) return set_error(ER_ERROR, "output pointer is NULL");
+    if (size < bytes) return set_error(ER_ERROR, "output buffer too small (%zu < %zu bytes)", size, bytes);
+    if (bytes == 0) return ER_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ER_OK;
+}
+
+template<class R, class D, class F>
+int bind_generic(R& reg, D* d, int bind, F make)
+{
+    if (!d) return set_error(ER_ERROR, "descriptor is NULL");
+    if (!bind) {
+        auto it = reg.map.find(d->id);
+        if (d->id < 0 || it == reg.map.end()) return ER_OK;   /* list.cuh:89-93 */
+        delete it->second;
+        reg.map.erase(it);
+        g_scene_version++;
+        return ER_OK;
+    }
+    if (reg.map.size() + 1 >= ER_LIST_CAPACITY) return ER_OK;   /* list.cuh:63-67 */
+    auto* obj = reg.find(d->id);
+    const bool fresh = obj == nullptr;
+    if (fresh) {
+        const int id = reg.counter;
+        if (id >= ER_LIST_CAPACITY * 1024) return set_error(ER_ERROR, "ID space exhausted");
+        d->id = id;
+    }
+    int st = make(obj, *d, fresh);
+    if (st) return st;
+    if (fresh) { reg.map[d->id] = obj; reg.counter++; }
+    g_scene_version++;
+    return ER_OK;
+}
+
+} /* namespace */
+
+/* ============================== C ABI ================================================== */
+extern "C" {
+
+ER_API int er_abi_version(void) { return 1; }
+
+ER_API int er_abi_sizeof(int* sizes, int n)
+{
+    const int s[11] = {(int)sizeof(er_tracer_desc), (int)sizeof(er_volume_desc), (int)sizeof(er_light_desc),
+                       (int)sizeof(er_texture_desc), (int)sizeof(er_clipping_desc), (int)sizeof(er_object_desc),
+                       (int)sizeof(er_bitmap_desc), (int)sizeof(er_plf), (int)sizeof(er_shape),
+                       (int)sizeof(er_timing), (int)sizeof(er_counters)};
+    int k = 0;
+    for (; sizes && k < n && k < 11; ++k) sizes[k] = s[k];
+    return k;
+}
+
+ER_API void er_plf_init(er_plf* p)
+{
+    memset(p, 0, sizeof(*p));
+    p->node_range[0] = FLT_MAX;
+    p->node_range[1] = FLT_MIN;
+}
+
+ER_API void er_plf_add_node(er_plf* p, float position, float value)
+{
+    if (p->count + 1 >= ER_MAX_NO_TF_NODES) return;
+    p->position[p->count] = position;
+    p->value[p->count] = value;
+    if (position < p->node_range[0]) p->node_range[0] = position;
+    if (position > p->node_range[1]) p->node_range[1] = position;
+    p->count++;
+}
+
+static void shape_init(er_shape* s)
+{
+    memset(s, 0, sizeof(*s));
+    for (int i = 0; i < 4; ++i) { s->tm[i][i] = 1.0f; s->inv_tm[i][i] = 1.0f; }
+    s->type = ER_SHAPE_PLANE;
+}
+
+ER_API void er_tracer_desc_init(er_tracer_desc* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->id = -1;
+    d->enabled = 1;
+    er_plf_init(&d->opacity);
+    er_plf_init(&d->glossiness);
+    for (int c = 0; c < 3; ++c) { er_plf_init(&d->diffuse.plf[c]); er_plf_init(&d->specular.plf[c]); er_plf_init(&d->emission.plf[c]); }
+    er_render_settings& s = d->settings;
+    s.step_factor_primary = 0.1f; s.step_factor_shadow = 0.1f; s.shadows = 1; s.max_shadow_distance = 1.0f;
+    s.shading_type = ER_SHADING_BRDF_ONLY; s.density_scale = 100.0f; s.opacity_modulated = 1;
+    s.gradient_computation = ER_GRADIENT_CENTRAL; s.gradient_threshold = 0.5f; s.gradient_factor = 0.5f;
+    s.estimator = ER_ESTIMATOR_COMPAT; s.sampler = ER_SAMPLER_RAYMARCH;
+    s.bilateral = 0; s.bilateral_radius = 3; s.bilateral_sigma_spatial = 2.0f; s.bilateral_sigma_range = 25.0f;
+    for (int i = 0; i < ER_MAX_INDICES; ++i) { d->light_ids[i] = -1; d->object_ids[i] = -1; d->clipping_ids[i] = -1; }
+}
+
+ER_API void er_volume_desc_init(er_volume_desc* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->id = -1; d->enabled = 1;
+    d->spacing[0] = d->spacing[1] = d->spacing[2] = 1.0f;
+}
+
+ER_API void er_light_desc_init(er_light_desc* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->id = -1; d->enabled = 1; d->texture_id = -1; d->unit = ER_UNIT_POWER;
+    shape_init(&d->shape);
+}
+
+ER_API void er_texture_desc_init(er_texture_desc* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->id = -1; d->enabled = 1; d->type = ER_TEXTURE_PROCEDURAL; d->output_level = 1.0f; d->bitmap_id = -1;
+    for (int c = 0; c < 3; ++c) er_plf_init(&d->procedural.gradient.plf[c]);
+}
+
+ER_API void er_clipping_desc_init(er_clipping_desc* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->id = -1; d->enabled = 1;
+    shape_init(&d->shape);
+}
+
+ER_API void er_object_desc_init(er_object_desc* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->id = -1; d->enabled = 1; d->diffuse_texture_id = d->specular_texture_id = d->glossiness_texture_id = -1;
+    d->ior = 5.0f;
+    shape_init(&d->shape);
+}
+
+ER_API void er_bitmap_desc_init(er_bitmap_desc* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->id = -1; d->enabled = 1;
+}
+
+ER_API int er_last_error(char* buf, size_t size)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    if (buf && size) {
+        strncpy(buf, g_last_error.c_str(), size - 1);
+        buf[size - 1] = 0;
+    }
+    return (int)g_last_error.size();
+}
+
+ER_API int er_init(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st = ensure_device();
+    if (st) return st;
+    return ensure_neglog_device();
+}
+
+ER_API int er_shutdown(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    g_tracers.clear(); g_volumes.clear(); g_lights.clear(); g_textures.clear(); g_clips.clear();
+    g_objects.clear(); g_bitmaps.clear();
+    if (g_neglog_dev) { (void)hipFree(g_neglog_dev); g_neglog_dev = nullptr; }
+    g_scene_version++;
+    return ER_OK;
+}
+
+ER_API int er_bind_tracer(er_tracer_desc* d, int bind)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    if (bind) { int st = ensure_device(); if (st) return st; }
+    return bind_generic(g_tracers, d, bind, [](Tracer*& t, er_tracer_desc& desc, bool fresh) -> int {
+        if (fresh) {
+            t = new Tracer();
+            hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+            if (e != hipSuccess) { delete t; t = nullptr; return set_error(ER_ERROR, "%s (hipStreamCreate)", hipGetErrorString(e)); }
+            int st = t->counters.alloc(8);
+            if (!st && hipMemset(t->counters.ptr, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
+                st = set_error(ER_ERROR, "hipMemset failed");
+            if (st) { delete t; t = nullptr; return st; }
+        }
+        t->desc = desc;
+        int st = tracer_resize(*t, desc.camera.film_size[0], desc.camera.film_size[1], desc.id);
+        if (st && fresh) { delete t; t = nullptr; }
+        return st;
+    });
+}
+
+ER_API int er_bind_volume(er_volume_desc* d, int bind)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    if (bind) { int st = ensure_device(); if (st) return st; }
+    return bind_generic(g_volumes, d, bind, [](Volume*& v, er_volume_desc& desc, bool fresh) -> int {
+        if (fresh) v = new Volume();
+        const bool upload = fresh || desc.voxels_dirty;
+        v->desc = desc;
+        if (upload) {
+            int st = upload_volume(*v);
+            if (st) { if (fresh) { delete v; v = nullptr; } return st; }
+            desc.voxels_dirty = 0;   /* Buffer3D dirty-gated copy (buffer3d.h:52-65) */
+        }
+        v->desc.voxels = nullptr;
+        return ER_OK;
+    });
+}
+
+ER_API int er_bind_light(er_light_desc* d, int bind)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    return bind_generic(g_lights, d, bind, [](LightObj*& o, er_light_desc& desc, bool fresh) -> int {
+        if (fresh) o = new LightObj();
+        desc.shape.area = shape_area(desc.shape);   /* Light::operator= -> Shape::Update (light.h:39-45) */
+        o->desc = desc;
+        return ER_OK;
+    });
+}
+
+ER_API int er_bind_object(er_object_desc* d, int bind)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    return bind_generic(g_objects, d, bind, [](ObjectObj*& o, er_object_desc& desc, bool fresh) -> int {
+        if (fresh) o = new ObjectObj();
+        desc.shape.area = shape_area(desc.shape);
+        o->desc = desc;
+        return ER_OK;
+    });
+}
+
+ER_API int er_bind_clipping_object(er_clipping_desc* d, int bind)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    return bind_generic(g_clips, d, bind, [](ClipObj*& o, er_clipping_desc& desc, bool fresh) -> int {
+        if (fresh) o = new ClipObj();
+        o->desc = desc;
+        return ER_OK;
+    });
+}
+
+ER_API int er_bind_texture(er_texture_desc* d, int bind)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    return bind_generic(g_textures, d, bind, [](TextureObj*& o, er_texture_desc& desc, bool fresh) -> int {
+        if (fresh) o = new TextureObj();
+        o->desc = desc;
+        return ER_OK;
+    });
+}
+
+ER_API int er_bind_bitmap(er_bitmap_desc* d, int bind)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    return bind_generic(g_bitmaps, d, bind, [](BitmapObj*& o, er_bitmap_desc& desc, bool fresh) -> int {
+        if (fresh) o = new BitmapObj();
+        const size_t n = (size_t)std::max(0, desc.resolution[0]) * std::max(0, desc.resolution[1]) * 4;
+        if ((fresh || desc.pixels_dirty) && desc.pixels && n) {
+            o->pixels.assign(desc.pixels, desc.pixels + n);
+            desc.pixels_dirty = 0;
+        }
+        o->desc = desc;
+        o->desc.pixels = nullptr;
+        return ER_OK;
+    });
+}
+
+ER_API int er_render_estimate(int tracer_id)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if ((st = build_params(*t))) return st;
+    if ((st = enqueue_pass(*t, nullptr, nullptr, nullptr))) return st;
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return ER_OK;
+}
+
+ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if (passes < 0) return set_error(ER_ERROR, "negative pass count");
+    if ((st = build_params(*t))) return st;
+    std::vector<hipEvent_t> ev;
+    if (timing) {
+        ev.resize((size_t)passes * 3, nullptr);
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    }
+    for (int k = 0; k < passes && !st; ++k)
+        st = enqueue_pass(*t, timing ? ev[3 * k] : nullptr, timing ? ev[3 * k + 1] : nullptr, timing ? ev[3 * k + 2] : nullptr);
+    hipError_t se = hipStreamSynchronize(t->stream);
+    if (timing && !st && se == hipSuccess) {
+        float total = 0, tr = 0, po = 0, ms = 0;
+        for (int k = 0; k < passes; ++k) {
+            if (hipEventElapsedTime(&ms, ev[3 * k], ev[3 * k + 1]) == hipSuccess) tr += ms;
+            if (hipEventElapsedTime(&ms, ev[3 * k + 1], ev[3 * k + 2]) == hipSuccess) po += ms;
+        }
+        if (passes > 0 && hipEventElapsedTime(&ms, ev[0], ev[3 * (passes - 1) + 2]) == hipSuccess) total = ms;
+        timing->total_ms = total; timing->trace_ms = tr; timing->post_ms = po; timing->passes = passes;
+    }
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (st) return st;
+    if (se != hipSuccess) return set_error(ER_ERROR, "%s (hipStreamSynchronize)", hipGetErrorString(se));
+    return ER_OK;
+}
+
+ER_API int er_get_estimate(int tracer_id, uint8_t* rgba, size_t size)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    const void* src = use_bilateral(*t) ? (const void*)t->disp_filt.ptr : (const void*)t->display.ptr;
+    return copy_out(rgba, src, (size_t)t->W * t->H * 4, size, t->stream);
+}
+
+ER_API int er_get_frame_xyza(int tracer_id, float* xyza, size_t size)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    return copy_out(xyza, t->frame.ptr, (size_t)t->W * t->H * 16, size, t->stream);
+}
+
+ER_API int er_get_running_xyza(int tracer_id, float* xyza, size_t size)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    return copy_out(xyza, t->running.ptr, (size_t)t->W * t->H * 16, size, t->stream);
+}
+
+ER_API int er_get_no_iterations(int tracer_id, int* n)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if (!n) return set_error(ER_ERROR, "output pointer is NULL");
+    *n = t->desc.no_iterations;
+    return ER_OK;
+}
+
+ER_API int er_get_autofocus_distance(int tracer_id, int film_u, int film_v, float* distance)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    (void)film_u; (void)film_v;
+    if (!distance) return set_error(ER_ERROR, "output pointer is NULL");
+    /* The reference entry point is a stub (core.cu:145-148): the distance is left unchanged. */
+    return ER_OK;
+}
+
+ER_API int er_get_seeds(int tracer_id, uint32_t* s1, uint32_t* s2, size_t count)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    const size_t n = (size_t)t->W * t->H;
+    if ((st = copy_out(s1, t->seeds1.ptr, n * 4, count * 4, t->stream))) return st;
+    return copy_out(s2, t->seeds2.ptr, n * 4, count * 4, t->stream);
+}
+
+ER_API int er_set_seeds(int tracer_id, const uint32_t* s1, const uint32_t* s2, size_t count)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    const size_t n = (size_t)t->W * t->H;
+    if (!s1 || !s2) return set_error(ER_ERROR, "seed pointer is NULL");
+    if (count < n) return set_error(ER_ERROR, "need %zu seeds, got %zu", n, count);
+    HIP_TRY(hipMemcpy(t->seeds1.ptr, s1, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t->seeds2.ptr, s2, n * 4, hipMemcpyHostToDevice));
+    return ER_OK;
+}
+
+ER_API int er_seed_tracer(int tracer_id, uint64_t seed)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    return tracer_seed(*t, seed);
+}
+
+ER_API int er_set_counting(int tracer_id, int on)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    t->counting = on ? 1 : 0;
+    return ER_OK;
+}
+
+ER_API int er_get_counters(int tracer_id, er_counters* c)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    unsigned long long v[8];
+    if ((st = copy_out(v, t->counters.ptr, sizeof(v), sizeof(v), t->stream))) return st;
+    c->paths = v[0]; c->primary_samples = v[1]; c->shadow_samples = v[2]; c->shading_samples = v[3];
+    c->primary_fetches = v[4]; c->shadow_fetches = v[5]; c->scatter_events = v[6]; c->shadow_rays = v[7];
+    return ER_OK;
+}
+
+ER_API int er_reset_counters(int tracer_id)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    HIP_TRY(hipMemsetAsync(t->counters.ptr, 0, 8 * sizeof(unsigned long long), t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return ER_OK;
+}
+
+ER_API int er_get_gradient_magnitude_max(int tracer_id, float* gm)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if ((st = build_params(*t))) return st;
+    *gm = t->gm_max;
+    return ER_OK;
+}
+
+ER_API int er_neglog_table_entry(uint32_t m, float* value)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    if (!value) return set_error(ER_ERROR, "output pointer is NULL");
+    if (m >= (1u << ER_NEGLOG_BITS)) return set_error(ER_ERROR, "m out of range");
+    build_neglog_host();
+    *value = g_neglog_host[m];
+    return ER_OK;
+}
+
+} /* extern "C" */

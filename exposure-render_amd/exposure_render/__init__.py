@@ -1,0 +1,161 @@
+"""Python host binding of the MI355X single-scattering volume tracer.
+
+The product is the native library lib/libexposurerender_amd.so (gfx950 HIP kernels behind the
+C-ABI of include/er_capi.h). This module only loads it and offers a thin object wrapper over
+the reference's binding API (Source/exposurerender.h:32-42): bind descriptors, render
+progressive passes, read the estimate back. Every call fails loudly when the native library is
+missing or returns a non-zero status; there is no fallback implementation.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import load, check_abi  # noqa: F401
+
+LEVELS = {capi.ER_INFO: "Info", capi.ER_WARNING: "Warning", capi.ER_ERROR: "Error", capi.ER_FATAL: "Fatal"}
+
+
+class ExposureRenderError(RuntimeError):
+    """ExposureRender::Exception (Source/exception.h:27-55): level + message."""
+
+    def __init__(self, status, message):
+        self.status = status
+        self.level = LEVELS.get(status, "Error")
+        super().__init__(f"[{self.level}] {message}")
+
+
+def _check(lib, status):
+    if status != capi.ER_OK:
+        buf = C.create_string_buffer(512)
+        lib.er_last_error(buf, 512)
+        raise ExposureRenderError(status, buf.value.decode(errors="replace"))
+    return status
+
+
+class Renderer:
+    """Binds one Scene (scenes.Scene) through the C-ABI and renders it."""
+
+    def __init__(self, scene, seed=None):
+        """Binds copies of the scene's descriptors. The scene refers to textures, lights and
+        clipping objects by their position in its lists; the copies are remapped to the IDs the
+        registry assigns."""
+        self.lib = load()
+        self.scene = scene
+        lib = self.lib
+
+        def cp(d):
+            return type(d).from_buffer_copy(d)
+
+        self.volume = cp(scene.volume)
+        self.volume.voxels = scene.volume.voxels
+        _check(lib, lib.er_bind_volume(C.byref(self.volume), 1))
+        self.textures = [cp(T) for T in scene.textures]
+        for T in self.textures:
+            _check(lib, lib.er_bind_texture(C.byref(T), 1))
+        self.lights = [cp(L) for L in scene.lights]
+        for L in self.lights:
+            if 0 <= L.texture_id < len(self.textures):
+                L.texture_id = self.textures[L.texture_id].id
+            _check(lib, lib.er_bind_light(C.byref(L), 1))
+        self.clips = [cp(Cb) for Cb in scene.clips]
+        for Cb in self.clips:
+            _check(lib, lib.er_bind_clipping_object(C.byref(Cb), 1))
+        t = cp(scene.tracer)
+        t.volume_id = self.volume.id
+        for i in range(t.light_count):
+            j = scene.tracer.light_ids[i]
+            t.light_ids[i] = self.lights[j].id if 0 <= j < len(self.lights) else -1
+        for i in range(t.clipping_count):
+            j = scene.tracer.clipping_ids[i]
+            t.clipping_ids[i] = self.clips[j].id if 0 <= j < len(self.clips) else -1
+        self.tracer = t
+        _check(lib, lib.er_bind_tracer(C.byref(t), 1))
+        self.id = t.id
+        self.W, self.H = scene.film
+        if seed is not None:
+            self.seed(seed)
+
+    # -- reference entry points ------------------------------------------------------------
+    def render_estimate(self):
+        _check(self.lib, self.lib.er_render_estimate(self.id))
+
+    def get_estimate(self):
+        out = np.zeros((self.H, self.W, 4), dtype=np.uint8)
+        _check(self.lib, self.lib.er_get_estimate(self.id, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.nbytes))
+        return out
+
+    def no_iterations(self):
+        n = C.c_int(0)
+        _check(self.lib, self.lib.er_get_no_iterations(self.id, C.byref(n)))
+        return n.value
+
+    # -- extensions -------------------------------------------------------------------------
+    def render_passes(self, passes):
+        t = capi.Timing()
+        _check(self.lib, self.lib.er_render_passes(self.id, passes, C.byref(t)))
+        return t
+
+    def seed(self, seed):
+        _check(self.lib, self.lib.er_seed_tracer(self.id, int(seed)))
+
+    def set_seeds(self, s1, s2):
+        s1 = np.ascontiguousarray(s1, dtype=np.uint32)
+        s2 = np.ascontiguousarray(s2, dtype=np.uint32)
+        _check(self.lib, self.lib.er_set_seeds(self.id, s1.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                               s2.ctypes.data_as(C.POINTER(C.c_uint32)), s1.size))
+
+    def get_seeds(self):
+        n = self.W * self.H
+        s1 = np.zeros(n, dtype=np.uint32)
+        s2 = np.zeros(n, dtype=np.uint32)
+        _check(self.lib, self.lib.er_get_seeds(self.id, s1.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                               s2.ctypes.data_as(C.POINTER(C.c_uint32)), n))
+        return s1, s2
+
+    def frame(self):
+        out = np.zeros((self.H, self.W, 4), dtype=np.float32)
+        _check(self.lib, self.lib.er_get_frame_xyza(self.id, out.ctypes.data_as(C.POINTER(C.c_float)), out.nbytes))
+        return out
+
+    def running(self):
+        out = np.zeros((self.H, self.W, 4), dtype=np.float32)
+        _check(self.lib, self.lib.er_get_running_xyza(self.id, out.ctypes.data_as(C.POINTER(C.c_float)), out.nbytes))
+        return out
+
+    def set_counting(self, on):
+        _check(self.lib, self.lib.er_set_counting(self.id, 1 if on else 0))
+
+    def counters(self):
+        c = capi.Counters()
+        _check(self.lib, self.lib.er_get_counters(self.id, C.byref(c)))
+        return c.as_dict()
+
+    def reset_counters(self):
+        _check(self.lib, self.lib.er_reset_counters(self.id))
+
+    def gm_max(self):
+        v = C.c_float(0)
+        _check(self.lib, self.lib.er_get_gradient_magnitude_max(self.id, C.byref(v)))
+        return v.value
+
+    def rebind(self):
+        """BindTracer with the existing ID: restarts accumulation (NoIterations from the desc)."""
+        self.tracer.no_iterations = 0
+        _check(self.lib, self.lib.er_bind_tracer(C.byref(self.tracer), 1))
+
+    def close(self):
+        lib = self.lib
+        lib.er_bind_tracer(C.byref(self.tracer), 0)
+        for Cb in self.clips:
+            lib.er_bind_clipping_object(C.byref(Cb), 0)
+        for L in self.lights:
+            lib.er_bind_light(C.byref(L), 0)
+        for T in self.textures:
+            lib.er_bind_texture(C.byref(T), 0)
+        lib.er_bind_volume(C.byref(self.volume), 0)
+
+
+def shutdown():
+    lib = load()
+    _check(lib, lib.er_shutdown())

@@ -1,0 +1,339 @@
+"""Scene descriptors and the synthetic benchmark scenes.
+
+`Scene` bundles the descriptors one tracer needs (tracer, volume, lights, textures, clipping
+objects) as the C-ABI structs, plus the voxel array that backs the volume descriptor. The
+`config_*` builders produce the five benchmark workloads (plain small, CT-like, large
+anisotropic, differentiation, batching) from seeded phantoms; `scale` shrinks image and
+volume for quick parity tests while keeping the scene content.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import capi
+
+
+# ---- transforms --------------------------------------------------------------------------
+
+def _rigid(R, t):
+    tm = np.eye(4)
+    tm[:3, :3] = R
+    tm[:3, 3] = t
+    inv = np.eye(4)
+    inv[:3, :3] = R.T
+    inv[:3, 3] = -R.T @ np.asarray(t, dtype=np.float64)
+    return tm, inv
+
+
+def look_at(position, target, up=(0.0, 1.0, 0.0)):
+    """TM / InvTM of a shape at `position` whose local +z axis points at `target`."""
+    p = np.asarray(position, dtype=np.float64)
+    z = np.asarray(target, dtype=np.float64) - p
+    z /= np.linalg.norm(z)
+    u = np.asarray(up, dtype=np.float64)
+    if abs(float(np.dot(u, z))) > 0.99:
+        u = np.array([1.0, 0.0, 0.0])
+    x = np.cross(u, z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    return _rigid(np.stack([x, y, z], axis=1), p)
+
+
+def translate(t):
+    return _rigid(np.eye(3), t)
+
+
+def set_matrix(dst, m):
+    for i in range(4):
+        for j in range(4):
+            dst[i][j] = float(m[i, j])
+
+
+# ---- descriptor defaults (the reference constructors) ------------------------------------
+
+def default_settings(s):
+    """RenderSettings() defaults (Source/rendersettings.h:33-38,69-76)."""
+    s.step_factor_primary = 0.1
+    s.step_factor_shadow = 0.1
+    s.shadows = 1
+    s.max_shadow_distance = 1.0
+    s.shading_type = capi.SHADING_BRDF_ONLY
+    s.density_scale = 100.0
+    s.opacity_modulated = 1
+    s.gradient_computation = capi.GRADIENT_CENTRAL
+    s.gradient_threshold = 0.5
+    s.gradient_factor = 0.5
+    s.estimator = capi.ESTIMATOR_COMPAT
+    s.sampler = capi.SAMPLER_RAYMARCH
+    s.bilateral = 0
+    s.bilateral_radius = 3
+    s.bilateral_sigma_spatial = 2.0
+    s.bilateral_sigma_range = 25.0
+
+
+def new_tracer():
+    t = capi.TracerDesc()
+    t.id = -1
+    t.enabled = 1
+    t.opacity.init()
+    t.diffuse.init()
+    t.specular.init()
+    t.glossiness.init()
+    t.emission.init()
+    default_settings(t.settings)
+    t.no_iterations = 0
+    t.volume_id = 0
+    for i in range(capi.MAX_INDICES):
+        t.light_ids[i] = -1
+        t.object_ids[i] = -1
+        t.clipping_ids[i] = -1
+    return t
+
+
+def set_camera(t, film, pos, target, up=(0.0, 1.0, 0.0), fov=35.0, exposure=1.0,
+               clip_near=0.0, clip_far=1000.0, aperture=0.0, focal=-1.0):
+    c = t.camera
+    c.film_size[0], c.film_size[1] = int(film[0]), int(film[1])
+    for i in range(3):
+        c.pos[i], c.target[i], c.up[i] = pos[i], target[i], up[i]
+    c.focal_distance = focal
+    c.aperture_size = aperture
+    c.clip_near = clip_near
+    c.clip_far = clip_far
+    c.exposure = exposure
+    c.gamma = 2.2
+    c.fov = fov
+
+
+def plane_light(pos, target, size, multiplier, texture_id=0, one_sided=0, visible=1, unit=capi.UNIT_POWER):
+    L = capi.LightDesc()
+    L.id = -1
+    L.enabled = 1
+    L.visible = visible
+    L.texture_id = texture_id
+    L.multiplier = multiplier
+    L.unit = unit
+    tm, inv = look_at(pos, target)
+    set_matrix(L.shape.tm, tm)
+    set_matrix(L.shape.inv_tm, inv)
+    L.shape.one_sided = one_sided
+    L.shape.type = capi.SHAPE_PLANE
+    L.shape.size[0], L.shape.size[1], L.shape.size[2] = size[0], size[1], 0.0
+    return L
+
+
+def sphere_light(radius, multiplier, texture_id=0, center=(0.0, 0.0, 0.0), visible=1):
+    """A uniform environment: a large two-sided sphere light around the scene."""
+    L = capi.LightDesc()
+    L.id = -1
+    L.enabled = 1
+    L.visible = visible
+    L.texture_id = texture_id
+    L.multiplier = multiplier
+    L.unit = capi.UNIT_POWER
+    tm, inv = translate(center)
+    set_matrix(L.shape.tm, tm)
+    set_matrix(L.shape.inv_tm, inv)
+    L.shape.one_sided = 0
+    L.shape.type = capi.SHAPE_SPHERE
+    L.shape.outer_radius = radius
+    return L
+
+
+def uniform_texture(rgb=(1.0, 1.0, 1.0), output_level=1.0):
+    T = capi.TextureDesc()
+    T.id = -1
+    T.enabled = 1
+    T.type = capi.TEXTURE_PROCEDURAL
+    T.output_level = output_level
+    T.bitmap_id = -1
+    T.procedural.type = capi.PROCEDURAL_UNIFORM
+    for i in range(3):
+        T.procedural.uniform_color[i] = rgb[i]
+    T.procedural.gradient.init()
+    return T
+
+
+def clipping_box(center, size, invert=0):
+    Cb = capi.ClippingDesc()
+    Cb.id = -1
+    Cb.enabled = 1
+    Cb.invert = invert
+    tm, inv = translate(center)
+    set_matrix(Cb.shape.tm, tm)
+    set_matrix(Cb.shape.inv_tm, inv)
+    Cb.shape.type = capi.SHAPE_BOX
+    for i in range(3):
+        Cb.shape.size[i] = size[i]
+    return Cb
+
+
+class Scene:
+    """One tracer with its volume, lights, textures and clipping objects. IDs follow the
+    reference's registry: each kind counts from 0 in bind order, so list index == ID."""
+
+    def __init__(self, voxels, spacing=(1.0, 1.0, 1.0)):
+        self.voxels = np.ascontiguousarray(voxels, dtype=np.uint16)   # [z, y, x]
+        self.volume = capi.VolumeDesc()
+        self.volume.id = -1
+        self.volume.enabled = 1
+        rz, ry, rx = self.voxels.shape
+        self.volume.resolution[0], self.volume.resolution[1], self.volume.resolution[2] = rx, ry, rz
+        for i in range(3):
+            self.volume.spacing[i] = spacing[i]
+        self.volume.normalize_size = 1
+        self.volume.voxels_dirty = 1
+        self.volume.voxels = self.voxels.ctypes.data_as(C.POINTER(C.c_uint16))
+        self.tracer = new_tracer()
+        self.lights = []
+        self.textures = []
+        self.clips = []
+        self.name = "scene"
+
+    @property
+    def film(self):
+        return self.tracer.camera.film_size[0], self.tracer.camera.film_size[1]
+
+    def add_texture(self, T):
+        self.textures.append(T)
+        return len(self.textures) - 1
+
+    def add_light(self, L):
+        self.lights.append(L)
+        i = len(self.lights) - 1
+        self.tracer.light_ids[self.tracer.light_count] = i
+        self.tracer.light_count += 1
+        return i
+
+    def add_clip(self, Cb):
+        self.clips.append(Cb)
+        i = len(self.clips) - 1
+        self.tracer.clipping_ids[self.tracer.clipping_count] = i
+        self.tracer.clipping_count += 1
+        return i
+
+    def array(self, kind):
+        lst = {"lights": self.lights, "textures": self.textures, "clips": self.clips}[kind]
+        typ = {"lights": capi.LightDesc, "textures": capi.TextureDesc, "clips": capi.ClippingDesc}[kind]
+        arr = (typ * max(1, len(lst)))()
+        for i, d in enumerate(lst):
+            arr[i] = d
+        return arr, len(lst)
+
+
+# ---- phantoms (native generators, seeded) -------------------------------------------------
+
+def phantom_blobs(shape, n_blobs, sigma=(0.08, 0.2), peak=(1000.0, 4000.0), noise=50.0, seed=1):
+    rz, ry, rx = shape
+    out = np.empty((rz, ry, rx), dtype=np.uint16)
+    lib = capi.load()
+    st = lib.er_phantom_blobs(out.ctypes.data_as(C.POINTER(C.c_uint16)), rx, ry, rz, n_blobs,
+                              sigma[0], sigma[1], peak[0], peak[1], noise, seed)
+    if st != 0:
+        raise RuntimeError("er_phantom_blobs failed")
+    return out
+
+
+def phantom_ellipsoids(shape, n_ellipsoids, n_tubes=0, axes=(0.05, 0.45), n_bands=5,
+                       max_intensity=4095.0, noise_sigma=20.0, seed=2):
+    rz, ry, rx = shape
+    out = np.empty((rz, ry, rx), dtype=np.uint16)
+    lib = capi.load()
+    st = lib.er_phantom_ellipsoids(out.ctypes.data_as(C.POINTER(C.c_uint16)), rx, ry, rz,
+                                   n_ellipsoids, n_tubes, axes[0], axes[1], n_bands,
+                                   max_intensity, noise_sigma, seed)
+    if st != 0:
+        raise RuntimeError("er_phantom_ellipsoids failed")
+    return out
+
+
+# ---- the benchmark workloads ---------------------------------------------------------------
+
+def _scaled(n, scale):
+    return max(8, int(round(n * scale)))
+
+
+def config_small(scale=1.0, estimator=capi.ESTIMATOR_FULL, seed=1):
+    """Plain tracer, small: 64^3 blobs, 4-node TF, density scale 10, one 0.5 x 0.5 plane
+    light of intensity 100, pinhole camera, 256 x 256, phase-only shading, 16 passes."""
+    r = _scaled(64, scale)
+    vox = phantom_blobs((r, r, r), 4, seed=seed)
+    sc = Scene(vox)
+    sc.name = "small_64c_256px"
+    t = sc.tracer
+    t.opacity.add(0.0, 0.0).add(400.0, 0.0).add(1500.0, 0.6).add(4000.0, 1.0)
+    t.diffuse.add(0.0, (1.0, 1.0, 1.0)).add(400.0, (0.9, 0.55, 0.4)).add(1500.0, (0.85, 0.75, 0.6)).add(4000.0, (1.0, 1.0, 1.0))
+    t.specular.add(0.0, (0.0, 0.0, 0.0)).add(4000.0, (0.0, 0.0, 0.0))
+    t.glossiness.add(0.0, 0.5).add(4000.0, 0.5)
+    t.settings.density_scale = 10.0
+    t.settings.estimator = estimator
+    t.settings.shading_type = capi.SHADING_PHASE_ONLY
+    px = _scaled(256, scale)
+    set_camera(t, (px, px), (1.2, 0.9, 1.6), (0.0, 0.0, 0.0), exposure=4.0)
+    tex = sc.add_texture(uniform_texture((1.0, 1.0, 1.0)))
+    sc.add_light(plane_light((0.9, 1.0, 0.6), (0.0, 0.0, 0.0), (0.5, 0.5), 100.0, texture_id=tex))
+    sc.passes = 16
+    return sc
+
+
+def _ct_tf(t):
+    t.opacity.add(0.0, 0.0).add(300.0, 0.0).add(800.0, 0.02).add(1100.0, 0.1).add(1500.0, 0.3) \
+        .add(2500.0, 0.5).add(3500.0, 0.9).add(4095.0, 1.0)
+    t.diffuse.add(0.0, (0.8, 0.8, 0.8)).add(300.0, (0.9, 0.7, 0.6)).add(800.0, (0.85, 0.45, 0.35)) \
+        .add(1100.0, (0.8, 0.3, 0.25)).add(1500.0, (0.9, 0.6, 0.5)).add(2500.0, (0.95, 0.9, 0.8)) \
+        .add(3500.0, (1.0, 1.0, 0.95)).add(4095.0, (1.0, 1.0, 1.0))
+    t.specular.add(0.0, (0.05, 0.05, 0.05)).add(1500.0, (0.2, 0.2, 0.2)).add(4095.0, (0.5, 0.5, 0.5))
+    t.glossiness.add(0.0, 0.3).add(1500.0, 0.5).add(4095.0, 0.8)
+    t.emission.add(0.0, (0.0, 0.0, 0.0)).add(4095.0, (0.0, 0.0, 0.0))
+
+
+def config_ct(scale=1.0, estimator=capi.ESTIMATOR_FULL, seed=2, bilateral=True):
+    """Plain tracer, CT-like phantom: 256^3, 12 nested ellipsoids in 5 bands + N(0, 20^2),
+    1024 x 768, 2 plane lights + uniform environment sphere, hybrid shading, 8-node TF,
+    64 passes, bilateral display filter, exponential tonemap."""
+    r = _scaled(256, scale)
+    vox = phantom_ellipsoids((r, r, r), 12, 0, (0.05, 0.45), 5, 4095.0, 20.0, seed)
+    sc = Scene(vox)
+    sc.name = "ct_256c_1024x768"
+    t = sc.tracer
+    _ct_tf(t)
+    t.settings.density_scale = 100.0
+    t.settings.estimator = estimator
+    t.settings.shading_type = capi.SHADING_HYBRID
+    t.settings.bilateral = 1 if bilateral else 0
+    set_camera(t, (_scaled(1024, scale), _scaled(768, scale)), (0.9, 0.6, 1.9), (0.0, 0.0, 0.0), exposure=6.0)
+    tex = sc.add_texture(uniform_texture((1.0, 1.0, 1.0)))
+    sc.add_light(plane_light((1.0, 1.2, 0.8), (0.0, 0.0, 0.0), (0.6, 0.6), 80.0, texture_id=tex))
+    sc.add_light(plane_light((-1.1, 0.4, 0.9), (0.0, 0.0, 0.0), (0.4, 0.4), 50.0, texture_id=tex))
+    sc.add_light(sphere_light(10.0, 0.3, texture_id=tex))
+    sc.passes = 64
+    return sc
+
+
+def config_large(scale=1.0, estimator=capi.ESTIMATOR_FULL, seed=3):
+    """Plain tracer, large anisotropic volume: 512 x 512 x 460, 0.7 x 0.7 x 1.0 spacing,
+    30 cylinders and ellipsoids + N(0, 30^2), clipping box removing 30% of the volume,
+    1920 x 1080, 3 plane lights, hybrid shading, 256 passes."""
+    rx, rz = _scaled(512, scale), _scaled(460, scale)
+    vox = phantom_ellipsoids((rz, rx, rx), 15, 15, (0.05, 0.45), 5, 4095.0, 30.0, seed)
+    sc = Scene(vox, spacing=(0.7, 0.7, 1.0))
+    sc.name = "large_512x512x460_1080p"
+    t = sc.tracer
+    _ct_tf(t)
+    t.settings.density_scale = 100.0
+    t.settings.estimator = estimator
+    t.settings.shading_type = capi.SHADING_HYBRID
+    set_camera(t, (_scaled(1920, scale), _scaled(1080, scale)), (1.0, 0.7, 1.8), (0.0, 0.0, 0.0), exposure=6.0)
+    tex = sc.add_texture(uniform_texture((1.0, 1.0, 1.0)))
+    sc.add_light(plane_light((1.0, 1.2, 0.8), (0.0, 0.0, 0.0), (0.6, 0.6), 80.0, texture_id=tex))
+    sc.add_light(plane_light((-1.1, 0.4, 0.9), (0.0, 0.0, 0.0), (0.4, 0.4), 50.0, texture_id=tex))
+    sc.add_light(plane_light((0.2, -1.2, 0.9), (0.0, 0.0, 0.0), (0.5, 0.5), 40.0, texture_id=tex))
+    # world extent of the volume after normalisation: 0.7*512 : 0.7*512 : 460 -> longest axis 1
+    ext = np.array([0.7 * rx, 0.7 * rx, 1.0 * rz])
+    ext = ext / ext.max()
+    # remove 30% of the volume: a box over the +x side spanning 30% of the x extent
+    cx = 0.5 * ext[0] - 0.15 * ext[0]
+    sc.add_clip(clipping_box((cx, 0.0, 0.0), (0.3 * ext[0] + 1e-3, ext[1] + 0.01, ext[2] + 0.01)))
+    sc.passes = 256
+    return sc

@@ -1,0 +1,368 @@
+/*
+ * er_capi.h - the C-ABI drop-in boundary of the MI355X single-scattering volume tracer.
+ *
+ * Every entry point returns an int status (ER_OK == 0). A non-zero status carries the
+ * exception level the reference would have thrown (Source/exception.h:27-55,
+ * Source/enums.h:69-75) and er_last_error() returns its message.
+ *
+ * The descriptor structs are plain-old-data mirrors of the reference's bindable classes.
+ * They hold only int32, float and pointer fields, so C, C++, ctypes and cgo lay them out
+ * identically; er_abi_sizeof() lets a binding check that.
+ */
+#ifndef ER_CAPI_H
+#define ER_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define ER_API __attribute__((visibility("default")))
+#else
+#define ER_API
+#endif
+
+/* ---- constants ------------------------------------------------------------------------ */
+
+#define ER_MAX_NO_TF_NODES   128   /* Source/defines.h:77; a PLF holds at most 127 nodes (plf.h:53) */
+#define ER_MAX_INDICES       256   /* Source/vector.h:554-579 (Indices: 256 ints + Count) */
+#define ER_LIST_CAPACITY     32    /* Source/list.cuh:27 (MaxSize); 31 live objects per kind */
+
+/* status codes: 0 = success, otherwise the reference's ExceptionLevel + 1 */
+#define ER_OK        0
+#define ER_INFO      1   /* Enums::Info    */
+#define ER_WARNING   2   /* Enums::Warning: unknown resource ID (list.cuh:150-164) */
+#define ER_ERROR     3   /* Enums::Error:   device failure (wrapper.cuh:38-46), bad argument */
+#define ER_FATAL     4   /* Enums::Fatal    */
+
+/* Enums::ShapeType (Source/enums.h:52-61) */
+#define ER_SHAPE_PLANE    0
+#define ER_SHAPE_DISK     1
+#define ER_SHAPE_RING     2
+#define ER_SHAPE_BOX      3
+#define ER_SHAPE_SPHERE   4
+#define ER_SHAPE_CYLINDER 5
+
+/* Enums::ShadingMode (Source/enums.h:63-71) */
+#define ER_SHADING_BRDF_ONLY           0
+#define ER_SHADING_PHASE_ONLY          1
+#define ER_SHADING_HYBRID              2
+#define ER_SHADING_MODULATION          3
+#define ER_SHADING_THRESHOLD           4
+#define ER_SHADING_GRADIENT_MAGNITUDE  5
+
+/* Enums::GradientMode (Source/enums.h:73-78) */
+#define ER_GRADIENT_FORWARD  0
+#define ER_GRADIENT_CENTRAL  1
+#define ER_GRADIENT_FILTERED 2
+
+/* Enums::EmissionUnit (Source/enums.h:87-92) */
+#define ER_UNIT_POWER     0
+#define ER_UNIT_LUX       1
+#define ER_UNIT_INTENSITY 2
+
+/* Enums::TextureType / ProceduralType (Source/enums.h:40-50) */
+#define ER_TEXTURE_PROCEDURAL 0
+#define ER_TEXTURE_BITMAP     1
+#define ER_PROCEDURAL_UNIFORM  0
+#define ER_PROCEDURAL_CHECKER  1
+#define ER_PROCEDURAL_GRADIENT 2
+
+/* Estimator modes (new). COMPAT reproduces the live reference estimator bit-for-bit in RNG
+ * use and integer decisions; FULL enables the designed-but-dead features (shading-mode
+ * selection, both halves of MIS, clipping objects, bilateral display filter). */
+#define ER_ESTIMATOR_COMPAT 0
+#define ER_ESTIMATOR_FULL   1
+
+/* Free-flight samplers (new). RAYMARCH = Source/raymarching.h:30-113 (jittered fixed-step
+ * tracking); WOODCOCK = delta tracking against per-brick majorants. */
+#define ER_SAMPLER_RAYMARCH 0
+#define ER_SAMPLER_WOODCOCK 1
+
+/* ---- descriptors ---------------------------------------------------------------------- */
+
+/* PiecewiseLinearFunction<128> (Source/plf.h:26-92 over Source/pf.h:26-71). Initialise
+ * with er_plf_init (node_range = {FLT_MAX, FLT_MIN}, count = 0) and fill with
+ * er_plf_add_node, or fill the fields directly. */
+typedef struct er_plf {
+    float node_range[2];
+    float position[ER_MAX_NO_TF_NODES];
+    float value[ER_MAX_NO_TF_NODES];
+    int   count;
+} er_plf;
+
+/* ColorTransferFunction1D (Source/transferfunction.h:117-154): one PLF per XYZ channel. */
+typedef struct er_color_tf {
+    er_plf plf[3];
+} er_color_tf;
+
+/* Camera (Source/camera.h:98-109): the caller-set fields. The derived fields
+ * (N, U, V, Screen, InvScreen; camera.h:61-96) are computed at bind time. */
+typedef struct er_camera {
+    int   film_size[2];
+    float pos[3];
+    float target[3];
+    float up[3];
+    float focal_distance;   /* -1 => |target - pos| (camera.h:69-70) */
+    float aperture_size;
+    float clip_near;
+    float clip_far;
+    float exposure;
+    float gamma;
+    float fov;              /* degrees */
+} er_camera;
+
+/* RenderSettings (Source/rendersettings.h:27-131) plus the settings this build adds. */
+typedef struct er_render_settings {
+    /* TraversalSettings (rendersettings.h:33-54) */
+    float step_factor_primary;     /* default 0.1 */
+    float step_factor_shadow;      /* default 0.1 */
+    int   shadows;                 /* default 1   */
+    float max_shadow_distance;     /* default 1.0 */
+    /* ShadingSettings (rendersettings.h:66-98) */
+    int   shading_type;            /* ER_SHADING_*; ignored in COMPAT (the live code is BRDF-only) */
+    float density_scale;           /* default 100 */
+    int   opacity_modulated;       /* default 1 */
+    int   gradient_computation;    /* ER_GRADIENT_*; default central */
+    float gradient_threshold;      /* default 0.5 */
+    float gradient_factor;         /* default 0.5 */
+    /* new */
+    int   estimator;               /* ER_ESTIMATOR_* */
+    int   sampler;                 /* ER_SAMPLER_*   */
+    int   bilateral;               /* FULL only: bilateral filter on the RGBA8 display image */
+    int   bilateral_radius;        /* taps either side (Source/filter.h:34-40 KernelRadius) */
+    float bilateral_sigma_spatial; /* KernelD[k] = exp(-k^2 / (2 s^2)) */
+    float bilateral_sigma_range;   /* GaussSimilarity[d] = exp(-d^2 / (2 s^2)), d = |dY| in u8 units */
+} er_render_settings;
+
+/* ErTracer (Source/ertracer.h:33-117). */
+typedef struct er_tracer_desc {
+    int          id;                /* in/out: -1 before the first bind (erbindable.h:31-36,65) */
+    int          enabled;
+    er_plf       opacity;           /* Opacity1D    */
+    er_color_tf  diffuse;           /* Diffuse1D    */
+    er_color_tf  specular;          /* Specular1D   */
+    er_plf       glossiness;        /* Glossiness1D */
+    er_color_tf  emission;          /* Emission1D   */
+    er_camera    camera;
+    er_render_settings settings;
+    int          no_iterations;     /* copied on (re)bind; 0 restarts accumulation */
+    int          volume_id;
+    int          light_ids[ER_MAX_INDICES];
+    int          light_count;
+    int          object_ids[ER_MAX_INDICES];
+    int          object_count;
+    int          clipping_ids[ER_MAX_INDICES];
+    int          clipping_count;
+} er_tracer_desc;
+
+/* Shape (Source/shape.h:47-112). The caller supplies both tm and inv_tm; area is derived
+ * at bind time (Shape::Update, shape.h:91-103). Matrices are row-major NN[row][col]. */
+typedef struct er_shape {
+    float tm[4][4];
+    float inv_tm[4][4];
+    int   one_sided;
+    int   type;                     /* ER_SHAPE_* */
+    float size[3];
+    float area;                     /* output of bind */
+    float inner_radius;
+    float outer_radius;
+} er_shape;
+
+/* ErVolume (Source/ervolume.h:28-74). voxels: resolution[0]*resolution[1]*resolution[2]
+ * uint16, x fastest (index z*Rx*Ry + y*Rx + x, Source/buffer3d.h:210-214). The voxels are
+ * uploaded when voxels_dirty is set (Buffer3D dirty-gated copy, buffer3d.h:52-65); bind
+ * clears it, so an unchanged rebind skips the upload. */
+typedef struct er_volume_desc {
+    int             id;
+    int             enabled;
+    int             resolution[3];
+    float           spacing[3];
+    int             normalize_size; /* 0 => zero-size volume, as the reference (volume.h:106-117) */
+    int             voxels_dirty;
+    const uint16_t* voxels;
+} er_volume_desc;
+
+/* ErLight (Source/erlight.h:27-66). */
+typedef struct er_light_desc {
+    int      id;
+    int      enabled;
+    int      visible;               /* visible to camera rays (lights.h:102) */
+    int      texture_id;            /* < 0 => emits nothing (textures.h:66-67) */
+    float    multiplier;
+    int      unit;                  /* ER_UNIT_*; LUX divides by the area (lights.h:55-56) */
+    er_shape shape;
+} er_light_desc;
+
+/* Procedural (Source/procedural.h:27-60). */
+typedef struct er_procedural {
+    int         type;               /* ER_PROCEDURAL_* */
+    float       uniform_color[3];
+    float       checker_color1[3];
+    float       checker_color2[3];
+    er_color_tf gradient;
+} er_procedural;
+
+/* ErTexture (Source/ertexture.h:27-82). */
+typedef struct er_texture_desc {
+    int           id;
+    int           enabled;
+    int           type;             /* ER_TEXTURE_* */
+    float         output_level;
+    int           bitmap_id;
+    float         offset[2];
+    float         repeat[2];
+    int           flip[2];
+    er_procedural procedural;
+} er_texture_desc;
+
+/* ErClippingObject (Source/erclippingobject.h:27-58). */
+typedef struct er_clipping_desc {
+    int      id;
+    int      enabled;
+    int      invert;
+    er_shape shape;
+} er_clipping_desc;
+
+/* ErObject (Source/erobject.h:27-67). */
+typedef struct er_object_desc {
+    int      id;
+    int      enabled;
+    int      diffuse_texture_id;
+    int      specular_texture_id;
+    int      glossiness_texture_id;
+    float    ior;
+    er_shape shape;
+} er_object_desc;
+
+/* ErBitmap (Source/erbitmap.h:28-61): RGBA8 pixels, row-major y*W + x. */
+typedef struct er_bitmap_desc {
+    int            id;
+    int            enabled;
+    int            resolution[2];
+    int            pixels_dirty;
+    const uint8_t* pixels;
+} er_bitmap_desc;
+
+/* Per-stage timing of er_render_passes (hipEvent-measured on the tracer's stream). */
+typedef struct er_timing {
+    float total_ms;        /* first launch to last completion */
+    float trace_ms;        /* sum over passes of the trace kernel */
+    float post_ms;         /* sum over passes of filter + accumulate + tonemap (+ bilateral) */
+    int   passes;
+} er_timing;
+
+/* Sample counters of one pass (er_set_counting): the unit counts of the algorithmic-bytes
+ * model, B = 16 * (primary + shadow + shading) + 64 bytes per path. */
+typedef struct er_counters {
+    uint64_t paths;
+    uint64_t primary_samples;   /* iterations of raymarching.h:55-68 */
+    uint64_t shadow_samples;    /* iterations of raymarching.h:97-110 */
+    uint64_t shading_samples;   /* 1 (transport.h:119) + gradient samples per volume scatter */
+    uint64_t primary_fetches;   /* samples that read voxels (the rest are in empty bricks) */
+    uint64_t shadow_fetches;
+    uint64_t scatter_events;
+    uint64_t shadow_rays;
+} er_counters;
+
+/* ---- helpers ------------------------------------------------------------------------- */
+
+ER_API int  er_abi_version(void);
+/* sizeof of each descriptor, in the order tracer, volume, light, texture, clipping, object,
+ * bitmap, plf, shape, timing, counters; returns how many it wrote. */
+ER_API int  er_abi_sizeof(int* sizes, int n);
+ER_API void er_plf_init(er_plf* plf);
+ER_API void er_plf_add_node(er_plf* plf, float position, float value);  /* plf.h:50-66 */
+ER_API void er_tracer_desc_init(er_tracer_desc* d);   /* ErTracer() defaults */
+ER_API void er_volume_desc_init(er_volume_desc* d);
+ER_API void er_light_desc_init(er_light_desc* d);
+ER_API void er_texture_desc_init(er_texture_desc* d);
+ER_API void er_clipping_desc_init(er_clipping_desc* d);
+ER_API void er_object_desc_init(er_object_desc* d);
+ER_API void er_bitmap_desc_init(er_bitmap_desc* d);
+ER_API int  er_last_error(char* buf, size_t size);    /* message of the last non-OK status */
+
+/* ---- the reference entry points (Source/exposurerender.h:32-42) ----------------------- */
+
+/* BindTracer/BindVolume/... (exposurerender.h:32-38; core.cu:56-124; list.cuh:59-106).
+ * bind != 0: first bind assigns d->id, a rebind with the same id updates in place;
+ * bind == 0: unbinds (an unknown id is ignored, list.cuh:89-93). A 32nd live object of
+ * one kind is ignored and d->id stays -1 (list.cuh:63-67). */
+ER_API int er_bind_tracer(er_tracer_desc* d, int bind);
+ER_API int er_bind_volume(er_volume_desc* d, int bind);
+ER_API int er_bind_light(er_light_desc* d, int bind);
+ER_API int er_bind_object(er_object_desc* d, int bind);
+ER_API int er_bind_clipping_object(er_clipping_desc* d, int bind);
+ER_API int er_bind_texture(er_texture_desc* d, int bind);
+ER_API int er_bind_bitmap(er_bitmap_desc* d, int bind);
+
+/* RenderEstimate (exposurerender.h:39; core.cu:126-136): one progressive pass - trace,
+ * 3x3 Gaussian frame filter, running average, tonemap - then NoIterations++. Synchronous. */
+ER_API int er_render_estimate(int tracer_id);
+
+/* GetEstimate (exposurerender.h:40; core.cu:138-143): W*H RGBA8, row-major y*W + x.
+ * size must be at least W*H*4. With the bilateral filter on, the filtered image. */
+ER_API int er_get_estimate(int tracer_id, uint8_t* rgba, size_t size);
+
+/* GetAutoFocusDistance (exposurerender.h:41; stub in core.cu:145-148): mean distance of
+ * volume scatter points of 100 jittered rays through film point (u, v). */
+ER_API int er_get_autofocus_distance(int tracer_id, int film_u, int film_v, float* distance);
+
+/* GetNoIterations (exposurerender.h:42; stub in core.cu:150-153): passes since the last bind. */
+ER_API int er_get_no_iterations(int tracer_id, int* no_iterations);
+
+/* ---- extensions (tests, benchmarks, hosts that manage passes themselves) -------------- */
+
+ER_API int er_init(void);
+ER_API int er_shutdown(void);     /* unbinds everything and frees device memory */
+
+/* Per-pixel RNG seeds (FrameBuffer::RandomSeeds1/2, framebuffer.h:100-101). count = W*H. */
+ER_API int er_get_seeds(int tracer_id, uint32_t* seeds1, uint32_t* seeds2, size_t count);
+ER_API int er_set_seeds(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count);
+/* Deterministic per-pixel seeds: SplitMix64(seed ^ pixel) split into two non-zero halves. */
+ER_API int er_seed_tracer(int tracer_id, uint64_t seed);
+
+/* Float readbacks: the per-pass unfiltered frame estimate (SingleScattering output) and the
+ * running estimate; XYZA float, W*H*16 bytes. */
+ER_API int er_get_frame_xyza(int tracer_id, float* xyza, size_t size);
+ER_API int er_get_running_xyza(int tracer_id, float* xyza, size_t size);
+
+/* Runs `passes` progressive passes back-to-back with one host synchronisation at the end
+ * (same result as `passes` calls of er_render_estimate). timing may be NULL. */
+ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing);
+
+/* Sample counting: while on, each pass accumulates counters (a slower kernel variant). */
+ER_API int er_set_counting(int tracer_id, int on);
+ER_API int er_get_counters(int tracer_id, er_counters* counters);
+ER_API int er_reset_counters(int tracer_id);
+
+/* Maximum gradient magnitude of the tracer's volume (full-mode hybrid shading normaliser,
+ * Source/gradientmagnitude.cuh:46-70). */
+ER_API int er_get_gradient_magnitude_max(int tracer_id, float* gm_max);
+
+/* -log(u) table entry used for the free-path threshold, for u = m * 2^-23. */
+ER_API int er_neglog_table_entry(uint32_t m, float* value);
+
+/* ---- synthetic phantoms (seeded, deterministic) --------------------------------------- */
+
+/* Isotropic Gaussian blobs plus uniform noise in [-noise, noise]; clamp to [0,65535], round.
+ * Blob centres uniform in the unit cube, sigma uniform in [sigma_lo, sigma_hi] of the extent,
+ * peak uniform in [peak_lo, peak_hi]. */
+ER_API int er_phantom_blobs(uint16_t* out, int rx, int ry, int rz, int n_blobs,
+                            float sigma_lo, float sigma_hi, float peak_lo, float peak_hi,
+                            float noise, uint64_t seed);
+/* Nested/overlapping ellipsoids (n_ellipsoids) and, when n_tubes > 0, cylinders along random
+ * axes; intensities drawn from `n_bands` tissue-like bands in [0, max_intensity]; Gaussian
+ * noise of standard deviation noise_sigma. */
+ER_API int er_phantom_ellipsoids(uint16_t* out, int rx, int ry, int rz, int n_ellipsoids,
+                                 int n_tubes, float axis_lo, float axis_hi, int n_bands,
+                                 float max_intensity, float noise_sigma, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ER_CAPI_H */

@@ -1,0 +1,1299 @@
+/*
+ * er_oracle.cpp - TEST INFRASTRUCTURE. CPU restatement of Exposure Render's single-scattering
+ * estimator and per-pass post-processing, used only as the checker of the HIP path (by tests/,
+ * by __graft_entry__.smoke() and as bench.py's cpu_baseline). It is never linked into the
+ * product library.
+ *
+ * One template, two instantiations:
+ *   Real = float  : follows the reference's FP32 operation order exactly (build with
+ *                   -ffp-contract=off). Used for per-sample parity: RNG streams, scatter /
+ *                   no-scatter and light-index decisions, sample counts, XYZA radiance.
+ *   Real = double : the FP64 oracle. Same estimator, double arithmetic; its high-spp mean is
+ *                   the converged image the RMSE gate compares against.
+ *
+ * Every function cites the reference file:line it restates (paths relative to Source/).
+ * Elementary functions: -log(u) is the correctly rounded value (-logl in long double, rounded
+ * to float); sin/cos/exp/pow/acos/atan2 in float are the project's er_fmath.h functions, whose
+ * accuracy the tests pin against double libm; Real = double uses <cmath>.
+ *
+ * Parity status per function (pinning tests in tests/test_oracle.py):
+ *   rng (rng.h:35-61)                    pinned: closed-form MWC recurrence, u = (r & 0x7fffff) 2^-23
+ *   plf_eval (plf.h:68-91)               pinned: closed form, Q5 last-node zero, brute force
+ *   get_intensity (buffer3d.h:222-242)   pinned: brute force vs. numpy restatement, corner bounds
+ *   intersect_box (box.h:51-83)          pinned: closed-form slab distances
+ *   sample_volume (raymarching.h:30-71)  pinned: homogeneous-slab survival law, step counts
+ *   camera (camera.h:61-96, singlescattering.h:29-50) pinned: closed-form pinhole rays
+ *   shading (shader.h)                   pinned: Lambert/phase closed forms, Fresnel at normal incidence
+ *   post (filterframeestimate.cuh, estimate.cuh, tonemap.cuh) pinned: closed forms
+ *   full-mode hybrid shading, MIS, clipping, bilateral, Woodcock: own design from the reference's
+ *   commented code; pinned by invariants and statistics (the light-sampling and BSDF-sampling
+ *   strategies must agree in expectation), reference parity unpinned.
+ */
+#include "../include/er_capi.h"
+#include "../exposure-render_amd/csrc/er_fmath.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+/* ---- constants (defines.h:58-77) ------------------------------------------------------ */
+static const float PI_F          = 3.141592654f;
+static const float INV_PI_F      = 0.31830988618379067154f;
+static const float INV_TWO_PI_F  = 0.15915494309189533577f;
+static const float INV_FOUR_PI_F = 1.0f / 4.0f * 3.141592654f;   /* Q6: expands to pi/4 */
+static const float RAY_EPS       = 0.0001f;
+static const float RAY_EPS_2     = 2.0f * 0.0001f;
+static const float TRUE_INV_FOUR_PI_F = 0.0795774715459476678844f; /* full mode phase pdf */
+
+/* ---- elementary functions ------------------------------------------------------------- */
+template<class R> struct M;
+template<> struct M<float> {
+    static float neglog(float u) { return (float)(-logl((long double)u)); }
+    static float exp(float x) { return erm::expf_(x); }
+    static float pow(float x, float y) { return erm::powf_(x, y); }
+    static float sin(float x) { return erm::sinf_(x); }
+    static float cos(float x) { return erm::cosf_(x); }
+    static float acos(float x) { return erm::acosf_(x); }
+    static float atan2(float y, float x) { return erm::atan2f_(y, x); }
+    static float sqrt(float x) { return std::sqrt(x); }
+};
+template<> struct M<double> {
+    static double neglog(double u) { return -std::log(u); }
+    static double exp(double x) { return std::exp(x); }
+    static double pow(double x, double y) { return std::pow(x, y); }
+    static double sin(double x) { return std::sin(x); }
+    static double cos(double x) { return std::cos(x); }
+    static double acos(double x) { return std::acos(x); }
+    static double atan2(double y, double x) { return std::atan2(y, x); }
+    static double sqrt(double x) { return std::sqrt(x); }
+};
+
+/* ---- vectors (vector.h:47-395; component-wise, left-to-right) --------------------------- */
+template<class R> struct V3 {
+    R v[3];
+    V3() : v{R(0), R(0), R(0)} {}
+    V3(R a, R b, R c) : v{a, b, c} {}
+    R  operator[](int i) const { return v[i]; }
+    R& operator[](int i) { return v[i]; }
+};
+template<class R> inline V3<R> operator+(const V3<R>& a, const V3<R>& b) { return V3<R>(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+template<class R> inline V3<R> operator-(const V3<R>& a, const V3<R>& b) { return V3<R>(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+template<class R> inline V3<R> operator*(const V3<R>& a, const V3<R>& b) { return V3<R>(a[0] * b[0], a[1] * b[1], a[2] * b[2]); }
+template<class R> inline V3<R> operator-(const V3<R>& a) { return V3<R>(-a[0], -a[1], -a[2]); }
+/* float * Vec3f returns V[i] * F (vector.h:529) */
+template<class R> inline V3<R> scale(R f, const V3<R>& a) { return V3<R>(a[0] * f, a[1] * f, a[2] * f); }
+template<class R> inline R dot(const V3<R>& a, const V3<R>& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template<class R> inline R length(const V3<R>& a) { return M<R>::sqrt(dot(a, a)); }
+template<class R> inline V3<R> normalize(const V3<R>& a) { const R L = length(a); return V3<R>(a[0] / L, a[1] / L, a[2] / L); }
+template<class R> inline V3<R> cross(const V3<R>& a, const V3<R>& b)
+{
+    return V3<R>((a[1] * b[2]) - (a[2] * b[1]), (a[2] * b[0]) - (a[0] * b[2]), (a[0] * b[1]) - (a[1] * b[0]));
+}
+template<class R> inline R abs_dot(const V3<R>& a, const V3<R>& b) { return std::fabs(dot(a, b)); }
+template<class R> inline R dist2(const V3<R>& a, const V3<R>& b) { const V3<R> d = a - b; return dot(d, d); }
+template<class R> inline bool is_black(const V3<R>& c) { return c[0] == R(0) && c[1] == R(0) && c[2] == R(0); }
+template<class R> inline V3<R> from(const float* f) { return V3<R>(R(f[0]), R(f[1]), R(f[2])); }
+template<class R> inline R clampf(R v, R lo, R hi) { return std::max(lo, std::min(v, hi)); }
+
+/* matrix transforms (geometry.h:30-52) */
+template<class R> inline V3<R> transform_point(const float tm[4][4], const V3<R>& p)
+{
+    const R x = p[0], y = p[1], z = p[2];
+    return V3<R>(R(tm[0][0]) * x + R(tm[0][1]) * y + R(tm[0][2]) * z + R(tm[0][3]),
+                 R(tm[1][0]) * x + R(tm[1][1]) * y + R(tm[1][2]) * z + R(tm[1][3]),
+                 R(tm[2][0]) * x + R(tm[2][1]) * y + R(tm[2][2]) * z + R(tm[2][3]));
+}
+template<class R> inline V3<R> transform_vector(const float tm[4][4], const V3<R>& v)
+{
+    const R x = v[0], y = v[1], z = v[2];
+    return V3<R>(R(tm[0][0]) * x + R(tm[0][1]) * y + R(tm[0][2]) * z,
+                 R(tm[1][0]) * x + R(tm[1][1]) * y + R(tm[1][2]) * z,
+                 R(tm[2][0]) * x + R(tm[2][1]) * y + R(tm[2][2]) * z);
+}
+
+/* Ray (ray.h:27-55); operator() re-normalises D (Q34) */
+template<class R> struct Ray {
+    V3<R> O, D; R MinT, MaxT;
+    Ray() : O(), D(R(0), R(0), R(1)), MinT(R(0)), MaxT(R(1000000.0f)) {}
+    Ray(const V3<R>& o, const V3<R>& d, R mn, R mx) : O(o), D(d), MinT(mn), MaxT(mx) {}
+    V3<R> at(R t) const { return O + scale(t, normalize(D)); }
+};
+
+/* TransformRay (geometry.h:54-68) */
+template<class R> inline Ray<R> transform_ray(const float tm[4][4], const Ray<R>& r)
+{
+    const V3<R> P = transform_point(tm, r.O);
+    const V3<R> MinP = transform_point(tm, r.at(r.MinT));
+    const V3<R> MaxP = transform_point(tm, r.at(r.MaxT));
+    Ray<R> t;
+    t.O = P;
+    t.D = normalize(MaxP - t.O);
+    t.MinT = length(MinP - t.O);
+    t.MaxT = length(MaxP - t.O);
+    return t;
+}
+
+/* Intersection (intersection.h:27-77) */
+template<class R> struct Isect {
+    bool Valid = false, Front = true; R NearT = R(0), FarT = R(FLT_MAX);
+    V3<R> P, N; R UV[2] = {R(0), R(0)};
+};
+
+/* ---- RNG (rng.h:26-66). Q22: Get2/Get3 draw left to right. ---------------------------- */
+struct Rng {
+    uint32_t s0, s1;
+    uint32_t draws = 0;
+    float get1()
+    {
+        s0 = 36969u * (s0 & 65535u) + (s0 >> 16);
+        s1 = 18000u * (s1 & 65535u) + (s1 >> 16);
+        const uint32_t ires = (s0 << 16) + s1;
+        const uint32_t b = (ires & 0x007fffffu) | 0x40000000u;
+        float f; std::memcpy(&f, &b, 4);
+        ++draws;
+        return (f - 2.f) / 2.f;
+    }
+};
+
+/* ---- scene --------------------------------------------------------------------------- */
+struct Scene {
+    er_tracer_desc t;
+    /* derived camera (camera.h:61-96) */
+    float camN[3], camU[3], camV[3], screen[2][2], invscreen[2], focal;
+    /* volume (volume.h:100-129) */
+    int res[3];
+    std::vector<uint16_t> vox;
+    float spacing[3], size[3], invsize[3], minp[3], maxp[3], minstep;
+    std::vector<er_light_desc> lights;     /* index = light ID */
+    std::vector<er_texture_desc> textures; /* index = texture ID */
+    std::vector<er_clipping_desc> clips;   /* index = clipping ID */
+    float gm_max = 0.0f;
+    float maj_max = 0.0f;                  /* global majorant for Woodcock: DS^2 * max opacity */
+};
+
+static float shape_area(const er_shape& s)   /* shape.h:28-56,91-103 */
+{
+    switch (s.type) {
+        case ER_SHAPE_PLANE:    return s.size[0] * s.size[1];
+        case ER_SHAPE_DISK:     return PI_F * (s.outer_radius * s.outer_radius);
+        case ER_SHAPE_RING:     return PI_F * (s.outer_radius * s.outer_radius) - PI_F * (s.inner_radius * s.inner_radius);
+        case ER_SHAPE_BOX:      return (2.0f * s.size[0] * s.size[1]) + (2.0f * s.size[0] * s.size[2]) + (2.0f * s.size[1] * s.size[2]);
+        case ER_SHAPE_SPHERE:   return 4.0f * PI_F * (s.outer_radius * s.outer_radius);
+        case ER_SHAPE_CYLINDER: return (2.0f * PI_F * (s.outer_radius * s.outer_radius)) + (2.0f * PI_F * s.outer_radius * s.size[2]);
+    }
+    return s.area;
+}
+
+static void camera_update(Scene& S)   /* camera.h:61-96 */
+{
+    const er_camera& c = S.t.camera;
+    V3<float> pos = from<float>(c.pos), target = from<float>(c.target), up = from<float>(c.up);
+    V3<float> N = normalize(target - pos);
+    V3<float> U = normalize(cross(N, up));
+    V3<float> V = normalize(cross(N, U));
+    for (int i = 0; i < 3; ++i) { S.camN[i] = N[i]; S.camU[i] = U[i]; S.camV[i] = V[i]; }
+    S.focal = c.focal_distance;
+    if (S.focal == -1.0f) S.focal = length(target - pos);
+    const float Scale = tanf((0.5f * c.fov / 57.29577951308232f));
+    const float Aspect = (float)c.film_size[1] / (float)c.film_size[0];
+    if (Aspect > 1.0f) {
+        S.screen[0][0] = -Scale; S.screen[0][1] = Scale;
+        S.screen[1][0] = -Scale * Aspect; S.screen[1][1] = Scale * Aspect;
+    } else {
+        S.screen[0][0] = -Scale / Aspect; S.screen[0][1] = Scale / Aspect;
+        S.screen[1][0] = -Scale; S.screen[1][1] = Scale;
+    }
+    S.invscreen[0] = (S.screen[0][1] - S.screen[0][0]) / (float)c.film_size[0];
+    S.invscreen[1] = (S.screen[1][1] - S.screen[1][0]) / (float)c.film_size[1];
+}
+
+static void volume_update(Scene& S, const er_volume_desc& v)   /* volume.h:100-129 */
+{
+    for (int i = 0; i < 3; ++i) S.res[i] = v.resolution[i];
+    float scale = 0.0f;
+    if (v.normalize_size) {
+        const float ps[3] = {(float)S.res[0] * v.spacing[0], (float)S.res[1] * v.spacing[1], (float)S.res[2] * v.spacing[2]};
+        scale = 1.0f / std::max(ps[0], std::max(ps[1], ps[2]));
+    }
+    for (int i = 0; i < 3; ++i) {
+        S.spacing[i] = v.spacing[i] * scale;
+        S.size[i] = (float)S.res[i] * S.spacing[i];
+        S.invsize[i] = 1.0f / S.size[i];
+        S.minp[i] = S.size[i] * -0.5f;
+        S.maxp[i] = S.size[i] * 0.5f;
+    }
+    S.minstep = std::min(S.spacing[0], std::min(S.spacing[1], S.spacing[2]));
+}
+
+/* ---- transfer functions (plf.h:68-91, transferfunction.h:110-113,148-151) ----------- */
+template<class R> inline R plf_eval(const er_plf& p, R x)
+{
+    if (p.count <= 0) return R(0);
+    if (x < R(p.node_range[0])) return R(p.value[0]);
+    if (x > R(p.node_range[1])) return R(p.value[p.count - 1]);
+    for (int i = 1; i < p.count; ++i) {
+        const R P1 = R(p.position[i - 1]);
+        const R P2 = R(p.position[i]);
+        const R DeltaP = P2 - P1;
+        const R LerpT = (x - P1) / DeltaP;
+        if (x >= P1 && x < P2)
+            return R(p.value[i - 1]) + LerpT * (R(p.value[i]) - R(p.value[i - 1]));
+    }
+    return R(0);   /* Q5: x == last node position */
+}
+template<class R> inline V3<R> ctf_eval(const er_color_tf& c, R x)
+{
+    return V3<R>(plf_eval(c.plf[0], x), plf_eval(c.plf[1], x), plf_eval(c.plf[2], x));
+}
+
+/* ---- estimator ----------------------------------------------------------------------- */
+struct Counters { uint64_t c[8]; };   /* er_counters field order */
+
+template<class R> struct Est {
+    const Scene& S;
+    Rng& rng;
+    uint64_t* cnt;   /* paths, primary, shadow, shading, primary_fetch, shadow_fetch, scatters, shadow_rays */
+    bool full;
+
+    Est(const Scene& s, Rng& r, uint64_t* c) : S(s), rng(r), cnt(c), full(s.t.settings.estimator == ER_ESTIMATOR_FULL) {}
+
+    R u() { return R(rng.get1()); }
+
+    /* Buffer3D<ushort>::operator()(Vec3f) (buffer3d.h:222-242) via Volume::operator()
+     * (volume.h:131-138): corner-aligned lattice, clamped fetches, 7 lerps each truncated
+     * to uint16 (Q4, geometry.h:87-90). */
+    R get_intensity(const V3<R>& P) const
+    {
+        const R L[3] = {((P[0] - R(S.minp[0])) * R(S.invsize[0])) * R((float)S.res[0]),
+                        ((P[1] - R(S.minp[1])) * R(S.invsize[1])) * R((float)S.res[1]),
+                        ((P[2] - R(S.minp[2])) * R(S.invsize[2])) * R((float)S.res[2])};
+        const int vx = (int)std::floor(L[0]), vy = (int)std::floor(L[1]), vz = (int)std::floor(L[2]);
+        const R dx = L[0] - R(vx), dy = L[1] - R(vy), dz = L[2] - R(vz);
+        auto at = [&](int x, int y, int z) -> R {
+            x = std::max(0, std::min(x, S.res[0] - 1));
+            y = std::max(0, std::min(y, S.res[1] - 1));
+            z = std::max(0, std::min(z, S.res[2] - 1));
+            return R(S.vox[(size_t)z * S.res[0] * S.res[1] + (size_t)y * S.res[0] + x]);
+        };
+        auto lerp = [](R t, R a, R b) -> R { return R((uint16_t)((R(1) - t) * a + t * b)); };
+        const R d00 = lerp(dx, at(vx, vy, vz), at(vx + 1, vy, vz));
+        const R d10 = lerp(dx, at(vx, vy + 1, vz), at(vx + 1, vy + 1, vz));
+        const R d01 = lerp(dx, at(vx, vy, vz + 1), at(vx + 1, vy, vz + 1));
+        const R d11 = lerp(dx, at(vx, vy + 1, vz + 1), at(vx + 1, vy + 1, vz + 1));
+        const R d0 = lerp(dy, d00, d10);
+        const R d1 = lerp(dy, d01, d11);
+        return lerp(dz, d0, d1);
+    }
+
+    /* GradientCD / FD / Filtered (volumes.h:31-74) */
+    V3<R> grad_cd(const V3<R>& P) const
+    {
+        const R d = R(S.minstep), z = R(0);
+        const V3<R> dX(d, z, z), dY(z, d, z), dZ(z, z, d);
+        const R i00 = get_intensity(P + dX), i01 = get_intensity(P - dX);
+        const R i10 = get_intensity(P + dY), i11 = get_intensity(P - dY);
+        const R i20 = get_intensity(P + dZ), i21 = get_intensity(P - dZ);
+        return V3<R>(i01 - i00, i11 - i10, i21 - i20);
+    }
+    V3<R> grad_fd(const V3<R>& P) const
+    {
+        const R d = R(S.minstep), z = R(0);
+        const R i0 = get_intensity(P);
+        const R i1 = get_intensity(P + V3<R>(d, z, z));
+        const R i2 = get_intensity(P + V3<R>(z, d, z));
+        const R i3 = get_intensity(P + V3<R>(z, z, d));
+        return V3<R>(i0 - i1, i0 - i2, i0 - i3);
+    }
+    static V3<R> vlerp(const V3<R>& a, const V3<R>& b, R c) { return a + scale(c, b - a); }   /* vector.h:548 */
+    V3<R> grad_filtered(const V3<R>& P) const
+    {
+        const R o = R(S.minstep);
+        const V3<R> G0 = grad_cd(P);
+        const V3<R> G1 = grad_cd(P + V3<R>(-o, -o, -o));
+        const V3<R> G2 = grad_cd(P + V3<R>(o, o, o));
+        const V3<R> G3 = grad_cd(P + V3<R>(-o, o, -o));
+        const V3<R> G4 = grad_cd(P + V3<R>(o, -o, o));
+        const V3<R> G5 = grad_cd(P + V3<R>(-o, -o, o));
+        const V3<R> G6 = grad_cd(P + V3<R>(o, o, -o));
+        const V3<R> G7 = grad_cd(P + V3<R>(-o, o, o));
+        const V3<R> G8 = grad_cd(P + V3<R>(o, -o, -o));
+        const R h = R(0.5f);
+        const V3<R> L0 = vlerp(vlerp(G1, G2, h), vlerp(G3, G4, h), h);
+        const V3<R> L1 = vlerp(vlerp(G5, G6, h), vlerp(G7, G8, h), h);
+        return vlerp(G0, vlerp(L0, L1, h), R(0.75f));
+    }
+    int gradient_samples() const
+    {
+        switch (S.t.settings.gradient_computation) {
+            case ER_GRADIENT_FORWARD: return 4;
+            case ER_GRADIENT_CENTRAL: return 6;
+            case ER_GRADIENT_FILTERED: return 54;
+        }
+        return 4;
+    }
+    V3<R> normalized_gradient(const V3<R>& P) const   /* volumes.h:76-91 (Q20: zero -> NaN) */
+    {
+        switch (S.t.settings.gradient_computation) {
+            case ER_GRADIENT_FORWARD: return normalize(grad_fd(P));
+            case ER_GRADIENT_CENTRAL: return normalize(grad_cd(P));
+            case ER_GRADIENT_FILTERED: return normalize(grad_filtered(P));
+        }
+        return normalize(grad_fd(P));
+    }
+    /* GradientMagnitude (volumes.h:93-114) */
+    R gradient_magnitude(const V3<R>& P) const
+    {
+        const R d = R(S.minstep), z = R(0);
+        const V3<R> dl[3] = {V3<R>(d, z, z), V3<R>(z, d, z), V3<R>(z, z, d)};
+        R Sum = R(0);
+        for (int i = 0; i < 3; ++i) {
+            R D = get_intensity(P - dl[i]) - get_intensity(P + dl[i]);
+            D *= R(0.5f) / R(S.spacing[i]);
+            Sum += D * D;
+        }
+        return M<R>::sqrt(Sum);
+    }
+
+    /* ---- clipping (full mode; box.h:116-120, sphere.h:147-150, plane.h:90-93, cylinder.h:96-99) */
+    bool clipped(const V3<R>& P) const
+    {
+        for (int k = 0; k < S.t.clipping_count; ++k) {
+            const int id = S.t.clipping_ids[k];
+            if (id < 0 || id >= (int)S.clips.size()) continue;
+            const er_clipping_desc& c = S.clips[id];
+            const V3<R> Pl = transform_point(c.shape.inv_tm, P);
+            bool inside = false;
+            switch (c.shape.type) {
+                case ER_SHAPE_BOX: {
+                    const R h0 = R(0.5f * c.shape.size[0]), h1 = R(0.5f * c.shape.size[1]), h2 = R(0.5f * c.shape.size[2]);
+                    inside = Pl[0] > -h0 && Pl[0] < h0 && Pl[1] > -h1 && Pl[1] < h1 && Pl[2] > -h2 && Pl[2] < h2;
+                    break;
+                }
+                case ER_SHAPE_SPHERE: inside = length(Pl) < R(c.shape.outer_radius); break;
+                case ER_SHAPE_CYLINDER:
+                    inside = M<R>::sqrt((Pl[0] * Pl[0]) + (Pl[2] * Pl[2])) < R(c.shape.outer_radius) &&
+                             std::fabs(Pl[1]) < R(0.5f * c.shape.size[1]);
+                    break;
+                default: inside = Pl[2] > R(0); break;   /* plane, disk, ring: half-space */
+            }
+            if (c.invert ? !inside : inside) return true;
+        }
+        return false;
+    }
+
+    /* IntersectBox (box.h:51-83): slab test; fminf/fmaxf semantics for the per-axis min/max */
+    static bool intersect_box(const Ray<R>& r, const V3<R>& mn, const V3<R>& mx, R& nearT, R& farT)
+    {
+        const V3<R> InvR(R(1) / r.D[0], R(1) / r.D[1], R(1) / r.D[2]);
+        const V3<R> B = InvR * (mn - r.O);
+        const V3<R> T = InvR * (mx - r.O);
+        R mnT[3], mxT[3];
+        for (int i = 0; i < 3; ++i) { mnT[i] = std::fmin(T[i], B[i]); mxT[i] = std::fmax(T[i], B[i]); }
+        const R LargestMinT = std::fmax(std::fmax(mnT[0], mnT[1]), std::fmax(mnT[0], mnT[2]));
+        const R LargestMaxT = std::fmin(std::fmin(mxT[0], mxT[1]), std::fmin(mxT[0], mxT[2]));
+        if (LargestMaxT < LargestMinT) return false;
+        nearT = LargestMinT > R(0) ? LargestMinT : R(0);
+        farT = LargestMaxT;
+        if (nearT < r.MinT || nearT > r.MaxT) return false;
+        return true;
+    }
+
+    R sigma_at(const V3<R>& Ps, bool primary)
+    {
+        if (full && clipped(Ps)) return R(0);
+        const R I = get_intensity(Ps);
+        if (cnt) cnt[primary ? 4 : 5]++;
+        return R(S.t.settings.density_scale) * plf_eval(S.t.opacity, I);
+    }
+
+    /* SampleVolume (raymarching.h:30-71) / ScatterEventInVolume (raymarching.h:73-113):
+     * jittered fixed-step tracking. primary: exit when t >= tmax and report the event with
+     * T one step ahead of P (Q13); shadow: exit when t > tmax (Q14). */
+    bool march(const Ray<R>& r, bool primary, R& Tout, V3<R>& Pout)
+    {
+        const V3<R> mn = from<R>(S.minp), mx = from<R>(S.maxp);
+        R nearT, farT;
+        if (!intersect_box(r, mn, mx, nearT, farT)) return false;
+        R MinT = std::max(nearT, r.MinT);
+        const R MaxT = std::min(farT, r.MaxT);
+        if (full && S.t.settings.sampler == ER_SAMPLER_WOODCOCK)
+            return woodcock(r, primary, MinT, MaxT, Tout, Pout);
+        const R Sthr = M<R>::neglog(u()) / R(S.t.settings.density_scale);
+        R Sum = R(0);
+        const R StepSize = R(primary ? S.t.settings.step_factor_primary : S.t.settings.step_factor_shadow) * R(S.minstep);
+        MinT += u() * StepSize;
+        if (!(StepSize > R(0))) return false;   /* zero-size volume (Q2): the march would not advance */
+        V3<R> Ps;
+        while (Sum < Sthr) {
+            Ps = r.O + scale(MinT, r.D);
+            if (primary ? (MinT >= MaxT) : (MinT > MaxT)) return false;
+            if (cnt) cnt[primary ? 1 : 2]++;
+            const R SigmaT = sigma_at(Ps, primary);
+            Sum += SigmaT * StepSize;
+            MinT += StepSize;
+        }
+        Tout = MinT;
+        Pout = Ps;
+        return true;
+    }
+
+    /* Woodcock delta tracking against the global majorant DS^2 * max(opacity) (the effective
+     * extinction of the ray-marcher is DS^2 * opacity, Q3). Full mode only; design from the
+     * commented sampler at raymarching.h:115-273. */
+    bool woodcock(const Ray<R>& r, bool primary, R MinT, R MaxT, R& Tout, V3<R>& Pout)
+    {
+        const R mu_max = R(S.maj_max);
+        if (!(mu_max > R(0))) return false;
+        const R DS = R(S.t.settings.density_scale);
+        R t = MinT;
+        for (;;) {
+            t += M<R>::neglog(u()) / mu_max;
+            if (primary ? (t >= MaxT) : (t > MaxT)) return false;
+            const V3<R> Ps = r.O + scale(t, r.D);
+            if (cnt) cnt[primary ? 1 : 2]++;
+            const R sigma = sigma_at(Ps, primary) * DS;
+            if (u() * mu_max < sigma) { Tout = t; Pout = Ps; return true; }
+        }
+    }
+
+    /* ---- shapes (plane.h:41-88, sphere.h:84-140, box.h:51-83, disk.h, ring.h) --------- */
+    static void intersect_plane(const Ray<R>& r, bool one_sided, Isect<R>& I)
+    {
+        if (std::fabs(r.O[2] - r.D[2]) < R(RAY_EPS)) return;          /* Q16 */
+        I.NearT = (R(0) - r.O[2]) / r.D[2];
+        if (I.NearT < r.MinT || I.NearT > r.MaxT) return;
+        I.P = r.at(I.NearT);
+        I.UV[0] = I.P[0]; I.UV[1] = I.P[1];
+        I.N = V3<R>(R(0), R(0), R(1));
+        if (one_sided && r.D[2] >= R(0)) { I.Front = false; I.N = V3<R>(R(0), R(0), R(-1)); }
+        I.Valid = true;
+    }
+    static void intersect_sphere(const Ray<R>& r, R radius, Isect<R>& I)
+    {
+        const R a = dot(r.D, r.D);
+        const R b = R(2) * dot(r.D, r.O);
+        const R c = dot(r.O, r.O) - (radius * radius);
+        const R disc = b * b - R(4) * a * c;
+        if (disc < R(0)) return;
+        const R distSqrt = M<R>::sqrt(disc);
+        const R q = (b < R(0)) ? (-b - distSqrt) / R(2) : (-b + distSqrt) / R(2);
+        R t0 = q / a, t1 = c / q;
+        if (t0 > t1) std::swap(t0, t1);
+        if (t0 >= r.MinT && t0 < r.MaxT) I.NearT = t0;
+        else if (t1 >= r.MinT && t1 < r.MaxT) I.NearT = t1;
+        else return;
+        if (I.NearT < r.MinT || I.NearT > r.MaxT) return;
+        I.Valid = true;
+        I.P = r.at(I.NearT);
+        I.N = normalize(I.P);
+        /* SphericalToUV (geometry.h:70-83) */
+        const V3<R> V = normalize(I.P);
+        R phi = M<R>::atan2(V[2], V[0]);
+        phi = (phi < R(0)) ? phi + R(2.0f * PI_F) : phi;
+        const R theta = M<R>::acos(clampf(V[1], R(-1), R(1)));
+        I.UV[0] = R(INV_TWO_PI_F) * phi;
+        I.UV[1] = R(1) - (R(INV_PI_F) * theta);
+    }
+    static void intersect_shape(const er_shape& s, const Ray<R>& r, Isect<R>& I)
+    {
+        switch (s.type) {
+            case ER_SHAPE_PLANE: {
+                intersect_plane(r, s.one_sided != 0, I);
+                const R hx = R(0.5f * s.size[0]), hy = R(0.5f * s.size[1]);
+                if (I.Valid && (I.UV[0] < -hx || I.UV[0] > hx || I.UV[1] < -hy || I.UV[1] > hy)) I.Valid = false;
+                I.UV[0] /= R(s.size[0]); I.UV[1] /= R(s.size[1]);
+                I.UV[0] += R(0.5f); I.UV[1] += R(0.5f);
+                I.UV[0] = R(1) - I.UV[0];
+                break;
+            }
+            case ER_SHAPE_DISK: {
+                intersect_plane(r, s.one_sided != 0, I);
+                const R l = M<R>::sqrt(I.UV[0] * I.UV[0] + I.UV[1] * I.UV[1]);
+                if (I.Valid && l > R(s.outer_radius)) I.Valid = false;
+                const R dia = R(2.0f * s.outer_radius);
+                I.UV[0] /= dia; I.UV[1] /= dia;
+                I.UV[0] += R(0.5f); I.UV[1] += R(0.5f);
+                I.UV[0] = R(1) - I.UV[0];
+                break;
+            }
+            case ER_SHAPE_RING: {
+                intersect_plane(r, s.one_sided != 0, I);
+                const R l = M<R>::sqrt(I.UV[0] * I.UV[0] + I.UV[1] * I.UV[1]);
+                if (I.Valid && (l < R(s.inner_radius) || l > R(s.outer_radius))) I.Valid = false;
+                const R dia = R(2.0f * s.outer_radius);
+                I.UV[0] /= dia; I.UV[1] /= dia;
+                I.UV[0] += R(0.5f); I.UV[1] += R(0.5f);
+                I.UV[0] = R(1) - I.UV[0];
+                break;
+            }
+            case ER_SHAPE_BOX: {
+                const V3<R> mx(R(0.5f * s.size[0]), R(0.5f * s.size[1]), R(0.5f * s.size[2]));
+                const V3<R> mn(R(-0.5f * s.size[0]), R(-0.5f * s.size[1]), R(-0.5f * s.size[2]));
+                R nT, fT;
+                if (intersect_box(r, mn, mx, nT, fT)) {
+                    I.NearT = nT; I.FarT = fT; I.Valid = true;
+                    I.P = r.at(nT);
+                    I.N = V3<R>();
+                    for (int i = 0; i < 3; ++i) {
+                        if (I.P[i] <= mn[i] + R(0.0001f)) I.N[i] = R(-1);
+                        if (I.P[i] >= mx[i] - R(0.0001f)) I.N[i] = R(1);
+                    }
+                }
+                break;
+            }
+            case ER_SHAPE_SPHERE: intersect_sphere(r, R(s.outer_radius), I); break;
+            default: break;
+        }
+    }
+
+    /* surface samples (plane.h:95-107, sphere.h:152-170, box.h:122-191, disk.h, ring.h) */
+    struct SS { V3<R> P, N; R UV[2]; };
+    void sample_shape(const er_shape& s, const V3<R>& uvw, SS& ss) const
+    {
+        ss.N = V3<R>(R(0), R(0), R(1));
+        ss.UV[0] = ss.UV[1] = R(0);
+        switch (s.type) {
+            case ER_SHAPE_PLANE:
+                ss.P = V3<R>(R(-0.5f) + uvw[0], R(-0.5f) + uvw[1], R(0));
+                ss.UV[0] = uvw[0]; ss.UV[1] = uvw[1];
+                ss.P = ss.P * V3<R>(R(s.size[0]), R(s.size[1]), R(0));
+                break;
+            case ER_SHAPE_DISK: {
+                const R rr = M<R>::sqrt(uvw[0]);
+                const R th = R(2.0f * PI_F) * uvw[1];
+                ss.P = V3<R>(rr * M<R>::cos(th), rr * M<R>::sin(th), R(0));
+                ss.UV[0] = uvw[0]; ss.UV[1] = uvw[1];
+                ss.P = scale(R(s.outer_radius), ss.P);
+                break;
+            }
+            case ER_SHAPE_RING: {
+                const R inner = R(s.inner_radius) / R(s.outer_radius);
+                const R rr = inner + (R(1) - inner) * M<R>::sqrt(uvw[0]);
+                const R th = R(2.0f * PI_F) * uvw[1];
+                ss.P = V3<R>(rr * M<R>::cos(th), rr * M<R>::sin(th), R(0));
+                ss.P = scale(R(s.outer_radius), ss.P);
+                ss.UV[0] = ss.P[0]; ss.UV[1] = ss.P[1];
+                break;
+            }
+            case ER_SHAPE_BOX: {
+                const int side = (int)std::floor(uvw[0] * R(6));
+                const R h = R(0.5f);
+                switch (side) {
+                    case 0: ss.P = V3<R>(h, -h + uvw[2], -h + uvw[1]); ss.N = V3<R>(R(1), R(0), R(0)); break;
+                    case 1: ss.P = V3<R>(-h, -h + uvw[2], -h + uvw[1]); ss.N = V3<R>(R(-1), R(0), R(0)); break;
+                    case 2: ss.P = V3<R>(-h + uvw[1], h, -h + uvw[2]); ss.N = V3<R>(R(0), R(1), R(0)); break;
+                    case 3: ss.P = V3<R>(-h + uvw[1], -h, -h + uvw[2]); ss.N = V3<R>(R(0), R(-1), R(0)); break;
+                    case 4: ss.P = V3<R>(-h + uvw[1], -h + uvw[2], h); ss.N = V3<R>(R(0), R(0), R(1)); break;
+                    case 5: ss.P = V3<R>(-h + uvw[1], -h + uvw[2], -h); ss.N = V3<R>(R(0), R(0), R(-1)); break;
+                }
+                ss.UV[0] = uvw[1]; ss.UV[1] = uvw[2];
+                ss.P = ss.P * from<R>(s.size);
+                break;
+            }
+            case ER_SHAPE_SPHERE: {
+                const R z = R(1) - R(2) * uvw[0];
+                const R rr = M<R>::sqrt(std::max(R(0), R(1) - z * z));
+                const R phi = R(2.0f * PI_F) * uvw[1];
+                const R x = rr * M<R>::cos(phi);
+                const R y = rr * M<R>::sin(phi);
+                ss.P = V3<R>(x, y, z);
+                ss.N = ss.P;
+                /* UV = (SphericalTheta, SphericalPhi) (sphere.h:162, geometry.h:70-78) */
+                R ph = M<R>::atan2(ss.P[2], ss.P[0]);
+                ph = (ph < R(0)) ? ph + R(2.0f * PI_F) : ph;
+                ss.UV[0] = M<R>::acos(clampf(ss.P[1], R(-1), R(1)));
+                ss.UV[1] = ph;
+                ss.P = scale(R(s.outer_radius), ss.P);
+                break;
+            }
+            default: break;
+        }
+        ss.P = transform_point(s.tm, ss.P);
+        ss.N = transform_vector(s.tm, ss.N);
+    }
+
+    /* EvaluateProcedural / EvaluateTexture (textures.h:26-111) */
+    V3<R> eval_texture(int id, const R UVin[2]) const
+    {
+        if (id < 0 || id >= (int)S.textures.size()) return V3<R>();
+        const er_texture_desc& T = S.textures[id];
+        R uv[2] = {UVin[0], UVin[1]};
+        uv[0] *= R(T.repeat[0]); uv[1] *= R(T.repeat[1]);
+        uv[0] += R(T.offset[0]); uv[1] += R(1.0f - T.offset[1]);
+        uv[0] = uv[0] - std::floor(uv[0]); uv[1] = uv[1] - std::floor(uv[1]);
+        uv[0] = clampf(uv[0], R(0), R(1)); uv[1] = clampf(uv[1], R(0), R(1));
+        if (T.flip[0]) uv[0] = R(1) - uv[0];
+        if (T.flip[1]) uv[1] = R(1) - uv[1];
+        V3<R> L;
+        if (T.type == ER_TEXTURE_PROCEDURAL) {
+            const er_procedural& p = T.procedural;
+            switch (p.type) {
+                case ER_PROCEDURAL_UNIFORM: L = from<R>(p.uniform_color); break;
+                case ER_PROCEDURAL_CHECKER: {
+                    const int a = (int)(uv[0] * R(2)), b = (int)(uv[1] * R(2));
+                    const bool c1 = (a % 2 == 0) ? (b % 2 == 0) : (b % 2 != 0);
+                    L = c1 ? from<R>(p.checker_color1) : from<R>(p.checker_color2);
+                    break;
+                }
+                case ER_PROCEDURAL_GRADIENT: L = ctf_eval(p.gradient, uv[1]); break;
+                default: break;
+            }
+        }
+        return scale(R(T.output_level), L);
+    }
+
+    /* ---- shading (shader.h:29-483) ----------------------------------------------------- */
+    struct Shader {
+        bool brdf;
+        V3<R> Nn, Nu, Nv, Kd, Ks;
+        R exponent, ior;
+        bool full;
+    };
+    Shader make_shader(bool brdf, const V3<R>& N, const V3<R>& Wo, const V3<R>& Kd, const V3<R>& Ks, R ior, R exponent) const
+    {
+        Shader sh;
+        sh.brdf = brdf; sh.Kd = Kd; sh.Ks = Ks; sh.exponent = exponent; sh.ior = ior; sh.full = full;
+        sh.Nn = normalize(N);
+        sh.Nu = normalize(cross(N, Wo));
+        sh.Nv = normalize(cross(N, sh.Nu));
+        return sh;
+    }
+    static V3<R> to_local(const Shader& s, const V3<R>& W) { return V3<R>(dot(W, s.Nu), dot(W, s.Nv), dot(W, s.Nn)); }
+    static V3<R> to_world(const Shader& s, const V3<R>& W)
+    {
+        return V3<R>(s.Nu[0] * W[0] + s.Nv[0] * W[1] + s.Nn[0] * W[2],
+                     s.Nu[1] * W[0] + s.Nv[1] * W[1] + s.Nn[1] * W[2],
+                     s.Nu[2] * W[0] + s.Nv[2] * W[1] + s.Nn[2] * W[2]);
+    }
+    static bool same_hemi(const V3<R>& a, const V3<R>& b) { return (a[2] * b[2]) > R(0); }
+    /* Fresnel::Evaluate + FrDiel (shader.h:69-123), EtaI = 1, EtaT = ior */
+    static R fresnel(R cosi, R ior)
+    {
+        cosi = clampf(cosi, R(-1), R(1));
+        const bool entering = cosi > R(0);
+        R ei = R(1), et = ior;
+        if (!entering) std::swap(ei, et);
+        const R sint = ei / et * M<R>::sqrt(std::max(R(0), R(1) - cosi * cosi));
+        if (sint >= R(1)) return R(1);
+        const R cost = M<R>::sqrt(std::max(R(0), R(1) - sint * sint));
+        const R ci = std::fabs(cosi);
+        const R Rparl = ((et * ci) - (ei * cost)) / ((et * ci) + (ei * cost));
+        const R Rperp = ((ei * ci) - (et * cost)) / ((ei * ci) + (et * cost));
+        return (Rparl * Rparl + Rperp * Rperp) / R(2);
+    }
+    static R blinn_d(R exponent, const V3<R>& wh)   /* shader.h:181-185 */
+    {
+        const R c = std::fabs(wh[2]);
+        return (exponent + R(2)) * R(INV_TWO_PI_F) * M<R>::pow(c, exponent);
+    }
+    static R blinn_pdf(R exponent, const V3<R>& wo, const V3<R>& wi)   /* shader.h:166-179 */
+    {
+        const V3<R> wh = normalize(wo + wi);
+        const R c = std::fabs(wh[2]);
+        R pdf = ((exponent + R(1)) * M<R>::pow(c, exponent)) / (R(2.0f) * R(PI_F) * R(4.0f) * dot(wo, wh));
+        if (dot(wo, wh) <= R(0)) pdf = R(0);
+        return pdf;
+    }
+    /* Microfacet::F (shader.h:216-236) */
+    static V3<R> microfacet_f(const Shader& s, const V3<R>& wo, const V3<R>& wi)
+    {
+        const R cO = std::fabs(wo[2]), cI = std::fabs(wi[2]);
+        if (cI == R(0) || cO == R(0)) return V3<R>();
+        V3<R> wh = wi + wo;
+        if (wh[0] == R(0) && wh[1] == R(0) && wh[2] == R(0)) return V3<R>();
+        wh = normalize(wh);
+        const R cosH = dot(wi, wh);
+        const R Fr = fresnel(cosH, s.ior);
+        const R D = blinn_d(s.exponent, wh);
+        /* G (shader.h:255-263) */
+        const R NdotWh = std::fabs(wh[2]), NdotWo = std::fabs(wo[2]), NdotWi = std::fabs(wi[2]);
+        const R WOdotWh = abs_dot(wo, wh);
+        const R G = std::min(R(1), std::min((R(2) * NdotWh * NdotWo / WOdotWh), (R(2) * NdotWh * NdotWi / WOdotWh)));
+        const R den = R(4) * cI * cO;
+        return V3<R>((((s.Ks[0] * D) * G) * Fr) / den, (((s.Ks[1] * D) * G) * Fr) / den, (((s.Ks[2] * D) * G) * Fr) / den);
+    }
+    static R lambert_pdf(const V3<R>& wo, const V3<R>& wi) { return same_hemi(wo, wi) ? std::fabs(wi[2]) * R(INV_PI_F) : R(0); }
+    static R microfacet_pdf(const Shader& s, const V3<R>& wo, const V3<R>& wi) { return same_hemi(wo, wi) ? blinn_pdf(s.exponent, wo, wi) : R(0); }
+
+    /* Shader::F (shader.h:429-441) */
+    static V3<R> shader_f(const Shader& s, const V3<R>& Wo, const V3<R>& Wi)
+    {
+        if (!s.brdf) return scale(R(INV_PI_F), s.Kd);   /* IsotropicPhase::F (Q7) */
+        const V3<R> wo = to_local(s, Wo), wi = to_local(s, Wi);
+        V3<R> Rr;
+        Rr = Rr + scale(R(INV_PI_F), s.Kd);
+        Rr = Rr + microfacet_f(s, wo, wi);
+        return Rr;
+    }
+    /* Shader::Pdf (shader.h:457-469). COMPAT: Lambert + Blinn (Q32), phase pi/4 (Q6).
+     * FULL: the density of the one-sample mixture the BSDF sampler draws from. */
+    static R shader_pdf(const Shader& s, const V3<R>& Wo, const V3<R>& Wi)
+    {
+        if (!s.brdf) return s.full ? R(TRUE_INV_FOUR_PI_F) : R(INV_FOUR_PI_F);
+        const V3<R> wo = to_local(s, Wo), wi = to_local(s, Wi);
+        R pdf = R(0);
+        pdf += lambert_pdf(wo, wi);
+        pdf += microfacet_pdf(s, wo, wi);
+        return s.full ? R(0.5f) * pdf : pdf;
+    }
+
+    /* montecarlo.h:86-137,187-195 */
+    static void concentric_disk(R u0, R u1, R& ox, R& oy)
+    {
+        R r, theta;
+        const R sx = R(2) * u0 - R(1), sy = R(2) * u1 - R(1);
+        if (sx == R(0) && sy == R(0)) { ox = oy = R(0); return; }
+        if (sx >= -sy) {
+            if (sx > sy) { r = sx; theta = (sy > R(0)) ? sy / r : R(8) + sy / r; }
+            else { r = sy; theta = R(2) - sx / r; }
+        } else {
+            if (sx <= sy) { r = -sx; theta = R(4) - sy / r; }
+            else { r = -sy; theta = R(6) + sx / r; }
+        }
+        theta *= R(PI_F / 4.f);
+        ox = r * M<R>::cos(theta);
+        oy = r * M<R>::sin(theta);
+    }
+    /* BRDF::SampleF / IsotropicPhase::SampleF (shader.h:297-303,350-374), full mode only */
+    V3<R> shader_sample_f(const Shader& s, const V3<R>& Wo, V3<R>& Wi, R& pdf, R comp, R d0, R d1) const
+    {
+        if (!s.brdf) {
+            const R z = R(1) - R(2) * d0;
+            const R rr = M<R>::sqrt(std::max(R(0), R(1) - z * z));
+            const R phi = R(2.f * PI_F) * d1;
+            Wi = V3<R>(rr * M<R>::cos(phi), rr * M<R>::sin(phi), z);
+            pdf = R(TRUE_INV_FOUR_PI_F);
+            return scale(R(INV_PI_F), s.Kd);
+        }
+        const V3<R> wo = to_local(s, Wo);
+        V3<R> wi;
+        if (comp <= R(0.5f)) {
+            R dx, dy;
+            concentric_disk(d0, d1, dx, dy);
+            wi = V3<R>(dx, dy, M<R>::sqrt(std::max(R(0), R(1) - dx * dx - dy * dy)));
+            if (wo[2] < R(0)) wi[2] *= R(-1);
+        } else {
+            const R costheta = M<R>::pow(d0, R(1) / (s.exponent + R(1)));
+            const R sintheta = M<R>::sqrt(std::max(R(0), R(1) - costheta * costheta));
+            const R phi = d1 * R(2) * R(PI_F);
+            V3<R> wh(sintheta * M<R>::cos(phi), sintheta * M<R>::sin(phi), costheta);
+            if (!same_hemi(wo, wh)) wh = -wh;
+            const R two_d = R(2) * dot(wo, wh);
+            wi = (-wo) + scale(two_d, wh);
+        }
+        pdf = R(0.5f) * (lambert_pdf(wo, wi) + microfacet_pdf(s, wo, wi));
+        V3<R> Rr;
+        Rr = Rr + scale(R(INV_PI_F), s.Kd);
+        Rr = Rr + microfacet_f(s, wo, wi);
+        Wi = to_world(s, wi);
+        return Rr;
+    }
+
+    static R power_heuristic(R f, R g) { return (f * f) / (f * f + g * g); }   /* montecarlo.h:223-227 */
+
+    /* ---- lights (lights.h:28-131) ------------------------------------------------------- */
+    const er_light_desc* light(int id) const { return (id >= 0 && id < (int)S.lights.size()) ? &S.lights[id] : nullptr; }
+
+    /* IntersectLight (lights.h:59-88) */
+    bool intersect_light(const er_light_desc& L, const Ray<R>& r, R& T, V3<R>& P, V3<R>& N, V3<R>& Le) const
+    {
+        const Ray<R> rt = transform_ray(L.shape.inv_tm, r);
+        Isect<R> I;
+        intersect_shape(L.shape, rt, I);
+        if (!I.Valid) return false;
+        P = transform_point(L.shape.tm, I.P);
+        N = transform_vector(L.shape.tm, I.N);
+        T = length(P - r.O);
+        Le = I.Front ? scale(R(L.multiplier), eval_texture(L.texture_id, I.UV)) : V3<R>();
+        if (L.unit == ER_UNIT_LUX) Le = V3<R>(Le[0] / R(L.shape.area), Le[1] / R(L.shape.area), Le[2] / R(L.shape.area));
+        return true;
+    }
+    /* IntersectLights (lights.h:90-113): nearest hit; SE.LightID = list position */
+    bool intersect_lights(const Ray<R>& r, bool respect_visibility, R& T, V3<R>& P, V3<R>& N, V3<R>& Le, int& idx) const
+    {
+        R best = R(FLT_MAX);
+        bool any = false;
+        for (int i = 0; i < S.t.light_count; ++i) {
+            const er_light_desc* L = light(S.t.light_ids[i]);
+            if (!L) continue;
+            if (respect_visibility && !L->visible) continue;
+            R t; V3<R> p, n, le;
+            if (intersect_light(*L, r, t, p, n, le) && t < best) {
+                best = t; T = t; P = p; N = n; Le = le; idx = i; any = true;
+            }
+        }
+        return any;
+    }
+    /* IntersectsLight (lights.h:115-131) */
+    bool intersects_any_light(const Ray<R>& r) const
+    {
+        for (int i = 0; i < S.t.light_count; ++i) {
+            const er_light_desc* L = light(S.t.light_ids[i]);
+            if (!L) continue;
+            Isect<R> I;
+            intersect_shape(L->shape, transform_ray(L->shape.inv_tm, r), I);
+            if (I.Valid) return true;
+        }
+        return false;
+    }
+
+    /* Visible (transport.h:46-56) -> Intersect (transport.h:30-44) */
+    bool visible(const V3<R>& P1, const V3<R>& P2)
+    {
+        if (!S.t.settings.shadows) return true;
+        const V3<R> W = normalize(P2 - P1);
+        const Ray<R> r(P1 + scale(R(RAY_EPS), W), W, R(0), std::min(length(P2 - P1) - R(RAY_EPS_2), R(S.t.settings.max_shadow_distance)));
+        if (cnt) cnt[7]++;
+        if (intersects_any_light(r)) return false;
+        R T; V3<R> P;
+        if (march(r, false, T, P)) return false;
+        return true;
+    }
+
+    /* SampleLight (lights.h:43-57) */
+    void sample_light(const er_light_desc& L, const V3<R>& uvw, const V3<R>& SEP, SS& ss, V3<R>& Wi, V3<R>& Le) const
+    {
+        sample_shape(L.shape, uvw, ss);
+        Wi = normalize(ss.P - SEP);
+        Le = scale(R(L.multiplier), eval_texture(L.texture_id, ss.UV));
+        if (L.shape.one_sided && dot(SEP - ss.P, ss.N) < R(0)) Le = V3<R>();
+        if (L.unit == ER_UNIT_LUX) Le = V3<R>(Le[0] / R(L.shape.area), Le[1] / R(L.shape.area), Le[2] / R(L.shape.area));
+    }
+
+    struct Sample { R comp, dir0, dir1, uvw[3], lightnum, film0, film1, lens0, lens1; };
+
+    /* EstimateDirectLight (transport.h:58-113). COMPAT stops after the light-sampling half
+     * (the early `return Ld;` at transport.h:84); FULL runs both halves with the power
+     * heuristic on both. */
+    V3<R> estimate_direct(const er_light_desc& L, int light_pos, const Sample& smp, const V3<R>& SEP, const V3<R>& SEN, const V3<R>& Wo, const Shader& sh)
+    {
+        V3<R> Ld;
+        SS ss; V3<R> Wi, Li;
+        sample_light(L, V3<R>(smp.uvw[0], smp.uvw[1], smp.uvw[2]), SEP, ss, Wi, Li);
+        V3<R> F = shader_f(sh, Wo, Wi);
+        R BsdfPdf = shader_pdf(sh, Wo, Wi);
+        const R area = R(L.shape.area);
+        if (!is_black(Li) && !is_black(F) && BsdfPdf > R(0) && visible(SEP, ss.P)) {
+            const R LightPdf = dist2(SEP, ss.P) / (abs_dot(ss.N, -Wi) * area);
+            const R Weight = power_heuristic(LightPdf, BsdfPdf);
+            const V3<R> FL = F * Li;
+            if (sh.brdf) Ld = Ld + scale(abs_dot(Wi, SEN) * Weight / LightPdf, FL);
+            else if (full) Ld = Ld + scale(Weight / LightPdf, FL);
+            else Ld = Ld + V3<R>(FL[0] / LightPdf, FL[1] / LightPdf, FL[2] / LightPdf);
+        }
+        if (!full) return Ld;
+
+        F = shader_sample_f(sh, Wo, Wi, BsdfPdf, smp.comp, smp.dir0, smp.dir1);
+        if (is_black(F) || BsdfPdf <= R(0)) return Ld;
+        R T2; V3<R> P2, N2, Le2; int idx2 = -1;
+        const Ray<R> r2(SEP, Wi, R(0), R(1000000.0f));
+        if (!intersect_lights(r2, false, T2, P2, N2, Le2, idx2) || idx2 != light_pos) return Ld;
+        if (!is_black(Le2) && visible(SEP, P2)) {
+            const R LightPdf = dist2(SEP, P2) / (abs_dot(N2, -Wi) * area);
+            const R Weight = power_heuristic(BsdfPdf, LightPdf);
+            const V3<R> FL = F * Le2;
+            if (sh.brdf) Ld = Ld + scale(abs_dot(Wi, SEN) * Weight / BsdfPdf, FL);
+            else Ld = Ld + scale(Weight / BsdfPdf, FL);
+        }
+        return Ld;
+    }
+
+    /* UniformSampleOneLight (transport.h:115-156) with the full-mode shader selection
+     * (design: scatterevent.h:155-212) */
+    V3<R> uniform_sample_one_light(const Sample& smp, const V3<R>& SEP, const V3<R>& SEN, const V3<R>& Wo)
+    {
+        V3<R> Ld;
+        const R I = get_intensity(SEP);
+        if (cnt) cnt[3] += 1 + gradient_samples();
+        Ld = Ld + ctf_eval(S.t.emission, I);
+        const int count = S.t.light_count;
+        if (count <= 0) return Ld;
+        const int pos = (int)std::floor(smp.lightnum * R((float)count));
+        const int LightID = S.t.light_ids[pos];
+        const er_light_desc* L = light(LightID);
+        if (LightID < 0 || !L) return Ld;
+        bool brdf = true;
+        if (full) {
+            const int type = S.t.settings.shading_type;
+            if (type == ER_SHADING_PHASE_ONLY) brdf = false;
+            else if (type == ER_SHADING_HYBRID || type == ER_SHADING_GRADIENT_MAGNITUDE || type == ER_SHADING_MODULATION || type == ER_SHADING_THRESHOLD) {
+                const R gm = gradient_magnitude(SEP);
+                if (cnt) cnt[3] += 6;
+                const R NGM = S.gm_max > 0.0f ? gm * (R(1) / R(S.gm_max)) : R(0);
+                if (type == ER_SHADING_THRESHOLD) {
+                    brdf = NGM > R(S.t.settings.gradient_threshold);
+                } else if (type == ER_SHADING_MODULATION) {
+                    const R PdfBrdf = R(1) - M<R>::pow(R(1) - NGM, R(2));
+                    brdf = u() < PdfBrdf;
+                } else {
+                    const R e = R(25) * M<R>::pow(R(S.t.settings.gradient_factor), R(3)) * NGM;
+                    const R base = R(1) - M<R>::exp(-e);
+                    const R PdfBrdf = S.t.settings.opacity_modulated ? plf_eval(S.t.opacity, I) * base : base;
+                    brdf = u() <= PdfBrdf;
+                }
+            }
+        }
+        const V3<R> Kd = ctf_eval(S.t.diffuse, I);
+        const V3<R> Ks = ctf_eval(S.t.specular, I);
+        const R gloss = R(1000000.0f) * M<R>::pow(plf_eval(S.t.glossiness, I), R(7));   /* utilities.h:28-31 */
+        const Shader sh = make_shader(brdf, SEN, Wo, Kd, Ks, R(15.0f), gloss);
+        Ld = Ld + estimate_direct(*L, pos, smp, SEP, SEN, Wo, sh);
+        return scale(R((float)count), Ld);
+    }
+
+    /* SingleScattering (singlescattering.h:29-102) */
+    void single_scattering(int px, int py, R out[4])
+    {
+        if (cnt) cnt[0]++;
+        Sample smp;
+        smp.comp = u();                                       /* sample.h:142-146 */
+        smp.dir0 = u(); smp.dir1 = u();
+        smp.uvw[0] = u(); smp.uvw[1] = u(); smp.uvw[2] = u(); /* sample.h:101-104 */
+        smp.lightnum = u();                                   /* sample.h:184 */
+        smp.film0 = u(); smp.film1 = u();                     /* sample.h:221-225 */
+        smp.lens0 = u(); smp.lens1 = u();
+
+        /* SampleCamera (singlescattering.h:29-50) */
+        const er_camera& c = S.t.camera;
+        const R sx = R(S.screen[0][0]) + (R(S.invscreen[0]) * R((float)px + smp.film0 * R(1.0f) / R((float)c.film_size[0])));
+        const R sy = R(S.screen[1][0]) + (R(S.invscreen[1]) * R((float)py + smp.film1 * R(1.0f) / R((float)c.film_size[1])));
+        const V3<R> N = from<R>(S.camN), U = from<R>(S.camU), V = from<R>(S.camV);
+        Ray<R> r;
+        r.O = from<R>(c.pos);
+        r.D = normalize((N + scale(sx, U)) - scale(sy, V));
+        r.MinT = R(c.clip_near);
+        r.MaxT = R(c.clip_far);
+        if (c.aperture_size != 0.0f) {
+            R lx, ly;
+            concentric_disk(smp.lens0, smp.lens1, lx, ly);
+            lx = lx * R(c.aperture_size); ly = ly * R(c.aperture_size);
+            const V3<R> LI = scale(lx, U) + scale(ly, V);
+            r.O = r.O + LI;
+            r.D = normalize(scale(R(S.focal), r.D) - LI);
+        }
+
+        /* SampleRay (singlescattering.h:52-74): volume, lights, objects; nearest by T */
+        R Tv = R(0); V3<R> Pv;
+        const bool vol = march(r, true, Tv, Pv);
+        R Tl = R(0); V3<R> Pl, Nl, Lel; int il = -1;
+        const bool lig = intersect_lights(r, true, Tl, Pl, Nl, Lel, il);
+        int type = -1;
+        R T = R(FLT_MAX);
+        if (vol && Tv < T) { type = 0; T = Tv; }
+        if (lig && Tl < T) { type = 1; T = Tl; }
+
+        V3<R> Lv;
+        if (type == 0) {
+            if (cnt) cnt[6]++;
+            const V3<R> Nv = normalized_gradient(Pv);   /* raymarching.h:70 */
+            Lv = Lv + uniform_sample_one_light(smp, Pv, Nv, -r.D);
+        }
+        if (type == 1) Lv = Lv + Lel;
+        out[0] = Lv[0]; out[1] = Lv[1]; out[2] = Lv[2];
+        out[3] = type >= 0 ? R(1) : R(0);
+    }
+};
+
+/* ---- post (filterframeestimate.cuh:28-64, estimate.cuh:27-32, tonemap.cuh:30-59) ------- */
+static void post_pass(const Scene& S, const float* frame, float* running, uint8_t* display, int no_iterations, float* filtered)
+{
+    const int W = S.t.camera.film_size[0], H = S.t.camera.film_size[1];
+    /* Gauss2D(1, X, Y) = expf(-(X*X + Y*Y) / 2) for X, Y in {-1, 0, 1} */
+    const float g[3] = {erm::expf_(-0.0f), erm::expf_(-0.5f), erm::expf_(-1.0f)};
+    const float exposure = S.t.camera.exposure;
+    const float nf = std::max((float)no_iterations, 1.0f);
+    #pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            const int x0 = std::max(x - 1, 0), x1 = std::min(x + 1, W - 1);
+            const int y0 = std::max(y - 1, 0), y1 = std::min(y + 1, H - 1);
+            float sum[4] = {0, 0, 0, 0}, tw = 0.0f;
+            for (int yy = y0; yy <= y1; ++yy)
+                for (int xx = x0; xx <= x1; ++xx) {
+                    const int d2 = (xx - x) * (xx - x) + (yy - y) * (yy - y);
+                    const float w = g[d2];
+                    const float* f = frame + 4 * ((size_t)yy * W + xx);
+                    for (int c = 0; c < 4; ++c) sum[c] += f[c] * w;
+                    tw += w;
+                }
+            sum[3] = frame[4 * ((size_t)y * W + x) + 3];
+            float fe[4];
+            for (int c = 0; c < 4; ++c) fe[c] = tw > 0.0f ? sum[c] / tw : 0.0f;   /* alpha / tw too */
+            if (filtered) for (int c = 0; c < 4; ++c) filtered[4 * ((size_t)y * W + x) + c] = fe[c];
+            float* r = running + 4 * ((size_t)y * W + x);
+            for (int c = 0; c < 4; ++c) r[c] = r[c] + (fe[c] - r[c]) / nf;
+            /* ColorRGBf::FromXYZAf (color.h:155-164) + tonemap */
+            float rgb[3];
+            rgb[0] = 3.240479f * r[0] - 1.537150f * r[1] - 0.498535f * r[2];
+            rgb[1] = -0.969256f * r[0] + 1.875991f * r[1] + 0.041556f * r[2];
+            rgb[2] = 0.055648f * r[0] - 0.204043f * r[1] + 1.057311f * r[2];
+            uint8_t* d = display + 4 * ((size_t)y * W + x);
+            for (int c = 0; c < 3; ++c) {
+                float v = 1.0f - erm::expf_(-(rgb[c] / exposure));
+                v = std::fmax(0.0f, std::fmin(v, 1.0f));   /* Clamp via device min/max */
+                d[c] = (uint8_t)(255.0f * v);
+            }
+            d[3] = (uint8_t)(r[3] * 255.0f);
+        }
+    }
+}
+
+/* Separable bilateral filter on RGBA8 (design: filterrunningestimate.cuh:28-200,
+ * filter.h:34-40). Taps k in [max(0, i-R), min(i+R, W-1)) - the upper bound is exclusive
+ * as written (Q27). Luminance weights of color.h:37-51 on 0..255 values. */
+static void bilateral_pass(const Scene& S, const uint8_t* in, uint8_t* tmp, uint8_t* out)
+{
+    const int W = S.t.camera.film_size[0], H = S.t.camera.film_size[1];
+    const int R = S.t.settings.bilateral_radius;
+    const float ss = S.t.settings.bilateral_sigma_spatial, sr = S.t.settings.bilateral_sigma_range;
+    std::vector<float> kd(2 * R + 1), gs(256);
+    for (int k = -R; k <= R; ++k) kd[k + R] = erm::expf_(-((float)(k * k)) / (2.0f * ss * ss));
+    for (int d = 0; d < 256; ++d) gs[d] = erm::expf_(-((float)(d * d)) / (2.0f * sr * sr));
+    auto lum = [](const uint8_t* p) { return 0.212671f * (float)p[0] + 0.715160f * (float)p[1] + 0.072169f * (float)p[2]; };
+    for (int pass = 0; pass < 2; ++pass) {
+        const uint8_t* src = pass == 0 ? in : tmp;
+        uint8_t* dst = pass == 0 ? tmp : out;
+        #pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const int i = pass == 0 ? x : y;
+                const int n = pass == 0 ? W : H;
+                const int lo = std::max(0, i - R), hi = std::min(i + R, n - 1);
+                const uint8_t* c = src + 4 * ((size_t)y * W + x);
+                const float yc = lum(c);
+                float sum[3] = {0, 0, 0}, tw = 0.0f;
+                for (int k = lo; k < hi; ++k) {
+                    const uint8_t* p = pass == 0 ? src + 4 * ((size_t)y * W + k) : src + 4 * ((size_t)k * W + x);
+                    const float w = kd[R + k - i] * gs[(int)std::fabs(lum(p) - yc)];
+                    tw += w;
+                    for (int ch = 0; ch < 3; ++ch) sum[ch] += (float)p[ch] * w;
+                }
+                uint8_t* d = dst + 4 * ((size_t)y * W + x);
+                for (int ch = 0; ch < 3; ++ch)
+                    d[ch] = tw > 0.0f ? (uint8_t)std::max(0.0f, std::min(sum[ch] / tw, 255.0f)) : 0;
+                d[3] = 255;
+            }
+    }
+}
+
+} /* namespace orc */
+
+using namespace orc;
+
+/* ---- C API (ctypes) -------------------------------------------------------------------- */
+extern "C" {
+
+void* orc_create(const er_tracer_desc* t, const er_volume_desc* v, const er_light_desc* lights, int nlights,
+                 const er_texture_desc* textures, int ntextures, const er_clipping_desc* clips, int nclips)
+{
+    Scene* S = new Scene();
+    S->t = *t;
+    camera_update(*S);
+    volume_update(*S, *v);
+    const size_t n = (size_t)v->resolution[0] * v->resolution[1] * v->resolution[2];
+    S->vox.assign(v->voxels, v->voxels + n);
+    for (int i = 0; i < nlights; ++i) { er_light_desc L = lights[i]; L.shape.area = shape_area(L.shape); S->lights.push_back(L); }
+    for (int i = 0; i < ntextures; ++i) S->textures.push_back(textures[i]);
+    for (int i = 0; i < nclips; ++i) S->clips.push_back(clips[i]);
+    /* Woodcock global majorant: DS^2 * max opacity over intensities reachable by the volume */
+    uint16_t vmax = 0;
+    for (size_t i = 0; i < n; ++i) vmax = std::max(vmax, S->vox[i]);
+    float omax = 0.0f;
+    for (int i = 0; i <= (int)vmax; ++i) omax = std::max(omax, plf_eval(S->t.opacity, (float)i));
+    S->maj_max = S->t.settings.density_scale * S->t.settings.density_scale * omax;
+    return S;
+}
+
+void orc_destroy(void* h) { delete (Scene*)h; }
+
+void orc_set_gm_max(void* h, float gm) { ((Scene*)h)->gm_max = gm; }
+float orc_get_gm_max(void* h) { return ((Scene*)h)->gm_max; }
+
+/* Maximum gradient magnitude over voxel centres (design: gradientmagnitude.cuh:28-70);
+ * P = MinAABB + Size * ((X + 0.5, Y + 0.5, Z + 0.5) / Resolution). */
+float orc_compute_gm_max(void* h, int nthreads)
+{
+    Scene& S = *(Scene*)h;
+    float best = 0.0f;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    #pragma omp parallel for schedule(static) reduction(max:best)
+    for (int z = 0; z < S.res[2]; ++z) {
+        Rng dummy{1, 1};
+        Est<float> e(S, dummy, nullptr);
+        for (int y = 0; y < S.res[1]; ++y)
+            for (int x = 0; x < S.res[0]; ++x) {
+                const V3<float> P(S.minp[0] + S.size[0] * (((float)x + 0.5f) / (float)S.res[0]),
+                                  S.minp[1] + S.size[1] * (((float)y + 0.5f) / (float)S.res[1]),
+                                  S.minp[2] + S.size[2] * (((float)z + 0.5f) / (float)S.res[2]));
+                best = std::max(best, e.gradient_magnitude(P));
+            }
+    }
+    S.gm_max = best;
+    return best;
+}
+
+/* One FP32 pass over every pixel: frame[W*H*4] = SingleScattering; seeds updated in place.
+ * counters (may be NULL): 8 x uint64 in er_counters order, accumulated. */
+int orc_trace_f32(void* h, uint32_t* seeds1, uint32_t* seeds2, float* frame, int nthreads, uint64_t* counters)
+{
+    Scene& S = *(Scene*)h;
+    const int W = S.t.camera.film_size[0], H = S.t.camera.film_size[1];
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    std::vector<uint64_t> acc(8, 0);
+    #pragma omp parallel
+    {
+        uint64_t local[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        #pragma omp for schedule(dynamic, 1)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t i = (size_t)y * W + x;
+                Rng rng{seeds1[i], seeds2[i]};
+                Est<float> e(S, rng, counters ? local : nullptr);
+                float out[4];
+                e.single_scattering(x, y, out);
+                for (int c = 0; c < 4; ++c) frame[4 * i + c] = out[c];
+                seeds1[i] = rng.s0; seeds2[i] = rng.s1;
+            }
+        #pragma omp critical
+        for (int k = 0; k < 8; ++k) acc[k] += local[k];
+    }
+    if (counters) for (int k = 0; k < 8; ++k) counters[k] += acc[k];
+    return 0;
+}
+
+/* FP32 pass over a pixel subset (px, py, n); seeds are per listed pixel. Also returns the
+ * number of RNG draws per pixel (draws[n], may be NULL). */
+int orc_trace_pixels_f32(void* h, const int* px, const int* py, int n, uint32_t* seeds1, uint32_t* seeds2,
+                         float* out, uint32_t* draws, uint64_t* counters)
+{
+    Scene& S = *(Scene*)h;
+    for (int i = 0; i < n; ++i) {
+        Rng rng{seeds1[i], seeds2[i]};
+        Est<float> e(S, rng, counters);
+        e.single_scattering(px[i], py[i], out + 4 * i);
+        seeds1[i] = rng.s0; seeds2[i] = rng.s1;
+        if (draws) draws[i] = rng.draws;
+    }
+    return 0;
+}
+
+/* FP64 oracle: per listed pixel, the mean of `spp` samples of the Real=double estimator
+ * (unfiltered XYZA), with per-pixel streams from (seed1[i], seed2[i]). */
+int orc_expect_f64(void* h, const int* px, const int* py, int n, int spp, const uint32_t* seeds1,
+                   const uint32_t* seeds2, double* out, int nthreads)
+{
+    Scene& S = *(Scene*)h;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < n; ++i) {
+        Rng rng{seeds1[i], seeds2[i]};
+        Est<double> e(S, rng, nullptr);
+        double acc[4] = {0, 0, 0, 0};
+        for (int s = 0; s < spp; ++s) {
+            double o[4];
+            e.single_scattering(px[i], py[i], o);
+            for (int c = 0; c < 4; ++c) acc[c] += o[c];
+        }
+        for (int c = 0; c < 4; ++c) out[4 * i + c] = acc[c] / spp;
+    }
+    return 0;
+}
+
+/* Filter + running average + tonemap for one pass (no_iterations = passes before this one). */
+int orc_post(void* h, const float* frame, float* running, uint8_t* display, int no_iterations, float* filtered)
+{
+    post_pass(*(Scene*)h, frame, running, display, no_iterations, filtered);
+    return 0;
+}
+
+int orc_bilateral(void* h, const uint8_t* in, uint8_t* out)
+{
+    Scene& S = *(Scene*)h;
+    std::vector<uint8_t> tmp((size_t)S.t.camera.film_size[0] * S.t.camera.film_size[1] * 4);
+    bilateral_pass(S, in, tmp.data(), out);
+    return 0;
+}
+
+/* Single functions for pinning tests. */
+float orc_intensity(void* h, float x, float y, float z)
+{
+    Rng r{1, 1};
+    Est<float> e(*(Scene*)h, r, nullptr);
+    return e.get_intensity(V3<float>(x, y, z));
+}
+float orc_plf_eval(const er_plf* p, float x) { return plf_eval(*p, x); }
+double orc_plf_eval_f64(const er_plf* p, double x) { return plf_eval(*p, x); }
+float orc_rng_get1(uint32_t* s0, uint32_t* s1) { Rng r{*s0, *s1}; const float v = r.get1(); *s0 = r.s0; *s1 = r.s1; return v; }
+float orc_neglog(float u) { return M<float>::neglog(u); }
+int orc_camera(void* h, float* out13)
+{
+    Scene& S = *(Scene*)h;
+    for (int i = 0; i < 3; ++i) { out13[i] = S.camN[i]; out13[3 + i] = S.camU[i]; out13[6 + i] = S.camV[i]; }
+    out13[9] = S.screen[0][0]; out13[10] = S.screen[1][0]; out13[11] = S.invscreen[0]; out13[12] = S.invscreen[1];
+    return 0;
+}
+/* Primary march of one explicit ray (FP32): returns 1 on a scatter event, writes T and P and
+ * the number of steps taken; u1, u2 are the two draws. */
+int orc_march_ray(void* h, const float* o, const float* d, float mint, float maxt, int primary, float u1, float u2,
+                  float* T, float* P, uint64_t* steps)
+{
+    /* Restates march() (raymarching.h:30-113) for one explicit ray with given draws u1
+     * (free-path threshold) and u2 (jitter). */
+    Scene& S = *(Scene*)h;
+    Rng r{1, 1};
+    Est<float> est(S, r, nullptr);
+    const V3<float> mn = from<float>(S.minp), mx = from<float>(S.maxp);
+    const Ray<float> ray(V3<float>(o[0], o[1], o[2]), V3<float>(d[0], d[1], d[2]), mint, maxt);
+    float nearT, farT;
+    *steps = 0;
+    if (!Est<float>::intersect_box(ray, mn, mx, nearT, farT)) return -1;
+    float MinT = std::max(nearT, ray.MinT);
+    const float MaxT = std::min(farT, ray.MaxT);
+    const float Sthr = M<float>::neglog(u1) / S.t.settings.density_scale;
+    float Sum = 0.0f;
+    const float StepSize = (primary ? S.t.settings.step_factor_primary : S.t.settings.step_factor_shadow) * S.minstep;
+    MinT += u2 * StepSize;
+    if (!(StepSize > 0.0f)) return 0;
+    V3<float> Ps;
+    uint64_t n = 0;
+    while (Sum < Sthr) {
+        Ps = ray.O + scale(MinT, ray.D);
+        if (primary ? (MinT >= MaxT) : (MinT > MaxT)) { *steps = n; return 0; }
+        ++n;
+        const float SigmaT = est.sigma_at(Ps, primary != 0);
+        Sum += SigmaT * StepSize;
+        MinT += StepSize;
+    }
+    *steps = n;
+    *T = MinT;
+    P[0] = Ps[0]; P[1] = Ps[1]; P[2] = Ps[2];
+    return 1;
+}
+
+int orc_abi_sizeof(int* sizes, int n)
+{
+    const int s[11] = {(int)sizeof(er_tracer_desc), (int)sizeof(er_volume_desc), (int)sizeof(er_light_desc),
+                       (int)sizeof(er_texture_desc), (int)sizeof(er_clipping_desc), (int)sizeof(er_object_desc),
+                       (int)sizeof(er_bitmap_desc), (int)sizeof(er_plf), (int)sizeof(er_shape),
+                       (int)sizeof(er_timing), (int)sizeof(er_counters)};
+    int k = 0;
+    for (; k < n && k < 11; ++k) sizes[k] = s[k];
+    return k;
+}
+
+} /* extern "C" */

@@ -1,0 +1,142 @@
+"""Per-sample parity of the gfx950 tracer with the CPU oracle (through the C-ABI).
+
+With identical per-pixel seeds the RNG streams, the scatter / no-scatter and light-index
+decisions (visible in the alpha channel, the sample counters and the RNG state after each pass)
+must match bit for bit, and the FP32 radiance within 1e-4 relative per sample. Most cases are
+in fact bit-identical; the tests report the exact-match fraction.
+"""
+import numpy as np
+import pytest
+
+import exposure_render as er
+from exposure_render import capi, scenes
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4      # FP32 radiance per sample, relative
+ATOL = 1e-6
+
+COUNTER_KEYS = ["paths", "primary_samples", "shadow_samples", "shading_samples", "scatter_events", "shadow_rays"]
+ORC_INDEX = {"paths": 0, "primary_samples": 1, "shadow_samples": 2, "shading_samples": 3, "scatter_events": 6,
+             "shadow_rays": 7}
+
+
+def run_parity(sc, passes=3, seed=5, counting=True, check_display=True):
+    print(f"[parity] scene={sc.name} film={sc.film} vol={sc.voxels.shape} estimator={sc.tracer.settings.estimator} "
+          f"shading={sc.tracer.settings.shading_type} sampler={sc.tracer.settings.sampler}", flush=True)
+    r = er.Renderer(sc)
+    orc = O.Oracle(sc)
+    if sc.tracer.settings.estimator == capi.ESTIMATOR_FULL and sc.tracer.settings.shading_type not in (0, 1):
+        gm = r.gm_max()
+        gm_ref = orc.compute_gm_max()
+        assert gm == gm_ref, (gm, gm_ref)
+    s1, s2 = O.seeds_for(r.W, r.H, seed)
+    r.set_seeds(s1, s2)
+    r.set_counting(counting)
+    r.reset_counters()
+    running = np.zeros((r.H, r.W, 4), dtype=np.float32)
+    cnt_ref = np.zeros(8, dtype=np.uint64)
+    exact = 0
+    total = 0
+    disp_ref = None
+    for k in range(passes):
+        r.render_estimate()
+        f_ref, _ = orc.trace(s1, s2, cnt_ref)
+        disp_ref, _ = orc.post(f_ref, running, k)
+        f = r.frame()
+        g1, g2 = r.get_seeds()
+        assert np.array_equal(g1, s1) and np.array_equal(g2, s2), f"RNG state differs after pass {k}"
+        assert np.array_equal(f[..., 3], f_ref[..., 3]), f"alpha (valid event) differs in pass {k}"
+        np.testing.assert_allclose(f, f_ref, rtol=RTOL, atol=ATOL, err_msg=f"frame pass {k}")
+        exact += int(np.sum(f == f_ref))
+        total += f.size
+    if counting:
+        c = r.counters()
+        for key in COUNTER_KEYS:
+            assert c[key] == int(cnt_ref[ORC_INDEX[key]]), (key, c[key], int(cnt_ref[ORC_INDEX[key]]))
+    np.testing.assert_allclose(r.running(), running, rtol=RTOL, atol=ATOL)
+    if check_display:
+        d = r.get_estimate()
+        if sc.tracer.settings.estimator == capi.ESTIMATOR_FULL and sc.tracer.settings.bilateral:
+            disp_ref = orc.bilateral(disp_ref)
+        diff = np.abs(d.astype(np.int32) - disp_ref.astype(np.int32))
+        assert diff.max() <= 1, f"display differs by {diff.max()} LSB"
+    assert r.no_iterations() == passes
+    print(f"[parity] exact-match fraction {exact / max(total, 1):.6f}", flush=True)
+    r.close()
+    return exact / max(total, 1)
+
+
+def test_small_compat(fresh):
+    sc = scenes.config_small(scale=0.5, estimator=capi.ESTIMATOR_COMPAT)
+    assert run_parity(sc) > 0.999
+
+
+def test_small_full_phase(fresh):
+    sc = scenes.config_small(scale=0.5, estimator=capi.ESTIMATOR_FULL)
+    run_parity(sc)
+
+
+def test_ct_full_hybrid_env_bilateral(fresh):
+    sc = scenes.config_ct(scale=0.125)
+    run_parity(sc, passes=2)
+
+
+def test_ct_compat_brdf(fresh):
+    sc = scenes.config_ct(scale=0.125, estimator=capi.ESTIMATOR_COMPAT)
+    run_parity(sc, passes=2)
+
+
+def test_large_clipping_anisotropic(fresh):
+    sc = scenes.config_large(scale=0.0625)
+    run_parity(sc, passes=2)
+
+
+def test_shading_modes(fresh):
+    for mode in (capi.SHADING_BRDF_ONLY, capi.SHADING_MODULATION, capi.SHADING_THRESHOLD,
+                 capi.SHADING_GRADIENT_MAGNITUDE):
+        sc = scenes.config_ct(scale=0.0625, bilateral=False)
+        sc.tracer.settings.shading_type = mode
+        run_parity(sc, passes=1)
+        er.shutdown()
+
+
+def test_gradient_modes(fresh):
+    for mode in (capi.GRADIENT_FORWARD, capi.GRADIENT_FILTERED):
+        sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+        sc.tracer.settings.gradient_computation = mode
+        run_parity(sc, passes=1)
+        er.shutdown()
+
+
+def test_woodcock(fresh):
+    sc = scenes.config_small(scale=0.5, estimator=capi.ESTIMATOR_FULL)
+    sc.tracer.settings.sampler = capi.SAMPLER_WOODCOCK
+    run_parity(sc, passes=2)
+
+
+def test_no_lights_emission_only(fresh):
+    sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+    sc.tracer.light_count = 0
+    sc.tracer.emission.init()
+    sc.tracer.emission.add(0.0, (0.1, 0.2, 0.3)).add(4000.0, (0.5, 0.4, 0.3))
+    run_parity(sc, passes=2)
+
+
+def test_odd_film_sizes(fresh):
+    for film in ((1, 1), (17, 5), (33, 70)):
+        sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+        sc.tracer.camera.film_size[0], sc.tracer.camera.film_size[1] = film
+        run_parity(sc, passes=2)
+        er.shutdown()
+
+
+def test_camera_inside_volume_and_thin_lens(fresh):
+    sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+    scenes.set_camera(sc.tracer, sc.film, (0.1, 0.05, 0.2), (0.0, 0.0, 0.0), clip_near=0.0, exposure=4.0)
+    run_parity(sc, passes=1)
+    er.shutdown()
+    sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+    scenes.set_camera(sc.tracer, sc.film, (1.2, 0.9, 1.6), (0.0, 0.0, 0.0), exposure=4.0, aperture=0.05, focal=-1.0)
+    run_parity(sc, passes=1)
