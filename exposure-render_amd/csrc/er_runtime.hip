@@ -696,6 +696,14 @@ ER_API int er_shutdown(void)
     return ER_OK;
 }
 
+ER_API int er_synchronize(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    if (!g_device_ready) return ER_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    return ER_OK;
+}
+
 ER_API int er_bind_tracer(er_tracer_desc* d, int bind)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mutex);

@@ -188,6 +188,7 @@ SIGNATURES = {
     "er_get_no_iterations": (C.c_int, [C.c_int, _P(C.c_int)]),
     "er_init": (C.c_int, []),
     "er_shutdown": (C.c_int, []),
+    "er_synchronize": (C.c_int, []),
     "er_get_seeds": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t]),
     "er_set_seeds": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t]),
     "er_seed_tracer": (C.c_int, [C.c_int, C.c_uint64]),

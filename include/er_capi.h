@@ -318,6 +318,7 @@ ER_API int er_get_no_iterations(int tracer_id, int* no_iterations);
 
 ER_API int er_init(void);
 ER_API int er_shutdown(void);     /* unbinds everything and frees device memory */
+ER_API int er_synchronize(void);  /* waits for all work on the device */
 
 /* Per-pixel RNG seeds (FrameBuffer::RandomSeeds1/2, framebuffer.h:100-101). count = W*H. */
 ER_API int er_get_seeds(int tracer_id, uint32_t* seeds1, uint32_t* seeds2, size_t count);

@@ -1285,6 +1285,23 @@ int orc_march_ray(void* h, const float* o, const float* d, float mint, float max
     return 1;
 }
 
+/* the project's float elementary functions (er_fmath.h), vectorised for the accuracy tests:
+ * fn 0 exp, 1 log, 2 pow, 3 sin, 4 cos, 5 atan2(x, y), 6 acos */
+void orc_fmath(int fn, const float* x, const float* y, float* out, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = erm::expf_(x[i]); break;
+            case 1: out[i] = erm::logf_(x[i]); break;
+            case 2: out[i] = erm::powf_(x[i], y[i]); break;
+            case 3: out[i] = erm::sinf_(x[i]); break;
+            case 4: out[i] = erm::cosf_(x[i]); break;
+            case 5: out[i] = erm::atan2f_(x[i], y[i]); break;
+            default: out[i] = erm::acosf_(x[i]); break;
+        }
+    }
+}
+
 int orc_abi_sizeof(int* sizes, int n)
 {
     const int s[11] = {(int)sizeof(er_tracer_desc), (int)sizeof(er_volume_desc), (int)sizeof(er_light_desc),

@@ -49,6 +49,7 @@ def load():
         "orc_march_ray": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, C.c_int,
                                     C.c_float, C.c_float, _P(C.c_float), _P(C.c_float), _P(C.c_uint64)]),
         "orc_abi_sizeof": (C.c_int, [_P(C.c_int), C.c_int]),
+        "orc_fmath": (None, [C.c_int, _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -219,3 +220,14 @@ def gauss3x3_f64(img):
             tw[yd, xd] += w
     out[..., 3] = img[..., 3]
     return out / tw[..., None]
+
+
+def fmath(fn, x, y=None):
+    """Evaluate the project's float elementary functions (0 exp, 1 log, 2 pow, 3 sin, 4 cos,
+    5 atan2, 6 acos) over float32 arrays."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float32)
+    out = np.zeros_like(x)
+    lib.orc_fmath(fn, _p(x, C.c_float), _p(y, C.c_float), _p(out, C.c_float), x.size)
+    return out

@@ -1,0 +1,382 @@
+"""Pins the CPU oracle (CPU only).
+
+The reference ships no tests, fixtures or golden vectors, and its hot-path sources cannot be
+built here without stand-in headers, so every check below pins an oracle function against
+what the mathematics fixes: closed forms, invariants, special cases, exhaustive sweeps of
+finite domains and brute force on tiny inputs. Each check is chosen so that a plausible slip
+(a transposed index, a dropped term, a wrong power of the density scale, a wrong draw order)
+fails it.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from exposure_render import capi, scenes
+
+
+def _u_from(s0, s1):
+    return ((((s0 << 16) + s1) & 0xFFFFFFFF) & 0x7FFFFF) / 8388608.0
+
+
+# ---- RNG (rng.h:35-51) -----------------------------------------------------------------
+
+def test_rng_mwc_closed_form(orc_lib):
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        a, b = (int(v) for v in rng.integers(1, 2**32, size=2))
+        s0 = C.c_uint32(a)
+        s1 = C.c_uint32(b)
+        for _ in range(20):
+            u = orc_lib.orc_rng_get1(C.byref(s0), C.byref(s1))
+            # Marsaglia's two 16-bit multiply-with-carry generators
+            a = (36969 * (a & 0xFFFF) + (a >> 16)) & 0xFFFFFFFF
+            b = (18000 * (b & 0xFFFF) + (b >> 16)) & 0xFFFFFFFF
+            assert s0.value == a and s1.value == b
+            assert u == _u_from(a, b)
+            assert 0.0 <= u < 1.0
+
+
+def test_rng_zero_seed_is_fixed_point(orc_lib):
+    s0, s1 = C.c_uint32(0), C.c_uint32(0)
+    for _ in range(5):
+        assert orc_lib.orc_rng_get1(C.byref(s0), C.byref(s1)) == 0.0
+    assert s0.value == 0 and s1.value == 0
+
+
+def test_seed_hash_never_zero():
+    s1, s2 = O.seeds_for(4096, 4096, 0)
+    assert s1.min() > 0 and s2.min() > 0
+    assert len(np.unique(s1[:100000])) > 99000
+
+
+# ---- -log(u) (raymarching.h:45,89) -----------------------------------------------------
+
+def test_neglog_correctly_rounded_exhaustive(orc_lib):
+    """Over all 2^23 values u = m 2^-23 the oracle's -log(u) must be the float nearest the exact
+    value. float64 log has < 1 ulp(double) error; the check asserts that no exact value lies
+    within 4 ulp(double) of a float rounding boundary, so rounding the float64 result is the
+    correctly rounded float for every m."""
+    m = np.arange(1, 2**23, dtype=np.float64)
+    v = -np.log(m * 2.0 ** -23)
+    f = v.astype(np.float32)
+    # distance of v to the nearest float midpoint, in units of ulp(double)
+    lo = np.nextafter(f, np.float32(-np.inf)).astype(np.float64)
+    hi = np.nextafter(f, np.float32(np.inf)).astype(np.float64)
+    mid_lo = (f.astype(np.float64) + lo) / 2
+    mid_hi = (f.astype(np.float64) + hi) / 2
+    gap = np.minimum(np.abs(v - mid_lo), np.abs(v - mid_hi)) / np.spacing(v)
+    assert gap.min() > 4.0
+    # the oracle on a dense sample, including the extremes
+    idx = np.unique(np.concatenate([np.arange(1, 5000), np.arange(2**23 - 5000, 2**23),
+                                    np.random.default_rng(1).integers(1, 2**23, 20000)]))
+    got = np.array([orc_lib.orc_neglog(float(k) * 2.0 ** -23) for k in idx], dtype=np.float32)
+    assert np.array_equal(got, f[idx - 1])
+    assert math.isinf(orc_lib.orc_neglog(0.0))
+
+
+# ---- float elementary functions (er_fmath.h) ----------------------------------------------
+
+def _ulp_err(got, want):
+    want32 = want.astype(np.float32)
+    ulp = np.spacing(np.abs(want32)).astype(np.float64)
+    return np.abs(got.astype(np.float64) - want) / np.maximum(ulp, 1e-45)
+
+
+def test_fmath_accuracy():
+    x = np.linspace(-103.0, 88.0, 400001, dtype=np.float32)
+    assert _ulp_err(O.fmath(0, x), np.exp(x.astype(np.float64))).max() < 2.0
+    x = np.concatenate([np.geomspace(1e-38, 1e38, 400001), np.linspace(0.5, 2.0, 100001)]).astype(np.float32)
+    assert _ulp_err(O.fmath(1, x), np.log(x.astype(np.float64))).max() < 2.5
+    x = np.linspace(0.0, 2 * np.pi, 400001, dtype=np.float32)
+    s = O.fmath(3, x)
+    c = O.fmath(4, x)
+    assert np.abs(s - np.sin(x.astype(np.float64))).max() < 2e-7
+    assert np.abs(c - np.cos(x.astype(np.float64))).max() < 2e-7
+    # pow on the shader's domain: cosines in [0, 1], exponents up to 1e6
+    rng = np.random.default_rng(2)
+    b = rng.uniform(0.0, 1.0, 200000).astype(np.float32)
+    e = np.concatenate([rng.uniform(0, 8, 100000), 10 ** rng.uniform(0, 6, 100000)]).astype(np.float32)
+    want = np.power(b.astype(np.float64), e.astype(np.float64))
+    got = O.fmath(2, b, e).astype(np.float64)
+    keep = want > 1e-30
+    rel = np.abs(got[keep] - want[keep]) / want[keep]
+    assert rel.max() < 2.5e-7 * np.maximum(np.abs(e[keep] * np.log(b[keep])), 1.0).max() + 1e-6
+    x = np.linspace(-1.0, 1.0, 200001, dtype=np.float32)
+    assert np.abs(O.fmath(6, x) - np.arccos(x.astype(np.float64))).max() < 5e-7
+    yy, xx = np.meshgrid(np.linspace(-1, 1, 401), np.linspace(-1, 1, 401))
+    yy = yy.astype(np.float32).ravel()
+    xx = xx.astype(np.float32).ravel()
+    assert np.abs(O.fmath(5, yy, xx) - np.arctan2(yy.astype(np.float64), xx.astype(np.float64))).max() < 5e-7
+
+
+# ---- transfer functions (plf.h:68-91) ---------------------------------------------------
+
+def test_plf_closed_form_and_last_node_zero(orc_lib):
+    p = capi.Plf().init()
+    p.add(0.0, 0.0).add(400.0, 0.0).add(1500.0, 0.6).add(4000.0, 1.0)
+    ev = lambda x: orc_lib.orc_plf_eval(C.byref(p), x)
+    assert ev(-5.0) == 0.0                 # below the first node -> first value
+    assert ev(5000.0) == 1.0               # above the last node -> last value
+    assert ev(4000.0) == 0.0               # exactly the last node -> 0 (Q5)
+    assert ev(1500.0) == np.float32(0.6)   # a node is the start of its segment
+    assert abs(ev(950.0) - 0.3) < 1e-7     # midpoint of a segment
+    for x in np.linspace(400.0, 1499.0, 50):
+        assert abs(ev(float(x)) - 0.6 * (x - 400.0) / 1100.0) < 1e-6
+    e = capi.Plf().init()
+    assert orc_lib.orc_plf_eval(C.byref(e), 10.0) == 0.0   # empty function
+    # the node capacity: 127 nodes (plf.h:53)
+    big = capi.Plf().init()
+    for k in range(200):
+        big.add(float(k), float(k))
+    assert big.count == 127
+
+
+def test_plf_brute_force_random(orc_lib):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        n = int(rng.integers(2, 12))
+        pos = np.sort(rng.uniform(0, 4000, n)).astype(np.float32)
+        val = rng.uniform(0, 1, n).astype(np.float32)
+        p = capi.Plf().init()
+        for a, b in zip(pos, val):
+            p.add(float(a), float(b))
+        xs = rng.uniform(-100, 4100, 200).astype(np.float32)
+        for x in xs:
+            got = orc_lib.orc_plf_eval(C.byref(p), float(x))
+            if x < pos[0]:
+                want = val[0]
+            elif x > pos[-1]:
+                want = val[-1]
+            elif x == pos[-1]:
+                want = 0.0
+            else:
+                k = np.searchsorted(pos, x, side="right") - 1
+                want = val[k] + (x - pos[k]) / (pos[k + 1] - pos[k]) * (val[k + 1] - val[k])
+            assert abs(got - want) < 2e-6
+
+
+# ---- trilinear sampling (buffer3d.h:210-242, volume.h:131-138) ----------------------------
+
+def _tiny_scene(vox):
+    sc = scenes.Scene(vox)
+    sc.tracer.opacity.add(0.0, 0.5).add(70000.0, 0.5)
+    scenes.set_camera(sc.tracer, (4, 4), (0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    return sc
+
+
+def test_trilinear_lattice_points_and_layout():
+    """At lattice points the truncating trilinear returns the voxel itself; a distinct value
+    per voxel pins the x-fastest, z-slowest layout and the corner-aligned lattice."""
+    rz, ry, rx = 2, 4, 8          # powers of two: the lattice coordinates of these points are exact
+    vox = (np.arange(rx * ry * rz, dtype=np.uint16) * 7 + 3).reshape(rz, ry, rx)
+    orc = O.Oracle(_tiny_scene(vox))
+    ext = np.array([rx, ry, rz], dtype=np.float64)
+    size = ext / ext.max()
+    for z in range(rz):
+        for y in range(ry):
+            for x in range(rx):
+                p = (np.array([x, y, z]) / ext - 0.5) * size
+                assert orc.intensity(p) == vox[z, y, x], (x, y, z)
+    # off the exact lattice the chain truncates: a point rounding to just below a lattice
+    # coordinate loses one unit (the reference's uint16 lerps, Q4)
+    assert orc.intensity(np.array([1.0 / 8 - 0.5 - 1e-7, -0.25, -0.125])) in (vox[0, 0, 1] - 1, vox[0, 0, 1])
+
+
+def test_trilinear_bounds_and_clamp():
+    rng = np.random.default_rng(4)
+    vox = rng.integers(0, 5000, size=(6, 6, 6)).astype(np.uint16)
+    orc = O.Oracle(_tiny_scene(vox))
+    for _ in range(2000):
+        L = rng.uniform(-1.5, 7.5, 3)          # lattice coordinates, also outside the volume
+        p = (L / 6.0 - 0.5) * 1.0
+        got = orc.intensity(p)
+        assert got == int(got)                  # every lerp truncates to uint16 (Q4)
+        v = np.floor(L).astype(int)
+        xs = np.clip([v[0], v[0] + 1], 0, 5)
+        ys = np.clip([v[1], v[1] + 1], 0, 5)
+        zs = np.clip([v[2], v[2] + 1], 0, 5)
+        corners = vox[np.ix_(zs, ys, xs)]
+        d = L - v
+        w = [[(1 - d[2]) if k == 0 else d[2] for k in range(2)], [(1 - d[1]) if k == 0 else d[1] for k in range(2)],
+             [(1 - d[0]) if k == 0 else d[0] for k in range(2)]]
+        exact = sum(corners[a, b, c] * w[0][a] * w[1][b] * w[2][c] for a in range(2) for b in range(2) for c in range(2))
+        assert corners.min() - 3 <= got <= corners.max()
+        assert exact - 3.0 - 1e-3 <= got <= exact + 1e-3   # three truncation levels lose < 1 each
+
+
+# ---- free flight (raymarching.h:30-113) ---------------------------------------------------
+
+def _slab(intensity, opacity, ds, res=16):
+    vox = np.full((res, res, res), intensity, dtype=np.uint16)
+    sc = scenes.Scene(vox)
+    sc.tracer.opacity.add(0.0, opacity).add(60000.0, opacity)
+    sc.tracer.settings.density_scale = ds
+    scenes.set_camera(sc.tracer, (4, 4), (0.0, 0.0, 3.0), (0.0, 0.0, 0.0))
+    return sc
+
+
+def test_march_homogeneous_step_count_and_T_offset():
+    """Constant sigma: the march scatters after n = ceil(S / (sigma dt)) samples, reports P at
+    the n-th sample and T one step further (Q13)."""
+    ds, op = 10.0, 0.25
+    orc = O.Oracle(_slab(1000, op, ds))
+    dt = np.float32(0.1) * np.float32(1.0 / 16)
+    sig = np.float32(ds) * np.float32(op)
+    for u1 in (0.9, 0.5, 0.1, 0.01):
+        hit, T, P, steps = orc.march_ray((0.0, 0.0, -2.0), (0.0, 0.0, 1.0), 0.0, 1000.0, 1, u1, 0.5)
+        S = np.float32(-np.log(np.float32(u1))) / np.float32(ds)
+        acc, n = np.float32(0), 0
+        while acc < S:
+            acc = np.float32(acc + np.float32(sig * dt))
+            n += 1
+        assert hit == 1 and steps == n
+        t_first = np.float32(np.float32(1.5) + np.float32(0.5) * dt)
+        assert abs(P[2] - (-2.0 + t_first + (n - 1) * dt)) < 1e-5
+        assert abs(T - (t_first + n * dt)) < 1e-5
+
+
+def test_march_transmittance_density_scale_squared():
+    """P(no scatter through a slab of length L) = exp(-DS^2 * opacity * L) (Q3: the threshold is
+    divided by DS and the accumulation multiplied by DS)."""
+    ds, op = 4.0, 0.2
+    orc = O.Oracle(_slab(1000, op, ds))
+    us = (np.arange(1, 2001) - 0.5) / 2000.0
+    passed = 0
+    for u1 in us:
+        hit, T, P, steps = orc.march_ray((0.1, 0.05, -2.0), (0.0, 0.0, 1.0), 0.0, 1000.0, 1, float(u1), 0.3)
+        passed += hit == 0
+    L = 1.0
+    assert abs(passed / len(us) - math.exp(-ds * ds * op * L)) < 0.01
+
+
+def test_march_misses_and_shadow_tmax():
+    orc = O.Oracle(_slab(1000, 0.5, 10.0))
+    hit, *_ = orc.march_ray((2.0, 2.0, -2.0), (0.0, 0.0, 1.0), 0.0, 1000.0, 1, 0.5, 0.5)
+    assert hit == -1                                   # misses the box
+    hit, *_ = orc.march_ray((0.0, 0.0, -2.0), (0.0, 0.0, 1.0), 0.0, 1.0, 0, 0.5, 0.5)
+    assert hit == -1                                   # box starts after MaxT
+    empty = O.Oracle(_slab(1000, 0.0, 10.0))
+    hit, T, P, steps = empty.march_ray((0.0, 0.0, -2.0), (0.0, 0.0, 1.0), 0.0, 1000.0, 1, 0.5, 0.5)
+    assert hit == 0 and steps == 160                    # 1.0 / (0.1 / 16) samples, no event
+
+
+# ---- camera (camera.h:61-96) ----------------------------------------------------------------
+
+def test_camera_frame():
+    sc = scenes.config_small(scale=0.25)
+    scenes.set_camera(sc.tracer, (64, 48), (1.0, 2.0, 3.0), (0.0, 0.5, 0.0), fov=40.0)
+    cam = O.Oracle(sc).camera()
+    N, U, V = cam[0:3], cam[3:6], cam[6:9]
+    d = -np.array([1.0, 1.5, 3.0])
+    assert np.allclose(N, d / np.linalg.norm(d), atol=1e-6)
+    assert abs(np.dot(N, U)) < 1e-6 and abs(np.dot(N, V)) < 1e-6 and abs(np.dot(U, V)) < 1e-6
+    assert np.allclose(U, np.cross(N, [0, 1, 0]) / np.linalg.norm(np.cross(N, [0, 1, 0])), atol=1e-6)
+    scale = math.tan(math.radians(20.0))
+    aspect = 48 / 64
+    assert abs(cam[9] + scale / aspect) < 1e-6 and abs(cam[10] + scale) < 1e-6
+    assert abs(cam[11] - 2 * scale / aspect / 64) < 1e-7 and abs(cam[12] - 2 * scale / 48) < 1e-7
+
+
+# ---- post-processing (filterframeestimate.cuh, estimate.cuh, tonemap.cuh) ---------------------
+
+def test_post_filter_cma_tonemap_closed_forms():
+    sc = scenes.config_small(scale=0.125)
+    orc = O.Oracle(sc)
+    W, H = sc.film
+    running = np.zeros((H, W, 4), dtype=np.float32)
+    vals = [0.7, 0.2, 0.5, 0.9]
+    for k, v in enumerate(vals):
+        frame = np.zeros((H, W, 4), dtype=np.float32)
+        frame[..., :3] = v
+        frame[..., 3] = 1.0
+        disp, filt = orc.post(frame, running, k)
+        # constant image: the renormalised Gaussian returns the constant; alpha / total weight
+        assert np.allclose(filt[..., :3], v, atol=1e-6)
+        tw_inner = 1 + 4 * math.exp(-0.5) + 4 * math.exp(-1.0)
+        assert abs(filt[H // 2, W // 2, 3] - 1.0 / tw_inner) < 1e-6
+        assert abs(filt[0, 0, 3] - 1.0 / (1 + 2 * math.exp(-0.5) + math.exp(-1.0))) < 1e-6
+    # running estimate = mean of passes 2..n (pass 1 discarded, Q1)
+    assert np.allclose(running[..., 0], np.mean(vals[1:]), atol=1e-6)
+    # tonemap: XYZ -> linear sRGB, 1 - exp(-c / exposure), truncation to u8, A = a * 255
+    m = np.array([[3.240479, -1.537150, -0.498535], [-0.969256, 1.875991, 0.041556], [0.055648, -0.204043, 1.057311]])
+    rgb = m @ np.full(3, np.mean(vals[1:]))
+    want = np.floor(255 * np.clip(1 - np.exp(-rgb / sc.tracer.camera.exposure), 0, 1))
+    assert np.all(np.abs(disp[H // 2, W // 2, :3].astype(float) - want) <= 1)
+
+
+def test_bilateral_constant_and_range_limits():
+    sc = scenes.config_ct(scale=0.0625)
+    orc = O.Oracle(sc)
+    W, H = sc.film
+    img = np.full((H, W, 4), 120, dtype=np.uint8)
+    out = orc.bilateral(img)
+    # normalisation by the summed weights, then truncation to u8: at most one unit per pass
+    assert np.all(out[..., :3] >= 118) and np.all(out[..., :3] <= 120) and np.all(out[..., 3] == 255)
+    # a step edge far above the range sigma survives (edge-preserving)
+    img[:, : W // 2, :3] = 10
+    img[:, W // 2:, :3] = 250
+    out = orc.bilateral(img)
+    assert out[H // 2, W // 2 - 2, 0] <= 12 and out[H // 2, W // 2 + 2, 0] >= 246
+
+
+# ---- estimator special cases ------------------------------------------------------------------
+
+def test_no_lights_gives_unscaled_emission():
+    sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+    sc.tracer.light_count = 0
+    sc.tracer.emission.init()
+    sc.tracer.emission.add(0.0, (0.25, 0.5, 0.75)).add(70000.0, (0.25, 0.5, 0.75))
+    orc = O.Oracle(sc)
+    s1, s2 = O.seeds_for(*sc.film, 3)
+    f, cnt = orc.trace(s1, s2)
+    hit = f[..., 3] == 1.0
+    assert hit.any()
+    assert np.allclose(f[hit][:, :3], [0.25, 0.5, 0.75])     # Q10: no light-count scaling without lights
+    assert np.all(f[~hit][:, :3] == 0.0)
+
+
+def test_transparent_volume_is_invisible():
+    sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+    sc.tracer.opacity.init()
+    sc.tracer.opacity.add(0.0, 0.0).add(70000.0, 0.0)
+    sc.lights[0].visible = 0
+    orc = O.Oracle(sc)
+    s1, s2 = O.seeds_for(*sc.film, 3)
+    f, cnt = orc.trace(s1, s2)
+    assert np.all(f == 0.0) and cnt[6] == 0
+
+
+def test_draw_schedule():
+    """11 draws per path, +2 when the camera ray meets the volume box, +2 per shadow march."""
+    sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
+    sc.tracer.opacity.init()
+    sc.tracer.opacity.add(0.0, 0.0).add(70000.0, 0.0)     # no scatter -> no shadow rays
+    orc = O.Oracle(sc)
+    W, H = sc.film
+    px = np.array([0, W // 2], dtype=np.int32)
+    py = np.array([0, H // 2], dtype=np.int32)
+    s1, s2 = O.seeds_for(2, 1, 9)
+    out, draws, cnt = orc.trace_pixels(px, py, s1, s2)
+    assert list(draws) == [11, 13]                      # corner misses the box, centre hits it
+
+
+def test_fp32_mean_matches_fp64_expectation():
+    sc = scenes.config_small(scale=0.125, estimator=capi.ESTIMATOR_FULL)
+    orc = O.Oracle(sc, nthreads=4)
+    W, H = sc.film
+    px = np.repeat(np.arange(W), 1)[: W].astype(np.int32)
+    py = np.full(W, H // 2, dtype=np.int32)
+    e64 = orc.expect_f64(px, py, 4000, seed=1)
+    acc = np.zeros((W, 4))
+    s1, s2 = O.seeds_for(W, 1, 2)
+    n = 4000
+    for _ in range(n):
+        out, _, _ = orc.trace_pixels(px, py, s1, s2)
+        acc += out
+    m32 = acc / n
+    sel = e64[:, 3] > 0.5
+    err = np.abs(m32[sel, :3] - e64[sel, :3]).mean()
+    assert err < 0.05 * np.abs(e64[sel, :3]).mean()
