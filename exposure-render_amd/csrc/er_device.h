@@ -7,7 +7,9 @@
  *             (Source/buffer3d.h:210-242). A sample is one 128-bit load instead of eight
  *             2-byte gathers; the clamp-to-edge of the reference is folded into the records.
  *             (Rx+1)(Ry+1)(Rz+1) x 16 B (8x the voxel bytes; 1.9 GB for 512x512x460).
- *   mc_*    : 8^3-cell macrocells. For each, the min/max voxel over the voxels its samples can
+ *   mc_*    : macrocells of 8^3 cells (16^3 or 32^3 for volumes with more than 2^17 macrocells,
+ *             so that one occupancy bit per macrocell fits in 16 KB of LDS). For each, the
+ *             min/max voxel over the voxels its samples can
  *             read, and (per tracer) the max of DensityScale * Opacity over the intensities a
  *             truncating trilinear sample can produce from that range ([min-3, max]). A sample
  *             whose macrocell maximum is 0 contributes exactly 0 to the optical depth, so the
@@ -28,7 +30,6 @@
 #include "../../include/er_capi.h"
 #include "er_fmath.h"
 
-#define ER_MC_SHIFT 3              /* macrocell edge = 8 cells */
 #define ER_NEGLOG_BITS 23          /* -log(u) table over u = m * 2^-23 */
 
 namespace erd {
@@ -86,7 +87,10 @@ struct DevParams {
     const float* sigma_lut;          /* DensityScale * Opacity(i), i in [0, lut_max] */
     int   lut_max;
     const float* mc_sigma;           /* per macrocell max sigma */
+    const uint32_t* mc_bits;         /* bit m set <=> mc_sigma[m] > 0 */
     int   mc_dim[3];
+    int   mc_shift;                  /* macrocell edge = 2^mc_shift cells */
+    int   mc_words;                  /* 32-bit words of mc_bits */
     const float* neglog;             /* -log(m 2^-23), m in [0, 2^23) */
     /* lights, textures, clipping objects; indexed by ID (< ER_LIST_CAPACITY) */
     int   light_count;
@@ -226,21 +230,6 @@ __device__ __forceinline__ float trilinear_cell(const DevParams& p, const Lattic
 }
 
 __device__ __forceinline__ float intensity(const DevParams& p, f3 P) { return trilinear_cell(p, to_lattice(p, P)); }
-
-/* macrocell of a sample: clamp(floor(L) >> 3) per axis */
-__device__ __forceinline__ int macrocell(const DevParams& p, const Lattice& l)
-{
-    const int mx = min(max(((int)floorf(l.Lx)) >> ER_MC_SHIFT, 0), p.mc_dim[0] - 1);
-    const int my = min(max(((int)floorf(l.Ly)) >> ER_MC_SHIFT, 0), p.mc_dim[1] - 1);
-    const int mz = min(max(((int)floorf(l.Lz)) >> ER_MC_SHIFT, 0), p.mc_dim[2] - 1);
-    return (mz * p.mc_dim[1] + my) * p.mc_dim[0] + mx;
-}
-
-/* sigma of one sample: DensityScale * Opacity1D(I) through the exact LUT */
-__device__ __forceinline__ float sigma_lut(const DevParams& p, float I)
-{
-    return p.sigma_lut[min((int)I, p.lut_max)];
-}
 
 /* GradientCD / FD / Filtered (volumes.h:31-74); Normalize (Q20: zero gradient -> NaN) */
 __device__ __forceinline__ f3 grad_cd(const DevParams& p, f3 P)

@@ -26,9 +26,12 @@ namespace erd {
 #define POST_TILE 16
 
 __global__ __launch_bounds__(256) void k_post(const DevParams* __restrict__ pp, const float4* __restrict__ frame,
-                                             float4* __restrict__ running, uchar4* __restrict__ display, int no_iterations)
+                                             float4* __restrict__ running, uchar4* __restrict__ display, int no_iterations,
+                                             unsigned int* __restrict__ qcount)
 {
     const DevParams& p = *pp;
+    /* the scatter queue of this pass has been consumed: empty it for the next pass */
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *qcount = 0u;
     const int W = p.film_w, H = p.film_h;
     __shared__ float4 tile[POST_TILE + 2][POST_TILE + 2];
     const int bx = blockIdx.x * POST_TILE, by = blockIdx.y * POST_TILE;
@@ -151,17 +154,18 @@ __global__ __launch_bounds__(256) void k_build_cells(const uint16_t* __restrict_
     }
 }
 
-/* min/max voxel over [m*8, m*8 + 8] (clamped) on each axis: every voxel a sample whose
- * floor(L) >> 3 == m (after clamping) can read */
+/* min/max voxel over [m*E, m*E + E] (E = 2^shift, clamped) on each axis: every voxel a sample
+ * whose floor(L) >> shift == m (after clamping) can read */
 __global__ __launch_bounds__(256) void k_macro_minmax(const uint16_t* __restrict__ vox, uint32_t* __restrict__ mm,
-                                                     int rx, int ry, int rz, int mx, int my, int mz)
+                                                     int rx, int ry, int rz, int mx, int my, int mz, int shift)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= mx * my * mz) return;
+    const int E = 1 << shift;
     const int bx = m % mx, by = (m / mx) % my, bz = m / (mx * my);
-    const int x0 = bx * 8, x1 = min(bx * 8 + 8, rx - 1);
-    const int y0 = by * 8, y1 = min(by * 8 + 8, ry - 1);
-    const int z0 = bz * 8, z1 = min(bz * 8 + 8, rz - 1);
+    const int x0 = bx * E, x1 = min(bx * E + E, rx - 1);
+    const int y0 = by * E, y1 = min(by * E + E, ry - 1);
+    const int z0 = bz * E, z1 = min(bz * E + E, rz - 1);
     uint32_t lo = 0xffff, hi = 0;
     for (int z = z0; z <= z1; ++z)
         for (int y = y0; y <= y1; ++y)
@@ -187,6 +191,20 @@ __global__ __launch_bounds__(256) void k_macro_sigma(const uint32_t* __restrict_
     out[m] = s;
 }
 
+/* occupancy bits: bit m of the mask <=> macrocell m can hold a sample with sigma > 0 */
+__global__ __launch_bounds__(256) void k_macro_bits(const float* __restrict__ mc_sigma, uint32_t* __restrict__ bits,
+                                                   int n, int words)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint32_t v = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int m = w * 32 + b;
+        if (m < n && mc_sigma[m] > 0.0f) v |= 1u << b;
+    }
+    bits[w] = v;
+}
+
 /* maximum GradientMagnitude over voxel centres (design: gradientmagnitude.cuh:28-70) */
 __global__ __launch_bounds__(256) void k_gm_max(const DevParams* __restrict__ pp, unsigned int* __restrict__ out)
 {
@@ -210,10 +228,10 @@ __global__ __launch_bounds__(256) void k_gm_max(const DevParams* __restrict__ pp
 } /* namespace erd */
 
 extern "C" hipError_t erd_launch_post(const erd::DevParams* p, const float4* frame, float4* running, uchar4* display,
-                                      int no_iterations, int W, int H, hipStream_t st)
+                                      int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st)
 {
     dim3 grid((W + POST_TILE - 1) / POST_TILE, (H + POST_TILE - 1) / POST_TILE), block(256);
-    hipLaunchKernelGGL(erd::k_post, grid, block, 0, st, p, frame, running, display, no_iterations);
+    hipLaunchKernelGGL(erd::k_post, grid, block, 0, st, p, frame, running, display, no_iterations, qcount);
     return hipGetLastError();
 }
 
@@ -235,10 +253,16 @@ extern "C" hipError_t erd_launch_build_cells(const uint16_t* vox, uint4* cells, 
 }
 
 extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm, int rx, int ry, int rz, int mx, int my,
-                                              int mz, hipStream_t st)
+                                              int mz, int shift, hipStream_t st)
 {
     const int n = mx * my * mz;
-    hipLaunchKernelGGL(erd::k_macro_minmax, dim3((n + 255) / 256), dim3(256), 0, st, vox, mm, rx, ry, rz, mx, my, mz);
+    hipLaunchKernelGGL(erd::k_macro_minmax, dim3((n + 255) / 256), dim3(256), 0, st, vox, mm, rx, ry, rz, mx, my, mz, shift);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, uint32_t* bits, int n, int words, hipStream_t st)
+{
+    hipLaunchKernelGGL(erd::k_macro_bits, dim3((words + 255) / 256), dim3(256), 0, st, mc_sigma, bits, n, words);
     return hipGetLastError();
 }
 

@@ -31,16 +31,19 @@
 
 using erd::DevParams;
 
-extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2,
-                                       unsigned long long* counters, int W, int H, int full, int count, hipStream_t st);
+extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
+                                       unsigned int* qcount, unsigned long long* counters, int W, int H, int full,
+                                       int count, int lut_lds, size_t lds_bytes, int shade_blocks, hipStream_t st);
+extern "C" size_t erd_scatter_rec_size(void);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
-                                      int no_iterations, int W, int H, hipStream_t st);
+                                      int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
+extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, uint32_t* bits, int n, int words, hipStream_t st);
 extern "C" hipError_t erd_launch_bilateral(const void* bp, const uchar4* in, uchar4* tmp, uchar4* out, int W, int H,
                                            hipStream_t st);
 extern "C" size_t erd_bilateral_params_size(void);
 extern "C" hipError_t erd_launch_build_cells(const uint16_t* vox, uint4* cells, int rx, int ry, int rz, hipStream_t st);
 extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm, int rx, int ry, int rz, int mx, int my,
-                                              int mz, hipStream_t st);
+                                              int mz, int shift, hipStream_t st);
 extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lut, int lut_max, float* out, int n,
                                              hipStream_t st);
 extern "C" hipError_t erd_launch_gm_max(const DevParams* p, unsigned int* out, hipStream_t st);
@@ -156,6 +159,7 @@ struct Volume {
     DevBuf<uint4> cells;
     DevBuf<uint32_t> mm;
     int mc[3] = {0, 0, 0};
+    int mc_shift = 3;
     uint16_t vmax = 0;
 };
 
@@ -166,6 +170,10 @@ struct Tracer {
     DevBuf<uchar4> display, disp_tmp, disp_filt;
     DevBuf<uint32_t> seeds1, seeds2;
     DevBuf<float> lut, mc_sigma;
+    DevBuf<uint32_t> mc_bits, qcount;
+    DevBuf<char> queue;
+    size_t lds_bytes = 0;
+    int lut_lds = 0;
     DevBuf<DevParams> params;
     DevBuf<unsigned long long> counters;
     DevBuf<unsigned int> gm_tmp;
@@ -180,6 +188,7 @@ struct Tracer {
     {
         frame.release(); running.release(); display.release(); disp_tmp.release(); disp_filt.release();
         seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); params.release();
+        mc_bits.release(); qcount.release(); queue.release();
         counters.release(); gm_tmp.release(); bilateral.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -261,7 +270,7 @@ int tracer_resize(Tracer& t, int W, int H, int id)
     int st;
     if ((st = t.frame.alloc(n)) || (st = t.running.alloc(n)) || (st = t.display.alloc(n)) ||
         (st = t.disp_tmp.alloc(n)) || (st = t.disp_filt.alloc(n)) || (st = t.seeds1.alloc(n)) ||
-        (st = t.seeds2.alloc(n)))
+        (st = t.seeds2.alloc(n)) || (st = t.queue.alloc(n * erd_scatter_rec_size())) || (st = t.qcount.alloc(1)))
         return st;
     t.W = W; t.H = H;
     if (n) {
@@ -269,6 +278,7 @@ int tracer_resize(Tracer& t, int W, int H, int id)
         HIP_TRY(hipMemset(t.running.ptr, 0, n * sizeof(float4)));
         HIP_TRY(hipMemset(t.display.ptr, 0, n * sizeof(uchar4)));
         HIP_TRY(hipMemset(t.disp_filt.ptr, 0, n * sizeof(uchar4)));
+        HIP_TRY(hipMemset(t.qcount.ptr, 0, sizeof(uint32_t)));
         return tracer_seed(t, 0x45787052ull + (uint64_t)id);
     }
     return ER_OK;
@@ -330,9 +340,16 @@ int upload_volume(Volume& v)
     const size_t nc = (size_t)(rx + 1) * (ry + 1) * (rz + 1);
     if ((st = v.cells.alloc(nc))) return st;
     HIP_TRY(erd_launch_build_cells(v.vox.ptr, v.cells.ptr, rx, ry, rz, 0));
-    v.mc[0] = (rx + 7) / 8; v.mc[1] = (ry + 7) / 8; v.mc[2] = (rz + 7) / 8;
+    /* macrocell edge: 8 cells, coarser when one occupancy bit per macrocell would exceed 16 KB */
+    v.mc_shift = 3;
+    for (;;) {
+        const int E = 1 << v.mc_shift;
+        v.mc[0] = (rx + E - 1) / E; v.mc[1] = (ry + E - 1) / E; v.mc[2] = (rz + E - 1) / E;
+        if ((size_t)v.mc[0] * v.mc[1] * v.mc[2] <= 131072u || v.mc_shift >= 8) break;
+        v.mc_shift++;
+    }
     if ((st = v.mm.alloc((size_t)v.mc[0] * v.mc[1] * v.mc[2]))) return st;
-    HIP_TRY(erd_launch_macro_minmax(v.vox.ptr, v.mm.ptr, rx, ry, rz, v.mc[0], v.mc[1], v.mc[2], 0));
+    HIP_TRY(erd_launch_macro_minmax(v.vox.ptr, v.mm.ptr, rx, ry, rz, v.mc[0], v.mc[1], v.mc[2], v.mc_shift, 0));
     HIP_TRY(hipDeviceSynchronize());
     return ER_OK;
 }
@@ -464,6 +481,13 @@ int build_params(Tracer& t)
     if ((st = t.mc_sigma.alloc(nmc))) return st;
     HIP_TRY(erd_launch_macro_sigma(vol->mm.ptr, t.lut.ptr, P.lut_max, t.mc_sigma.ptr, nmc, t.stream));
     P.mc_sigma = t.mc_sigma.ptr;
+    P.mc_shift = vol->mc_shift;
+    P.mc_words = (nmc + 31) / 32;
+    if ((st = t.mc_bits.alloc(P.mc_words))) return st;
+    HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, t.mc_bits.ptr, nmc, P.mc_words, t.stream));
+    P.mc_bits = t.mc_bits.ptr;
+    t.lut_lds = (size_t)lut_n * sizeof(float) <= 32768u;
+    t.lds_bytes = (size_t)P.mc_words * 4 + (t.lut_lds ? (size_t)lut_n * sizeof(float) : 0);
 
     if ((st = t.params.alloc(1))) return st;
     P.gm_inv = 0.0f;
@@ -510,12 +534,13 @@ int enqueue_pass(Tracer& t, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
 {
     if (t.W == 0 || t.H == 0) return ER_OK;
     const int full = t.desc.settings.estimator == ER_ESTIMATOR_FULL;
+    const int shade_blocks = std::max(1, std::min((t.W * t.H + 255) / 256, 2048));
     if (e0) HIP_TRY(hipEventRecord(e0, t.stream));
-    HIP_TRY(erd_launch_trace(t.params.ptr, t.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, t.counters.ptr, t.W, t.H, full,
-                             t.counting, t.stream));
+    HIP_TRY(erd_launch_trace(t.params.ptr, t.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, t.queue.ptr, t.qcount.ptr,
+                             t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds, t.lds_bytes, shade_blocks, t.stream));
     if (e1) HIP_TRY(hipEventRecord(e1, t.stream));
-    HIP_TRY(erd_launch_post(t.params.ptr, t.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations, t.W, t.H,
-                            t.stream));
+    HIP_TRY(erd_launch_post(t.params.ptr, t.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations,
+                            t.qcount.ptr, t.W, t.H, t.stream));
     if (use_bilateral(t))
         HIP_TRY(erd_launch_bilateral(t.bilateral.ptr, t.display.ptr, t.disp_tmp.ptr, t.disp_filt.ptr, t.W, t.H, t.stream));
     if (e2) HIP_TRY(hipEventRecord(e2, t.stream));
@@ -713,8 +738,8 @@ ER_API int er_bind_tracer(er_tracer_desc* d, int bind)
             t = new Tracer();
             hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
             if (e != hipSuccess) { delete t; t = nullptr; return set_error(ER_ERROR, "%s (hipStreamCreate)", hipGetErrorString(e)); }
-            int st = t->counters.alloc(8);
-            if (!st && hipMemset(t->counters.ptr, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
+            int st = t->counters.alloc(12);
+            if (!st && hipMemset(t->counters.ptr, 0, 12 * sizeof(unsigned long long)) != hipSuccess)
                 st = set_error(ER_ERROR, "hipMemset failed");
             if (st) { delete t; t = nullptr; return st; }
         }
@@ -945,10 +970,11 @@ ER_API int er_get_counters(int tracer_id, er_counters* c)
     int st;
     Tracer* t = get_tracer(tracer_id, st);
     if (!t) return st;
-    unsigned long long v[8];
+    unsigned long long v[12];
     if ((st = copy_out(v, t->counters.ptr, sizeof(v), sizeof(v), t->stream))) return st;
     c->paths = v[0]; c->primary_samples = v[1]; c->shadow_samples = v[2]; c->shading_samples = v[3];
     c->primary_fetches = v[4]; c->shadow_fetches = v[5]; c->scatter_events = v[6]; c->shadow_rays = v[7];
+    c->march_iterations = v[8]; c->skipped_samples = v[9]; c->skips_undone = v[10]; c->reserved = 0;
     return ER_OK;
 }
 
@@ -958,7 +984,7 @@ ER_API int er_reset_counters(int tracer_id)
     int st;
     Tracer* t = get_tracer(tracer_id, st);
     if (!t) return st;
-    HIP_TRY(hipMemsetAsync(t->counters.ptr, 0, 8 * sizeof(unsigned long long), t->stream));
+    HIP_TRY(hipMemsetAsync(t->counters.ptr, 0, 12 * sizeof(unsigned long long), t->stream));
     HIP_TRY(hipStreamSynchronize(t->stream));
     return ER_OK;
 }

@@ -17,7 +17,7 @@
 namespace erd {
 
 struct Cnt {
-    uint32_t primary, shadow, shading, pfetch, sfetch, scatter, shadow_rays;
+    uint32_t primary, shadow, shading, pfetch, sfetch, scatter, shadow_rays, iters, skipped, undone;
 };
 
 struct Isect {
@@ -58,16 +58,77 @@ __device__ __forceinline__ bool clipped(const DevParams& p, f3 P)
 
 /* ---- free-flight sampling --------------------------------------------------------------- */
 
-/* sigma of a sample at Ps with lattice coordinates l; 0 without a fetch when its macrocell
- * holds no opacity or (full mode) the point is clipped */
+/* Per-block copies of the macrocell occupancy bits and (when it fits) the opacity LUT. */
+struct Tables {
+    const uint32_t* bits;   /* bit m set <=> macrocell m holds a sample with sigma > 0 */
+    const float* lut;       /* DensityScale * Opacity(i) */
+};
+
+__device__ __forceinline__ bool mc_occupied(const Tables& T, int m) { return (T.bits[m >> 5] >> (m & 31)) & 1u; }
+
+__device__ __forceinline__ int mc_axis(float L, int shift, int dim)
+{
+    return min(max(((int)floorf(L)) >> shift, 0), dim - 1);
+}
+
+/* sigma of one sample (Woodcock path) */
 template<int FULL, int COUNT>
-__device__ __forceinline__ float sample_sigma(const DevParams& p, f3 Ps, bool primary, Cnt& cnt)
+__device__ __forceinline__ float sample_sigma(const DevParams& p, const Tables& T, f3 Ps, bool primary, Cnt& cnt)
 {
     const Lattice l = to_lattice(p, Ps);
-    if (!(p.mc_sigma[macrocell(p, l)] > 0.0f)) return 0.0f;
+    const int m = (mc_axis(l.Lz, p.mc_shift, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(l.Ly, p.mc_shift, p.mc_dim[1])) * p.mc_dim[0] +
+                  mc_axis(l.Lx, p.mc_shift, p.mc_dim[0]);
+    if (!mc_occupied(T, m)) return 0.0f;
     if (FULL && p.clip_count > 0 && clipped(p, Ps)) return 0.0f;
     if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-    return sigma_lut(p, trilinear_cell(p, l));
+    return T.lut[min((int)trilinear_cell(p, l), p.lut_max)];
+}
+
+/* Advance t by repeated float additions of b while t < t_lim, without performing them one by
+ * one: returns the number of additions n and sets t to the value after the last one and t_prev
+ * to the value before it - bit-identical to `while (t < t_lim) { t_prev = t; t += b; ++n; }`.
+ * Within one binade [2^e, 2^(e+1)) of t every value is an integer multiple A of the spacing
+ * u = 2^(e-23), so t + b rounds to (A + D) u with the constant D = round(b / u) unless b / u
+ * has fractional part exactly 1/2 (then the rounding depends on the parity of A and the
+ * addition is done plainly). The additions inside a binade are counted with integer arithmetic;
+ * the one that crosses into the next binade is done as a float addition. Requires b > 0 and
+ * t_lim + b > t_lim (the march checks it), so D >= 1 wherever t < t_lim. */
+__device__ __forceinline__ uint32_t skip_additions(float& t, float b, float t_lim, float& t_prev)
+{
+    uint32_t n = 0;
+    while (t < t_lim) {
+        const uint32_t tb = __float_as_uint(t);
+        const int e = (int)((tb >> 23) & 0xffu);
+        if (e <= 24 || e >= 254 || (tb >> 31)) { t_prev = t; t += b; ++n; continue; }
+        const float u = __uint_as_float((uint32_t)(e - 23) << 23);
+        const float inv_u = __uint_as_float((uint32_t)(254 - (e - 23)) << 23);   /* 1 / u, exact */
+        const float q = b * inv_u;                                                 /* exact: power-of-2 scaling */
+        const float fq = floorf(q);
+        const float fr = q - fq;
+        if (fr == 0.5f || q >= 8388608.0f) { t_prev = t; t += b; ++n; continue; }
+        const uint32_t D = (uint32_t)fq + (fr > 0.5f ? 1u : 0u);
+        const uint32_t A = (tb & 0x007fffffu) | 0x00800000u;                      /* t / u */
+        const uint32_t room = 0x00ffffffu - A;                                     /* additions stay below 2^24 u */
+        uint32_t kbin = (uint32_t)((float)room * __builtin_amdgcn_rcpf((float)D));
+        while ((uint64_t)kbin * D > room) --kbin;
+        while ((uint64_t)(kbin + 1) * D <= room) ++kbin;
+        /* J = number of j >= 0 with (A + j D) u < t_lim, found from an estimate and exact checks */
+        uint32_t J = (uint32_t)fminf((t_lim - t) * __builtin_amdgcn_rcpf((float)D * u), 16777216.0f);
+        if (J > kbin + 1) J = kbin + 1;
+        while (J > 0 && (float)(A + (J - 1) * D) * u >= t_lim) --J;
+        while (J <= kbin && (float)(A + J * D) * u < t_lim) ++J;
+        if (J <= kbin) {
+            n += J;
+            t_prev = (float)(A + (J - 1) * D) * u;
+            t = (float)(A + J * D) * u;
+            break;
+        }
+        /* every addition inside the binade is taken; the next one crosses into the next binade */
+        n += kbin + 1;
+        t_prev = (float)(A + kbin * D) * u;
+        t = t_prev + b;
+    }
+    return n;
 }
 
 /* SampleVolume (raymarching.h:30-71) for primary rays, ScatterEventInVolume
@@ -75,7 +136,7 @@ __device__ __forceinline__ float sample_sigma(const DevParams& p, f3 Ps, bool pr
  * S = -log(u) / DensityScale; T is reported one step beyond P (Q13); the primary march ends
  * at t >= tmax, the shadow march at t > tmax (Q14). */
 template<int FULL, int COUNT>
-__device__ bool march(const DevParams& p, const Ray& r, bool primary, Rng& rng, float& Tout, f3& Pout, Cnt& cnt)
+__device__ bool march(const DevParams& p, const Tables& T, const Ray& r, bool primary, Rng& rng, float& Tout, f3& Pout, Cnt& cnt)
 {
     float nearT, farT;
     if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return false;
@@ -92,7 +153,7 @@ __device__ bool march(const DevParams& p, const Ray& r, bool primary, Rng& rng, 
             if (primary ? (t >= MaxT) : (t > MaxT)) return false;
             const f3 Ps = add(r.O, scl(t, r.D));
             if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; }
-            const float sigma = sample_sigma<FULL, COUNT>(p, Ps, primary, cnt) * p.density_scale;
+            const float sigma = sample_sigma<FULL, COUNT>(p, T, Ps, primary, cnt) * p.density_scale;
             if (rng.u() * mu < sigma) { Tout = t; Pout = Ps; return true; }
         }
     }
@@ -100,16 +161,68 @@ __device__ bool march(const DevParams& p, const Ray& r, bool primary, Rng& rng, 
     const float S = p.neglog[rng.next_bits()] / p.density_scale;
     const float StepSize = (primary ? p.step_primary : p.step_shadow) * p.minstep;
     MinT += rng.u() * StepSize;
-    if (!(StepSize > 0.0f)) return false;   /* zero-size volume (Q2): the march would not advance */
+    /* a march that could not advance (zero-size volume, Q2, or a step below the spacing of
+     * floats at MaxT) would not terminate in the reference; it is reported as no event */
+    if (!(StepSize > 0.0f) || !(MaxT + StepSize > MaxT)) return false;
+    /* lattice-space direction, used only to estimate where the ray leaves a macrocell */
+    const float Ldx = (r.D.x * p.invsize[0]) * p.resf[0];
+    const float Ldy = (r.D.y * p.invsize[1]) * p.resf[1];
+    const float Ldz = (r.D.z * p.invsize[2]) * p.resf[2];
+    const float Lmax = fmaxf(fabsf(Ldx), fmaxf(fabsf(Ldy), fabsf(Ldz)));
+    /* reciprocals for the exit estimate only (hardware approximation: the skip is verified) */
+    const float iLdx = __builtin_amdgcn_rcpf(Ldx), iLdy = __builtin_amdgcn_rcpf(Ldy), iLdz = __builtin_amdgcn_rcpf(Ldz);
+    const float margin = 1.0e-3f * __builtin_amdgcn_rcpf(Lmax);
+    const int sh = p.mc_shift;
     float Sum = 0.0f;
     f3 Ps = mk(0.0f, 0.0f, 0.0f);
     while (Sum < S) {
         Ps = add(r.O, scl(MinT, r.D));
         if (primary ? (MinT >= MaxT) : (MinT > MaxT)) return false;
-        if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; }
-        const float SigmaT = sample_sigma<FULL, COUNT>(p, Ps, primary, cnt);
-        Sum += SigmaT * StepSize;
+        if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
+        const Lattice l = to_lattice(p, Ps);
+        const int mxi = mc_axis(l.Lx, sh, p.mc_dim[0]);
+        const int myi = mc_axis(l.Ly, sh, p.mc_dim[1]);
+        const int mzi = mc_axis(l.Lz, sh, p.mc_dim[2]);
+        const int mc = (mzi * p.mc_dim[1] + myi) * p.mc_dim[0] + mxi;
+        if (mc_occupied(T, mc)) {
+            float SigmaT = 0.0f;
+            if (!(FULL && p.clip_count > 0 && clipped(p, Ps))) {
+                if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
+                SigmaT = T.lut[min((int)trilinear_cell(p, l), p.lut_max)];
+            }
+            Sum += SigmaT * StepSize;
+            MinT += StepSize;
+            continue;
+        }
+        /* Empty macrocell: this sample adds exactly 0. Skip the following samples that stay in
+         * the same macrocell without computing them, keeping the exact sequence of MinT
+         * additions. Each operation of P = O + t D, (P - Min) * InvSize * Res is monotone in t,
+         * so if the first and the last skipped sample lie in this macrocell's lattice box,
+         * every sample between them does too; the last one is checked with the same
+         * expression as above, and the skip is undone if it left the box. */
         MinT += StepSize;
+        const float big = 3.0e38f;
+        const float lo_x = mxi == 0 ? -big : (float)(mxi << sh), hi_x = mxi == p.mc_dim[0] - 1 ? big : (float)((mxi + 1) << sh);
+        const float lo_y = myi == 0 ? -big : (float)(myi << sh), hi_y = myi == p.mc_dim[1] - 1 ? big : (float)((myi + 1) << sh);
+        const float lo_z = mzi == 0 ? -big : (float)(mzi << sh), hi_z = mzi == p.mc_dim[2] - 1 ? big : (float)((mzi + 1) << sh);
+        const float tx = Ldx > 0.0f ? (hi_x - l.Lx) * iLdx : (Ldx < 0.0f ? (lo_x - l.Lx) * iLdx : big);
+        const float ty = Ldy > 0.0f ? (hi_y - l.Ly) * iLdy : (Ldy < 0.0f ? (lo_y - l.Ly) * iLdy : big);
+        const float tz = Ldz > 0.0f ? (hi_z - l.Lz) * iLdz : (Ldz < 0.0f ? (lo_z - l.Lz) * iLdz : big);
+        const float dt_exit = fminf(tx, fminf(ty, tz)) - margin;
+        const float t_lim = fminf((MinT - StepSize) + dt_exit, MaxT);
+        const float t_first = MinT;
+        float t_prev = MinT;
+        uint32_t n = skip_additions(MinT, StepSize, t_lim, t_prev);
+        if (n > 0) {
+            const Lattice lp = to_lattice(p, add(r.O, scl(t_prev, r.D)));
+            if (mc_axis(lp.Lx, sh, p.mc_dim[0]) != mxi || mc_axis(lp.Ly, sh, p.mc_dim[1]) != myi ||
+                mc_axis(lp.Lz, sh, p.mc_dim[2]) != mzi) {
+                MinT = t_first;
+                n = 0;
+                if (COUNT) cnt.undone++;
+            }
+        }
+        if (COUNT) { if (primary) cnt.primary += n; else cnt.shadow += n; cnt.skipped += n; }
     }
     Tout = MinT;
     Pout = Ps;
@@ -423,7 +536,7 @@ __device__ bool intersects_any_light(const DevParams& p, const Ray& r)
 
 /* Visible (transport.h:46-56) -> Intersect (transport.h:30-44) */
 template<int FULL, int COUNT>
-__device__ bool visible(const DevParams& p, f3 P1, f3 P2, Rng& rng, Cnt& cnt)
+__device__ bool visible(const DevParams& p, const Tables& T, f3 P1, f3 P2, Rng& rng, Cnt& cnt)
 {
     if (!p.shadows) return true;
     const f3 W = nrm(sub(P2, P1));
@@ -434,8 +547,9 @@ __device__ bool visible(const DevParams& p, f3 P1, f3 P2, Rng& rng, Cnt& cnt)
     r.MaxT = fminf(len(sub(P2, P1)) - (2.0f * ER_RAY_EPS), p.max_shadow);
     if (COUNT) cnt.shadow_rays++;
     if (intersects_any_light(p, r)) return false;
-    float T; f3 P;
-    if (march<FULL, COUNT>(p, r, false, rng, T, P, cnt)) return false;
+    float Tt; f3 P;
+    const Tables& Tb = T;
+    if (march<FULL, COUNT>(p, Tb, r, false, rng, Tt, P, cnt)) return false;
     return true;
 }
 
@@ -502,7 +616,7 @@ struct Sample { float comp, dir0, dir1, uvw0, uvw1, uvw2, lightnum, film0, film1
 
 /* EstimateDirectLight (transport.h:58-113) */
 template<int FULL, int COUNT>
-__device__ f3 estimate_direct(const DevParams& p, const DevLight& L, int light_pos, const Sample& smp, f3 SEP, f3 SEN, f3 Wo,
+__device__ f3 estimate_direct(const DevParams& p, const Tables& T, const DevLight& L, int light_pos, const Sample& smp, f3 SEP, f3 SEN, f3 Wo,
                               const Shader& sh, Rng& rng, Cnt& cnt)
 {
     f3 Ld = mk(0.0f, 0.0f, 0.0f);
@@ -514,7 +628,7 @@ __device__ f3 estimate_direct(const DevParams& p, const DevLight& L, int light_p
     if (L.unit == ER_UNIT_LUX) Li = mk(Li.x / L.area, Li.y / L.area, Li.z / L.area);
     f3 F = shader_f<FULL>(sh, Wo, Wi);
     float BsdfPdf = shader_pdf<FULL>(sh, Wo, Wi);
-    if (!black(Li) && !black(F) && BsdfPdf > 0.0f && visible<FULL, COUNT>(p, SEP, SP, rng, cnt)) {
+    if (!black(Li) && !black(F) && BsdfPdf > 0.0f && visible<FULL, COUNT>(p, T, SEP, SP, rng, cnt)) {
         const f3 d = sub(SEP, SP);
         const float LightPdf = dot(d, d) / (absdot(SN, neg(Wi)) * L.area);
         const float Weight = power_heuristic(LightPdf, BsdfPdf);
@@ -532,7 +646,7 @@ __device__ f3 estimate_direct(const DevParams& p, const DevLight& L, int light_p
     float T2; f3 P2, N2, Le2;
     const int idx2 = intersect_lights(p, r2, false, T2, P2, N2, Le2);
     if (idx2 < 0 || idx2 != light_pos) return Ld;
-    if (!black(Le2) && visible<FULL, COUNT>(p, SEP, P2, rng, cnt)) {
+    if (!black(Le2) && visible<FULL, COUNT>(p, T, SEP, P2, rng, cnt)) {
         const f3 d = sub(SEP, P2);
         const float LightPdf = dot(d, d) / (absdot(N2, neg(Wi)) * L.area);
         const float Weight = power_heuristic(BsdfPdf, LightPdf);
@@ -546,7 +660,7 @@ __device__ f3 estimate_direct(const DevParams& p, const DevLight& L, int light_p
 /* UniformSampleOneLight (transport.h:115-156) with the full-mode shader selection
  * (design at scatterevent.h:155-212) */
 template<int FULL, int COUNT>
-__device__ f3 uniform_sample_one_light(const DevParams& p, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt)
+__device__ f3 uniform_sample_one_light(const DevParams& p, const Tables& T, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt)
 {
     f3 Ld = mk(0.0f, 0.0f, 0.0f);
     const float I = intensity(p, SEP);
@@ -588,133 +702,231 @@ __device__ f3 uniform_sample_one_light(const DevParams& p, const Sample& smp, f3
     sh.Nn = nrm(SEN);
     sh.Nu = nrm(crs(SEN, Wo));
     sh.Nv = nrm(crs(SEN, sh.Nu));
-    Ld = add(Ld, estimate_direct<FULL, COUNT>(p, *L, pos, smp, SEP, SEN, Wo, sh, rng, cnt));
+    Ld = add(Ld, estimate_direct<FULL, COUNT>(p, T, *L, pos, smp, SEP, SEN, Wo, sh, rng, cnt));
     return scl((float)count, Ld);
 }
 
-/* SingleScattering (singlescattering.h:29-102) */
-template<int FULL, int COUNT>
-__device__ float4 single_scattering(const DevParams& p, int px, int py, Rng& rng, Cnt& cnt)
-{
-    Sample smp;
-    smp.comp = rng.u();
-    smp.dir0 = rng.u(); smp.dir1 = rng.u();
-    smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
-    smp.lightnum = rng.u();
-    smp.film0 = rng.u(); smp.film1 = rng.u();
-    smp.lens0 = rng.u(); smp.lens1 = rng.u();
-
-    /* SampleCamera (singlescattering.h:29-50) */
-    const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
-    const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
-    const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
-    Ray r;
-    r.O = ld3(p.cam_pos);
-    r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
-    r.MinT = p.clip_near;
-    r.MaxT = p.clip_far;
-    if (p.aperture != 0.0f) {
-        float lx, ly;
-        concentric_disk(smp.lens0, smp.lens1, lx, ly);
-        lx = lx * p.aperture; ly = ly * p.aperture;
-        const f3 LI = add(scl(lx, U), scl(ly, V));
-        r.O = add(r.O, LI);
-        r.D = nrm(sub(scl(p.focal, r.D), LI));
-    }
-
-    /* SampleRay (singlescattering.h:52-74) */
-    float Tv = 0.0f; f3 Pv = mk(0.0f, 0.0f, 0.0f);
-    const bool vol = march<FULL, COUNT>(p, r, true, rng, Tv, Pv, cnt);
-    float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
-    const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
-    int type = -1;
-    float T = 3.402823466e+38f;
-    if (vol && Tv < T) { type = 0; T = Tv; }
-    if (il >= 0 && Tl < T) { type = 1; T = Tl; }
-
-    f3 Lv = mk(0.0f, 0.0f, 0.0f);
-    if (type == 0) {
-        if (COUNT) cnt.scatter++;
-        const f3 Nv = normalized_gradient(p, Pv);   /* raymarching.h:70 */
-        Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, smp, Pv, Nv, neg(r.D), rng, cnt));
-    }
-    if (type == 1) Lv = add(Lv, Lel);
-    return make_float4(Lv.x, Lv.y, Lv.z, type >= 0 ? 1.0f : 0.0f);
-}
+/* One scattering pixel handed from the primary kernel to the shading kernel (64 bytes). */
+struct ScatterRec {
+    uint32_t pixel, s0, s1, pad;
+    float Px, Py, Pz, Wox, Woy, Woz;
+    float comp, dir0, dir1, uvw0, uvw1, uvw2;
+    float lightnum, spare;
+};
 
 /* XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous band */
 __device__ __forceinline__ int tile_of_block(int b, int n)
 {
     const int per = (n + 7) >> 3;
-    const int xcd = b & 7, local = b >> 3;
-    const int t = xcd * per + local;
-    return t;
+    return (b & 7) * per + (b >> 3);
 }
 
-template<int FULL, int COUNT>
-__global__ __launch_bounds__(256) void k_trace(const DevParams* __restrict__ pp, float4* __restrict__ frame,
-                                              uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
-                                              unsigned long long* __restrict__ counters, int tiles_x, int ntiles)
+/* Stage the macrocell bits (always) and the opacity LUT (when LUT_LDS) in LDS. */
+template<int LUT_LDS>
+__device__ __forceinline__ Tables stage_tables(const DevParams& p, uint32_t* lds)
 {
+    const int nw = p.mc_words;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = p.mc_bits[k];
+    Tables T;
+    T.bits = lds;
+    if (LUT_LDS) {
+        float* lut = (float*)(lds + nw);
+        for (int k = threadIdx.x; k <= p.lut_max; k += blockDim.x) lut[k] = p.sigma_lut[k];
+        T.lut = lut;
+    } else {
+        T.lut = p.sigma_lut;
+    }
+    __syncthreads();
+    return T;
+}
+
+template<int COUNT>
+__device__ __forceinline__ void flush_counters(const Cnt& cnt, bool active, int paths, unsigned long long* counters)
+{
+    if (!COUNT) return;
+    __shared__ unsigned long long acc[12];
+    if (threadIdx.x < 12) acc[threadIdx.x] = 0ull;
+    __syncthreads();
+    if (active) {
+        if (paths) atomicAdd(&acc[0], (unsigned long long)paths);
+        atomicAdd(&acc[1], (unsigned long long)cnt.primary);
+        atomicAdd(&acc[2], (unsigned long long)cnt.shadow);
+        atomicAdd(&acc[3], (unsigned long long)cnt.shading);
+        atomicAdd(&acc[4], (unsigned long long)cnt.pfetch);
+        atomicAdd(&acc[5], (unsigned long long)cnt.sfetch);
+        atomicAdd(&acc[6], (unsigned long long)cnt.scatter);
+        atomicAdd(&acc[7], (unsigned long long)cnt.shadow_rays);
+        atomicAdd(&acc[8], (unsigned long long)cnt.iters);
+        atomicAdd(&acc[9], (unsigned long long)cnt.skipped);
+        atomicAdd(&acc[10], (unsigned long long)cnt.undone);
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) atomicAdd(&counters[threadIdx.x], acc[threadIdx.x]);
+}
+
+/* Primary kernel: the 11 sample draws (sample.h:258-262), the camera ray (singlescattering.h:29-50),
+ * SampleRay (singlescattering.h:52-74). Pixels whose nearest event is a light or nothing are
+ * finished here; pixels with a volume event are appended to the scatter queue (wave-aggregated
+ * atomic, so each wave's scattering pixels stay contiguous). */
+template<int FULL, int COUNT, int LUT_LDS>
+__global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
+                                                uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
+                                                ScatterRec* __restrict__ queue, unsigned int* __restrict__ qcount,
+                                                unsigned long long* __restrict__ counters, int tiles_x, int ntiles)
+{
+    extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
-    /* blocks beyond the band layout of the last XCD get remapped tiles >= ntiles: skip */
+    const Tables T = stage_tables<LUT_LDS>(p, lds);
     const int tile = tile_of_block(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    Cnt cnt = {0, 0, 0, 0, 0, 0, 0};
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool active = tile < ntiles;
     int px = 0, py = 0;
     if (active) {
-        const int tx = tile % tiles_x, ty = tile / tiles_x;
-        px = tx * 16 + (wave & 1) * 8 + (lane & 7);
-        py = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
+        px = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+        py = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
         active = px < p.film_w && py < p.film_h;
     }
+    const size_t i = (size_t)py * p.film_w + px;
+    bool scatter = false;
+    Sample smp;
+    Rng rng;
+    f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f);
     if (active) {
-        const size_t i = (size_t)py * p.film_w + px;
-        Rng rng;
         rng.s0 = seeds1[i];
         rng.s1 = seeds2[i];
-        frame[i] = single_scattering<FULL, COUNT>(p, px, py, rng, cnt);
-        seeds1[i] = rng.s0;
-        seeds2[i] = rng.s1;
-    }
-    if (COUNT) {
-        __shared__ unsigned long long acc[8];
-        if (threadIdx.x < 8) acc[threadIdx.x] = 0ull;
-        __syncthreads();
-        if (active) {
-            atomicAdd(&acc[0], 1ull);
-            atomicAdd(&acc[1], (unsigned long long)cnt.primary);
-            atomicAdd(&acc[2], (unsigned long long)cnt.shadow);
-            atomicAdd(&acc[3], (unsigned long long)cnt.shading);
-            atomicAdd(&acc[4], (unsigned long long)cnt.pfetch);
-            atomicAdd(&acc[5], (unsigned long long)cnt.sfetch);
-            atomicAdd(&acc[6], (unsigned long long)cnt.scatter);
-            atomicAdd(&acc[7], (unsigned long long)cnt.shadow_rays);
+        smp.comp = rng.u();
+        smp.dir0 = rng.u(); smp.dir1 = rng.u();
+        smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
+        smp.lightnum = rng.u();
+        smp.film0 = rng.u(); smp.film1 = rng.u();
+        smp.lens0 = rng.u(); smp.lens1 = rng.u();
+        const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
+        const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
+        const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
+        Ray r;
+        r.O = ld3(p.cam_pos);
+        r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
+        r.MinT = p.clip_near;
+        r.MaxT = p.clip_far;
+        if (p.aperture != 0.0f) {
+            float lx, ly;
+            concentric_disk(smp.lens0, smp.lens1, lx, ly);
+            lx = lx * p.aperture; ly = ly * p.aperture;
+            const f3 LI = add(scl(lx, U), scl(ly, V));
+            r.O = add(r.O, LI);
+            r.D = nrm(sub(scl(p.focal, r.D), LI));
         }
-        __syncthreads();
-        if (threadIdx.x < 8) atomicAdd(&counters[threadIdx.x], acc[threadIdx.x]);
+        D = r.D;
+        float Tv = 0.0f;
+        const bool vol = march<FULL, COUNT>(p, T, r, true, rng, Tv, Pv, cnt);
+        float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
+        const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
+        int type = -1;
+        float Tn = 3.402823466e+38f;
+        if (vol && Tv < Tn) { type = 0; Tn = Tv; }
+        if (il >= 0 && Tl < Tn) { type = 1; Tn = Tl; }
+        if (type == 0) {
+            scatter = true;
+            if (COUNT) cnt.scatter++;
+        } else {
+            f3 Lv = mk(0.0f, 0.0f, 0.0f);
+            if (type == 1) Lv = add(Lv, Lel);
+            frame[i] = make_float4(Lv.x, Lv.y, Lv.z, type >= 0 ? 1.0f : 0.0f);
+            seeds1[i] = rng.s0;
+            seeds2[i] = rng.s1;
+        }
     }
+    /* wave-aggregated append to the scatter queue */
+    const unsigned long long mask = __ballot(scatter);
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(qcount, (unsigned int)__popcll(mask));
+        base = __shfl(base, leader);
+        if (scatter) {
+            const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+            ScatterRec rec;
+            rec.pixel = (uint32_t)i; rec.s0 = rng.s0; rec.s1 = rng.s1; rec.pad = 0;
+            rec.Px = Pv.x; rec.Py = Pv.y; rec.Pz = Pv.z;
+            rec.Wox = -D.x; rec.Woy = -D.y; rec.Woz = -D.z;
+            rec.comp = smp.comp; rec.dir0 = smp.dir0; rec.dir1 = smp.dir1;
+            rec.uvw0 = smp.uvw0; rec.uvw1 = smp.uvw1; rec.uvw2 = smp.uvw2;
+            rec.lightnum = smp.lightnum; rec.spare = 0.0f;
+            queue[slot] = rec;
+        }
+    }
+    flush_counters<COUNT>(cnt, active, active ? 1 : 0, counters);
+}
+
+/* Shading kernel: for each queued volume event, NormalizedGradient at the event
+ * (raymarching.h:70) and UniformSampleOneLight (transport.h:115-156) with the shadow march. */
+template<int FULL, int COUNT, int LUT_LDS>
+__global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp, float4* __restrict__ frame,
+                                              uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
+                                              const ScatterRec* __restrict__ queue, const unsigned int* __restrict__ qcount,
+                                              unsigned long long* __restrict__ counters)
+{
+    extern __shared__ uint32_t lds[];
+    const DevParams& p = *pp;
+    const Tables T = stage_tables<LUT_LDS>(p, lds);
+    const unsigned int n = *qcount;
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const ScatterRec rec = queue[q];
+        Rng rng;
+        rng.s0 = rec.s0;
+        rng.s1 = rec.s1;
+        Sample smp;
+        smp.comp = rec.comp; smp.dir0 = rec.dir0; smp.dir1 = rec.dir1;
+        smp.uvw0 = rec.uvw0; smp.uvw1 = rec.uvw1; smp.uvw2 = rec.uvw2;
+        smp.lightnum = rec.lightnum;
+        const f3 Pv = mk(rec.Px, rec.Py, rec.Pz);
+        const f3 Wo = mk(rec.Wox, rec.Woy, rec.Woz);
+        const f3 Nv = normalized_gradient(p, Pv);
+        f3 Lv = mk(0.0f, 0.0f, 0.0f);
+        Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, T, smp, Pv, Nv, Wo, rng, cnt));
+        frame[rec.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
+        seeds1[rec.pixel] = rng.s0;
+        seeds2[rec.pixel] = rng.s1;
+    }
+    flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
 } /* namespace erd */
 
-/* launcher used by the runtime */
+/* launchers used by the runtime */
+extern "C" size_t erd_scatter_rec_size(void) { return sizeof(erd::ScatterRec); }
+
+template<int FULL, int COUNT, int LUT_LDS>
+static void launch_pair(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, erd::ScatterRec* queue,
+                        unsigned int* qcount, unsigned long long* counters, int tiles_x, int ntiles, int nblocks,
+                        int shade_blocks, size_t lds, hipStream_t st)
+{
+    hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS>), dim3(nblocks), dim3(256), lds, st, p, frame, s1, s2, queue,
+                       qcount, counters, tiles_x, ntiles);
+    hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS>), dim3(shade_blocks), dim3(256), lds, st, p, frame, s1, s2,
+                       (const erd::ScatterRec*)queue, (const unsigned int*)qcount, counters);
+}
+
 extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2,
-                                       unsigned long long* counters, int W, int H, int full, int count, hipStream_t st)
+                                       void* queue, unsigned int* qcount, unsigned long long* counters, int W, int H,
+                                       int full, int count, int lut_lds, size_t lds_bytes, int shade_blocks,
+                                       hipStream_t st)
 {
     const int tiles_x = (W + 15) / 16, tiles_y = (H + 15) / 16;
     const int ntiles = tiles_x * tiles_y;
     /* round the grid up to a multiple of 8 so every XCD band has the same length */
-    const int per = (ntiles + 7) / 8;
-    const int nblocks = per * 8;
-    dim3 grid(nblocks), block(256);
+    const int nblocks = ((ntiles + 7) / 8) * 8;
+    erd::ScatterRec* q = (erd::ScatterRec*)queue;
+#define ER_PAIR(F, C, L) launch_pair<F, C, L>(p, frame, s1, s2, q, qcount, counters, tiles_x, ntiles, nblocks, shade_blocks, lds_bytes, st)
     if (full) {
-        if (count) hipLaunchKernelGGL((erd::k_trace<1, 1>), grid, block, 0, st, p, frame, s1, s2, counters, tiles_x, ntiles);
-        else hipLaunchKernelGGL((erd::k_trace<1, 0>), grid, block, 0, st, p, frame, s1, s2, counters, tiles_x, ntiles);
+        if (count) { if (lut_lds) ER_PAIR(1, 1, 1); else ER_PAIR(1, 1, 0); }
+        else { if (lut_lds) ER_PAIR(1, 0, 1); else ER_PAIR(1, 0, 0); }
     } else {
-        if (count) hipLaunchKernelGGL((erd::k_trace<0, 1>), grid, block, 0, st, p, frame, s1, s2, counters, tiles_x, ntiles);
-        else hipLaunchKernelGGL((erd::k_trace<0, 0>), grid, block, 0, st, p, frame, s1, s2, counters, tiles_x, ntiles);
+        if (count) { if (lut_lds) ER_PAIR(0, 1, 1); else ER_PAIR(0, 1, 0); }
+        else { if (lut_lds) ER_PAIR(0, 0, 1); else ER_PAIR(0, 0, 0); }
     }
+#undef ER_PAIR
     return hipGetLastError();
 }

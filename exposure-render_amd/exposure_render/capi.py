@@ -148,7 +148,9 @@ class Counters(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("primary_samples", C.c_uint64),
                 ("shadow_samples", C.c_uint64), ("shading_samples", C.c_uint64),
                 ("primary_fetches", C.c_uint64), ("shadow_fetches", C.c_uint64),
-                ("scatter_events", C.c_uint64), ("shadow_rays", C.c_uint64)]
+                ("scatter_events", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("march_iterations", C.c_uint64), ("skipped_samples", C.c_uint64),
+                ("skips_undone", C.c_uint64), ("reserved", C.c_uint64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -431,7 +431,9 @@ template<class R> struct Est {
         R Sum = R(0);
         const R StepSize = R(primary ? S.t.settings.step_factor_primary : S.t.settings.step_factor_shadow) * R(S.minstep);
         MinT += u() * StepSize;
-        if (!(StepSize > R(0))) return false;   /* zero-size volume (Q2): the march would not advance */
+        /* a march that could not advance (zero-size volume, Q2, or a step below the spacing of
+         * floats at MaxT) would not terminate; it is reported as no event */
+        if (!(StepSize > R(0)) || !(MaxT + StepSize > MaxT)) return false;
         V3<R> Ps;
         while (Sum < Sthr) {
             Ps = r.O + scale(MinT, r.D);
@@ -1268,7 +1270,7 @@ int orc_march_ray(void* h, const float* o, const float* d, float mint, float max
     float Sum = 0.0f;
     const float StepSize = (primary ? S.t.settings.step_factor_primary : S.t.settings.step_factor_shadow) * S.minstep;
     MinT += u2 * StepSize;
-    if (!(StepSize > 0.0f)) return 0;
+    if (!(StepSize > 0.0f) || !(MaxT + StepSize > MaxT)) return 0;
     V3<float> Ps;
     uint64_t n = 0;
     while (Sum < Sthr) {
