@@ -17,7 +17,7 @@
  * Source/filterrunningestimate.cuh:55-191, Source/filter.h:34-40) with taps
  * [max(0, i-R), min(i+R, n-1)) - exclusive upper bound as written (Q27).
  *
- * k_build_cells / k_macro_minmax / k_macro_sigma / k_gm_max: volume preprocessing.
+ * k_build_cells / k_macro_minmax / k_macro_sigma / k_macro_bits / k_gm_max: volume preprocessing.
  */
 #include "er_device.h"
 
@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void k_post(const DevParams* __restrict__ pp, 
 {
     const DevParams& p = *pp;
     /* the scatter queue of this pass has been consumed: empty it for the next pass */
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *qcount = 0u;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) qcount[threadIdx.x] = 0u;
     const int W = p.film_w, H = p.film_h;
     __shared__ float4 tile[POST_TILE + 2][POST_TILE + 2];
     const int bx = blockIdx.x * POST_TILE, by = blockIdx.y * POST_TILE;

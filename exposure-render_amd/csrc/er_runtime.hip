@@ -32,9 +32,11 @@
 using erd::DevParams;
 
 extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
-                                       unsigned int* qcount, unsigned long long* counters, int W, int H, int full,
-                                       int count, int lut_lds, size_t lds_bytes, int shade_blocks, hipStream_t st);
-extern "C" size_t erd_scatter_rec_size(void);
+                                       unsigned int* counts, void* events, void* jobs, unsigned char* vis,
+                                       unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
+                                       int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
+                                       int primary_blocks, hipStream_t st);
+extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
 extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, uint32_t* bits, int n, int words, hipStream_t st);
@@ -171,7 +173,8 @@ struct Tracer {
     DevBuf<uint32_t> seeds1, seeds2;
     DevBuf<float> lut, mc_sigma;
     DevBuf<uint32_t> mc_bits, qcount;
-    DevBuf<char> queue;
+    DevBuf<char> queue, events, jobs;
+    DevBuf<unsigned char> vis;
     size_t lds_bytes = 0;
     int lut_lds = 0;
     DevBuf<DevParams> params;
@@ -188,7 +191,7 @@ struct Tracer {
     {
         frame.release(); running.release(); display.release(); disp_tmp.release(); disp_filt.release();
         seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); params.release();
-        mc_bits.release(); qcount.release(); queue.release();
+        mc_bits.release(); qcount.release(); queue.release(); events.release(); jobs.release(); vis.release();
         counters.release(); gm_tmp.release(); bilateral.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -225,6 +228,7 @@ Registry<ClipObj> g_clips;
 Registry<ObjectObj> g_objects;
 Registry<BitmapObj> g_bitmaps;
 bool g_device_ready = false;
+int g_primary_blocks = 2048;   /* persistent primary workgroups: 7 per CU (LDS and VGPR bound) */
 
 int ensure_device()
 {
@@ -232,6 +236,9 @@ int ensure_device()
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n <= 0) return set_error(ER_ERROR, "no HIP device available (%s)", hipGetErrorString(e));
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+        g_primary_blocks = cus * 7;
     g_device_ready = true;
     return ER_OK;
 }
@@ -270,7 +277,12 @@ int tracer_resize(Tracer& t, int W, int H, int id)
     int st;
     if ((st = t.frame.alloc(n)) || (st = t.running.alloc(n)) || (st = t.display.alloc(n)) ||
         (st = t.disp_tmp.alloc(n)) || (st = t.disp_filt.alloc(n)) || (st = t.seeds1.alloc(n)) ||
-        (st = t.seeds2.alloc(n)) || (st = t.queue.alloc(n * erd_scatter_rec_size())) || (st = t.qcount.alloc(1)))
+        (st = t.seeds2.alloc(n)))
+        return st;
+    size_t rs = 0, re = 0, rj = 0;
+    erd_pass_record_sizes(&rs, &re, &rj);
+    if ((st = t.queue.alloc(n * rs)) || (st = t.events.alloc(n * re)) || (st = t.jobs.alloc(2 * n * rj)) ||
+        (st = t.vis.alloc(2 * n)) || (st = t.qcount.alloc(4)))
         return st;
     t.W = W; t.H = H;
     if (n) {
@@ -278,7 +290,7 @@ int tracer_resize(Tracer& t, int W, int H, int id)
         HIP_TRY(hipMemset(t.running.ptr, 0, n * sizeof(float4)));
         HIP_TRY(hipMemset(t.display.ptr, 0, n * sizeof(uchar4)));
         HIP_TRY(hipMemset(t.disp_filt.ptr, 0, n * sizeof(uchar4)));
-        HIP_TRY(hipMemset(t.qcount.ptr, 0, sizeof(uint32_t)));
+        HIP_TRY(hipMemset(t.qcount.ptr, 0, 4 * sizeof(uint32_t)));
         return tracer_seed(t, 0x45787052ull + (uint64_t)id);
     }
     return ER_OK;
@@ -536,8 +548,11 @@ int enqueue_pass(Tracer& t, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
     const int full = t.desc.settings.estimator == ER_ESTIMATOR_FULL;
     const int shade_blocks = std::max(1, std::min((t.W * t.H + 255) / 256, 2048));
     if (e0) HIP_TRY(hipEventRecord(e0, t.stream));
+    const int split = !(full && t.desc.settings.sampler == ER_SAMPLER_WOODCOCK);
+    const size_t lds_bits = (size_t)t.host_params.mc_words * 4;
     HIP_TRY(erd_launch_trace(t.params.ptr, t.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, t.queue.ptr, t.qcount.ptr,
-                             t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds, t.lds_bytes, shade_blocks, t.stream));
+                             t.events.ptr, t.jobs.ptr, t.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
+                             split, t.lds_bytes, lds_bits, shade_blocks, g_primary_blocks, t.stream));
     if (e1) HIP_TRY(hipEventRecord(e1, t.stream));
     HIP_TRY(erd_launch_post(t.params.ptr, t.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations,
                             t.qcount.ptr, t.W, t.H, t.stream));

@@ -14,6 +14,8 @@
  */
 #include "er_device.h"
 
+#include <algorithm>
+
 namespace erd {
 
 struct Cnt {
@@ -534,17 +536,51 @@ __device__ bool intersects_any_light(const DevParams& p, const Ray& r)
     return false;
 }
 
-/* Visible (transport.h:46-56) -> Intersect (transport.h:30-44) */
-template<int FULL, int COUNT>
-__device__ bool visible(const DevParams& p, const Tables& T, f3 P1, f3 P2, Rng& rng, Cnt& cnt)
+/* the shadow ray of Visible (transport.h:50-53) */
+__device__ __forceinline__ Ray shadow_ray(const DevParams& p, f3 P1, f3 P2)
 {
-    if (!p.shadows) return true;
     const f3 W = nrm(sub(P2, P1));
     Ray r;
     r.O = add(P1, scl(ER_RAY_EPS, W));
     r.D = W;
     r.MinT = 0.0f;
     r.MaxT = fminf(len(sub(P2, P1)) - (2.0f * ER_RAY_EPS), p.max_shadow);
+    return r;
+}
+
+/* A shadow ray whose volume march is deferred to k_shadow (48 bytes). */
+struct ShadowJob {
+    float Ox, Oy, Oz, Dx, Dy, Dz, MaxT;
+    uint32_t s0, s1, event, slot, pad;
+};
+
+/* Visible (transport.h:46-56) up to the volume march. Returns 0 (occluded by a light shape),
+ * 1 (visible without marching) or 2 (march needed: the ray and the RNG state the march starts
+ * from are written to job, and rng is advanced past the march's two draws, raymarching.h:89,95). */
+template<int COUNT>
+__device__ int visible_defer(const DevParams& p, f3 P1, f3 P2, Rng& rng, Cnt& cnt, ShadowJob& job)
+{
+    if (!p.shadows) return 1;
+    const Ray r = shadow_ray(p, P1, P2);
+    if (COUNT) cnt.shadow_rays++;
+    if (intersects_any_light(p, r)) return 0;
+    float nearT, farT;
+    if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return 1;
+    job.Ox = r.O.x; job.Oy = r.O.y; job.Oz = r.O.z;
+    job.Dx = r.D.x; job.Dy = r.D.y; job.Dz = r.D.z;
+    job.MaxT = r.MaxT;
+    job.s0 = rng.s0; job.s1 = rng.s1;
+    (void)rng.next_bits();
+    (void)rng.next_bits();
+    return 2;
+}
+
+/* Visible (transport.h:46-56) -> Intersect (transport.h:30-44) */
+template<int FULL, int COUNT>
+__device__ bool visible(const DevParams& p, const Tables& T, f3 P1, f3 P2, Rng& rng, Cnt& cnt)
+{
+    if (!p.shadows) return true;
+    const Ray r = shadow_ray(p, P1, P2);
     if (COUNT) cnt.shadow_rays++;
     if (intersects_any_light(p, r)) return false;
     float Tt; f3 P;
@@ -706,6 +742,123 @@ __device__ f3 uniform_sample_one_light(const DevParams& p, const Tables& T, cons
     return scl((float)count, Ld);
 }
 
+/* Shading of one volume event with the shadow-ray marches deferred (ray-march sampler only,
+ * whose shadow march consumes exactly two draws whenever it starts). Same operations and
+ * order as uniform_sample_one_light / estimate_direct; the contributions that wait for a
+ * visibility result are kept with their state. */
+struct EventRec {
+    uint32_t pixel, state;        /* state: 2 bits per contribution: 0 none, 1 add, 2 add if visible */
+    float count;                  /* light count scaling, or 0 when the result is not scaled */
+    float pad;
+    float Ld0x, Ld0y, Ld0z;       /* 0 + emission */
+    float c1x, c1y, c1z, c2x, c2y, c2z;
+    float pad2;
+};
+
+template<int FULL, int COUNT>
+__device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt,
+                            EventRec& ev, ShadowJob& job0, ShadowJob& job1, bool& has0, bool& has1)
+{
+    f3 Ld = mk(0.0f, 0.0f, 0.0f);
+    const float I = intensity(p, SEP);
+    if (COUNT) cnt.shading += 1 + gradient_samples(p);
+    Ld = add(Ld, ctf_eval(p.emission, I));
+    ev.Ld0x = Ld.x; ev.Ld0y = Ld.y; ev.Ld0z = Ld.z;
+    ev.state = 0;
+    ev.count = 0.0f;
+    has0 = false;
+    has1 = false;
+    const int count = p.light_count;
+    if (count <= 0) return;
+    const int pos = (int)floorf(smp.lightnum * (float)count);
+    const int LightID = p.light_ids[pos];
+    const DevLight* Lp = light_by_id(p, LightID);
+    if (LightID < 0 || !Lp) return;
+    const DevLight& L = *Lp;
+    ev.count = (float)count;
+    bool brdf = true;
+    if (FULL) {
+        const int type = p.shading_type;
+        if (type == ER_SHADING_PHASE_ONLY) {
+            brdf = false;
+        } else if (type != ER_SHADING_BRDF_ONLY) {
+            const float gm = gradient_magnitude(p, SEP);
+            if (COUNT) cnt.shading += 6;
+            const float NGM = gm * p.gm_inv;
+            if (type == ER_SHADING_THRESHOLD) {
+                brdf = NGM > p.grad_threshold;
+            } else if (type == ER_SHADING_MODULATION) {
+                const float PdfBrdf = 1.0f - erm::powf_(1.0f - NGM, 2.0f);
+                brdf = rng.u() < PdfBrdf;
+            } else {
+                const float e = 25.0f * erm::powf_(p.grad_factor, 3.0f) * NGM;
+                const float base = 1.0f - erm::expf_(-e);
+                const float PdfBrdf = p.opacity_modulated ? plf_eval(p.opacity, I) * base : base;
+                brdf = rng.u() <= PdfBrdf;
+            }
+        }
+    }
+    Shader sh;
+    sh.brdf = brdf;
+    sh.Kd = ctf_eval(p.diffuse, I);
+    sh.Ks = ctf_eval(p.specular, I);
+    sh.exponent = 1000000.0f * erm::powf_(plf_eval(p.glossiness, I), 7.0f);
+    sh.Nn = nrm(SEN);
+    sh.Nu = nrm(crs(SEN, Wo));
+    sh.Nv = nrm(crs(SEN, sh.Nu));
+
+    /* light-sampling half */
+    f3 SP, SN; float su, sv;
+    sample_light_surface(p, L, smp.uvw0, smp.uvw1, smp.uvw2, SP, SN, su, sv);
+    f3 Wi = nrm(sub(SP, SEP));
+    f3 Li = scl(L.multiplier, eval_texture(p, L.texture_id, su, sv));
+    if (L.one_sided && dot(sub(SEP, SP), SN) < 0.0f) Li = mk(0.0f, 0.0f, 0.0f);
+    if (L.unit == ER_UNIT_LUX) Li = mk(Li.x / L.area, Li.y / L.area, Li.z / L.area);
+    f3 F = shader_f<FULL>(sh, Wo, Wi);
+    float BsdfPdf = shader_pdf<FULL>(sh, Wo, Wi);
+    if (!black(Li) && !black(F) && BsdfPdf > 0.0f) {
+        const int v = visible_defer<COUNT>(p, SEP, SP, rng, cnt, job0);
+        if (v) {
+            const f3 d = sub(SEP, SP);
+            const float LightPdf = dot(d, d) / (absdot(SN, neg(Wi)) * L.area);
+            const float Weight = power_heuristic(LightPdf, BsdfPdf);
+            const f3 FL = mul(F, Li);
+            f3 c;
+            if (sh.brdf) c = scl(absdot(Wi, SEN) * Weight / LightPdf, FL);
+            else if (FULL) c = scl(Weight / LightPdf, FL);
+            else c = mk(FL.x / LightPdf, FL.y / LightPdf, FL.z / LightPdf);
+            ev.c1x = c.x; ev.c1y = c.y; ev.c1z = c.z;
+            ev.state |= (uint32_t)v;
+            if (v == 2) { job0.slot = 0; has0 = true; }
+        }
+    }
+    if (!FULL) return;
+
+    /* BSDF-sampling half */
+    F = shader_sample_f(sh, Wo, Wi, BsdfPdf, smp.comp, smp.dir0, smp.dir1);
+    if (black(F) || BsdfPdf <= 0.0f) return;
+    Ray r2;
+    r2.O = SEP; r2.D = Wi; r2.MinT = 0.0f; r2.MaxT = 1000000.0f;
+    float T2; f3 P2, N2, Le2;
+    const int idx2 = intersect_lights(p, r2, false, T2, P2, N2, Le2);
+    if (idx2 < 0 || idx2 != pos) return;
+    if (!black(Le2)) {
+        const int v = visible_defer<COUNT>(p, SEP, P2, rng, cnt, job1);
+        if (v) {
+            const f3 d = sub(SEP, P2);
+            const float LightPdf = dot(d, d) / (absdot(N2, neg(Wi)) * L.area);
+            const float Weight = power_heuristic(BsdfPdf, LightPdf);
+            const f3 FL = mul(F, Le2);
+            f3 c;
+            if (sh.brdf) c = scl(absdot(Wi, SEN) * Weight / BsdfPdf, FL);
+            else c = scl(Weight / BsdfPdf, FL);
+            ev.c2x = c.x; ev.c2y = c.y; ev.c2z = c.z;
+            ev.state |= (uint32_t)v << 2;
+            if (v == 2) { job1.slot = 1; has1 = true; }
+        }
+    }
+}
+
 /* One scattering pixel handed from the primary kernel to the shading kernel (64 bytes). */
 struct ScatterRec {
     uint32_t pixel, s0, s1, pad;
@@ -713,13 +866,6 @@ struct ScatterRec {
     float comp, dir0, dir1, uvw0, uvw1, uvw2;
     float lightnum, spare;
 };
-
-/* XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a contiguous band */
-__device__ __forceinline__ int tile_of_block(int b, int n)
-{
-    const int per = (n + 7) >> 3;
-    return (b & 7) * per + (b >> 3);
-}
 
 /* Stage the macrocell bits (always) and the opacity LUT (when LUT_LDS) in LDS. */
 template<int LUT_LDS>
@@ -765,168 +911,282 @@ __device__ __forceinline__ void flush_counters(const Cnt& cnt, bool active, int 
 }
 
 /* Primary kernel: the 11 sample draws (sample.h:258-262), the camera ray (singlescattering.h:29-50),
- * SampleRay (singlescattering.h:52-74). Pixels whose nearest event is a light or nothing are
- * finished here; pixels with a volume event are appended to the scatter queue (wave-aggregated
- * atomic, so each wave's scattering pixels stay contiguous). */
+ * SampleRay (singlescattering.h:52-74). Persistent waves take 8 x 8 pixel tiles from an atomic
+ * counter (counts[2]), so the cost of a tile - from a few samples for background pixels to
+ * thousands behind low-density tissue - is balanced over the whole device. Pixels whose nearest
+ * event is a light or nothing are finished here; pixels with a volume event are appended to the
+ * scatter queue (wave-aggregated atomic, so each wave's scattering pixels stay contiguous). */
 template<int FULL, int COUNT, int LUT_LDS>
 __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
-                                                ScatterRec* __restrict__ queue, unsigned int* __restrict__ qcount,
+                                                ScatterRec* __restrict__ queue, unsigned int* __restrict__ counts,
                                                 unsigned long long* __restrict__ counters, int tiles_x, int ntiles)
 {
     extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
     const Tables T = stage_tables<LUT_LDS>(p, lds);
-    const int tile = tile_of_block(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    bool active = tile < ntiles;
-    int px = 0, py = 0;
-    if (active) {
-        px = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-        py = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
-        active = px < p.film_w && py < p.film_h;
-    }
-    const size_t i = (size_t)py * p.film_w + px;
-    bool scatter = false;
-    Sample smp;
-    Rng rng;
-    f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f);
-    if (active) {
-        rng.s0 = seeds1[i];
-        rng.s1 = seeds2[i];
-        smp.comp = rng.u();
-        smp.dir0 = rng.u(); smp.dir1 = rng.u();
-        smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
-        smp.lightnum = rng.u();
-        smp.film0 = rng.u(); smp.film1 = rng.u();
-        smp.lens0 = rng.u(); smp.lens1 = rng.u();
-        const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
-        const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
-        const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
-        Ray r;
-        r.O = ld3(p.cam_pos);
-        r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
-        r.MinT = p.clip_near;
-        r.MaxT = p.clip_far;
-        if (p.aperture != 0.0f) {
-            float lx, ly;
-            concentric_disk(smp.lens0, smp.lens1, lx, ly);
-            lx = lx * p.aperture; ly = ly * p.aperture;
-            const f3 LI = add(scl(lx, U), scl(ly, V));
-            r.O = add(r.O, LI);
-            r.D = nrm(sub(scl(p.focal, r.D), LI));
+    int paths = 0;
+    for (;;) {
+        unsigned int tile = 0;
+        if (lane == 0) tile = atomicAdd(&counts[2], 1u);
+        tile = __shfl(tile, 0);
+        if (tile >= (unsigned int)ntiles) break;
+        const int px = (int)(tile % (unsigned int)tiles_x) * 8 + (lane & 7);
+        const int py = (int)(tile / (unsigned int)tiles_x) * 8 + (lane >> 3);
+        const bool active = px < p.film_w && py < p.film_h;
+        const size_t i = (size_t)py * p.film_w + px;
+        bool scatter = false;
+        Sample smp;
+        Rng rng;
+        f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f);
+        if (active) {
+            paths++;
+            rng.s0 = seeds1[i];
+            rng.s1 = seeds2[i];
+            smp.comp = rng.u();
+            smp.dir0 = rng.u(); smp.dir1 = rng.u();
+            smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
+            smp.lightnum = rng.u();
+            smp.film0 = rng.u(); smp.film1 = rng.u();
+            smp.lens0 = rng.u(); smp.lens1 = rng.u();
+            const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
+            const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
+            const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
+            Ray r;
+            r.O = ld3(p.cam_pos);
+            r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
+            r.MinT = p.clip_near;
+            r.MaxT = p.clip_far;
+            if (p.aperture != 0.0f) {
+                float lx, ly;
+                concentric_disk(smp.lens0, smp.lens1, lx, ly);
+                lx = lx * p.aperture; ly = ly * p.aperture;
+                const f3 LI = add(scl(lx, U), scl(ly, V));
+                r.O = add(r.O, LI);
+                r.D = nrm(sub(scl(p.focal, r.D), LI));
+            }
+            D = r.D;
+            float Tv = 0.0f;
+            const bool vol = march<FULL, COUNT>(p, T, r, true, rng, Tv, Pv, cnt);
+            float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
+            const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
+            int type = -1;
+            float Tn = 3.402823466e+38f;
+            if (vol && Tv < Tn) { type = 0; Tn = Tv; }
+            if (il >= 0 && Tl < Tn) { type = 1; Tn = Tl; }
+            if (type == 0) {
+                scatter = true;
+                if (COUNT) cnt.scatter++;
+            } else {
+                f3 Lv = mk(0.0f, 0.0f, 0.0f);
+                if (type == 1) Lv = add(Lv, Lel);
+                frame[i] = make_float4(Lv.x, Lv.y, Lv.z, type >= 0 ? 1.0f : 0.0f);
+                seeds1[i] = rng.s0;
+                seeds2[i] = rng.s1;
+            }
         }
-        D = r.D;
-        float Tv = 0.0f;
-        const bool vol = march<FULL, COUNT>(p, T, r, true, rng, Tv, Pv, cnt);
-        float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
-        const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
-        int type = -1;
-        float Tn = 3.402823466e+38f;
-        if (vol && Tv < Tn) { type = 0; Tn = Tv; }
-        if (il >= 0 && Tl < Tn) { type = 1; Tn = Tl; }
-        if (type == 0) {
-            scatter = true;
-            if (COUNT) cnt.scatter++;
-        } else {
-            f3 Lv = mk(0.0f, 0.0f, 0.0f);
-            if (type == 1) Lv = add(Lv, Lel);
-            frame[i] = make_float4(Lv.x, Lv.y, Lv.z, type >= 0 ? 1.0f : 0.0f);
-            seeds1[i] = rng.s0;
-            seeds2[i] = rng.s1;
+        /* wave-aggregated append to the scatter queue */
+        const unsigned long long mask = __ballot(scatter);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(&counts[0], (unsigned int)__popcll(mask));
+            base = __shfl(base, leader);
+            if (scatter) {
+                const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+                ScatterRec rec;
+                rec.pixel = (uint32_t)i; rec.s0 = rng.s0; rec.s1 = rng.s1; rec.pad = 0;
+                rec.Px = Pv.x; rec.Py = Pv.y; rec.Pz = Pv.z;
+                rec.Wox = -D.x; rec.Woy = -D.y; rec.Woz = -D.z;
+                rec.comp = smp.comp; rec.dir0 = smp.dir0; rec.dir1 = smp.dir1;
+                rec.uvw0 = smp.uvw0; rec.uvw1 = smp.uvw1; rec.uvw2 = smp.uvw2;
+                rec.lightnum = smp.lightnum; rec.spare = 0.0f;
+                queue[slot] = rec;
+            }
         }
     }
-    /* wave-aggregated append to the scatter queue */
-    const unsigned long long mask = __ballot(scatter);
-    if (mask) {
-        const int leader = __ffsll((long long)mask) - 1;
-        unsigned int base = 0;
-        if (lane == leader) base = atomicAdd(qcount, (unsigned int)__popcll(mask));
-        base = __shfl(base, leader);
-        if (scatter) {
-            const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
-            ScatterRec rec;
-            rec.pixel = (uint32_t)i; rec.s0 = rng.s0; rec.s1 = rng.s1; rec.pad = 0;
-            rec.Px = Pv.x; rec.Py = Pv.y; rec.Pz = Pv.z;
-            rec.Wox = -D.x; rec.Woy = -D.y; rec.Woz = -D.z;
-            rec.comp = smp.comp; rec.dir0 = smp.dir0; rec.dir1 = smp.dir1;
-            rec.uvw0 = smp.uvw0; rec.uvw1 = smp.uvw1; rec.uvw2 = smp.uvw2;
-            rec.lightnum = smp.lightnum; rec.spare = 0.0f;
-            queue[slot] = rec;
-        }
-    }
-    flush_counters<COUNT>(cnt, active, active ? 1 : 0, counters);
+    flush_counters<COUNT>(cnt, true, paths, counters);
 }
 
 /* Shading kernel: for each queued volume event, NormalizedGradient at the event
- * (raymarching.h:70) and UniformSampleOneLight (transport.h:115-156) with the shadow march. */
-template<int FULL, int COUNT, int LUT_LDS>
+ * (raymarching.h:70) and UniformSampleOneLight (transport.h:115-156). SPLIT: the shadow
+ * marches are appended to the shadow-job queue and the event waits in an EventRec; otherwise
+ * (Woodcock, whose draw count per march is not known in advance) the marches run inline. */
+template<int FULL, int COUNT, int LUT_LDS, int SPLIT>
 __global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                               uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
-                                              const ScatterRec* __restrict__ queue, const unsigned int* __restrict__ qcount,
+                                              const ScatterRec* __restrict__ queue, unsigned int* __restrict__ counts,
+                                              EventRec* __restrict__ events, ShadowJob* __restrict__ jobq,
                                               unsigned long long* __restrict__ counters)
 {
     extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
-    const Tables T = stage_tables<LUT_LDS>(p, lds);
-    const unsigned int n = *qcount;
+    const Tables T = stage_tables<SPLIT ? 0 : LUT_LDS>(p, lds);
+    const unsigned int n = counts[0];
+    const int lane = threadIdx.x & 63;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const ScatterRec rec = queue[q];
-        Rng rng;
-        rng.s0 = rec.s0;
-        rng.s1 = rec.s1;
-        Sample smp;
-        smp.comp = rec.comp; smp.dir0 = rec.dir0; smp.dir1 = rec.dir1;
-        smp.uvw0 = rec.uvw0; smp.uvw1 = rec.uvw1; smp.uvw2 = rec.uvw2;
-        smp.lightnum = rec.lightnum;
-        const f3 Pv = mk(rec.Px, rec.Py, rec.Pz);
-        const f3 Wo = mk(rec.Wox, rec.Woy, rec.Woz);
-        const f3 Nv = normalized_gradient(p, Pv);
-        f3 Lv = mk(0.0f, 0.0f, 0.0f);
-        Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, T, smp, Pv, Nv, Wo, rng, cnt));
-        frame[rec.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
-        seeds1[rec.pixel] = rng.s0;
-        seeds2[rec.pixel] = rng.s1;
+    const unsigned int stride = gridDim.x * blockDim.x;
+    /* every lane of a wave runs the same number of iterations (the wave-wide append needs it) */
+    const unsigned int base_q = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (unsigned int q0 = base_q; q0 < n; q0 += stride) {
+        const unsigned int q = q0 + lane;
+        const bool active = q < n;
+        bool has0 = false, has1 = false;
+        ShadowJob job0, job1;
+        if (active) {
+            const ScatterRec rec = queue[q];
+            Rng rng;
+            rng.s0 = rec.s0;
+            rng.s1 = rec.s1;
+            Sample smp;
+            smp.comp = rec.comp; smp.dir0 = rec.dir0; smp.dir1 = rec.dir1;
+            smp.uvw0 = rec.uvw0; smp.uvw1 = rec.uvw1; smp.uvw2 = rec.uvw2;
+            smp.lightnum = rec.lightnum;
+            const f3 Pv = mk(rec.Px, rec.Py, rec.Pz);
+            const f3 Wo = mk(rec.Wox, rec.Woy, rec.Woz);
+            const f3 Nv = normalized_gradient(p, Pv);
+            if (SPLIT) {
+                EventRec ev;
+                ev.pixel = rec.pixel;
+                ev.c1x = ev.c1y = ev.c1z = ev.c2x = ev.c2y = ev.c2z = 0.0f;
+                ev.pad = ev.pad2 = 0.0f;
+                shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1);
+                events[q] = ev;
+            } else {
+                f3 Lv = mk(0.0f, 0.0f, 0.0f);
+                Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, T, smp, Pv, Nv, Wo, rng, cnt));
+                frame[rec.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
+            }
+            seeds1[rec.pixel] = rng.s0;
+            seeds2[rec.pixel] = rng.s1;
+        }
+        if (SPLIT) {
+            /* wave-aggregated append of 0..2 shadow jobs per lane */
+            const unsigned long long m1 = __ballot(has0), m2 = __ballot(has1);
+            const unsigned int total = (unsigned int)(__popcll(m1) + __popcll(m2));
+            if (total) {
+                const int leader = __ffsll((long long)(m1 | m2)) - 1;
+                unsigned int base = 0;
+                if (lane == leader) base = atomicAdd(&counts[1], total);
+                base = __shfl(base, leader);
+                const unsigned long long below = (1ull << lane) - 1ull;
+                const unsigned int slot = base + (unsigned int)(__popcll(m1 & below) + __popcll(m2 & below));
+                if (has0) { job0.event = q; job0.pad = 0; jobq[slot] = job0; }
+                if (has1) { job1.event = q; job1.pad = 0; jobq[slot + (has0 ? 1u : 0u)] = job1; }
+            }
+        }
     }
     flush_counters<COUNT>(cnt, true, 0, counters);
+}
+
+/* Shadow-march kernel: ScatterEventInVolume (raymarching.h:73-113) for each deferred shadow
+ * ray; writes the visibility of the event's contribution. */
+template<int FULL, int COUNT, int LUT_LDS>
+__global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp, const unsigned int* __restrict__ counts,
+                                               const ShadowJob* __restrict__ jobq, unsigned char* __restrict__ vis,
+                                               unsigned long long* __restrict__ counters)
+{
+    extern __shared__ uint32_t lds[];
+    const DevParams& p = *pp;
+    const Tables T = stage_tables<LUT_LDS>(p, lds);
+    const unsigned int n = counts[1];
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const ShadowJob job = jobq[j];
+        Ray r;
+        r.O = mk(job.Ox, job.Oy, job.Oz);
+        r.D = mk(job.Dx, job.Dy, job.Dz);
+        r.MinT = 0.0f;
+        r.MaxT = job.MaxT;
+        Rng rng;
+        rng.s0 = job.s0;
+        rng.s1 = job.s1;
+        float Tt; f3 P;
+        const bool hit = march<FULL, COUNT>(p, T, r, false, rng, Tt, P, cnt);
+        vis[2 * job.event + job.slot] = hit ? 0 : 1;
+    }
+    flush_counters<COUNT>(cnt, true, 0, counters);
+}
+
+/* Completes each deferred event in the order of estimate_direct and UniformSampleOneLight:
+ * Ld = (0 + emission) + ((0 + c1) + c2), times the light count. */
+__global__ __launch_bounds__(256) void k_finish(const unsigned int* __restrict__ counts, const EventRec* __restrict__ events,
+                                               const unsigned char* __restrict__ vis, float4* __restrict__ frame)
+{
+    const unsigned int n = counts[0];
+    for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const EventRec ev = events[q];
+        const uint32_t s1 = ev.state & 3u, s2 = (ev.state >> 2) & 3u;
+        const bool use1 = s1 == 1u || (s1 == 2u && vis[2 * q] != 0);
+        const bool use2 = s2 == 1u || (s2 == 2u && vis[2 * q + 1] != 0);
+        f3 Ld = mk(ev.Ld0x, ev.Ld0y, ev.Ld0z);
+        if (ev.count > 0.0f) {
+            f3 Ed = mk(0.0f, 0.0f, 0.0f);
+            if (use1) Ed = add(Ed, mk(ev.c1x, ev.c1y, ev.c1z));
+            if (use2) Ed = add(Ed, mk(ev.c2x, ev.c2y, ev.c2z));
+            Ld = scl(ev.count, add(Ld, Ed));
+        }
+        f3 Lv = mk(0.0f, 0.0f, 0.0f);
+        Lv = add(Lv, Ld);
+        frame[ev.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
+    }
 }
 
 } /* namespace erd */
 
 /* launchers used by the runtime */
-extern "C" size_t erd_scatter_rec_size(void) { return sizeof(erd::ScatterRec); }
-
-template<int FULL, int COUNT, int LUT_LDS>
-static void launch_pair(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, erd::ScatterRec* queue,
-                        unsigned int* qcount, unsigned long long* counters, int tiles_x, int ntiles, int nblocks,
-                        int shade_blocks, size_t lds, hipStream_t st)
+extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job)
 {
-    hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS>), dim3(nblocks), dim3(256), lds, st, p, frame, s1, s2, queue,
-                       qcount, counters, tiles_x, ntiles);
-    hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS>), dim3(shade_blocks), dim3(256), lds, st, p, frame, s1, s2,
-                       (const erd::ScatterRec*)queue, (const unsigned int*)qcount, counters);
+    *scatter = sizeof(erd::ScatterRec);
+    *event = sizeof(erd::EventRec);
+    *job = sizeof(erd::ShadowJob);
 }
 
-extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2,
-                                       void* queue, unsigned int* qcount, unsigned long long* counters, int W, int H,
-                                       int full, int count, int lut_lds, size_t lds_bytes, int shade_blocks,
-                                       hipStream_t st)
+template<int FULL, int COUNT, int LUT_LDS>
+static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, erd::ScatterRec* queue,
+                       unsigned int* counts, erd::EventRec* events, erd::ShadowJob* jobs, unsigned char* vis,
+                       unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks, int split,
+                       size_t lds, size_t lds_bits, hipStream_t st)
 {
-    const int tiles_x = (W + 15) / 16, tiles_y = (H + 15) / 16;
-    const int ntiles = tiles_x * tiles_y;
-    /* round the grid up to a multiple of 8 so every XCD band has the same length */
-    const int nblocks = ((ntiles + 7) / 8) * 8;
-    erd::ScatterRec* q = (erd::ScatterRec*)queue;
-#define ER_PAIR(F, C, L) launch_pair<F, C, L>(p, frame, s1, s2, q, qcount, counters, tiles_x, ntiles, nblocks, shade_blocks, lds_bytes, st)
-    if (full) {
-        if (count) { if (lut_lds) ER_PAIR(1, 1, 1); else ER_PAIR(1, 1, 0); }
-        else { if (lut_lds) ER_PAIR(1, 0, 1); else ER_PAIR(1, 0, 0); }
+    hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS>), dim3(nblocks), dim3(256), lds, st, p, frame, s1, s2, queue,
+                       counts, counters, tiles_x, ntiles);
+    if (split) {
+        hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1,
+                           s2, (const erd::ScatterRec*)queue, counts, events, jobs, counters);
+        hipLaunchKernelGGL((erd::k_shadow<FULL, COUNT, LUT_LDS>), dim3(2 * shade_blocks), dim3(256), lds, st, p,
+                           (const unsigned int*)counts, (const erd::ShadowJob*)jobs, vis, counters);
+        hipLaunchKernelGGL(erd::k_finish, dim3(shade_blocks), dim3(256), 0, st, (const unsigned int*)counts,
+                           (const erd::EventRec*)events, (const unsigned char*)vis, frame);
     } else {
-        if (count) { if (lut_lds) ER_PAIR(0, 1, 1); else ER_PAIR(0, 1, 0); }
-        else { if (lut_lds) ER_PAIR(0, 0, 1); else ER_PAIR(0, 0, 0); }
+        hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS, 0>), dim3(shade_blocks), dim3(256), lds, st, p, frame, s1, s2,
+                           (const erd::ScatterRec*)queue, counts, events, jobs, counters);
     }
-#undef ER_PAIR
+}
+
+extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
+                                       unsigned int* counts, void* events, void* jobs, unsigned char* vis,
+                                       unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
+                                       int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
+                                       int primary_blocks, hipStream_t st)
+{
+    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const int ntiles = tiles_x * tiles_y;
+    /* persistent workgroups: enough to fill every CU, never more than there are tiles */
+    const int nblocks = std::max(1, std::min((ntiles + 3) / 4, primary_blocks));
+    erd::ScatterRec* q = (erd::ScatterRec*)queue;
+    erd::EventRec* ev = (erd::EventRec*)events;
+    erd::ShadowJob* jb = (erd::ShadowJob*)jobs;
+#define ER_ALL(F, C, L) launch_all<F, C, L>(p, frame, s1, s2, q, counts, ev, jb, vis, counters, tiles_x, ntiles, nblocks, \
+                                            shade_blocks, split, lds_bytes, lds_bits, st)
+    if (full) {
+        if (count) { if (lut_lds) ER_ALL(1, 1, 1); else ER_ALL(1, 1, 0); }
+        else { if (lut_lds) ER_ALL(1, 0, 1); else ER_ALL(1, 0, 0); }
+    } else {
+        if (count) { if (lut_lds) ER_ALL(0, 1, 1); else ER_ALL(0, 1, 0); }
+        else { if (lut_lds) ER_ALL(0, 0, 1); else ER_ALL(0, 0, 0); }
+    }
+#undef ER_ALL
     return hipGetLastError();
 }
