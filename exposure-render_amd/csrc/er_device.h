@@ -189,6 +189,15 @@ __device__ __forceinline__ f3 ctf_eval(const er_color_tf& c, float x)
     return mk(plf_eval(c.plf[0], x), plf_eval(c.plf[1], x), plf_eval(c.plf[2], x));
 }
 
+/* DevParams lives in device memory, so the compiler sees the pointers stored in it as generic
+ * and would access them with flat instructions. A flat load also counts against the LDS
+ * counter, so every later wait for an LDS read would drain all loads in flight. These pointers
+ * always address global memory; gmem() says so. */
+template<class T> __device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* ptr)
+{
+    return (const __attribute__((address_space(1))) T*)ptr;
+}
+
 /* ---- volume sampling ----------------------------------------------------------------- */
 struct Lattice { float Lx, Ly, Lz; };
 
@@ -204,17 +213,31 @@ __device__ __forceinline__ Lattice to_lattice(const DevParams& p, f3 P)
 
 __device__ __forceinline__ float trunc16(float v) { return (float)(unsigned short)v; }
 
-/* Buffer3D<ushort>::operator()(Vec3f) (buffer3d.h:222-242) from one cell record: 7 lerps,
- * each truncated to uint16 (geometry.h:87-90). */
-__device__ __forceinline__ float trilinear_cell(const DevParams& p, const Lattice& l)
+/* A trilinear sample split in two halves so that several cell loads can be in flight before
+ * their values are used: the cell record and the fractional lattice offsets. */
+struct CellFetch { uint4 c; float dx, dy, dz; };
+
+__device__ __forceinline__ CellFetch cell_fetch(const DevParams& p, const Lattice& l)
 {
     const float fx = floorf(l.Lx), fy = floorf(l.Ly), fz = floorf(l.Lz);
     const int vx = (int)fx, vy = (int)fy, vz = (int)fz;
-    const float dx = l.Lx - (float)vx, dy = l.Ly - (float)vy, dz = l.Lz - (float)vz;
+    CellFetch f;
+    f.dx = l.Lx - (float)vx; f.dy = l.Ly - (float)vy; f.dz = l.Lz - (float)vz;
     const int cx = min(max(vx, -1), p.res[0] - 1) + 1;
     const int cy = min(max(vy, -1), p.res[1] - 1) + 1;
     const int cz = min(max(vz, -1), p.res[2] - 1) + 1;
-    const uint4 c = p.cells[(size_t)cz * (size_t)p.cstride_z + (size_t)(cy * p.cstride_y + cx)];
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = gmem((const u32x4*)p.cells)[(size_t)cz * (size_t)p.cstride_z + (size_t)(cy * p.cstride_y + cx)];
+    f.c = make_uint4(v.x, v.y, v.z, v.w);
+    return f;
+}
+
+/* Buffer3D<ushort>::operator()(Vec3f) (buffer3d.h:222-242) from one cell record: 7 lerps,
+ * each truncated to uint16 (geometry.h:87-90). */
+__device__ __forceinline__ float cell_value(const CellFetch& f)
+{
+    const uint4 c = f.c;
+    const float dx = f.dx, dy = f.dy, dz = f.dz;
     const float v000 = (float)(c.x & 0xffffu), v100 = (float)(c.x >> 16);
     const float v010 = (float)(c.y & 0xffffu), v110 = (float)(c.y >> 16);
     const float v001 = (float)(c.z & 0xffffu), v101 = (float)(c.z >> 16);
@@ -228,6 +251,8 @@ __device__ __forceinline__ float trilinear_cell(const DevParams& p, const Lattic
     const float d1 = trunc16(oy * d01 + dy * d11);
     return trunc16(oz * d0 + dz * d1);
 }
+
+__device__ __forceinline__ float trilinear_cell(const DevParams& p, const Lattice& l) { return cell_value(cell_fetch(p, l)); }
 
 __device__ __forceinline__ float intensity(const DevParams& p, f3 P) { return trilinear_cell(p, to_lattice(p, P)); }
 

@@ -58,12 +58,23 @@ __device__ __forceinline__ bool clipped(const DevParams& p, f3 P)
     return false;
 }
 
+/* dense samples per march iteration (loads in flight together) in the primary and shadow kernels */
+#ifndef ER_BATCH_PRIMARY
+#define ER_BATCH_PRIMARY 6
+#endif
+#ifndef ER_BATCH_SHADOW
+#define ER_BATCH_SHADOW 8
+#endif
+
 /* ---- free-flight sampling --------------------------------------------------------------- */
 
 /* Per-block copies of the macrocell occupancy bits and (when it fits) the opacity LUT. */
 struct Tables {
     const uint32_t* bits;   /* bit m set <=> macrocell m holds a sample with sigma > 0 */
-    const float* lut;       /* DensityScale * Opacity(i) */
+    /* DensityScale * Opacity(i): in LDS when it fits (lut_lds), else in global memory */
+    const float* lut_lds;
+    const __attribute__((address_space(1))) float* lut_glb;
+    __device__ __forceinline__ float lut(int i) const { return lut_glb ? lut_glb[i] : lut_lds[i]; }
 };
 
 __device__ __forceinline__ bool mc_occupied(const Tables& T, int m) { return (T.bits[m >> 5] >> (m & 31)) & 1u; }
@@ -83,7 +94,7 @@ __device__ __forceinline__ float sample_sigma(const DevParams& p, const Tables& 
     if (!mc_occupied(T, m)) return 0.0f;
     if (FULL && p.clip_count > 0 && clipped(p, Ps)) return 0.0f;
     if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-    return T.lut[min((int)trilinear_cell(p, l), p.lut_max)];
+    return T.lut(min((int)trilinear_cell(p, l), p.lut_max));
 }
 
 /* Advance t by repeated float additions of b while t < t_lim, without performing them one by
@@ -137,7 +148,7 @@ __device__ __forceinline__ uint32_t skip_additions(float& t, float b, float t_li
  * (raymarching.h:73-113) for shadow rays: jittered fixed-step tracking with threshold
  * S = -log(u) / DensityScale; T is reported one step beyond P (Q13); the primary march ends
  * at t >= tmax, the shadow march at t > tmax (Q14). */
-template<int FULL, int COUNT>
+template<int FULL, int COUNT, int K = 1>
 __device__ bool march(const DevParams& p, const Tables& T, const Ray& r, bool primary, Rng& rng, float& Tout, f3& Pout, Cnt& cnt)
 {
     float nearT, farT;
@@ -151,7 +162,7 @@ __device__ bool march(const DevParams& p, const Tables& T, const Ray& r, bool pr
         if (!(mu > 0.0f)) return false;
         float t = MinT;
         for (;;) {
-            t += p.neglog[rng.next_bits()] / mu;
+            t += gmem(p.neglog)[rng.next_bits()] / mu;
             if (primary ? (t >= MaxT) : (t > MaxT)) return false;
             const f3 Ps = add(r.O, scl(t, r.D));
             if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; }
@@ -160,7 +171,7 @@ __device__ bool march(const DevParams& p, const Tables& T, const Ray& r, bool pr
         }
     }
 
-    const float S = p.neglog[rng.next_bits()] / p.density_scale;
+    const float S = gmem(p.neglog)[rng.next_bits()] / p.density_scale;
     const float StepSize = (primary ? p.step_primary : p.step_shadow) * p.minstep;
     MinT += rng.u() * StepSize;
     /* a march that could not advance (zero-size volume, Q2, or a step below the spacing of
@@ -186,14 +197,61 @@ __device__ bool march(const DevParams& p, const Tables& T, const Ray& r, bool pr
         const int myi = mc_axis(l.Ly, sh, p.mc_dim[1]);
         const int mzi = mc_axis(l.Lz, sh, p.mc_dim[2]);
         const int mc = (mzi * p.mc_dim[1] + myi) * p.mc_dim[0] + mxi;
-        if (mc_occupied(T, mc)) {
+        if (K == 1 && mc_occupied(T, mc)) {
             float SigmaT = 0.0f;
             if (!(FULL && p.clip_count > 0 && clipped(p, Ps))) {
                 if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-                SigmaT = T.lut[min((int)trilinear_cell(p, l), p.lut_max)];
+                SigmaT = T.lut(min((int)trilinear_cell(p, l), p.lut_max));
             }
             Sum += SigmaT * StepSize;
             MinT += StepSize;
+            continue;
+        }
+        if (K > 1 && mc_occupied(T, mc)) {
+            /* K consecutive samples at once: their positions follow from the same sequence of
+             * MinT additions and do not depend on the values, so all K cell loads are issued
+             * before the first value is used. The values are then consumed in order, with the
+             * loop's own end and threshold tests between them; loads past the point where the
+             * loop stops are discarded (their addresses are clamped into the cell array). A
+             * sample in an empty macrocell has sigma exactly 0 and is consumed without a fetch. */
+            CellFetch f[K];
+            bool live[K];
+            float tk[K];
+            int kn = K;
+            tk[0] = MinT;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                f3 Pk = Ps;
+                Lattice lk = l;
+                bool occ = true;
+                if (k > 0) {
+                    tk[k] = tk[k - 1] + StepSize;
+                    if (kn == K && (primary ? (tk[k] >= MaxT) : (tk[k] > MaxT))) kn = k;
+                    Pk = add(r.O, scl(tk[k], r.D));
+                    lk = to_lattice(p, Pk);
+                    occ = mc_occupied(T, (mc_axis(lk.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(lk.Ly, sh, p.mc_dim[1])) * p.mc_dim[0] +
+                                             mc_axis(lk.Lx, sh, p.mc_dim[0]));
+                }
+                live[k] = occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
+                f[k] = cell_fetch(p, lk);
+            }
+            bool done = false;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (k > 0) {
+                    if (k >= kn) return false;
+                    if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
+                }
+                float SigmaT = 0.0f;
+                if (live[k]) {
+                    if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
+                    SigmaT = T.lut(min((int)cell_value(f[k]), p.lut_max));
+                }
+                Sum += SigmaT * StepSize;
+                MinT += StepSize;
+                if (!(Sum < S)) { Ps = add(r.O, scl(tk[k], r.D)); done = true; break; }
+            }
+            if (done) break;
             continue;
         }
         /* Empty macrocell: this sample adds exactly 0. Skip the following samples that stay in
@@ -872,15 +930,17 @@ template<int LUT_LDS>
 __device__ __forceinline__ Tables stage_tables(const DevParams& p, uint32_t* lds)
 {
     const int nw = p.mc_words;
-    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = p.mc_bits[k];
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = gmem(p.mc_bits)[k];
     Tables T;
     T.bits = lds;
     if (LUT_LDS) {
         float* lut = (float*)(lds + nw);
-        for (int k = threadIdx.x; k <= p.lut_max; k += blockDim.x) lut[k] = p.sigma_lut[k];
-        T.lut = lut;
+        for (int k = threadIdx.x; k <= p.lut_max; k += blockDim.x) lut[k] = gmem(p.sigma_lut)[k];
+        T.lut_lds = lut;
+        T.lut_glb = nullptr;
     } else {
-        T.lut = p.sigma_lut;
+        T.lut_lds = nullptr;
+        T.lut_glb = gmem(p.sigma_lut);
     }
     __syncthreads();
     return T;
@@ -969,7 +1029,7 @@ __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ p
             }
             D = r.D;
             float Tv = 0.0f;
-            const bool vol = march<FULL, COUNT>(p, T, r, true, rng, Tv, Pv, cnt);
+            const bool vol = march<FULL, COUNT, ER_BATCH_PRIMARY>(p, T, r, true, rng, Tv, Pv, cnt);
             float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
             const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
             int type = -1;
@@ -1104,7 +1164,7 @@ __global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp
         rng.s0 = job.s0;
         rng.s1 = job.s1;
         float Tt; f3 P;
-        const bool hit = march<FULL, COUNT>(p, T, r, false, rng, Tt, P, cnt);
+        const bool hit = march<FULL, COUNT, ER_BATCH_SHADOW>(p, T, r, false, rng, Tt, P, cnt);
         vis[2 * job.event + job.slot] = hit ? 0 : 1;
     }
     flush_counters<COUNT>(cnt, true, 0, counters);
