@@ -104,6 +104,10 @@ struct DevParams {
     er_plf      glossiness;
     er_color_tf diffuse, specular, emission;
     er_color_tf tex_gradients[4];
+    /* node-table transfer function (er_set_tf_lut): when set, opacity and diffuse colour come
+     * from ER_TF_LUT_NODES nodes x 4 channels (opacity, diffuse X, Y, Z) instead of the PLFs */
+    const float* tf_lut;
+    float tf_lut_scale;              /* (ER_TF_LUT_NODES - 1) / intensity range */
 };
 
 /* ---- small vector type; component-wise, reference operand order ---------------------- */
@@ -187,6 +191,38 @@ __device__ __forceinline__ float plf_eval(const er_plf& p, float x)
 __device__ __forceinline__ f3 ctf_eval(const er_color_tf& c, float x)
 {
     return mk(plf_eval(c.plf[0], x), plf_eval(c.plf[1], x), plf_eval(c.plf[2], x));
+}
+
+/* Node-table transfer function: node n sits at intensity n / scale; between nodes the value
+ * is linear, below zero it is clamped to zero (the derivative there is zero). */
+__host__ __device__ inline void tf_lut_coord(float I, float scale, int& b, float& f)
+{
+    const float x = fminf(I * scale, (float)(ER_TF_LUT_NODES - 1));
+    b = (int)x;
+    if (b > ER_TF_LUT_NODES - 2) b = ER_TF_LUT_NODES - 2;
+    f = x - (float)b;
+}
+__host__ __device__ inline float tf_lut_raw(const float* lut, int b, float f, int ch)
+{
+    return (1.0f - f) * lut[b * 4 + ch] + f * lut[(b + 1) * 4 + ch];
+}
+__host__ __device__ inline float tf_lut_eval(const float* lut, float scale, float I, int ch)
+{
+    int b; float f;
+    tf_lut_coord(I, scale, b, f);
+    return fmaxf(tf_lut_raw(lut, b, f, ch), 0.0f);
+}
+__device__ __forceinline__ float opacity_at(const DevParams& p, float I)
+{
+    if (p.tf_lut) return tf_lut_eval(p.tf_lut, p.tf_lut_scale, I, 0);
+    return plf_eval(p.opacity, I);
+}
+__device__ __forceinline__ f3 diffuse_at(const DevParams& p, float I)
+{
+    if (p.tf_lut)
+        return mk(tf_lut_eval(p.tf_lut, p.tf_lut_scale, I, 1), tf_lut_eval(p.tf_lut, p.tf_lut_scale, I, 2),
+                  tf_lut_eval(p.tf_lut, p.tf_lut_scale, I, 3));
+    return ctf_eval(p.diffuse, I);
 }
 
 /* DevParams lives in device memory, so the compiler sees the pointers stored in it as generic

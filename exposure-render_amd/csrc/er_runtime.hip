@@ -49,6 +49,12 @@ extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm,
 extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lut, int lut_max, float* out, int n,
                                              hipStream_t st);
 extern "C" hipError_t erd_launch_gm_max(const DevParams* p, unsigned int* out, hipStream_t st);
+extern "C" size_t erd_adj_partials_per_block(void);
+extern "C" hipError_t erd_launch_adjoint(const DevParams* p, int mode, const float* lut_eval, const float* sigma_eval,
+                                         const float4* adj, float4* value, float* partials, uint32_t* s1, uint32_t* s2,
+                                         int W, int H, size_t lds_bytes, hipStream_t st);
+extern "C" hipError_t erd_launch_adj_filter_t(const float* dldr, float4* out, int W, int H, float weight, hipStream_t st);
+extern "C" hipError_t erd_launch_adj_reduce(const float* partials, int nblocks, double* out, hipStream_t st);
 
 namespace {
 
@@ -186,6 +192,9 @@ struct Tracer {
     int built_volume = -1;
     float gm_max = 0.0f;
     int counting = 0;
+    std::vector<float> tf_lut;       /* node table (er_set_tf_lut), empty when the PLFs are used */
+    float tf_scale = 0.0f;
+    DevBuf<float> tf_lut_dev;
     hipStream_t stream = nullptr;
     ~Tracer()
     {
@@ -475,12 +484,22 @@ int build_params(Tracer& t)
     P.opacity = d.opacity; P.glossiness = d.glossiness;
     P.diffuse = d.diffuse; P.specular = d.specular; P.emission = d.emission;
 
+    /* node-table transfer function */
+    P.tf_lut = nullptr;
+    P.tf_lut_scale = t.tf_scale;
+    if (!t.tf_lut.empty()) {
+        if ((st = t.tf_lut_dev.alloc(t.tf_lut.size()))) return st;
+        HIP_TRY(hipMemcpy(t.tf_lut_dev.ptr, t.tf_lut.data(), t.tf_lut.size() * sizeof(float), hipMemcpyHostToDevice));
+        P.tf_lut = t.tf_lut_dev.ptr;
+    }
+
     /* exact opacity LUT over the intensities the volume can produce */
     const int lut_n = (int)vol->vmax + 1;
     std::vector<float> lut(lut_n);
     float omax = 0.0f;
     for (int i = 0; i < lut_n; ++i) {
-        const float o = plf_eval_host(d.opacity, (float)i);
+        const float o = t.tf_lut.empty() ? plf_eval_host(d.opacity, (float)i)
+                                         : erd::tf_lut_eval(t.tf_lut.data(), t.tf_scale, (float)i, 0);
         lut[i] = s.density_scale * o;
         omax = std::max(omax, o);
     }
@@ -881,6 +900,127 @@ ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (st) return st;
     if (se != hipSuccess) return set_error(ER_ERROR, "%s (hipStreamSynchronize)", hipGetErrorString(se));
+    return ER_OK;
+}
+
+ER_API int er_set_tf_lut(int tracer_id, const float* lut, float range)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if (lut) {
+        if (!(range > 0.0f) || !std::isfinite(range)) return set_error(ER_ERROR, "node-table range must be positive");
+        t->tf_lut.assign(lut, lut + ER_TF_LUT_NODES * 4);
+        t->tf_scale = (float)(ER_TF_LUT_NODES - 1) / range;
+    } else {
+        t->tf_lut.clear();
+        t->tf_scale = 0.0f;
+    }
+    t->desc.no_iterations = 0;
+    ++g_scene_version;
+    return ER_OK;
+}
+
+/* checks shared by the adjoint entry points; builds the tracer's parameters */
+static int adjoint_setup(Tracer& t, const uint32_t* s1, const uint32_t* s2, size_t count, int passes)
+{
+    int st;
+    if (t.tf_lut.empty()) return set_error(ER_ERROR, "the adjoint needs a node-table transfer function (er_set_tf_lut)");
+    const er_render_settings& s = t.desc.settings;
+    if (s.estimator != ER_ESTIMATOR_FULL || s.shading_type != ER_SHADING_PHASE_ONLY || s.sampler != ER_SAMPLER_RAYMARCH)
+        return set_error(ER_ERROR, "the adjoint supports the full estimator with phase-function shading and the ray-march sampler");
+    if (passes < 1) return set_error(ER_ERROR, "the adjoint needs at least one pass");
+    if (!s1 || !s2) return set_error(ER_ERROR, "seed pointer is NULL");
+    if (count < (size_t)t.W * t.H) return set_error(ER_ERROR, "need %zu seeds, got %zu", (size_t)t.W * t.H, count);
+    if ((st = build_params(t))) return st;
+    if (!t.lut_lds) return set_error(ER_ERROR, "the adjoint needs the opacity table in LDS (volume maximum <= 8191)");
+    return ER_OK;
+}
+
+ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
+                          const float* dL_drunning_xyz, double* dL_dlut)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if ((st = adjoint_setup(*t, seeds1, seeds2, count, passes))) return st;
+    if (!dL_drunning_xyz || !dL_dlut) return set_error(ER_ERROR, "adjoint pointer is NULL");
+    const size_t n = (size_t)t->W * t->H;
+    const int nblocks = ((t->W + 15) / 16) * ((t->H + 15) / 16);
+    const size_t per_block = erd_adj_partials_per_block();
+    DevBuf<uint32_t> d1, d2;
+    DevBuf<float> dldr, partials;
+    DevBuf<float4> adj;
+    DevBuf<double> out;
+    if ((st = d1.alloc(n)) || (st = d2.alloc(n)) || (st = dldr.alloc(n * 3)) || (st = adj.alloc(n)) ||
+        (st = partials.alloc((size_t)nblocks * per_block)) || (st = out.alloc(per_block)))
+        return st;
+    HIP_TRY(hipMemcpy(d1.ptr, seeds1, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d2.ptr, seeds2, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dldr.ptr, dL_drunning_xyz, n * 12, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(out.ptr, 0, per_block * sizeof(double)));
+    const size_t lds = t->lds_bytes;
+    float cur_w = -1.0f;
+    /* the running estimate after N passes is the mean of the filtered frames of passes 2..N
+     * (the first pass is discarded, Q1), or the first frame when N == 1 */
+    for (int k = 1; k <= passes; ++k) {
+        const float w = passes == 1 ? 1.0f : (k >= 2 ? 1.0f / (float)(passes - 1) : 0.0f);
+        if (w != cur_w) {
+            HIP_TRY(erd_launch_adj_filter_t(dldr.ptr, adj.ptr, t->W, t->H, w, t->stream));
+            cur_w = w;
+        }
+        HIP_TRY(erd_launch_adjoint(t->params.ptr, 0, nullptr, nullptr, adj.ptr, nullptr, partials.ptr, d1.ptr, d2.ptr,
+                                   t->W, t->H, lds, t->stream));
+        if (w != 0.0f) HIP_TRY(erd_launch_adj_reduce(partials.ptr, nblocks, out.ptr, t->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(dL_dlut, out.ptr, per_block * sizeof(double), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return ER_OK;
+}
+
+ER_API int er_adjoint_value(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
+                            const float* eval_lut, float* running_xyz)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if ((st = adjoint_setup(*t, seeds1, seeds2, count, passes))) return st;
+    if (!eval_lut || !running_xyz) return set_error(ER_ERROR, "pointer is NULL");
+    const size_t n = (size_t)t->W * t->H;
+    const DevParams& P = t->host_params;
+    std::vector<float> sig((size_t)P.lut_max + 1);
+    for (int i = 0; i <= P.lut_max; ++i)
+        sig[i] = P.density_scale * erd::tf_lut_eval(eval_lut, t->tf_scale, (float)i, 0);
+    DevBuf<uint32_t> d1, d2, qc;
+    DevBuf<float> lut_e, sig_e;
+    DevBuf<float4> frame, running;
+    DevBuf<uchar4> display;
+    if ((st = d1.alloc(n)) || (st = d2.alloc(n)) || (st = qc.alloc(4)) || (st = lut_e.alloc(ER_TF_LUT_NODES * 4)) ||
+        (st = sig_e.alloc(sig.size())) || (st = frame.alloc(n)) || (st = running.alloc(n)) || (st = display.alloc(n)))
+        return st;
+    HIP_TRY(hipMemcpy(d1.ptr, seeds1, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d2.ptr, seeds2, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(lut_e.ptr, eval_lut, ER_TF_LUT_NODES * 4 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sig_e.ptr, sig.data(), sig.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(running.ptr, 0, n * sizeof(float4)));
+    /* value mode reads every sample: all occupancy bits set, plus the decision table */
+    const size_t lds = (size_t)P.mc_words * 4 + ((size_t)P.lut_max + 1) * sizeof(float);
+    for (int k = 0; k < passes; ++k) {
+        HIP_TRY(erd_launch_adjoint(t->params.ptr, 1, lut_e.ptr, sig_e.ptr, nullptr, frame.ptr, nullptr, d1.ptr, d2.ptr,
+                                   t->W, t->H, lds, t->stream));
+        HIP_TRY(erd_launch_post(t->params.ptr, frame.ptr, running.ptr, display.ptr, k, qc.ptr, t->W, t->H, t->stream));
+    }
+    std::vector<float4> host(n);
+    HIP_TRY(hipMemcpyAsync(host.data(), running.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    for (size_t i = 0; i < n; ++i) {
+        running_xyz[3 * i] = host[i].x;
+        running_xyz[3 * i + 1] = host[i].y;
+        running_xyz[3 * i + 2] = host[i].z;
+    }
     return ER_OK;
 }
 

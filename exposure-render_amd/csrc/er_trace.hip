@@ -1,921 +1,27 @@
 /*
- * er_trace.hip - the per-pixel single-scattering estimator kernel for gfx950.
+ * er_trace.hip - the forward kernels of one progressive pass on gfx950.
  *
- * One thread per pixel and pass (KrnlSingleScattering, Source/singlescattering.cuh:27-38).
- * A 256-thread workgroup covers a 16 x 16 pixel tile; each wave64 an 8 x 8 sub-tile, so the
- * 64 rays of a wave start from neighbouring pixels and march through neighbouring cells.
- * Workgroups are remapped so that each of the 8 XCDs (which receive workgroups round-robin)
- * renders a contiguous band of tiles and keeps that band's volume cells in its own L2.
+ * A pass is a short pipeline of kernels over the whole film (KrnlSingleScattering,
+ * Source/singlescattering.cuh:27-38, split where the paths diverge):
+ *   k_primary  persistent waves take 8 x 8 pixel tiles from an atomic counter; each lane
+ *              generates its camera ray and marches it (SampleVolume, raymarching.h:30-71);
+ *              pixels without a volume event are final, the others are appended to a
+ *              scatter queue (wave-aggregated atomics);
+ *   k_shade    shades each queued event (UniformSampleOneLight, transport.h:115-156) and
+ *              defers its shadow-ray marches to a job queue;
+ *   k_shadow   marches each shadow ray (ScatterEventInVolume, raymarching.h:73-113);
+ *   k_finish   combines the contributions with their visibility into the frame estimate.
  * RNG state lives in registers for the whole path and is written back once (the reference
- * read-modify-writes it in global memory on every draw, Source/rng.h:37-38).
+ * read-modify-writes it in global memory on every draw, Source/rng.h:37-38); a deferred
+ * shadow march takes the RNG state it starts from along and the shading kernel advances past
+ * its two draws, so every path consumes exactly the reference draw sequence.
  *
  * Template parameters: FULL selects the full estimator (shading-mode selection, both halves
  * of MIS, clipping objects, Woodcock sampler); COUNT adds the sample counters.
  */
-#include "er_device.h"
-
-#include <algorithm>
+#include "er_path.h"
 
 namespace erd {
-
-struct Cnt {
-    uint32_t primary, shadow, shading, pfetch, sfetch, scatter, shadow_rays, iters, skipped, undone;
-};
-
-struct Isect {
-    bool Valid, Front;
-    float NearT;
-    f3 P, N;
-    float U, V;
-};
-
-__device__ __forceinline__ void isect_init(Isect& I)
-{
-    I.Valid = false; I.Front = true; I.NearT = 0.0f;
-    I.P = mk(0.0f, 0.0f, 0.0f); I.N = mk(0.0f, 0.0f, 0.0f); I.U = 0.0f; I.V = 0.0f;
-}
-
-/* ---- clipping objects (full mode; box.h:116-120, sphere.h:147-150, plane.h:90-93,
- *      cylinder.h:96-99) -------------------------------------------------------------- */
-__device__ __forceinline__ bool clipped(const DevParams& p, f3 P)
-{
-    for (int k = 0; k < p.clip_count; ++k) {
-        const DevClip& c = p.clips[k];
-        const f3 Pl = xform_point(c.inv_tm, P);
-        bool inside;
-        if (c.type == ER_SHAPE_BOX) {
-            const float h0 = 0.5f * c.size[0], h1 = 0.5f * c.size[1], h2 = 0.5f * c.size[2];
-            inside = Pl.x > -h0 && Pl.x < h0 && Pl.y > -h1 && Pl.y < h1 && Pl.z > -h2 && Pl.z < h2;
-        } else if (c.type == ER_SHAPE_SPHERE) {
-            inside = len(Pl) < c.outer_radius;
-        } else if (c.type == ER_SHAPE_CYLINDER) {
-            inside = sqrtf((Pl.x * Pl.x) + (Pl.z * Pl.z)) < c.outer_radius && fabsf(Pl.y) < (0.5f * c.size[1]);
-        } else {
-            inside = Pl.z > 0.0f;
-        }
-        if (c.invert ? !inside : inside) return true;
-    }
-    return false;
-}
-
-/* dense samples per march iteration (loads in flight together) in the primary and shadow kernels */
-#ifndef ER_BATCH_PRIMARY
-#define ER_BATCH_PRIMARY 6
-#endif
-#ifndef ER_BATCH_SHADOW
-#define ER_BATCH_SHADOW 8
-#endif
-
-/* ---- free-flight sampling --------------------------------------------------------------- */
-
-/* Per-block copies of the macrocell occupancy bits and (when it fits) the opacity LUT. */
-struct Tables {
-    const uint32_t* bits;   /* bit m set <=> macrocell m holds a sample with sigma > 0 */
-    /* DensityScale * Opacity(i): in LDS when it fits (lut_lds), else in global memory */
-    const float* lut_lds;
-    const __attribute__((address_space(1))) float* lut_glb;
-    __device__ __forceinline__ float lut(int i) const { return lut_glb ? lut_glb[i] : lut_lds[i]; }
-};
-
-__device__ __forceinline__ bool mc_occupied(const Tables& T, int m) { return (T.bits[m >> 5] >> (m & 31)) & 1u; }
-
-__device__ __forceinline__ int mc_axis(float L, int shift, int dim)
-{
-    return min(max(((int)floorf(L)) >> shift, 0), dim - 1);
-}
-
-/* sigma of one sample (Woodcock path) */
-template<int FULL, int COUNT>
-__device__ __forceinline__ float sample_sigma(const DevParams& p, const Tables& T, f3 Ps, bool primary, Cnt& cnt)
-{
-    const Lattice l = to_lattice(p, Ps);
-    const int m = (mc_axis(l.Lz, p.mc_shift, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(l.Ly, p.mc_shift, p.mc_dim[1])) * p.mc_dim[0] +
-                  mc_axis(l.Lx, p.mc_shift, p.mc_dim[0]);
-    if (!mc_occupied(T, m)) return 0.0f;
-    if (FULL && p.clip_count > 0 && clipped(p, Ps)) return 0.0f;
-    if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-    return T.lut(min((int)trilinear_cell(p, l), p.lut_max));
-}
-
-/* Advance t by repeated float additions of b while t < t_lim, without performing them one by
- * one: returns the number of additions n and sets t to the value after the last one and t_prev
- * to the value before it - bit-identical to `while (t < t_lim) { t_prev = t; t += b; ++n; }`.
- * Within one binade [2^e, 2^(e+1)) of t every value is an integer multiple A of the spacing
- * u = 2^(e-23), so t + b rounds to (A + D) u with the constant D = round(b / u) unless b / u
- * has fractional part exactly 1/2 (then the rounding depends on the parity of A and the
- * addition is done plainly). The additions inside a binade are counted with integer arithmetic;
- * the one that crosses into the next binade is done as a float addition. Requires b > 0 and
- * t_lim + b > t_lim (the march checks it), so D >= 1 wherever t < t_lim. */
-__device__ __forceinline__ uint32_t skip_additions(float& t, float b, float t_lim, float& t_prev)
-{
-    uint32_t n = 0;
-    while (t < t_lim) {
-        const uint32_t tb = __float_as_uint(t);
-        const int e = (int)((tb >> 23) & 0xffu);
-        if (e <= 24 || e >= 254 || (tb >> 31)) { t_prev = t; t += b; ++n; continue; }
-        const float u = __uint_as_float((uint32_t)(e - 23) << 23);
-        const float inv_u = __uint_as_float((uint32_t)(254 - (e - 23)) << 23);   /* 1 / u, exact */
-        const float q = b * inv_u;                                                 /* exact: power-of-2 scaling */
-        const float fq = floorf(q);
-        const float fr = q - fq;
-        if (fr == 0.5f || q >= 8388608.0f) { t_prev = t; t += b; ++n; continue; }
-        const uint32_t D = (uint32_t)fq + (fr > 0.5f ? 1u : 0u);
-        const uint32_t A = (tb & 0x007fffffu) | 0x00800000u;                      /* t / u */
-        const uint32_t room = 0x00ffffffu - A;                                     /* additions stay below 2^24 u */
-        uint32_t kbin = (uint32_t)((float)room * __builtin_amdgcn_rcpf((float)D));
-        while ((uint64_t)kbin * D > room) --kbin;
-        while ((uint64_t)(kbin + 1) * D <= room) ++kbin;
-        /* J = number of j >= 0 with (A + j D) u < t_lim, found from an estimate and exact checks */
-        uint32_t J = (uint32_t)fminf((t_lim - t) * __builtin_amdgcn_rcpf((float)D * u), 16777216.0f);
-        if (J > kbin + 1) J = kbin + 1;
-        while (J > 0 && (float)(A + (J - 1) * D) * u >= t_lim) --J;
-        while (J <= kbin && (float)(A + J * D) * u < t_lim) ++J;
-        if (J <= kbin) {
-            n += J;
-            t_prev = (float)(A + (J - 1) * D) * u;
-            t = (float)(A + J * D) * u;
-            break;
-        }
-        /* every addition inside the binade is taken; the next one crosses into the next binade */
-        n += kbin + 1;
-        t_prev = (float)(A + kbin * D) * u;
-        t = t_prev + b;
-    }
-    return n;
-}
-
-/* SampleVolume (raymarching.h:30-71) for primary rays, ScatterEventInVolume
- * (raymarching.h:73-113) for shadow rays: jittered fixed-step tracking with threshold
- * S = -log(u) / DensityScale; T is reported one step beyond P (Q13); the primary march ends
- * at t >= tmax, the shadow march at t > tmax (Q14). */
-template<int FULL, int COUNT, int K = 1>
-__device__ bool march(const DevParams& p, const Tables& T, const Ray& r, bool primary, Rng& rng, float& Tout, f3& Pout, Cnt& cnt)
-{
-    float nearT, farT;
-    if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return false;
-    float MinT = fmaxf(nearT, r.MinT);
-    const float MaxT = fminf(farT, r.MaxT);
-
-    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
-        /* delta tracking against the global majorant DS^2 * max opacity */
-        const float mu = p.woodcock_majorant;
-        if (!(mu > 0.0f)) return false;
-        float t = MinT;
-        for (;;) {
-            t += gmem(p.neglog)[rng.next_bits()] / mu;
-            if (primary ? (t >= MaxT) : (t > MaxT)) return false;
-            const f3 Ps = add(r.O, scl(t, r.D));
-            if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; }
-            const float sigma = sample_sigma<FULL, COUNT>(p, T, Ps, primary, cnt) * p.density_scale;
-            if (rng.u() * mu < sigma) { Tout = t; Pout = Ps; return true; }
-        }
-    }
-
-    const float S = gmem(p.neglog)[rng.next_bits()] / p.density_scale;
-    const float StepSize = (primary ? p.step_primary : p.step_shadow) * p.minstep;
-    MinT += rng.u() * StepSize;
-    /* a march that could not advance (zero-size volume, Q2, or a step below the spacing of
-     * floats at MaxT) would not terminate in the reference; it is reported as no event */
-    if (!(StepSize > 0.0f) || !(MaxT + StepSize > MaxT)) return false;
-    /* lattice-space direction, used only to estimate where the ray leaves a macrocell */
-    const float Ldx = (r.D.x * p.invsize[0]) * p.resf[0];
-    const float Ldy = (r.D.y * p.invsize[1]) * p.resf[1];
-    const float Ldz = (r.D.z * p.invsize[2]) * p.resf[2];
-    const float Lmax = fmaxf(fabsf(Ldx), fmaxf(fabsf(Ldy), fabsf(Ldz)));
-    /* reciprocals for the exit estimate only (hardware approximation: the skip is verified) */
-    const float iLdx = __builtin_amdgcn_rcpf(Ldx), iLdy = __builtin_amdgcn_rcpf(Ldy), iLdz = __builtin_amdgcn_rcpf(Ldz);
-    const float margin = 1.0e-3f * __builtin_amdgcn_rcpf(Lmax);
-    const int sh = p.mc_shift;
-    float Sum = 0.0f;
-    f3 Ps = mk(0.0f, 0.0f, 0.0f);
-    while (Sum < S) {
-        Ps = add(r.O, scl(MinT, r.D));
-        if (primary ? (MinT >= MaxT) : (MinT > MaxT)) return false;
-        if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
-        const Lattice l = to_lattice(p, Ps);
-        const int mxi = mc_axis(l.Lx, sh, p.mc_dim[0]);
-        const int myi = mc_axis(l.Ly, sh, p.mc_dim[1]);
-        const int mzi = mc_axis(l.Lz, sh, p.mc_dim[2]);
-        const int mc = (mzi * p.mc_dim[1] + myi) * p.mc_dim[0] + mxi;
-        if (K == 1 && mc_occupied(T, mc)) {
-            float SigmaT = 0.0f;
-            if (!(FULL && p.clip_count > 0 && clipped(p, Ps))) {
-                if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-                SigmaT = T.lut(min((int)trilinear_cell(p, l), p.lut_max));
-            }
-            Sum += SigmaT * StepSize;
-            MinT += StepSize;
-            continue;
-        }
-        if (K > 1 && mc_occupied(T, mc)) {
-            /* K consecutive samples at once: their positions follow from the same sequence of
-             * MinT additions and do not depend on the values, so all K cell loads are issued
-             * before the first value is used. The values are then consumed in order, with the
-             * loop's own end and threshold tests between them; loads past the point where the
-             * loop stops are discarded (their addresses are clamped into the cell array). A
-             * sample in an empty macrocell has sigma exactly 0 and is consumed without a fetch. */
-            CellFetch f[K];
-            bool live[K];
-            float tk[K];
-            int kn = K;
-            tk[0] = MinT;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                f3 Pk = Ps;
-                Lattice lk = l;
-                bool occ = true;
-                if (k > 0) {
-                    tk[k] = tk[k - 1] + StepSize;
-                    if (kn == K && (primary ? (tk[k] >= MaxT) : (tk[k] > MaxT))) kn = k;
-                    Pk = add(r.O, scl(tk[k], r.D));
-                    lk = to_lattice(p, Pk);
-                    occ = mc_occupied(T, (mc_axis(lk.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(lk.Ly, sh, p.mc_dim[1])) * p.mc_dim[0] +
-                                             mc_axis(lk.Lx, sh, p.mc_dim[0]));
-                }
-                live[k] = occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
-                f[k] = cell_fetch(p, lk);
-            }
-            bool done = false;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (k > 0) {
-                    if (k >= kn) return false;
-                    if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
-                }
-                float SigmaT = 0.0f;
-                if (live[k]) {
-                    if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-                    SigmaT = T.lut(min((int)cell_value(f[k]), p.lut_max));
-                }
-                Sum += SigmaT * StepSize;
-                MinT += StepSize;
-                if (!(Sum < S)) { Ps = add(r.O, scl(tk[k], r.D)); done = true; break; }
-            }
-            if (done) break;
-            continue;
-        }
-        /* Empty macrocell: this sample adds exactly 0. Skip the following samples that stay in
-         * the same macrocell without computing them, keeping the exact sequence of MinT
-         * additions. Each operation of P = O + t D, (P - Min) * InvSize * Res is monotone in t,
-         * so if the first and the last skipped sample lie in this macrocell's lattice box,
-         * every sample between them does too; the last one is checked with the same
-         * expression as above, and the skip is undone if it left the box. */
-        MinT += StepSize;
-        const float big = 3.0e38f;
-        const float lo_x = mxi == 0 ? -big : (float)(mxi << sh), hi_x = mxi == p.mc_dim[0] - 1 ? big : (float)((mxi + 1) << sh);
-        const float lo_y = myi == 0 ? -big : (float)(myi << sh), hi_y = myi == p.mc_dim[1] - 1 ? big : (float)((myi + 1) << sh);
-        const float lo_z = mzi == 0 ? -big : (float)(mzi << sh), hi_z = mzi == p.mc_dim[2] - 1 ? big : (float)((mzi + 1) << sh);
-        const float tx = Ldx > 0.0f ? (hi_x - l.Lx) * iLdx : (Ldx < 0.0f ? (lo_x - l.Lx) * iLdx : big);
-        const float ty = Ldy > 0.0f ? (hi_y - l.Ly) * iLdy : (Ldy < 0.0f ? (lo_y - l.Ly) * iLdy : big);
-        const float tz = Ldz > 0.0f ? (hi_z - l.Lz) * iLdz : (Ldz < 0.0f ? (lo_z - l.Lz) * iLdz : big);
-        const float dt_exit = fminf(tx, fminf(ty, tz)) - margin;
-        const float t_lim = fminf((MinT - StepSize) + dt_exit, MaxT);
-        const float t_first = MinT;
-        float t_prev = MinT;
-        uint32_t n = skip_additions(MinT, StepSize, t_lim, t_prev);
-        if (n > 0) {
-            const Lattice lp = to_lattice(p, add(r.O, scl(t_prev, r.D)));
-            if (mc_axis(lp.Lx, sh, p.mc_dim[0]) != mxi || mc_axis(lp.Ly, sh, p.mc_dim[1]) != myi ||
-                mc_axis(lp.Lz, sh, p.mc_dim[2]) != mzi) {
-                MinT = t_first;
-                n = 0;
-                if (COUNT) cnt.undone++;
-            }
-        }
-        if (COUNT) { if (primary) cnt.primary += n; else cnt.shadow += n; cnt.skipped += n; }
-    }
-    Tout = MinT;
-    Pout = Ps;
-    return true;
-}
-
-/* ---- shapes (plane.h:41-88, disk.h, ring.h, box.h:51-83, sphere.h:84-140) ------------ */
-__device__ __forceinline__ void intersect_plane(const Ray& r, bool one_sided, Isect& I)
-{
-    if (fabsf(r.O.z - r.D.z) < ER_RAY_EPS) return;   /* Q16 */
-    I.NearT = (0.0f - r.O.z) / r.D.z;
-    if (I.NearT < r.MinT || I.NearT > r.MaxT) return;
-    I.P = ray_at(r, I.NearT);
-    I.U = I.P.x; I.V = I.P.y;
-    I.N = mk(0.0f, 0.0f, 1.0f);
-    if (one_sided && r.D.z >= 0.0f) { I.Front = false; I.N = mk(0.0f, 0.0f, -1.0f); }
-    I.Valid = true;
-}
-
-__device__ __forceinline__ void intersect_sphere(const Ray& r, float radius, bool need_uv, Isect& I)
-{
-    const float a = dot(r.D, r.D);
-    const float b = 2.0f * dot(r.D, r.O);
-    const float c = dot(r.O, r.O) - (radius * radius);
-    const float disc = b * b - 4.0f * a * c;
-    if (disc < 0.0f) return;
-    const float distSqrt = sqrtf(disc);
-    const float q = (b < 0.0f) ? (-b - distSqrt) / 2.0f : (-b + distSqrt) / 2.0f;
-    float t0 = q / a, t1 = c / q;
-    if (t0 > t1) { const float tmp = t0; t0 = t1; t1 = tmp; }
-    if (t0 >= r.MinT && t0 < r.MaxT) I.NearT = t0;
-    else if (t1 >= r.MinT && t1 < r.MaxT) I.NearT = t1;
-    else return;
-    if (I.NearT < r.MinT || I.NearT > r.MaxT) return;
-    I.Valid = true;
-    I.P = ray_at(r, I.NearT);
-    I.N = nrm(I.P);
-    if (need_uv) {   /* SphericalToUV (geometry.h:70-83) */
-        const f3 V = nrm(I.P);
-        float phi = erm::atan2f_(V.z, V.x);
-        phi = (phi < 0.0f) ? phi + (2.0f * ER_PI_F) : phi;
-        const float theta = erm::acosf_(clampf_(V.y, -1.0f, 1.0f));
-        I.U = ER_INV_TWO_PI_F * phi;
-        I.V = 1.0f - (ER_INV_PI_F * theta);
-    }
-}
-
-__device__ __forceinline__ void intersect_shape(const DevLight& s, const Ray& r, bool need_uv, Isect& I)
-{
-    if (s.type == ER_SHAPE_PLANE) {
-        intersect_plane(r, s.one_sided != 0, I);
-        const float hx = 0.5f * s.size[0], hy = 0.5f * s.size[1];
-        if (I.Valid && (I.U < -hx || I.U > hx || I.V < -hy || I.V > hy)) I.Valid = false;
-        I.U /= s.size[0]; I.V /= s.size[1];
-        I.U += 0.5f; I.V += 0.5f;
-        I.U = 1.0f - I.U;
-    } else if (s.type == ER_SHAPE_DISK || s.type == ER_SHAPE_RING) {
-        intersect_plane(r, s.one_sided != 0, I);
-        const float l = sqrtf(I.U * I.U + I.V * I.V);
-        if (s.type == ER_SHAPE_DISK) { if (I.Valid && l > s.outer_radius) I.Valid = false; }
-        else if (I.Valid && (l < s.inner_radius || l > s.outer_radius)) I.Valid = false;
-        const float dia = 2.0f * s.outer_radius;
-        I.U /= dia; I.V /= dia;
-        I.U += 0.5f; I.V += 0.5f;
-        I.U = 1.0f - I.U;
-    } else if (s.type == ER_SHAPE_BOX) {
-        const f3 mx = mk(0.5f * s.size[0], 0.5f * s.size[1], 0.5f * s.size[2]);
-        const f3 mn = mk(-0.5f * s.size[0], -0.5f * s.size[1], -0.5f * s.size[2]);
-        float nT, fT;
-        if (intersect_box(r, mn, mx, nT, fT)) {
-            I.NearT = nT; I.Valid = true;
-            I.P = ray_at(r, nT);
-            float n[3] = {0.0f, 0.0f, 0.0f};
-            const float pp[3] = {I.P.x, I.P.y, I.P.z}, lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
-            for (int i = 0; i < 3; ++i) {
-                if (pp[i] <= lo[i] + 0.0001f) n[i] = -1.0f;
-                if (pp[i] >= hi[i] - 0.0001f) n[i] = 1.0f;
-            }
-            I.N = mk(n[0], n[1], n[2]);
-        }
-    } else if (s.type == ER_SHAPE_SPHERE) {
-        intersect_sphere(r, s.outer_radius, need_uv, I);
-    }
-}
-
-/* EvaluateTexture / EvaluateProcedural (textures.h:26-111) */
-__device__ __forceinline__ bool texture_needs_uv(const DevParams& p, int id)
-{
-    if (id < 0 || id >= ER_LIST_CAPACITY) return false;
-    const DevTexture& T = p.textures[id];
-    return T.valid && !(T.type == ER_TEXTURE_PROCEDURAL && T.proc_type == ER_PROCEDURAL_UNIFORM);
-}
-
-__device__ f3 eval_texture(const DevParams& p, int id, float u, float v)
-{
-    if (id < 0 || id >= ER_LIST_CAPACITY || !p.textures[id].valid) return mk(0.0f, 0.0f, 0.0f);
-    const DevTexture& T = p.textures[id];
-    u *= T.repeat[0]; v *= T.repeat[1];
-    u += T.offset[0]; v += 1.0f - T.offset[1];
-    u = u - floorf(u); v = v - floorf(v);
-    u = clampf_(u, 0.0f, 1.0f); v = clampf_(v, 0.0f, 1.0f);
-    if (T.flip[0]) u = 1.0f - u;
-    if (T.flip[1]) v = 1.0f - v;
-    f3 L = mk(0.0f, 0.0f, 0.0f);
-    if (T.type == ER_TEXTURE_PROCEDURAL) {
-        if (T.proc_type == ER_PROCEDURAL_UNIFORM) {
-            L = ld3(T.uniform);
-        } else if (T.proc_type == ER_PROCEDURAL_CHECKER) {
-            const int a = (int)(u * 2.0f), b = (int)(v * 2.0f);
-            const bool c1 = (a % 2 == 0) ? (b % 2 == 0) : (b % 2 != 0);
-            L = c1 ? ld3(T.checker1) : ld3(T.checker2);
-        } else if (T.proc_type == ER_PROCEDURAL_GRADIENT && T.gradient_index >= 0) {
-            L = ctf_eval(p.tex_gradients[T.gradient_index], v);
-        }
-    }
-    return scl(T.output_level, L);
-}
-
-/* ---- shading (shader.h:29-483) ------------------------------------------------------- */
-struct Shader {
-    bool brdf;
-    f3 Nn, Nu, Nv, Kd, Ks;
-    float exponent;
-};
-
-__device__ __forceinline__ f3 to_local(const Shader& s, f3 W) { return mk(dot(W, s.Nu), dot(W, s.Nv), dot(W, s.Nn)); }
-__device__ __forceinline__ f3 to_world(const Shader& s, f3 W)
-{
-    return mk(s.Nu.x * W.x + s.Nv.x * W.y + s.Nn.x * W.z,
-              s.Nu.y * W.x + s.Nv.y * W.y + s.Nn.y * W.z,
-              s.Nu.z * W.x + s.Nv.z * W.y + s.Nn.z * W.z);
-}
-__device__ __forceinline__ bool same_hemi(f3 a, f3 b) { return (a.z * b.z) > 0.0f; }
-
-/* Fresnel::Evaluate + FrDiel with EtaI = 1, EtaT = 15 (shader.h:69-123, transport.h:138) */
-__device__ __forceinline__ float fresnel(float cosi)
-{
-    cosi = clampf_(cosi, -1.0f, 1.0f);
-    const bool entering = cosi > 0.0f;
-    float ei = 1.0f, et = 15.0f;
-    if (!entering) { ei = 15.0f; et = 1.0f; }
-    const float sint = ei / et * sqrtf(fmaxf(0.0f, 1.0f - cosi * cosi));
-    if (sint >= 1.0f) return 1.0f;
-    const float cost = sqrtf(fmaxf(0.0f, 1.0f - sint * sint));
-    const float ci = fabsf(cosi);
-    const float Rparl = ((et * ci) - (ei * cost)) / ((et * ci) + (ei * cost));
-    const float Rperp = ((ei * ci) - (et * cost)) / ((ei * ci) + (et * cost));
-    return (Rparl * Rparl + Rperp * Rperp) / 2.0f;
-}
-
-__device__ __forceinline__ float blinn_pdf(float exponent, f3 wo, f3 wi)   /* shader.h:166-179 */
-{
-    const f3 wh = nrm(add(wo, wi));
-    const float c = fabsf(wh.z);
-    float pdf = ((exponent + 1.0f) * erm::powf_(c, exponent)) / (2.0f * ER_PI_F * 4.0f * dot(wo, wh));
-    if (dot(wo, wh) <= 0.0f) pdf = 0.0f;
-    return pdf;
-}
-
-__device__ f3 microfacet_f(const Shader& s, f3 wo, f3 wi)   /* shader.h:216-263 */
-{
-    const float cO = fabsf(wo.z), cI = fabsf(wi.z);
-    if (cI == 0.0f || cO == 0.0f) return mk(0.0f, 0.0f, 0.0f);
-    f3 wh = add(wi, wo);
-    if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return mk(0.0f, 0.0f, 0.0f);
-    wh = nrm(wh);
-    const float cosH = dot(wi, wh);
-    const float Fr = fresnel(cosH);
-    const float D = (s.exponent + 2.0f) * ER_INV_TWO_PI_F * erm::powf_(fabsf(wh.z), s.exponent);
-    const float NdotWh = fabsf(wh.z), NdotWo = fabsf(wo.z), NdotWi = fabsf(wi.z);
-    const float WOdotWh = absdot(wo, wh);
-    const float G = fminf(1.0f, fminf((2.0f * NdotWh * NdotWo / WOdotWh), (2.0f * NdotWh * NdotWi / WOdotWh)));
-    const float den = 4.0f * cI * cO;
-    return mk((((s.Ks.x * D) * G) * Fr) / den, (((s.Ks.y * D) * G) * Fr) / den, (((s.Ks.z * D) * G) * Fr) / den);
-}
-
-__device__ __forceinline__ float lambert_pdf(f3 wo, f3 wi) { return same_hemi(wo, wi) ? fabsf(wi.z) * ER_INV_PI_F : 0.0f; }
-__device__ __forceinline__ float microfacet_pdf(const Shader& s, f3 wo, f3 wi) { return same_hemi(wo, wi) ? blinn_pdf(s.exponent, wo, wi) : 0.0f; }
-
-template<int FULL>
-__device__ f3 shader_f(const Shader& s, f3 Wo, f3 Wi)
-{
-    if (!s.brdf) return scl(ER_INV_PI_F, s.Kd);   /* IsotropicPhase::F (Q7) */
-    const f3 wo = to_local(s, Wo), wi = to_local(s, Wi);
-    f3 R = mk(0.0f, 0.0f, 0.0f);
-    R = add(R, scl(ER_INV_PI_F, s.Kd));
-    R = add(R, microfacet_f(s, wo, wi));
-    return R;
-}
-
-template<int FULL>
-__device__ float shader_pdf(const Shader& s, f3 Wo, f3 Wi)
-{
-    if (!s.brdf) return FULL ? 0.0795774715459476678844f : (1.0f / 4.0f * 3.141592654f);   /* Q6 in compat */
-    const f3 wo = to_local(s, Wo), wi = to_local(s, Wi);
-    float pdf = 0.0f;
-    pdf += lambert_pdf(wo, wi);
-    pdf += microfacet_pdf(s, wo, wi);
-    return FULL ? 0.5f * pdf : pdf;
-}
-
-/* ConcentricSampleDisk (montecarlo.h:86-137) */
-__device__ __forceinline__ void concentric_disk(float u0, float u1, float& ox, float& oy)
-{
-    float r, theta;
-    const float sx = 2.0f * u0 - 1.0f, sy = 2.0f * u1 - 1.0f;
-    if (sx == 0.0f && sy == 0.0f) { ox = 0.0f; oy = 0.0f; return; }
-    if (sx >= -sy) {
-        if (sx > sy) { r = sx; theta = (sy > 0.0f) ? sy / r : 8.0f + sy / r; }
-        else { r = sy; theta = 2.0f - sx / r; }
-    } else {
-        if (sx <= sy) { r = -sx; theta = 4.0f - sy / r; }
-        else { r = -sy; theta = 6.0f + sx / r; }
-    }
-    theta *= ER_PI_F / 4.0f;
-    float sn, cs;
-    erm::sincosf_(theta, &sn, &cs);
-    ox = r * cs;
-    oy = r * sn;
-}
-
-/* BRDF::SampleF / IsotropicPhase::SampleF (shader.h:297-303,350-374), full mode: the pdf is
- * the one-sample mixture density 0.5 (pdf_L + pdf_M); the phase pdf is 1 / (4 pi). */
-__device__ f3 shader_sample_f(const Shader& s, f3 Wo, f3& Wi, float& pdf, float comp, float d0, float d1)
-{
-    if (!s.brdf) {
-        const float z = 1.0f - 2.0f * d0;
-        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        const float phi = (2.0f * ER_PI_F) * d1;
-        float sn, cs;
-        erm::sincosf_(phi, &sn, &cs);
-        Wi = mk(rr * cs, rr * sn, z);
-        pdf = 0.0795774715459476678844f;
-        return scl(ER_INV_PI_F, s.Kd);
-    }
-    const f3 wo = to_local(s, Wo);
-    f3 wi;
-    if (comp <= 0.5f) {
-        float dx, dy;
-        concentric_disk(d0, d1, dx, dy);
-        wi = mk(dx, dy, sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy)));
-        if (wo.z < 0.0f) wi.z *= -1.0f;
-    } else {
-        const float costheta = erm::powf_(d0, 1.0f / (s.exponent + 1.0f));
-        const float sintheta = sqrtf(fmaxf(0.0f, 1.0f - costheta * costheta));
-        const float phi = d1 * 2.0f * ER_PI_F;
-        float sn, cs;
-        erm::sincosf_(phi, &sn, &cs);
-        f3 wh = mk(sintheta * cs, sintheta * sn, costheta);
-        if (!same_hemi(wo, wh)) wh = neg(wh);
-        const float two_d = 2.0f * dot(wo, wh);
-        wi = add(neg(wo), scl(two_d, wh));
-    }
-    pdf = 0.5f * (lambert_pdf(wo, wi) + microfacet_pdf(s, wo, wi));
-    f3 R = mk(0.0f, 0.0f, 0.0f);
-    R = add(R, scl(ER_INV_PI_F, s.Kd));
-    R = add(R, microfacet_f(s, wo, wi));
-    Wi = to_world(s, wi);
-    return R;
-}
-
-__device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
-
-/* ---- lights (lights.h:28-131) -------------------------------------------------------- */
-__device__ __forceinline__ const DevLight* light_by_id(const DevParams& p, int id)
-{
-    if (id < 0 || id >= ER_LIST_CAPACITY || !p.lights[id].valid) return nullptr;
-    return &p.lights[id];
-}
-
-/* IntersectLights (lights.h:90-113); returns the list position of the nearest hit, or -1 */
-__device__ int intersect_lights(const DevParams& p, const Ray& r, bool respect_visibility, float& T, f3& P, f3& N, f3& Le)
-{
-    float best = 3.402823466e+38f;
-    int hit = -1;
-    for (int i = 0; i < p.light_count; ++i) {
-        const DevLight* L = light_by_id(p, p.light_ids[i]);
-        if (!L) continue;
-        if (respect_visibility && !L->visible) continue;
-        const Ray rt = xform_ray(L->inv_tm, r);
-        Isect I;
-        isect_init(I);
-        intersect_shape(*L, rt, texture_needs_uv(p, L->texture_id), I);
-        if (!I.Valid) continue;
-        const f3 Pw = xform_point(L->tm, I.P);
-        const float t = len(sub(Pw, r.O));
-        if (t < best) {
-            best = t; hit = i; T = t; P = Pw;
-            N = xform_vector(L->tm, I.N);
-            f3 le = I.Front ? scl(L->multiplier, eval_texture(p, L->texture_id, I.U, I.V)) : mk(0.0f, 0.0f, 0.0f);
-            if (L->unit == ER_UNIT_LUX) le = mk(le.x / L->area, le.y / L->area, le.z / L->area);
-            Le = le;
-        }
-    }
-    return hit;
-}
-
-/* IntersectsLight (lights.h:115-131) */
-__device__ bool intersects_any_light(const DevParams& p, const Ray& r)
-{
-    for (int i = 0; i < p.light_count; ++i) {
-        const DevLight* L = light_by_id(p, p.light_ids[i]);
-        if (!L) continue;
-        Isect I;
-        isect_init(I);
-        intersect_shape(*L, xform_ray(L->inv_tm, r), false, I);
-        if (I.Valid) return true;
-    }
-    return false;
-}
-
-/* the shadow ray of Visible (transport.h:50-53) */
-__device__ __forceinline__ Ray shadow_ray(const DevParams& p, f3 P1, f3 P2)
-{
-    const f3 W = nrm(sub(P2, P1));
-    Ray r;
-    r.O = add(P1, scl(ER_RAY_EPS, W));
-    r.D = W;
-    r.MinT = 0.0f;
-    r.MaxT = fminf(len(sub(P2, P1)) - (2.0f * ER_RAY_EPS), p.max_shadow);
-    return r;
-}
-
-/* A shadow ray whose volume march is deferred to k_shadow (48 bytes). */
-struct ShadowJob {
-    float Ox, Oy, Oz, Dx, Dy, Dz, MaxT;
-    uint32_t s0, s1, event, slot, pad;
-};
-
-/* Visible (transport.h:46-56) up to the volume march. Returns 0 (occluded by a light shape),
- * 1 (visible without marching) or 2 (march needed: the ray and the RNG state the march starts
- * from are written to job, and rng is advanced past the march's two draws, raymarching.h:89,95). */
-template<int COUNT>
-__device__ int visible_defer(const DevParams& p, f3 P1, f3 P2, Rng& rng, Cnt& cnt, ShadowJob& job)
-{
-    if (!p.shadows) return 1;
-    const Ray r = shadow_ray(p, P1, P2);
-    if (COUNT) cnt.shadow_rays++;
-    if (intersects_any_light(p, r)) return 0;
-    float nearT, farT;
-    if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return 1;
-    job.Ox = r.O.x; job.Oy = r.O.y; job.Oz = r.O.z;
-    job.Dx = r.D.x; job.Dy = r.D.y; job.Dz = r.D.z;
-    job.MaxT = r.MaxT;
-    job.s0 = rng.s0; job.s1 = rng.s1;
-    (void)rng.next_bits();
-    (void)rng.next_bits();
-    return 2;
-}
-
-/* Visible (transport.h:46-56) -> Intersect (transport.h:30-44) */
-template<int FULL, int COUNT>
-__device__ bool visible(const DevParams& p, const Tables& T, f3 P1, f3 P2, Rng& rng, Cnt& cnt)
-{
-    if (!p.shadows) return true;
-    const Ray r = shadow_ray(p, P1, P2);
-    if (COUNT) cnt.shadow_rays++;
-    if (intersects_any_light(p, r)) return false;
-    float Tt; f3 P;
-    const Tables& Tb = T;
-    if (march<FULL, COUNT>(p, Tb, r, false, rng, Tt, P, cnt)) return false;
-    return true;
-}
-
-/* SampleLightSurface (lights.h:28-41) */
-__device__ void sample_light_surface(const DevParams& p, const DevLight& L, float u0, float u1, float u2, f3& P, f3& N, float& U, float& V)
-{
-    N = mk(0.0f, 0.0f, 1.0f);
-    U = 0.0f; V = 0.0f;
-    if (L.type == ER_SHAPE_PLANE) {
-        P = mk(-0.5f + u0, -0.5f + u1, 0.0f);
-        U = u0; V = u1;
-        P = mul(P, mk(L.size[0], L.size[1], 0.0f));
-    } else if (L.type == ER_SHAPE_DISK) {
-        const float rr = sqrtf(u0);
-        const float th = (2.0f * ER_PI_F) * u1;
-        float sn, cs;
-        erm::sincosf_(th, &sn, &cs);
-        P = mk(rr * cs, rr * sn, 0.0f);
-        U = u0; V = u1;
-        P = scl(L.outer_radius, P);
-    } else if (L.type == ER_SHAPE_RING) {
-        const float inner = L.inner_radius / L.outer_radius;
-        const float rr = inner + (1.0f - inner) * sqrtf(u0);
-        const float th = (2.0f * ER_PI_F) * u1;
-        float sn, cs;
-        erm::sincosf_(th, &sn, &cs);
-        P = mk(rr * cs, rr * sn, 0.0f);
-        P = scl(L.outer_radius, P);
-        U = P.x; V = P.y;
-    } else if (L.type == ER_SHAPE_BOX) {
-        const int side = (int)floorf(u0 * 6.0f);
-        const float h = 0.5f;
-        switch (side) {
-            case 0: P = mk(h, -h + u2, -h + u1); N = mk(1.0f, 0.0f, 0.0f); break;
-            case 1: P = mk(-h, -h + u2, -h + u1); N = mk(-1.0f, 0.0f, 0.0f); break;
-            case 2: P = mk(-h + u1, h, -h + u2); N = mk(0.0f, 1.0f, 0.0f); break;
-            case 3: P = mk(-h + u1, -h, -h + u2); N = mk(0.0f, -1.0f, 0.0f); break;
-            case 4: P = mk(-h + u1, -h + u2, h); N = mk(0.0f, 0.0f, 1.0f); break;
-            default: P = mk(-h + u1, -h + u2, -h); N = mk(0.0f, 0.0f, -1.0f); break;
-        }
-        U = u1; V = u2;
-        P = mul(P, ld3(L.size));
-    } else if (L.type == ER_SHAPE_SPHERE) {
-        const float z = 1.0f - 2.0f * u0;
-        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        const float phi = (2.0f * ER_PI_F) * u1;
-        float sn, cs;
-        erm::sincosf_(phi, &sn, &cs);
-        P = mk(rr * cs, rr * sn, z);
-        N = P;
-        if (texture_needs_uv(p, L.texture_id)) {
-            float ph = erm::atan2f_(P.z, P.x);
-            ph = (ph < 0.0f) ? ph + (2.0f * ER_PI_F) : ph;
-            U = erm::acosf_(clampf_(P.y, -1.0f, 1.0f));
-            V = ph;
-        }
-        P = scl(L.outer_radius, P);
-    }
-    P = xform_point(L.tm, P);
-    N = xform_vector(L.tm, N);
-}
-
-struct Sample { float comp, dir0, dir1, uvw0, uvw1, uvw2, lightnum, film0, film1, lens0, lens1; };
-
-/* EstimateDirectLight (transport.h:58-113) */
-template<int FULL, int COUNT>
-__device__ f3 estimate_direct(const DevParams& p, const Tables& T, const DevLight& L, int light_pos, const Sample& smp, f3 SEP, f3 SEN, f3 Wo,
-                              const Shader& sh, Rng& rng, Cnt& cnt)
-{
-    f3 Ld = mk(0.0f, 0.0f, 0.0f);
-    f3 SP, SN; float su, sv;
-    sample_light_surface(p, L, smp.uvw0, smp.uvw1, smp.uvw2, SP, SN, su, sv);
-    f3 Wi = nrm(sub(SP, SEP));
-    f3 Li = scl(L.multiplier, eval_texture(p, L.texture_id, su, sv));
-    if (L.one_sided && dot(sub(SEP, SP), SN) < 0.0f) Li = mk(0.0f, 0.0f, 0.0f);
-    if (L.unit == ER_UNIT_LUX) Li = mk(Li.x / L.area, Li.y / L.area, Li.z / L.area);
-    f3 F = shader_f<FULL>(sh, Wo, Wi);
-    float BsdfPdf = shader_pdf<FULL>(sh, Wo, Wi);
-    if (!black(Li) && !black(F) && BsdfPdf > 0.0f && visible<FULL, COUNT>(p, T, SEP, SP, rng, cnt)) {
-        const f3 d = sub(SEP, SP);
-        const float LightPdf = dot(d, d) / (absdot(SN, neg(Wi)) * L.area);
-        const float Weight = power_heuristic(LightPdf, BsdfPdf);
-        const f3 FL = mul(F, Li);
-        if (sh.brdf) Ld = add(Ld, scl(absdot(Wi, SEN) * Weight / LightPdf, FL));
-        else if (FULL) Ld = add(Ld, scl(Weight / LightPdf, FL));
-        else Ld = add(Ld, mk(FL.x / LightPdf, FL.y / LightPdf, FL.z / LightPdf));
-    }
-    if (!FULL) return Ld;
-
-    F = shader_sample_f(sh, Wo, Wi, BsdfPdf, smp.comp, smp.dir0, smp.dir1);
-    if (black(F) || BsdfPdf <= 0.0f) return Ld;
-    Ray r2;
-    r2.O = SEP; r2.D = Wi; r2.MinT = 0.0f; r2.MaxT = 1000000.0f;
-    float T2; f3 P2, N2, Le2;
-    const int idx2 = intersect_lights(p, r2, false, T2, P2, N2, Le2);
-    if (idx2 < 0 || idx2 != light_pos) return Ld;
-    if (!black(Le2) && visible<FULL, COUNT>(p, T, SEP, P2, rng, cnt)) {
-        const f3 d = sub(SEP, P2);
-        const float LightPdf = dot(d, d) / (absdot(N2, neg(Wi)) * L.area);
-        const float Weight = power_heuristic(BsdfPdf, LightPdf);
-        const f3 FL = mul(F, Le2);
-        if (sh.brdf) Ld = add(Ld, scl(absdot(Wi, SEN) * Weight / BsdfPdf, FL));
-        else Ld = add(Ld, scl(Weight / BsdfPdf, FL));
-    }
-    return Ld;
-}
-
-/* UniformSampleOneLight (transport.h:115-156) with the full-mode shader selection
- * (design at scatterevent.h:155-212) */
-template<int FULL, int COUNT>
-__device__ f3 uniform_sample_one_light(const DevParams& p, const Tables& T, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt)
-{
-    f3 Ld = mk(0.0f, 0.0f, 0.0f);
-    const float I = intensity(p, SEP);
-    if (COUNT) cnt.shading += 1 + gradient_samples(p);
-    Ld = add(Ld, ctf_eval(p.emission, I));
-    const int count = p.light_count;
-    if (count <= 0) return Ld;
-    const int pos = (int)floorf(smp.lightnum * (float)count);
-    const int LightID = p.light_ids[pos];
-    const DevLight* L = light_by_id(p, LightID);
-    if (LightID < 0 || !L) return Ld;
-    bool brdf = true;
-    if (FULL) {
-        const int type = p.shading_type;
-        if (type == ER_SHADING_PHASE_ONLY) {
-            brdf = false;
-        } else if (type != ER_SHADING_BRDF_ONLY) {
-            const float gm = gradient_magnitude(p, SEP);
-            if (COUNT) cnt.shading += 6;
-            const float NGM = gm * p.gm_inv;
-            if (type == ER_SHADING_THRESHOLD) {
-                brdf = NGM > p.grad_threshold;
-            } else if (type == ER_SHADING_MODULATION) {
-                const float PdfBrdf = 1.0f - erm::powf_(1.0f - NGM, 2.0f);
-                brdf = rng.u() < PdfBrdf;
-            } else {
-                const float e = 25.0f * erm::powf_(p.grad_factor, 3.0f) * NGM;
-                const float base = 1.0f - erm::expf_(-e);
-                const float PdfBrdf = p.opacity_modulated ? plf_eval(p.opacity, I) * base : base;
-                brdf = rng.u() <= PdfBrdf;
-            }
-        }
-    }
-    Shader sh;
-    sh.brdf = brdf;
-    sh.Kd = ctf_eval(p.diffuse, I);
-    sh.Ks = ctf_eval(p.specular, I);
-    sh.exponent = 1000000.0f * erm::powf_(plf_eval(p.glossiness, I), 7.0f);   /* utilities.h:28-31 */
-    sh.Nn = nrm(SEN);
-    sh.Nu = nrm(crs(SEN, Wo));
-    sh.Nv = nrm(crs(SEN, sh.Nu));
-    Ld = add(Ld, estimate_direct<FULL, COUNT>(p, T, *L, pos, smp, SEP, SEN, Wo, sh, rng, cnt));
-    return scl((float)count, Ld);
-}
-
-/* Shading of one volume event with the shadow-ray marches deferred (ray-march sampler only,
- * whose shadow march consumes exactly two draws whenever it starts). Same operations and
- * order as uniform_sample_one_light / estimate_direct; the contributions that wait for a
- * visibility result are kept with their state. */
-struct EventRec {
-    uint32_t pixel, state;        /* state: 2 bits per contribution: 0 none, 1 add, 2 add if visible */
-    float count;                  /* light count scaling, or 0 when the result is not scaled */
-    float pad;
-    float Ld0x, Ld0y, Ld0z;       /* 0 + emission */
-    float c1x, c1y, c1z, c2x, c2y, c2z;
-    float pad2;
-};
-
-template<int FULL, int COUNT>
-__device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt,
-                            EventRec& ev, ShadowJob& job0, ShadowJob& job1, bool& has0, bool& has1)
-{
-    f3 Ld = mk(0.0f, 0.0f, 0.0f);
-    const float I = intensity(p, SEP);
-    if (COUNT) cnt.shading += 1 + gradient_samples(p);
-    Ld = add(Ld, ctf_eval(p.emission, I));
-    ev.Ld0x = Ld.x; ev.Ld0y = Ld.y; ev.Ld0z = Ld.z;
-    ev.state = 0;
-    ev.count = 0.0f;
-    has0 = false;
-    has1 = false;
-    const int count = p.light_count;
-    if (count <= 0) return;
-    const int pos = (int)floorf(smp.lightnum * (float)count);
-    const int LightID = p.light_ids[pos];
-    const DevLight* Lp = light_by_id(p, LightID);
-    if (LightID < 0 || !Lp) return;
-    const DevLight& L = *Lp;
-    ev.count = (float)count;
-    bool brdf = true;
-    if (FULL) {
-        const int type = p.shading_type;
-        if (type == ER_SHADING_PHASE_ONLY) {
-            brdf = false;
-        } else if (type != ER_SHADING_BRDF_ONLY) {
-            const float gm = gradient_magnitude(p, SEP);
-            if (COUNT) cnt.shading += 6;
-            const float NGM = gm * p.gm_inv;
-            if (type == ER_SHADING_THRESHOLD) {
-                brdf = NGM > p.grad_threshold;
-            } else if (type == ER_SHADING_MODULATION) {
-                const float PdfBrdf = 1.0f - erm::powf_(1.0f - NGM, 2.0f);
-                brdf = rng.u() < PdfBrdf;
-            } else {
-                const float e = 25.0f * erm::powf_(p.grad_factor, 3.0f) * NGM;
-                const float base = 1.0f - erm::expf_(-e);
-                const float PdfBrdf = p.opacity_modulated ? plf_eval(p.opacity, I) * base : base;
-                brdf = rng.u() <= PdfBrdf;
-            }
-        }
-    }
-    Shader sh;
-    sh.brdf = brdf;
-    sh.Kd = ctf_eval(p.diffuse, I);
-    sh.Ks = ctf_eval(p.specular, I);
-    sh.exponent = 1000000.0f * erm::powf_(plf_eval(p.glossiness, I), 7.0f);
-    sh.Nn = nrm(SEN);
-    sh.Nu = nrm(crs(SEN, Wo));
-    sh.Nv = nrm(crs(SEN, sh.Nu));
-
-    /* light-sampling half */
-    f3 SP, SN; float su, sv;
-    sample_light_surface(p, L, smp.uvw0, smp.uvw1, smp.uvw2, SP, SN, su, sv);
-    f3 Wi = nrm(sub(SP, SEP));
-    f3 Li = scl(L.multiplier, eval_texture(p, L.texture_id, su, sv));
-    if (L.one_sided && dot(sub(SEP, SP), SN) < 0.0f) Li = mk(0.0f, 0.0f, 0.0f);
-    if (L.unit == ER_UNIT_LUX) Li = mk(Li.x / L.area, Li.y / L.area, Li.z / L.area);
-    f3 F = shader_f<FULL>(sh, Wo, Wi);
-    float BsdfPdf = shader_pdf<FULL>(sh, Wo, Wi);
-    if (!black(Li) && !black(F) && BsdfPdf > 0.0f) {
-        const int v = visible_defer<COUNT>(p, SEP, SP, rng, cnt, job0);
-        if (v) {
-            const f3 d = sub(SEP, SP);
-            const float LightPdf = dot(d, d) / (absdot(SN, neg(Wi)) * L.area);
-            const float Weight = power_heuristic(LightPdf, BsdfPdf);
-            const f3 FL = mul(F, Li);
-            f3 c;
-            if (sh.brdf) c = scl(absdot(Wi, SEN) * Weight / LightPdf, FL);
-            else if (FULL) c = scl(Weight / LightPdf, FL);
-            else c = mk(FL.x / LightPdf, FL.y / LightPdf, FL.z / LightPdf);
-            ev.c1x = c.x; ev.c1y = c.y; ev.c1z = c.z;
-            ev.state |= (uint32_t)v;
-            if (v == 2) { job0.slot = 0; has0 = true; }
-        }
-    }
-    if (!FULL) return;
-
-    /* BSDF-sampling half */
-    F = shader_sample_f(sh, Wo, Wi, BsdfPdf, smp.comp, smp.dir0, smp.dir1);
-    if (black(F) || BsdfPdf <= 0.0f) return;
-    Ray r2;
-    r2.O = SEP; r2.D = Wi; r2.MinT = 0.0f; r2.MaxT = 1000000.0f;
-    float T2; f3 P2, N2, Le2;
-    const int idx2 = intersect_lights(p, r2, false, T2, P2, N2, Le2);
-    if (idx2 < 0 || idx2 != pos) return;
-    if (!black(Le2)) {
-        const int v = visible_defer<COUNT>(p, SEP, P2, rng, cnt, job1);
-        if (v) {
-            const f3 d = sub(SEP, P2);
-            const float LightPdf = dot(d, d) / (absdot(N2, neg(Wi)) * L.area);
-            const float Weight = power_heuristic(BsdfPdf, LightPdf);
-            const f3 FL = mul(F, Le2);
-            f3 c;
-            if (sh.brdf) c = scl(absdot(Wi, SEN) * Weight / BsdfPdf, FL);
-            else c = scl(Weight / BsdfPdf, FL);
-            ev.c2x = c.x; ev.c2y = c.y; ev.c2z = c.z;
-            ev.state |= (uint32_t)v << 2;
-            if (v == 2) { job1.slot = 1; has1 = true; }
-        }
-    }
-}
 
 /* One scattering pixel handed from the primary kernel to the shading kernel (64 bytes). */
 struct ScatterRec {

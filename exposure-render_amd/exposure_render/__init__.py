@@ -139,6 +139,40 @@ class Renderer:
         _check(self.lib, self.lib.er_get_gradient_magnitude_max(self.id, C.byref(v)))
         return v.value
 
+    # -- node-table transfer function and its adjoint --------------------------------------
+    def set_tf_lut(self, lut, range_):
+        """lut: (ER_TF_LUT_NODES, 4) float32 (opacity, diffuse X, Y, Z) or None for the PLFs."""
+        if lut is None:
+            _check(self.lib, self.lib.er_set_tf_lut(self.id, None, 0.0))
+            return
+        lut = np.ascontiguousarray(lut, dtype=np.float32).reshape(capi.TF_LUT_NODES, 4)
+        _check(self.lib, self.lib.er_set_tf_lut(self.id, lut.ctypes.data_as(C.POINTER(C.c_float)), float(range_)))
+
+    def adjoint_lut(self, s1, s2, passes, dl_drunning):
+        """dL/d(node table) (ER_TF_LUT_NODES, 4) float64 for dL/d(running XYZ) (H, W, 3)."""
+        s1 = np.ascontiguousarray(s1, dtype=np.uint32)
+        s2 = np.ascontiguousarray(s2, dtype=np.uint32)
+        g = np.ascontiguousarray(dl_drunning, dtype=np.float32).reshape(self.H, self.W, 3)
+        out = np.zeros((capi.TF_LUT_NODES, 4), dtype=np.float64)
+        _check(self.lib, self.lib.er_adjoint_lut(self.id, s1.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                 s2.ctypes.data_as(C.POINTER(C.c_uint32)), s1.size, int(passes),
+                                                 g.ctypes.data_as(C.POINTER(C.c_float)),
+                                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def adjoint_value(self, s1, s2, passes, eval_lut):
+        """The smooth estimator's running XYZ (H, W, 3) with values from eval_lut and the
+        decisions of the current node table."""
+        s1 = np.ascontiguousarray(s1, dtype=np.uint32)
+        s2 = np.ascontiguousarray(s2, dtype=np.uint32)
+        lut = np.ascontiguousarray(eval_lut, dtype=np.float32).reshape(capi.TF_LUT_NODES, 4)
+        out = np.zeros((self.H, self.W, 3), dtype=np.float32)
+        _check(self.lib, self.lib.er_adjoint_value(self.id, s1.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                   s2.ctypes.data_as(C.POINTER(C.c_uint32)), s1.size, int(passes),
+                                                   lut.ctypes.data_as(C.POINTER(C.c_float)),
+                                                   out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     def rebind(self):
         """BindTracer with the existing ID: restarts accumulation (NoIterations from the desc)."""
         self.tracer.no_iterations = 0

@@ -161,6 +161,7 @@ ABI_STRUCTS = [TracerDesc, VolumeDesc, LightDesc, TextureDesc, ClippingDesc, Obj
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libexposurerender_amd.so")
+TF_LUT_NODES = 256   # er_capi.h ER_TF_LUT_NODES
 
 # Every extern "C" symbol include/er_capi.h declares, with its ctypes signature.
 _P = C.POINTER
@@ -197,6 +198,11 @@ SIGNATURES = {
     "er_get_frame_xyza": (C.c_int, [C.c_int, _P(C.c_float), C.c_size_t]),
     "er_get_running_xyza": (C.c_int, [C.c_int, _P(C.c_float), C.c_size_t]),
     "er_render_passes": (C.c_int, [C.c_int, C.c_int, _P(Timing)]),
+    "er_set_tf_lut": (C.c_int, [C.c_int, _P(C.c_float), C.c_float]),
+    "er_adjoint_lut": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
+                                 _P(C.c_double)]),
+    "er_adjoint_value": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
+                                   _P(C.c_float)]),
     "er_set_counting": (C.c_int, [C.c_int, C.c_int]),
     "er_get_counters": (C.c_int, [C.c_int, _P(Counters)]),
     "er_reset_counters": (C.c_int, [C.c_int]),

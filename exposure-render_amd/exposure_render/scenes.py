@@ -337,3 +337,60 @@ def config_large(scale=1.0, estimator=capi.ESTIMATOR_FULL, seed=3):
     sc.add_clip(clipping_box((cx, 0.0, 0.0), (0.3 * ext[0] + 1e-3, ext[1] + 0.01, ext[2] + 0.01)))
     sc.passes = 256
     return sc
+
+
+# ---- differentiation workload ----------------------------------------------------------------
+
+def tf_lut_from_nodes(positions, opacity, colors, range_=4095.0):
+    """A node table (ER_TF_LUT_NODES x 4) sampling a piecewise-linear transfer function given
+    by node positions (intensities), opacities and XYZ colours."""
+    n = capi.TF_LUT_NODES
+    x = np.arange(n, dtype=np.float64) * (range_ / (n - 1))
+    lut = np.zeros((n, 4), dtype=np.float32)
+    lut[:, 0] = np.interp(x, positions, opacity)
+    for c in range(3):
+        lut[:, 1 + c] = np.interp(x, positions, np.asarray(colors)[:, c])
+    return lut
+
+
+def random_tf_lut(seed, range_=4095.0):
+    """A seeded random transfer function with 4-8 nodes: transparent below a random
+    threshold in [200, 800], random opacities in [0.05, 1] and colours in [0.3, 1] above."""
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(4, 9))
+    thr = float(rng.uniform(200.0, 800.0))
+    pos = np.sort(rng.uniform(thr + 50.0, range_, size=k - 2))
+    positions = np.concatenate([[0.0, thr], pos, [range_]])
+    opacity = np.concatenate([[0.0, 0.0], rng.uniform(0.05, 1.0, size=k - 1)])
+    colors = rng.uniform(0.3, 1.0, size=(k + 1, 3))
+    return tf_lut_from_nodes(positions, opacity, colors, range_)
+
+
+def initial_tf_lut(range_=4095.0):
+    """The starting point of a transfer-function recovery: a grey ramp, transparent at 0."""
+    return tf_lut_from_nodes([0.0, 300.0, range_], [0.0, 0.05, 0.5], [[0.7, 0.7, 0.7]] * 3, range_)
+
+
+def config_adjoint(scale=1.0, seed=4):
+    """Differentiation: 128^3 volume of 8 Gaussian blobs, 512 x 512, 32 passes, phase-function
+    shading, full estimator, one plane light; the transfer function is a 256-node table over
+    intensities [0, 4095] (er_set_tf_lut), set by the caller."""
+    r = _scaled(128, scale)
+    vox = phantom_blobs((r, r, r), 8, seed=seed)
+    sc = Scene(vox)
+    sc.name = "adjoint_128c_512px"
+    t = sc.tracer
+    t.opacity.add(0.0, 0.0).add(4095.0, 1.0)
+    t.diffuse.add(0.0, (1.0, 1.0, 1.0)).add(4095.0, (1.0, 1.0, 1.0))
+    t.specular.add(0.0, (0.0, 0.0, 0.0)).add(4095.0, (0.0, 0.0, 0.0))
+    t.glossiness.add(0.0, 0.5).add(4095.0, 0.5)
+    t.settings.density_scale = 10.0
+    t.settings.estimator = capi.ESTIMATOR_FULL
+    t.settings.shading_type = capi.SHADING_PHASE_ONLY
+    px = _scaled(512, scale)
+    set_camera(t, (px, px), (1.2, 0.9, 1.6), (0.0, 0.0, 0.0), exposure=4.0)
+    tex = sc.add_texture(uniform_texture((1.0, 1.0, 1.0)))
+    sc.add_light(plane_light((0.9, 1.0, 0.6), (0.0, 0.0, 0.0), (0.5, 0.5), 100.0, texture_id=tex))
+    sc.passes = 32
+    sc.tf_range = 4095.0
+    return sc

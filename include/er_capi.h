@@ -30,6 +30,7 @@ extern "C" {
 #define ER_MAX_NO_TF_NODES   128   /* Source/defines.h:77; a PLF holds at most 127 nodes (plf.h:53) */
 #define ER_MAX_INDICES       256   /* Source/vector.h:554-579 (Indices: 256 ints + Count) */
 #define ER_LIST_CAPACITY     32    /* Source/list.cuh:27 (MaxSize); 31 live objects per kind */
+#define ER_TF_LUT_NODES      256   /* node-table transfer function (er_set_tf_lut), 4 channels per node */
 
 /* status codes: 0 = success, otherwise the reference's ExceptionLevel + 1 */
 #define ER_OK        0
@@ -350,6 +351,36 @@ ER_API int er_get_gradient_magnitude_max(int tracer_id, float* gm_max);
 
 /* -log(u) table entry used for the free-path threshold, for u = m * 2^-23. */
 ER_API int er_neglog_table_entry(uint32_t m, float* value);
+
+/* ---- differentiable transfer function (new; no counterpart in the reference) ----------- */
+
+/* Node-table transfer function. lut holds ER_TF_LUT_NODES x 4 floats, node-major: opacity,
+ * diffuse X, Y, Z. Node n sits at intensity n * range / (ER_TF_LUT_NODES - 1); between nodes
+ * the value is linear, beyond the last node it is the last node's, and negative values clamp
+ * to zero. While set, it replaces the tracer's Opacity1D and Diffuse1D (specular, glossiness
+ * and emission keep their PLFs). lut == NULL returns to the PLFs. Restarts accumulation
+ * (NoIterations = 0), as a tracer rebind does. */
+ER_API int er_set_tf_lut(int tracer_id, const float* lut, float range);
+
+/* Path-replay adjoint of `passes` progressive passes rendered from the per-pixel seeds
+ * (seeds1, seeds2; count = W*H) with the current node table, starting from NoIterations = 0.
+ * Input: dL/d(running XYZ estimate after the passes), W*H*3 floats. Output: dL/d(node table),
+ * ER_TF_LUT_NODES*4 doubles (summed in FP64). Each path is replayed from its seed with the same
+ * decisions as the forward pass; shadow visibility and the primary free flight are
+ * differentiated through their expected values (transmittance exp(-DS * sum sigma dt) on the
+ * same samples, and the log-probability of the sampled scatter sample). Requires the full
+ * estimator, phase-function-only shading and the ray-march sampler. The tracer's own seeds,
+ * frames and running estimate are left unchanged. */
+ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
+                          const float* dL_drunning_xyz, double* dL_dlut);
+
+/* The smooth estimator the adjoint differentiates, evaluated with the node table eval_lut
+ * while every decision (scatter sample, light choice, directions) is taken with the current
+ * table: the running XYZ estimate (W*H*3 floats) after `passes` passes from the given seeds.
+ * With eval_lut equal to the current table its expectation equals the forward estimate's;
+ * central differences of it in eval_lut check er_adjoint_lut. */
+ER_API int er_adjoint_value(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
+                            const float* eval_lut, float* running_xyz);
 
 /* ---- synthetic phantoms (seeded, deterministic) --------------------------------------- */
 

@@ -178,6 +178,8 @@ struct Scene {
     std::vector<er_clipping_desc> clips;   /* index = clipping ID */
     float gm_max = 0.0f;
     float maj_max = 0.0f;                  /* global majorant for Woodcock: DS^2 * max opacity */
+    std::vector<float> tf_lut;             /* node table (er_set_tf_lut), empty: the PLFs */
+    float tf_scale = 0.0f;
 };
 
 static float shape_area(const er_shape& s)   /* shape.h:28-56,91-103 */
@@ -253,6 +255,29 @@ template<class R> inline R plf_eval(const er_plf& p, R x)
 template<class R> inline V3<R> ctf_eval(const er_color_tf& c, R x)
 {
     return V3<R>(plf_eval(c.plf[0], x), plf_eval(c.plf[1], x), plf_eval(c.plf[2], x));
+}
+
+/* Node-table transfer function (er_capi.h er_set_tf_lut): node n at intensity n / scale,
+ * linear between nodes, the last node's value beyond it, negative values clamped to 0. */
+template<class R> inline R tf_lut_eval(const std::vector<float>& lut, float scale, R I, int ch)
+{
+    const int last = ER_TF_LUT_NODES - 1;
+    const R x = std::min(I * R(scale), R(last));
+    int b = (int)x;
+    if (b > last - 1) b = last - 1;
+    const R f = x - R(b);
+    const R v = (R(1) - f) * R(lut[b * 4 + ch]) + f * R(lut[(b + 1) * 4 + ch]);
+    return std::max(v, R(0));
+}
+template<class R> inline R opacity_at(const Scene& S, R I)
+{
+    return S.tf_lut.empty() ? plf_eval(S.t.opacity, I) : tf_lut_eval(S.tf_lut, S.tf_scale, I, 0);
+}
+template<class R> inline V3<R> diffuse_at(const Scene& S, R I)
+{
+    if (S.tf_lut.empty()) return ctf_eval(S.t.diffuse, I);
+    return V3<R>(tf_lut_eval(S.tf_lut, S.tf_scale, I, 1), tf_lut_eval(S.tf_lut, S.tf_scale, I, 2),
+                 tf_lut_eval(S.tf_lut, S.tf_scale, I, 3));
 }
 
 /* ---- estimator ----------------------------------------------------------------------- */
@@ -412,7 +437,7 @@ template<class R> struct Est {
         if (full && clipped(Ps)) return R(0);
         const R I = get_intensity(Ps);
         if (cnt) cnt[primary ? 4 : 5]++;
-        return R(S.t.settings.density_scale) * plf_eval(S.t.opacity, I);
+        return R(S.t.settings.density_scale) * opacity_at(S, I);
     }
 
     /* SampleVolume (raymarching.h:30-71) / ScatterEventInVolume (raymarching.h:73-113):
@@ -939,12 +964,12 @@ template<class R> struct Est {
                 } else {
                     const R e = R(25) * M<R>::pow(R(S.t.settings.gradient_factor), R(3)) * NGM;
                     const R base = R(1) - M<R>::exp(-e);
-                    const R PdfBrdf = S.t.settings.opacity_modulated ? plf_eval(S.t.opacity, I) * base : base;
+                    const R PdfBrdf = S.t.settings.opacity_modulated ? opacity_at(S, I) * base : base;
                     brdf = u() <= PdfBrdf;
                 }
             }
         }
-        const V3<R> Kd = ctf_eval(S.t.diffuse, I);
+        const V3<R> Kd = diffuse_at(S, I);
         const V3<R> Ks = ctf_eval(S.t.specular, I);
         const R gloss = R(1000000.0f) * M<R>::pow(plf_eval(S.t.glossiness, I), R(7));   /* utilities.h:28-31 */
         const Shader sh = make_shader(brdf, SEN, Wo, Kd, Ks, R(15.0f), gloss);
@@ -1116,6 +1141,19 @@ void* orc_create(const er_tracer_desc* t, const er_volume_desc* v, const er_ligh
 }
 
 void orc_destroy(void* h) { delete (Scene*)h; }
+
+/* node-table transfer function; lut == NULL returns to the PLFs */
+void orc_set_tf_lut(void* h, const float* lut, float range)
+{
+    Scene& S = *(Scene*)h;
+    if (!lut) { S.tf_lut.clear(); S.tf_scale = 0.0f; }
+    else { S.tf_lut.assign(lut, lut + ER_TF_LUT_NODES * 4); S.tf_scale = (float)(ER_TF_LUT_NODES - 1) / range; }
+    uint16_t vmax = 0;
+    for (uint16_t v : S.vox) vmax = std::max(vmax, v);
+    float omax = 0.0f;
+    for (int i = 0; i <= (int)vmax; ++i) omax = std::max(omax, opacity_at(S, (float)i));
+    S.maj_max = S.t.settings.density_scale * S.t.settings.density_scale * omax;
+}
 
 void orc_set_gm_max(void* h, float gm) { ((Scene*)h)->gm_max = gm; }
 float orc_get_gm_max(void* h) { return ((Scene*)h)->gm_max; }

@@ -30,6 +30,7 @@ def load():
         "orc_create": (C.c_void_p, [_P(capi.TracerDesc), _P(capi.VolumeDesc), _P(capi.LightDesc), C.c_int,
                                     _P(capi.TextureDesc), C.c_int, _P(capi.ClippingDesc), C.c_int]),
         "orc_destroy": (None, [C.c_void_p]),
+        "orc_set_tf_lut": (None, [C.c_void_p, _P(C.c_float), C.c_float]),
         "orc_set_gm_max": (None, [C.c_void_p, C.c_float]),
         "orc_get_gm_max": (C.c_float, [C.c_void_p]),
         "orc_compute_gm_max": (C.c_float, [C.c_void_p, C.c_int]),
@@ -108,6 +109,13 @@ class Oracle:
             self.lib.orc_destroy(self.h)
         except Exception:
             pass
+
+    def set_tf_lut(self, lut, range_):
+        if lut is None:
+            self.lib.orc_set_tf_lut(self.h, None, 0.0)
+            return
+        self._lut = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
+        self.lib.orc_set_tf_lut(self.h, _p(self._lut, C.c_float), float(range_))
 
     def compute_gm_max(self):
         return self.lib.orc_compute_gm_max(self.h, self.nthreads)

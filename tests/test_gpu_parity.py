@@ -22,11 +22,15 @@ ORC_INDEX = {"paths": 0, "primary_samples": 1, "shadow_samples": 2, "shading_sam
              "shadow_rays": 7}
 
 
-def run_parity(sc, passes=3, seed=5, counting=True, check_display=True):
+def run_parity(sc, passes=3, seed=5, counting=True, check_display=True, lut=None, lut_range=4095.0):
     print(f"[parity] scene={sc.name} film={sc.film} vol={sc.voxels.shape} estimator={sc.tracer.settings.estimator} "
-          f"shading={sc.tracer.settings.shading_type} sampler={sc.tracer.settings.sampler}", flush=True)
+          f"shading={sc.tracer.settings.shading_type} sampler={sc.tracer.settings.sampler} lut={lut is not None}",
+          flush=True)
     r = er.Renderer(sc)
     orc = O.Oracle(sc)
+    if lut is not None:
+        r.set_tf_lut(lut, lut_range)
+        orc.set_tf_lut(lut, lut_range)
     if sc.tracer.settings.estimator == capi.ESTIMATOR_FULL and sc.tracer.settings.shading_type not in (0, 1):
         gm = r.gm_max()
         gm_ref = orc.compute_gm_max()
@@ -140,3 +144,9 @@ def test_camera_inside_volume_and_thin_lens(fresh):
     sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_COMPAT)
     scenes.set_camera(sc.tracer, sc.film, (1.2, 0.9, 1.6), (0.0, 0.0, 0.0), exposure=4.0, aperture=0.05, focal=-1.0)
     run_parity(sc, passes=1)
+
+
+def test_node_table_transfer_function(fresh):
+    """er_set_tf_lut: opacity and diffuse colour from a 256-node table, bit-for-bit decisions."""
+    sc = scenes.config_adjoint(scale=0.25)
+    run_parity(sc, passes=3, lut=scenes.random_tf_lut(7))
