@@ -149,6 +149,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevParams* __restrict__ p
     __syncthreads();
     Tables T;
     T.bits = lds;
+    T.bits_glb = nullptr;
     T.lut_lds = lut;
     T.lut_glb = nullptr;
 

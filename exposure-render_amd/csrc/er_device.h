@@ -92,7 +92,7 @@ struct DevParams {
     int   mc_shift;                  /* macrocell edge = 2^mc_shift cells */
     int   mc_words;                  /* 32-bit words of mc_bits */
     const float* neglog;             /* -log(m 2^-23), m in [0, 2^23) */
-    /* lights, textures, clipping objects; indexed by ID (< ER_LIST_CAPACITY) */
+    /* the tracer's lights and textures by slot (light_ids maps list position -> slot), clips */
     int   light_count;
     int   light_ids[ER_MAX_INDICES];
     DevLight   lights[ER_LIST_CAPACITY];

@@ -66,14 +66,21 @@ __device__ __forceinline__ bool clipped(const DevParams& p, f3 P)
 
 /* Per-block copies of the macrocell occupancy bits and (when it fits) the opacity LUT. */
 struct Tables {
-    const uint32_t* bits;   /* bit m set <=> macrocell m holds a sample with sigma > 0 */
+    /* bit m set <=> macrocell m holds a sample with sigma > 0: in LDS, or (bits_glb set) in
+     * global memory */
+    const uint32_t* bits;
+    const __attribute__((address_space(1))) uint32_t* bits_glb;
     /* DensityScale * Opacity(i): in LDS when it fits (lut_lds), else in global memory */
     const float* lut_lds;
     const __attribute__((address_space(1))) float* lut_glb;
     __device__ __forceinline__ float lut(int i) const { return lut_glb ? lut_glb[i] : lut_lds[i]; }
 };
 
-__device__ __forceinline__ bool mc_occupied(const Tables& T, int m) { return (T.bits[m >> 5] >> (m & 31)) & 1u; }
+__device__ __forceinline__ bool mc_occupied(const Tables& T, int m)
+{
+    const uint32_t w = T.bits_glb ? T.bits_glb[m >> 5] : T.bits[m >> 5];
+    return (w >> (m & 31)) & 1u;
+}
 
 __device__ __forceinline__ int mc_axis(float L, int shift, int dim)
 {
@@ -941,6 +948,50 @@ __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SE
             if (v == 2) { job1.slot = 1; has1 = true; }
         }
     }
+}
+
+/* One scattering pixel handed from the primary kernel to the shading kernel (64 bytes). */
+struct ScatterRec {
+    uint32_t pixel, s0, s1, pad;
+    float Px, Py, Pz, Wox, Woy, Woz;
+    float comp, dir0, dir1, uvw0, uvw1, uvw2;
+    float lightnum, spare;
+};
+
+/* One tracer of a batch (er_render_batch): its parameter block and per-pass buffers, its
+ * share of the batch's 8 x 8 primary tiles (tile_base .. tile_base + ntiles) and 16 x 16
+ * post tiles, and its pass count. Jobs are sorted by pass count, largest first, so the jobs
+ * still active in pass round r are a prefix of the table. */
+struct BatchJob {
+    const DevParams* params;
+    float4* frame;
+    float4* running;
+    uchar4* display;
+    uint32_t* seeds1;
+    uint32_t* seeds2;
+    ScatterRec* queue;
+    EventRec* events;
+    ShadowJob* jobs;
+    unsigned char* vis;
+    unsigned int* counts;       /* [0] scatter queue, [1] shadow jobs */
+    int tiles_x, ntiles, tile_base;
+    int post_tiles_x, post_ntiles, post_base;
+    int passes, no_iter0;
+};
+
+/* device-side progress of a batch: the pass round and the shared primary-tile counter */
+struct BatchState {
+    unsigned int round;
+    unsigned int tiles;
+    unsigned int pad[2];
+};
+
+/* number of jobs still rendering in this round (a prefix of the table) */
+__device__ __forceinline__ int batch_active(const BatchJob* __restrict__ J, int n, unsigned int round)
+{
+    int a = 0;
+    while (a < n && (unsigned int)J[a].passes > round) ++a;
+    return a;
 }
 
 } /* namespace erd */

@@ -19,22 +19,18 @@
  *
  * k_build_cells / k_macro_minmax / k_macro_sigma / k_macro_bits / k_gm_max: volume preprocessing.
  */
-#include "er_device.h"
+#include "er_path.h"
 
 namespace erd {
 
 #define POST_TILE 16
 
-__global__ __launch_bounds__(256) void k_post(const DevParams* __restrict__ pp, const float4* __restrict__ frame,
-                                             float4* __restrict__ running, uchar4* __restrict__ display, int no_iterations,
-                                             unsigned int* __restrict__ qcount)
+/* One 16 x 16 tile of one pass's post-processing; every thread of the workgroup calls it. */
+__device__ __forceinline__ void post_tile(const DevParams& p, const float4* __restrict__ frame, float4* __restrict__ running,
+                                          uchar4* __restrict__ display, int no_iterations, int bx, int by,
+                                          float4 (*tile)[POST_TILE + 2])
 {
-    const DevParams& p = *pp;
-    /* the scatter queue of this pass has been consumed: empty it for the next pass */
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) qcount[threadIdx.x] = 0u;
     const int W = p.film_w, H = p.film_h;
-    __shared__ float4 tile[POST_TILE + 2][POST_TILE + 2];
-    const int bx = blockIdx.x * POST_TILE, by = blockIdx.y * POST_TILE;
     for (int k = threadIdx.x; k < (POST_TILE + 2) * (POST_TILE + 2); k += blockDim.x) {
         const int ty = k / (POST_TILE + 2), tx = k % (POST_TILE + 2);
         const int gx = min(max(bx + tx - 1, 0), W - 1), gy = min(max(by + ty - 1, 0), H - 1);
@@ -86,6 +82,33 @@ __global__ __launch_bounds__(256) void k_post(const DevParams* __restrict__ pp, 
     d.z = (unsigned char)(255.0f * tb);
     d.w = (unsigned char)(r.w * 255.0f);
     display[i] = d;
+}
+
+__global__ __launch_bounds__(256) void k_post(const DevParams* __restrict__ pp, const float4* __restrict__ frame,
+                                             float4* __restrict__ running, uchar4* __restrict__ display, int no_iterations,
+                                             unsigned int* __restrict__ qcount)
+{
+    /* the scatter queue of this pass has been consumed: empty it for the next pass */
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) qcount[threadIdx.x] = 0u;
+    __shared__ float4 tile[POST_TILE + 2][POST_TILE + 2];
+    post_tile(*pp, frame, running, display, no_iterations, blockIdx.x * POST_TILE, blockIdx.y * POST_TILE, tile);
+}
+
+/* post-processing of a batch round: one workgroup per 16 x 16 tile of every active job */
+__global__ __launch_bounds__(256) void k_post_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+{
+    const unsigned int round = st->round;
+    const int na = batch_active(J, njobs, round);
+    const int b = blockIdx.x;
+    int job = -1;
+    for (int k = 0; k < na; ++k)
+        if (b >= J[k].post_base && b < J[k].post_base + J[k].post_ntiles) { job = k; break; }
+    if (job < 0) return;
+    const BatchJob& B = J[job];
+    const int t = b - B.post_base;
+    __shared__ float4 tile[POST_TILE + 2][POST_TILE + 2];
+    post_tile(*B.params, B.frame, B.running, B.display, B.no_iter0 + (int)round, (t % B.post_tiles_x) * POST_TILE,
+              (t / B.post_tiles_x) * POST_TILE, tile);
 }
 
 struct BilateralParams {
@@ -275,5 +298,12 @@ extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lu
 extern "C" hipError_t erd_launch_gm_max(const erd::DevParams* p, unsigned int* out, hipStream_t st)
 {
     hipLaunchKernelGGL(erd::k_gm_max, dim3(2048), dim3(256), 0, st, p, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, const erd::BatchState* st, int total_tiles,
+                                            hipStream_t stream)
+{
+    hipLaunchKernelGGL(erd::k_post_b, dim3(total_tiles), dim3(256), 0, stream, J, njobs, st);
     return hipGetLastError();
 }

@@ -27,7 +27,7 @@
 #include <vector>
 
 #include "../../include/er_capi.h"
-#include "er_device.h"
+#include "er_path.h"
 
 using erd::DevParams;
 
@@ -49,6 +49,11 @@ extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm,
 extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lut, int lut_max, float* out, int n,
                                              hipStream_t st);
 extern "C" hipError_t erd_launch_gm_max(const DevParams* p, unsigned int* out, hipStream_t st);
+extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, erd::BatchState* st, int full,
+                                             int primary_blocks, int shade_blocks, hipStream_t stream);
+extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, const erd::BatchState* st, int total_tiles,
+                                            hipStream_t stream);
+extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, erd::BatchState* st, hipStream_t stream);
 extern "C" size_t erd_adj_partials_per_block(void);
 extern "C" hipError_t erd_launch_adjoint(const DevParams* p, int mode, const float* lut_eval, const float* sigma_eval,
                                          const float4* adj, float4* value, float* partials, uint32_t* s1, uint32_t* s2,
@@ -160,6 +165,7 @@ int ensure_neglog_device()
 
 /* ---- resources --------------------------------------------------------------------------- */
 uint64_t g_scene_version = 1;
+hipStream_t g_batch_stream = nullptr;
 
 struct Volume {
     er_volume_desc desc{};
@@ -237,6 +243,7 @@ Registry<ClipObj> g_clips;
 Registry<ObjectObj> g_objects;
 Registry<BitmapObj> g_bitmaps;
 bool g_device_ready = false;
+int g_list_capacity = ER_LIST_CAPACITY;   /* list.cuh:27 MaxSize; er_set_list_capacity */
 int g_primary_blocks = 2048;   /* persistent primary workgroups: 7 per CU (LDS and VGPR bound) */
 
 int ensure_device()
@@ -431,26 +438,46 @@ int build_params(Tracer& t)
     P.neglog = g_neglog_dev;
     for (int i = 0; i < 3; ++i) P.mc_dim[i] = vol->mc[i];
 
-    /* lights, textures, clipping objects by ID */
+    /* The tracer's lights, compacted into device slots in the order of its LightIDs, and the
+     * textures they use, compacted likewise. The device indexes them by slot, so any number of
+     * registered objects (er_set_list_capacity) and any IDs work; a tracer uses at most
+     * ER_LIST_CAPACITY distinct lights and textures. */
     P.light_count = std::min(d.light_count, ER_MAX_INDICES);
-    for (int i = 0; i < ER_MAX_INDICES; ++i) P.light_ids[i] = d.light_ids[i];
-    for (auto& kv : g_lights.map) {
-        if (kv.first >= ER_LIST_CAPACITY) continue;
-        const er_light_desc& L = kv.second->desc;
-        erd::DevLight& D = P.lights[kv.first];
+    for (int i = 0; i < ER_MAX_INDICES; ++i) P.light_ids[i] = -1;
+    std::map<int, int> light_slot, tex_slot;
+    for (int i = 0; i < P.light_count; ++i) {
+        LightObj* lo = g_lights.find(d.light_ids[i]);
+        if (!lo) continue;
+        auto it = light_slot.find(d.light_ids[i]);
+        if (it != light_slot.end()) { P.light_ids[i] = it->second; continue; }
+        const int slot = (int)light_slot.size();
+        if (slot >= ER_LIST_CAPACITY) return set_error(ER_ERROR, "a tracer uses at most %d lights", ER_LIST_CAPACITY);
+        light_slot[d.light_ids[i]] = slot;
+        P.light_ids[i] = slot;
+        const er_light_desc& L = lo->desc;
+        erd::DevLight& D = P.lights[slot];
         mat34(D.tm, L.shape.tm);
         mat34(D.inv_tm, L.shape.inv_tm);
         D.type = L.shape.type; D.one_sided = L.shape.one_sided; D.visible = L.visible; D.unit = L.unit;
-        for (int i = 0; i < 3; ++i) D.size[i] = L.shape.size[i];
+        for (int k = 0; k < 3; ++k) D.size[k] = L.shape.size[k];
         D.area = shape_area(L.shape);
         D.inner_radius = L.shape.inner_radius; D.outer_radius = L.shape.outer_radius;
-        D.multiplier = L.multiplier; D.texture_id = L.texture_id; D.valid = 1;
+        D.multiplier = L.multiplier; D.valid = 1;
+        D.texture_id = -1;
+        if (g_textures.find(L.texture_id)) {
+            auto tt = tex_slot.find(L.texture_id);
+            if (tt == tex_slot.end()) {
+                const int ts = (int)tex_slot.size();
+                if (ts >= ER_LIST_CAPACITY) return set_error(ER_ERROR, "a tracer uses at most %d textures", ER_LIST_CAPACITY);
+                tt = tex_slot.emplace(L.texture_id, ts).first;
+            }
+            D.texture_id = tt->second;
+        }
     }
     int ngrad = 0;
-    for (auto& kv : g_textures.map) {
-        if (kv.first >= ER_LIST_CAPACITY) continue;
-        const er_texture_desc& T = kv.second->desc;
-        erd::DevTexture& D = P.textures[kv.first];
+    for (auto& kv : tex_slot) {
+        const er_texture_desc& T = g_textures.find(kv.first)->desc;
+        erd::DevTexture& D = P.textures[kv.second];
         D.type = T.type; D.proc_type = T.procedural.type; D.bitmap_id = T.bitmap_id; D.valid = 1;
         D.output_level = T.output_level;
         for (int i = 0; i < 2; ++i) { D.offset[i] = T.offset[i]; D.repeat[i] = T.repeat[i]; D.flip[i] = T.flip[i]; }
@@ -611,12 +638,12 @@ int bind_generic(R& reg, D* d, int bind, F make)
         g_scene_version++;
         return ER_OK;
     }
-    if (reg.map.size() + 1 >= ER_LIST_CAPACITY) return ER_OK;   /* list.cuh:63-67 */
+    if (reg.map.size() + 1 >= (size_t)g_list_capacity) return ER_OK;   /* list.cuh:63-67 */
     auto* obj = reg.find(d->id);
     const bool fresh = obj == nullptr;
     if (fresh) {
         const int id = reg.counter;
-        if (id >= ER_LIST_CAPACITY * 1024) return set_error(ER_ERROR, "ID space exhausted");
+        if (id >= (1 << 30)) return set_error(ER_ERROR, "ID space exhausted");
         d->id = id;
     }
     int st = make(obj, *d, fresh);
@@ -751,6 +778,7 @@ ER_API int er_shutdown(void)
     g_tracers.clear(); g_volumes.clear(); g_lights.clear(); g_textures.clear(); g_clips.clear();
     g_objects.clear(); g_bitmaps.clear();
     if (g_neglog_dev) { (void)hipFree(g_neglog_dev); g_neglog_dev = nullptr; }
+    if (g_batch_stream) { (void)hipStreamDestroy(g_batch_stream); g_batch_stream = nullptr; }
     g_scene_version++;
     return ER_OK;
 }
@@ -900,6 +928,119 @@ ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (st) return st;
     if (se != hipSuccess) return set_error(ER_ERROR, "%s (hipStreamSynchronize)", hipGetErrorString(se));
+    return ER_OK;
+}
+
+ER_API int er_set_list_capacity(int capacity)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    if (capacity < 2 || capacity > (1 << 20)) return set_error(ER_ERROR, "capacity must be in [2, 2^20]");
+    g_list_capacity = capacity;
+    return ER_OK;
+}
+
+ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    if (n <= 0 || !items) return set_error(ER_ERROR, "empty batch");
+    std::vector<std::pair<int, Tracer*>> jobs;   /* (passes, tracer) */
+    int full = -1;
+    for (int k = 0; k < n; ++k) {
+        Tracer* t = get_tracer(items[k].tracer_id, st);
+        if (!t) return st;
+        if (items[k].passes < 0) return set_error(ER_ERROR, "negative pass count");
+        for (auto& j : jobs)
+            if (j.second == t) return set_error(ER_ERROR, "tracer %d appears twice in the batch", items[k].tracer_id);
+        const er_render_settings& s = t->desc.settings;
+        const int f = s.estimator == ER_ESTIMATOR_FULL;
+        if (full >= 0 && f != full) return set_error(ER_ERROR, "all tracers of a batch must use the same estimator mode");
+        if (f && s.sampler == ER_SAMPLER_WOODCOCK) return set_error(ER_ERROR, "the Woodcock sampler is not batched");
+        full = f;
+        if ((st = build_params(*t))) return st;
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        if (items[k].passes > 0 && t->W > 0 && t->H > 0) jobs.push_back({items[k].passes, t});
+    }
+    if (jobs.empty()) return ER_OK;
+    std::stable_sort(jobs.begin(), jobs.end(), [](const std::pair<int, Tracer*>& a, const std::pair<int, Tracer*>& b) {
+        return a.first > b.first;
+    });
+    const int nj = (int)jobs.size();
+    std::vector<erd::BatchJob> table(nj);
+    int tile_base = 0, post_base = 0;
+    for (int k = 0; k < nj; ++k) {
+        Tracer& t = *jobs[k].second;
+        erd::BatchJob& B = table[k];
+        B.params = t.params.ptr;
+        B.frame = t.frame.ptr; B.running = t.running.ptr; B.display = t.display.ptr;
+        B.seeds1 = t.seeds1.ptr; B.seeds2 = t.seeds2.ptr;
+        B.queue = (erd::ScatterRec*)t.queue.ptr; B.events = (erd::EventRec*)t.events.ptr;
+        B.jobs = (erd::ShadowJob*)t.jobs.ptr; B.vis = t.vis.ptr;
+        B.counts = t.qcount.ptr;
+        B.tiles_x = (t.W + 7) / 8;
+        B.ntiles = B.tiles_x * ((t.H + 7) / 8);
+        B.tile_base = tile_base;
+        tile_base += B.ntiles;
+        B.post_tiles_x = (t.W + 15) / 16;
+        B.post_ntiles = B.post_tiles_x * ((t.H + 15) / 16);
+        B.post_base = post_base;
+        post_base += B.post_ntiles;
+        B.passes = jobs[k].first;
+        B.no_iter0 = t.desc.no_iterations;
+    }
+    DevBuf<erd::BatchJob> dJ;
+    DevBuf<erd::BatchState> dS;
+    if ((st = dJ.alloc(nj)) || (st = dS.alloc(1))) return st;
+    HIP_TRY(hipMemcpy(dJ.ptr, table.data(), nj * sizeof(erd::BatchJob), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dS.ptr, 0, sizeof(erd::BatchState)));
+    if (!g_batch_stream) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream, hipStreamNonBlocking));
+    hipStream_t bs = g_batch_stream;
+    const int shade_blocks = 2048;
+    /* one pass round as a graph: trace kernels, post, round end (no memset nodes) */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = erd_launch_batch_trace(dJ.ptr, nj, dS.ptr, full, g_primary_blocks, shade_blocks, bs);
+    hipError_t e2 = erd_launch_batch_post(dJ.ptr, nj, dS.ptr, post_base, bs);
+    hipError_t e3 = erd_launch_round_end(dJ.ptr, nj, dS.ptr, bs);
+    hipError_t ec = hipStreamEndCapture(bs, &graph);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_error(ER_ERROR, "batch graph capture failed: %s", hipGetErrorString(ec != hipSuccess ? ec : (e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3))));
+    }
+    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return set_error(ER_ERROR, "%s (hipGraphInstantiate)", hipGetErrorString(ei));
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing) {
+        HIP_TRY(hipEventCreate(&t0));
+        HIP_TRY(hipEventCreate(&t1));
+        HIP_TRY(hipEventRecord(t0, bs));
+    }
+    const int rounds = jobs[0].first;
+    hipError_t el = hipSuccess;
+    for (int r = 0; r < rounds && el == hipSuccess; ++r) el = hipGraphLaunch(exec, bs);
+    if (timing && el == hipSuccess) el = hipEventRecord(t1, bs);
+    const hipError_t es = hipStreamSynchronize(bs);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    float ms = 0.0f;
+    if (timing && el == hipSuccess && es == hipSuccess) (void)hipEventElapsedTime(&ms, t0, t1);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (el != hipSuccess) return set_error(ER_ERROR, "%s (hipGraphLaunch)", hipGetErrorString(el));
+    if (es != hipSuccess) return set_error(ER_ERROR, "%s (hipStreamSynchronize)", hipGetErrorString(es));
+    for (auto& j : jobs) {
+        Tracer& t = *j.second;
+        t.desc.no_iterations += j.first;
+        /* the bilateral display filter only shapes the displayed image: once, after the last pass */
+        if (use_bilateral(t))
+            HIP_TRY(erd_launch_bilateral(t.bilateral.ptr, t.display.ptr, t.disp_tmp.ptr, t.disp_filt.ptr, t.W, t.H, t.stream));
+    }
+    for (auto& j : jobs) HIP_TRY(hipStreamSynchronize(j.second->stream));
+    if (timing) { timing->total_ms = ms; timing->trace_ms = ms; timing->post_ms = 0.0f; timing->passes = rounds; }
     return ER_OK;
 }
 

@@ -23,14 +23,6 @@
 
 namespace erd {
 
-/* One scattering pixel handed from the primary kernel to the shading kernel (64 bytes). */
-struct ScatterRec {
-    uint32_t pixel, s0, s1, pad;
-    float Px, Py, Pz, Wox, Woy, Woz;
-    float comp, dir0, dir1, uvw0, uvw1, uvw2;
-    float lightnum, spare;
-};
-
 /* Stage the macrocell bits (always) and the opacity LUT (when LUT_LDS) in LDS. */
 template<int LUT_LDS>
 __device__ __forceinline__ Tables stage_tables(const DevParams& p, uint32_t* lds)
@@ -39,6 +31,7 @@ __device__ __forceinline__ Tables stage_tables(const DevParams& p, uint32_t* lds
     for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = gmem(p.mc_bits)[k];
     Tables T;
     T.bits = lds;
+    T.bits_glb = nullptr;
     if (LUT_LDS) {
         float* lut = (float*)(lds + nw);
         for (int k = threadIdx.x; k <= p.lut_max; k += blockDim.x) lut[k] = gmem(p.sigma_lut)[k];
@@ -76,12 +69,194 @@ __device__ __forceinline__ void flush_counters(const Cnt& cnt, bool active, int 
     if (threadIdx.x < 12) atomicAdd(&counters[threadIdx.x], acc[threadIdx.x]);
 }
 
-/* Primary kernel: the 11 sample draws (sample.h:258-262), the camera ray (singlescattering.h:29-50),
- * SampleRay (singlescattering.h:52-74). Persistent waves take 8 x 8 pixel tiles from an atomic
- * counter (counts[2]), so the cost of a tile - from a few samples for background pixels to
- * thousands behind low-density tissue - is balanced over the whole device. Pixels whose nearest
- * event is a light or nothing are finished here; pixels with a volume event are appended to the
- * scatter queue (wave-aggregated atomic, so each wave's scattering pixels stay contiguous). */
+/* One wave renders one 8 x 8 pixel tile: the 11 sample draws (sample.h:258-262), the camera
+ * ray (singlescattering.h:29-50), SampleRay (singlescattering.h:52-74). Pixels whose nearest
+ * event is a light or nothing are finished here; pixels with a volume event are appended to
+ * the scatter queue (wave-aggregated atomic on *qcount) with their RNG state and the draws
+ * the shading still needs. */
+template<int FULL, int COUNT>
+__device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T, unsigned int tile, int tiles_x, int lane,
+                                             float4* __restrict__ frame, uint32_t* __restrict__ seeds1,
+                                             uint32_t* __restrict__ seeds2, ScatterRec* __restrict__ queue,
+                                             unsigned int* __restrict__ qcount, Cnt& cnt, int& paths)
+{
+    const int px = (int)(tile % (unsigned int)tiles_x) * 8 + (lane & 7);
+    const int py = (int)(tile / (unsigned int)tiles_x) * 8 + (lane >> 3);
+    const bool active = px < p.film_w && py < p.film_h;
+    const size_t i = (size_t)py * p.film_w + px;
+    bool scatter = false;
+    Sample smp;
+    Rng rng;
+    f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f);
+    if (active) {
+        paths++;
+        rng.s0 = seeds1[i];
+        rng.s1 = seeds2[i];
+        smp.comp = rng.u();
+        smp.dir0 = rng.u(); smp.dir1 = rng.u();
+        smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
+        smp.lightnum = rng.u();
+        smp.film0 = rng.u(); smp.film1 = rng.u();
+        smp.lens0 = rng.u(); smp.lens1 = rng.u();
+        const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
+        const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
+        const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
+        Ray r;
+        r.O = ld3(p.cam_pos);
+        r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
+        r.MinT = p.clip_near;
+        r.MaxT = p.clip_far;
+        if (p.aperture != 0.0f) {
+            float lx, ly;
+            concentric_disk(smp.lens0, smp.lens1, lx, ly);
+            lx = lx * p.aperture; ly = ly * p.aperture;
+            const f3 LI = add(scl(lx, U), scl(ly, V));
+            r.O = add(r.O, LI);
+            r.D = nrm(sub(scl(p.focal, r.D), LI));
+        }
+        D = r.D;
+        float Tv = 0.0f;
+        const bool vol = march<FULL, COUNT, ER_BATCH_PRIMARY>(p, T, r, true, rng, Tv, Pv, cnt);
+        float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
+        const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
+        int type = -1;
+        float Tn = 3.402823466e+38f;
+        if (vol && Tv < Tn) { type = 0; Tn = Tv; }
+        if (il >= 0 && Tl < Tn) { type = 1; Tn = Tl; }
+        if (type == 0) {
+            scatter = true;
+            if (COUNT) cnt.scatter++;
+        } else {
+            f3 Lv = mk(0.0f, 0.0f, 0.0f);
+            if (type == 1) Lv = add(Lv, Lel);
+            frame[i] = make_float4(Lv.x, Lv.y, Lv.z, type >= 0 ? 1.0f : 0.0f);
+            seeds1[i] = rng.s0;
+            seeds2[i] = rng.s1;
+        }
+    }
+    /* wave-aggregated append to the scatter queue */
+    const unsigned long long mask = __ballot(scatter);
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(qcount, (unsigned int)__popcll(mask));
+        base = __shfl(base, leader);
+        if (scatter) {
+            const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+            ScatterRec rec;
+            rec.pixel = (uint32_t)i; rec.s0 = rng.s0; rec.s1 = rng.s1; rec.pad = 0;
+            rec.Px = Pv.x; rec.Py = Pv.y; rec.Pz = Pv.z;
+            rec.Wox = -D.x; rec.Woy = -D.y; rec.Woz = -D.z;
+            rec.comp = smp.comp; rec.dir0 = smp.dir0; rec.dir1 = smp.dir1;
+            rec.uvw0 = smp.uvw0; rec.uvw1 = smp.uvw1; rec.uvw2 = smp.uvw2;
+            rec.lightnum = smp.lightnum; rec.spare = 0.0f;
+            queue[slot] = rec;
+        }
+    }
+}
+
+/* One wave shades queue entries q0 .. q0 + 63 (those below n): NormalizedGradient at the
+ * event (raymarching.h:70) and UniformSampleOneLight (transport.h:115-156). SPLIT: the shadow
+ * marches are appended to the shadow-job queue (*jcount) and the event waits in an EventRec;
+ * otherwise (Woodcock, whose draw count per march is not known in advance) the marches run
+ * inline. Every lane of the wave must call this (the append is wave-wide). */
+template<int FULL, int COUNT, int SPLIT>
+__device__ __forceinline__ void shade_chunk(const DevParams& p, const Tables& T, unsigned int q0, unsigned int n, int lane,
+                                            float4* __restrict__ frame, uint32_t* __restrict__ seeds1,
+                                            uint32_t* __restrict__ seeds2, const ScatterRec* __restrict__ queue,
+                                            EventRec* __restrict__ events, ShadowJob* __restrict__ jobq,
+                                            unsigned int* __restrict__ jcount, Cnt& cnt)
+{
+    const unsigned int q = q0 + lane;
+    const bool active = q < n;
+    bool has0 = false, has1 = false;
+    ShadowJob job0, job1;
+    if (active) {
+        const ScatterRec rec = queue[q];
+        Rng rng;
+        rng.s0 = rec.s0;
+        rng.s1 = rec.s1;
+        Sample smp;
+        smp.comp = rec.comp; smp.dir0 = rec.dir0; smp.dir1 = rec.dir1;
+        smp.uvw0 = rec.uvw0; smp.uvw1 = rec.uvw1; smp.uvw2 = rec.uvw2;
+        smp.lightnum = rec.lightnum;
+        const f3 Pv = mk(rec.Px, rec.Py, rec.Pz);
+        const f3 Wo = mk(rec.Wox, rec.Woy, rec.Woz);
+        const f3 Nv = normalized_gradient(p, Pv);
+        if (SPLIT) {
+            EventRec ev;
+            ev.pixel = rec.pixel;
+            ev.c1x = ev.c1y = ev.c1z = ev.c2x = ev.c2y = ev.c2z = 0.0f;
+            ev.pad = ev.pad2 = 0.0f;
+            shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1);
+            events[q] = ev;
+        } else {
+            f3 Lv = mk(0.0f, 0.0f, 0.0f);
+            Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, T, smp, Pv, Nv, Wo, rng, cnt));
+            frame[rec.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
+        }
+        seeds1[rec.pixel] = rng.s0;
+        seeds2[rec.pixel] = rng.s1;
+    }
+    if (SPLIT) {
+        /* wave-aggregated append of 0..2 shadow jobs per lane */
+        const unsigned long long m1 = __ballot(has0), m2 = __ballot(has1);
+        const unsigned int total = (unsigned int)(__popcll(m1) + __popcll(m2));
+        if (total) {
+            const int leader = __ffsll((long long)(m1 | m2)) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(jcount, total);
+            base = __shfl(base, leader);
+            const unsigned long long below = (1ull << lane) - 1ull;
+            const unsigned int slot = base + (unsigned int)(__popcll(m1 & below) + __popcll(m2 & below));
+            if (has0) { job0.event = q; job0.pad = 0; jobq[slot] = job0; }
+            if (has1) { job1.event = q; job1.pad = 0; jobq[slot + (has0 ? 1u : 0u)] = job1; }
+        }
+    }
+}
+
+/* ScatterEventInVolume (raymarching.h:73-113) for one deferred shadow ray; writes the
+ * visibility of the event's contribution. */
+template<int FULL, int COUNT>
+__device__ __forceinline__ void shadow_item(const DevParams& p, const Tables& T, const ShadowJob& job,
+                                            unsigned char* __restrict__ vis, Cnt& cnt)
+{
+    Ray r;
+    r.O = mk(job.Ox, job.Oy, job.Oz);
+    r.D = mk(job.Dx, job.Dy, job.Dz);
+    r.MinT = 0.0f;
+    r.MaxT = job.MaxT;
+    Rng rng;
+    rng.s0 = job.s0;
+    rng.s1 = job.s1;
+    float Tt; f3 P;
+    const bool hit = march<FULL, COUNT, ER_BATCH_SHADOW>(p, T, r, false, rng, Tt, P, cnt);
+    vis[2 * job.event + job.slot] = hit ? 0 : 1;
+}
+
+/* The event's radiance from its contributions and their visibility (the tail of
+ * UniformSampleOneLight, transport.h:152-155, and SingleScattering, singlescattering.h:96-99). */
+__device__ __forceinline__ void finish_item(const EventRec& ev, unsigned int q, const unsigned char* __restrict__ vis,
+                                            float4* __restrict__ frame)
+{
+    const uint32_t s1 = ev.state & 3u, s2 = (ev.state >> 2) & 3u;
+    const bool use1 = s1 == 1u || (s1 == 2u && vis[2 * q] != 0);
+    const bool use2 = s2 == 1u || (s2 == 2u && vis[2 * q + 1] != 0);
+    f3 Ld = mk(ev.Ld0x, ev.Ld0y, ev.Ld0z);
+    if (ev.count > 0.0f) {
+        f3 Ed = mk(0.0f, 0.0f, 0.0f);
+        if (use1) Ed = add(Ed, mk(ev.c1x, ev.c1y, ev.c1z));
+        if (use2) Ed = add(Ed, mk(ev.c2x, ev.c2y, ev.c2z));
+        Ld = scl(ev.count, add(Ld, Ed));
+    }
+    f3 Lv = mk(0.0f, 0.0f, 0.0f);
+    Lv = add(Lv, Ld);
+    frame[ev.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
+}
+
+/* Primary kernel: persistent waves take 8 x 8 pixel tiles from an atomic counter (counts[2]),
+ * so the cost of a tile - from a few samples for background pixels to thousands behind
+ * low-density tissue - is balanced over the whole device. */
 template<int FULL, int COUNT, int LUT_LDS>
 __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
@@ -99,87 +274,12 @@ __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ p
         if (lane == 0) tile = atomicAdd(&counts[2], 1u);
         tile = __shfl(tile, 0);
         if (tile >= (unsigned int)ntiles) break;
-        const int px = (int)(tile % (unsigned int)tiles_x) * 8 + (lane & 7);
-        const int py = (int)(tile / (unsigned int)tiles_x) * 8 + (lane >> 3);
-        const bool active = px < p.film_w && py < p.film_h;
-        const size_t i = (size_t)py * p.film_w + px;
-        bool scatter = false;
-        Sample smp;
-        Rng rng;
-        f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f);
-        if (active) {
-            paths++;
-            rng.s0 = seeds1[i];
-            rng.s1 = seeds2[i];
-            smp.comp = rng.u();
-            smp.dir0 = rng.u(); smp.dir1 = rng.u();
-            smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
-            smp.lightnum = rng.u();
-            smp.film0 = rng.u(); smp.film1 = rng.u();
-            smp.lens0 = rng.u(); smp.lens1 = rng.u();
-            const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
-            const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
-            const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
-            Ray r;
-            r.O = ld3(p.cam_pos);
-            r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
-            r.MinT = p.clip_near;
-            r.MaxT = p.clip_far;
-            if (p.aperture != 0.0f) {
-                float lx, ly;
-                concentric_disk(smp.lens0, smp.lens1, lx, ly);
-                lx = lx * p.aperture; ly = ly * p.aperture;
-                const f3 LI = add(scl(lx, U), scl(ly, V));
-                r.O = add(r.O, LI);
-                r.D = nrm(sub(scl(p.focal, r.D), LI));
-            }
-            D = r.D;
-            float Tv = 0.0f;
-            const bool vol = march<FULL, COUNT, ER_BATCH_PRIMARY>(p, T, r, true, rng, Tv, Pv, cnt);
-            float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
-            const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
-            int type = -1;
-            float Tn = 3.402823466e+38f;
-            if (vol && Tv < Tn) { type = 0; Tn = Tv; }
-            if (il >= 0 && Tl < Tn) { type = 1; Tn = Tl; }
-            if (type == 0) {
-                scatter = true;
-                if (COUNT) cnt.scatter++;
-            } else {
-                f3 Lv = mk(0.0f, 0.0f, 0.0f);
-                if (type == 1) Lv = add(Lv, Lel);
-                frame[i] = make_float4(Lv.x, Lv.y, Lv.z, type >= 0 ? 1.0f : 0.0f);
-                seeds1[i] = rng.s0;
-                seeds2[i] = rng.s1;
-            }
-        }
-        /* wave-aggregated append to the scatter queue */
-        const unsigned long long mask = __ballot(scatter);
-        if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(&counts[0], (unsigned int)__popcll(mask));
-            base = __shfl(base, leader);
-            if (scatter) {
-                const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
-                ScatterRec rec;
-                rec.pixel = (uint32_t)i; rec.s0 = rng.s0; rec.s1 = rng.s1; rec.pad = 0;
-                rec.Px = Pv.x; rec.Py = Pv.y; rec.Pz = Pv.z;
-                rec.Wox = -D.x; rec.Woy = -D.y; rec.Woz = -D.z;
-                rec.comp = smp.comp; rec.dir0 = smp.dir0; rec.dir1 = smp.dir1;
-                rec.uvw0 = smp.uvw0; rec.uvw1 = smp.uvw1; rec.uvw2 = smp.uvw2;
-                rec.lightnum = smp.lightnum; rec.spare = 0.0f;
-                queue[slot] = rec;
-            }
-        }
+        primary_tile<FULL, COUNT>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths);
     }
     flush_counters<COUNT>(cnt, true, paths, counters);
 }
 
-/* Shading kernel: for each queued volume event, NormalizedGradient at the event
- * (raymarching.h:70) and UniformSampleOneLight (transport.h:115-156). SPLIT: the shadow
- * marches are appended to the shadow-job queue and the event waits in an EventRec; otherwise
- * (Woodcock, whose draw count per march is not known in advance) the marches run inline. */
+/* Shading kernel over the scatter queue (counts[0] entries), wave-aligned grid stride. */
 template<int FULL, int COUNT, int LUT_LDS, int SPLIT>
 __global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                               uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
@@ -194,61 +294,13 @@ __global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp,
     const int lane = threadIdx.x & 63;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const unsigned int stride = gridDim.x * blockDim.x;
-    /* every lane of a wave runs the same number of iterations (the wave-wide append needs it) */
     const unsigned int base_q = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (unsigned int q0 = base_q; q0 < n; q0 += stride) {
-        const unsigned int q = q0 + lane;
-        const bool active = q < n;
-        bool has0 = false, has1 = false;
-        ShadowJob job0, job1;
-        if (active) {
-            const ScatterRec rec = queue[q];
-            Rng rng;
-            rng.s0 = rec.s0;
-            rng.s1 = rec.s1;
-            Sample smp;
-            smp.comp = rec.comp; smp.dir0 = rec.dir0; smp.dir1 = rec.dir1;
-            smp.uvw0 = rec.uvw0; smp.uvw1 = rec.uvw1; smp.uvw2 = rec.uvw2;
-            smp.lightnum = rec.lightnum;
-            const f3 Pv = mk(rec.Px, rec.Py, rec.Pz);
-            const f3 Wo = mk(rec.Wox, rec.Woy, rec.Woz);
-            const f3 Nv = normalized_gradient(p, Pv);
-            if (SPLIT) {
-                EventRec ev;
-                ev.pixel = rec.pixel;
-                ev.c1x = ev.c1y = ev.c1z = ev.c2x = ev.c2y = ev.c2z = 0.0f;
-                ev.pad = ev.pad2 = 0.0f;
-                shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1);
-                events[q] = ev;
-            } else {
-                f3 Lv = mk(0.0f, 0.0f, 0.0f);
-                Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, T, smp, Pv, Nv, Wo, rng, cnt));
-                frame[rec.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
-            }
-            seeds1[rec.pixel] = rng.s0;
-            seeds2[rec.pixel] = rng.s1;
-        }
-        if (SPLIT) {
-            /* wave-aggregated append of 0..2 shadow jobs per lane */
-            const unsigned long long m1 = __ballot(has0), m2 = __ballot(has1);
-            const unsigned int total = (unsigned int)(__popcll(m1) + __popcll(m2));
-            if (total) {
-                const int leader = __ffsll((long long)(m1 | m2)) - 1;
-                unsigned int base = 0;
-                if (lane == leader) base = atomicAdd(&counts[1], total);
-                base = __shfl(base, leader);
-                const unsigned long long below = (1ull << lane) - 1ull;
-                const unsigned int slot = base + (unsigned int)(__popcll(m1 & below) + __popcll(m2 & below));
-                if (has0) { job0.event = q; job0.pad = 0; jobq[slot] = job0; }
-                if (has1) { job1.event = q; job1.pad = 0; jobq[slot + (has0 ? 1u : 0u)] = job1; }
-            }
-        }
-    }
+    for (unsigned int q0 = base_q; q0 < n; q0 += stride)
+        shade_chunk<FULL, COUNT, SPLIT>(p, T, q0, n, lane, frame, seeds1, seeds2, queue, events, jobq, &counts[1], cnt);
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
-/* Shadow-march kernel: ScatterEventInVolume (raymarching.h:73-113) for each deferred shadow
- * ray; writes the visibility of the event's contribution. */
+/* Shadow-march kernel over the shadow-job queue (counts[1] entries). */
 template<int FULL, int COUNT, int LUT_LDS>
 __global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp, const unsigned int* __restrict__ counts,
                                                const ShadowJob* __restrict__ jobq, unsigned char* __restrict__ vis,
@@ -259,44 +311,136 @@ __global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp
     const Tables T = stage_tables<LUT_LDS>(p, lds);
     const unsigned int n = counts[1];
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const ShadowJob job = jobq[j];
-        Ray r;
-        r.O = mk(job.Ox, job.Oy, job.Oz);
-        r.D = mk(job.Dx, job.Dy, job.Dz);
-        r.MinT = 0.0f;
-        r.MaxT = job.MaxT;
-        Rng rng;
-        rng.s0 = job.s0;
-        rng.s1 = job.s1;
-        float Tt; f3 P;
-        const bool hit = march<FULL, COUNT, ER_BATCH_SHADOW>(p, T, r, false, rng, Tt, P, cnt);
-        vis[2 * job.event + job.slot] = hit ? 0 : 1;
-    }
+    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        shadow_item<FULL, COUNT>(p, T, jobq[j], vis, cnt);
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
-/* Completes each deferred event in the order of estimate_direct and UniformSampleOneLight:
- * Ld = (0 + emission) + ((0 + c1) + c2), times the light count. */
 __global__ __launch_bounds__(256) void k_finish(const unsigned int* __restrict__ counts, const EventRec* __restrict__ events,
                                                const unsigned char* __restrict__ vis, float4* __restrict__ frame)
 {
     const unsigned int n = counts[0];
-    for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const EventRec ev = events[q];
-        const uint32_t s1 = ev.state & 3u, s2 = (ev.state >> 2) & 3u;
-        const bool use1 = s1 == 1u || (s1 == 2u && vis[2 * q] != 0);
-        const bool use2 = s2 == 1u || (s2 == 2u && vis[2 * q + 1] != 0);
-        f3 Ld = mk(ev.Ld0x, ev.Ld0y, ev.Ld0z);
-        if (ev.count > 0.0f) {
-            f3 Ed = mk(0.0f, 0.0f, 0.0f);
-            if (use1) Ed = add(Ed, mk(ev.c1x, ev.c1y, ev.c1z));
-            if (use2) Ed = add(Ed, mk(ev.c2x, ev.c2y, ev.c2z));
-            Ld = scl(ev.count, add(Ld, Ed));
-        }
-        f3 Lv = mk(0.0f, 0.0f, 0.0f);
-        Lv = add(Lv, Ld);
-        frame[ev.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
+    for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        finish_item(events[q], q, vis, frame);
+}
+
+/* ---- batches: the same pass over many tracers per launch (er_render_batch) ------------ */
+
+/* tables of a batch job: occupancy bits and opacity table read from global memory (the
+ * jobs of one workgroup change from tile to tile) */
+__device__ __forceinline__ Tables global_tables(const DevParams& p)
+{
+    Tables T;
+    T.bits = nullptr;
+    T.bits_glb = gmem(p.mc_bits);
+    T.lut_lds = nullptr;
+    T.lut_glb = gmem(p.sigma_lut);
+    return T;
+}
+
+/* Primary kernel of a batch round: persistent waves take 8 x 8 tiles from one counter over
+ * the tiles of every active job; a wave finds its tile's job in the table. */
+template<int FULL>
+__global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ J, int njobs, BatchState* __restrict__ st)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned int round = st->round;
+    const int na = batch_active(J, njobs, round);
+    if (na == 0) return;
+    const unsigned int total = (unsigned int)(J[na - 1].tile_base + J[na - 1].ntiles);
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int paths = 0;
+    int j = 0;
+    for (;;) {
+        unsigned int tile = 0;
+        if (lane == 0) tile = atomicAdd(&st->tiles, 1u);
+        tile = __shfl(tile, 0);
+        if (tile >= total) break;
+        while (tile >= (unsigned int)(J[j].tile_base + J[j].ntiles)) ++j;   /* tiles arrive in increasing order */
+        const BatchJob& B = J[j];
+        const DevParams& p = *B.params;
+        const Tables T = global_tables(p);
+        primary_tile<FULL, 0>(p, T, tile - (unsigned int)B.tile_base, B.tiles_x, lane, B.frame, B.seeds1, B.seeds2,
+                              B.queue, &B.counts[0], cnt, paths);
+    }
+}
+
+/* wave slot w of a per-job work list of count[which] items per job: job index and the
+ * slot's first item; false when w is past the last job */
+__device__ __forceinline__ bool batch_slot(const BatchJob* __restrict__ J, int na, int which, unsigned int w, int& job,
+                                           unsigned int& first, unsigned int& n)
+{
+    unsigned int base = 0;
+    for (int k = 0; k < na; ++k) {
+        const unsigned int c = J[k].counts[which];
+        const unsigned int waves = (c + 63u) >> 6;
+        if (w < base + waves) { job = k; first = (w - base) << 6; n = c; return true; }
+        base += waves;
+    }
+    return false;
+}
+
+template<int FULL>
+__global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+{
+    const int na = batch_active(J, njobs, st->round);
+    const int lane = threadIdx.x & 63;
+    const unsigned int waves = gridDim.x * (blockDim.x >> 6);
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);; w += waves) {
+        int job; unsigned int q0, n;
+        if (!batch_slot(J, na, 0, w, job, q0, n)) break;
+        const BatchJob& B = J[job];
+        const DevParams& p = *B.params;
+        const Tables T = global_tables(p);
+        shade_chunk<FULL, 0, 1>(p, T, q0, n, lane, B.frame, B.seeds1, B.seeds2, B.queue, B.events, B.jobs, &B.counts[1], cnt);
+    }
+}
+
+template<int FULL>
+__global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+{
+    const int na = batch_active(J, njobs, st->round);
+    const int lane = threadIdx.x & 63;
+    const unsigned int waves = gridDim.x * (blockDim.x >> 6);
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);; w += waves) {
+        int job; unsigned int j0, n;
+        if (!batch_slot(J, na, 1, w, job, j0, n)) break;
+        const BatchJob& B = J[job];
+        const DevParams& p = *B.params;
+        const Tables T = global_tables(p);
+        const unsigned int jq = j0 + lane;
+        if (jq < n) shadow_item<FULL, 0>(p, T, B.jobs[jq], B.vis, cnt);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+{
+    const int na = batch_active(J, njobs, st->round);
+    const int lane = threadIdx.x & 63;
+    const unsigned int waves = gridDim.x * (blockDim.x >> 6);
+    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);; w += waves) {
+        int job; unsigned int q0, n;
+        if (!batch_slot(J, na, 0, w, job, q0, n)) break;
+        const BatchJob& B = J[job];
+        const unsigned int q = q0 + lane;
+        if (q < n) finish_item(B.events[q], q, B.vis, B.frame);
+    }
+}
+
+/* end of a round: empty every job's queues and advance the round (one workgroup) */
+__global__ __launch_bounds__(64) void k_round_end(const BatchJob* __restrict__ J, int njobs, BatchState* __restrict__ st)
+{
+    const unsigned int round = st->round;
+    for (int k = threadIdx.x; k < njobs; k += blockDim.x) {
+        J[k].counts[0] = 0u;
+        J[k].counts[1] = 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->round = round + 1u;
+        st->tiles = 0u;
     }
 }
 
@@ -354,5 +498,27 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
         else { if (lut_lds) ER_ALL(0, 0, 1); else ER_ALL(0, 0, 0); }
     }
 #undef ER_ALL
+    return hipGetLastError();
+}
+
+extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, erd::BatchState* st, int full,
+                                             int primary_blocks, int shade_blocks, hipStream_t stream)
+{
+    if (full) {
+        hipLaunchKernelGGL(erd::k_primary_b<1>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st);
+        hipLaunchKernelGGL(erd::k_shade_b<1>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+        hipLaunchKernelGGL(erd::k_shadow_b<1>, dim3(2 * shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+    } else {
+        hipLaunchKernelGGL(erd::k_primary_b<0>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st);
+        hipLaunchKernelGGL(erd::k_shade_b<0>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+        hipLaunchKernelGGL(erd::k_shadow_b<0>, dim3(2 * shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+    }
+    hipLaunchKernelGGL(erd::k_finish_b, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, erd::BatchState* st, hipStream_t stream)
+{
+    hipLaunchKernelGGL(erd::k_round_end, dim3(1), dim3(64), 0, stream, J, njobs, st);
     return hipGetLastError();
 }

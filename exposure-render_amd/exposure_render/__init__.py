@@ -190,6 +190,19 @@ class Renderer:
         lib.er_bind_volume(C.byref(self.volume), 0)
 
 
+def render_batch(renderers, passes):
+    """er_render_batch: passes[i] progressive passes of renderers[i], all in one graph-captured
+    pass pipeline per round. Returns the capi.Timing of the whole batch."""
+    lib = load()
+    items = (capi.BatchItem * len(renderers))()
+    for k, (r, n) in enumerate(zip(renderers, passes)):
+        items[k].tracer_id = r.id
+        items[k].passes = int(n)
+    t = capi.Timing()
+    _check(lib, lib.er_render_batch(items, len(renderers), C.byref(t)))
+    return t
+
+
 def shutdown():
     lib = load()
     _check(lib, lib.er_shutdown())

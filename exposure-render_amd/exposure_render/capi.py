@@ -165,6 +165,10 @@ TF_LUT_NODES = 256   # er_capi.h ER_TF_LUT_NODES
 
 # Every extern "C" symbol include/er_capi.h declares, with its ctypes signature.
 _P = C.POINTER
+class BatchItem(C.Structure):
+    _fields_ = [("tracer_id", C.c_int), ("passes", C.c_int)]
+
+
 SIGNATURES = {
     "er_abi_version": (C.c_int, []),
     "er_abi_sizeof": (C.c_int, [_P(C.c_int), C.c_int]),
@@ -199,6 +203,8 @@ SIGNATURES = {
     "er_get_running_xyza": (C.c_int, [C.c_int, _P(C.c_float), C.c_size_t]),
     "er_render_passes": (C.c_int, [C.c_int, C.c_int, _P(Timing)]),
     "er_set_tf_lut": (C.c_int, [C.c_int, _P(C.c_float), C.c_float]),
+    "er_render_batch": (C.c_int, [_P(BatchItem), C.c_int, _P(Timing)]),
+    "er_set_list_capacity": (C.c_int, [C.c_int]),
     "er_adjoint_lut": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
                                  _P(C.c_double)]),
     "er_adjoint_value": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
@@ -215,6 +221,9 @@ SIGNATURES = {
                                         C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
                                         C.c_float, C.c_uint64]),
 }
+
+
+
 
 _lib = None
 

@@ -394,3 +394,61 @@ def config_adjoint(scale=1.0, seed=4):
     sc.passes = 32
     sc.tf_range = 4095.0
     return sc
+
+
+# ---- batching workload -----------------------------------------------------------------------
+
+def _random_plf_tf(t, rng, range_=4095.0):
+    """A seeded 4-8-node transfer function on the tracer's PLFs: transparent below a random
+    threshold, random opacities and colours above."""
+    k = int(rng.integers(4, 9))
+    thr = float(rng.uniform(150.0, 700.0))
+    pos = np.sort(rng.uniform(thr + 20.0, range_, size=k - 2))
+    positions = np.concatenate([[0.0, thr], pos])
+    op = np.concatenate([[0.0, 0.0], rng.uniform(0.05, 1.0, size=k - 2)])
+    for x, o in zip(positions, op):
+        t.opacity.add(float(x), float(o))
+        c = rng.uniform(0.3, 1.0, size=3)
+        t.diffuse.add(float(x), tuple(float(v) for v in c))
+    t.specular.add(0.0, (0.1, 0.1, 0.1)).add(range_, (0.3, 0.3, 0.3))
+    t.glossiness.add(0.0, 0.4).add(range_, 0.6)
+
+
+def _unit(v):
+    v = np.asarray(v, dtype=np.float64)
+    return v / np.linalg.norm(v)
+
+
+def batch_job(seed, scale=1.0):
+    """One job of the batching workload: a blob or ellipsoid phantom with an edge length from
+    {32, 64, 128, 192, 256}, an image side uniform in [128, 1024], a 4-8-node transfer function,
+    1-3 plane lights and 8-128 passes (sc.passes)."""
+    rng = np.random.default_rng(seed)
+    edge = _scaled(int(rng.choice([32, 64, 128, 192, 256])), scale)
+    if rng.random() < 0.5:
+        vox = phantom_blobs((edge, edge, edge), int(rng.integers(2, 9)), seed=int(rng.integers(1 << 30)))
+    else:
+        vox = phantom_ellipsoids((edge, edge, edge), int(rng.integers(4, 13)), 0, (0.05, 0.45), 5, 4095.0, 20.0,
+                                 int(rng.integers(1 << 30)))
+    sc = Scene(vox)
+    sc.name = f"batch_job_{seed}"
+    t = sc.tracer
+    _random_plf_tf(t, rng)
+    t.settings.density_scale = float(rng.uniform(10.0, 100.0))
+    t.settings.estimator = capi.ESTIMATOR_FULL
+    t.settings.shading_type = capi.SHADING_PHASE_ONLY if rng.random() < 0.5 else capi.SHADING_HYBRID
+    side = _scaled(int(rng.integers(128, 1025)), scale)
+    cam = 2.2 * _unit(rng.normal(size=3))
+    set_camera(t, (side, side), tuple(cam), (0.0, 0.0, 0.0), exposure=float(rng.uniform(2.0, 8.0)))
+    tex = sc.add_texture(uniform_texture((1.0, 1.0, 1.0)))
+    for _ in range(int(rng.integers(1, 4))):
+        pos = 1.5 * _unit(rng.normal(size=3))
+        s = float(rng.uniform(0.3, 0.6))
+        sc.add_light(plane_light(tuple(pos), (0.0, 0.0, 0.0), (s, s), float(rng.uniform(50.0, 150.0)), texture_id=tex))
+    sc.passes = int(rng.integers(8, 129))
+    return sc
+
+
+def config_batch(n_jobs=64, scale=1.0, seed=5):
+    """Batching: `n_jobs` independent jobs (batch_job) from one seed."""
+    return [batch_job(seed * 1000 + k, scale) for k in range(n_jobs)]

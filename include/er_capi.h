@@ -352,6 +352,28 @@ ER_API int er_get_gradient_magnitude_max(int tracer_id, float* gm_max);
 /* -log(u) table entry used for the free-path threshold, for u = m * 2^-23. */
 ER_API int er_neglog_table_entry(uint32_t m, float* value);
 
+/* Capacity of each resource list (Source/list.cuh:27 MaxSize = 32: at most capacity - 1 live
+ * objects per kind, further binds are ignored). The default is the reference's; hosts that
+ * keep many scenes bound at once (batches) raise it. A single tracer still uses at most
+ * ER_LIST_CAPACITY distinct lights and textures. */
+ER_API int er_set_list_capacity(int capacity);
+
+/* ---- batches (new; no counterpart in the reference) ----------------------------------- */
+
+typedef struct er_batch_item {
+    int tracer_id;
+    int passes;
+} er_batch_item;
+
+/* Renders items[i].passes progressive passes of each tracer - the same result, bit for bit,
+ * as er_render_passes on each tracer in turn - with one pass round of every tracer per launch
+ * sequence: a persistent primary kernel over the tiles of all tracers still rendering, then
+ * shading, shadow-march, combine and post kernels over all of their queues. The round is
+ * captured once in a HIP graph and replayed max(passes) times. Each tracer may appear once;
+ * all must use the same estimator mode, and none the Woodcock sampler. timing may be NULL
+ * (trace_ms then covers whole rounds). */
+ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing);
+
 /* ---- differentiable transfer function (new; no counterpart in the reference) ----------- */
 
 /* Node-table transfer function. lut holds ER_TF_LUT_NODES x 4 floats, node-major: opacity,
