@@ -35,7 +35,8 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
                                        unsigned int* counts, void* events, void* jobs, unsigned char* vis,
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
-                                       int primary_blocks, hipStream_t st);
+                                       int primary_blocks, hipStream_t st, hipStream_t st2, hipEvent_t shaded,
+                                       hipEvent_t after_primary);
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
@@ -177,16 +178,36 @@ struct Volume {
     uint16_t vmax = 0;
 };
 
+/* The state of one pass in flight: its frame estimate and its work queues. A tracer keeps
+ * two, so that a pass's shadow marches, combine and post kernels (second stream) overlap the
+ * next pass's primary and shading kernels (first stream). */
+struct PassSet {
+    DevBuf<float4> frame;
+    DevBuf<uint32_t> qcount;        /* [0] scatter queue, [1] shadow jobs, [2] primary tiles */
+    DevBuf<char> queue, events, jobs;
+    DevBuf<unsigned char> vis;
+    hipEvent_t shaded = nullptr;    /* first stream: primary + shading of this set's pass done */
+    hipEvent_t posted = nullptr;    /* second stream: the pass's post done, the set is free */
+    bool pending = false;           /* posted has been recorded */
+    void release()
+    {
+        frame.release(); qcount.release(); queue.release(); events.release(); jobs.release(); vis.release();
+        if (shaded) (void)hipEventDestroy(shaded);
+        if (posted) (void)hipEventDestroy(posted);
+        shaded = posted = nullptr;
+    }
+};
+
 struct Tracer {
     er_tracer_desc desc{};
     int W = 0, H = 0;
-    DevBuf<float4> frame, running;
+    PassSet set[2];
+    int cur = 0;                    /* set of the last pass */
+    DevBuf<float4> running;
     DevBuf<uchar4> display, disp_tmp, disp_filt;
     DevBuf<uint32_t> seeds1, seeds2;
     DevBuf<float> lut, mc_sigma;
-    DevBuf<uint32_t> mc_bits, qcount;
-    DevBuf<char> queue, events, jobs;
-    DevBuf<unsigned char> vis;
+    DevBuf<uint32_t> mc_bits;
     size_t lds_bytes = 0;
     int lut_lds = 0;
     DevBuf<DevParams> params;
@@ -201,16 +222,26 @@ struct Tracer {
     std::vector<float> tf_lut;       /* node table (er_set_tf_lut), empty when the PLFs are used */
     float tf_scale = 0.0f;
     DevBuf<float> tf_lut_dev;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     /* primary + shading, uploads, readbacks */
+    hipStream_t stream2 = nullptr;    /* shadow marches, combine, post */
     ~Tracer()
     {
-        frame.release(); running.release(); display.release(); disp_tmp.release(); disp_filt.release();
+        set[0].release(); set[1].release();
+        running.release(); display.release(); disp_tmp.release(); disp_filt.release();
         seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); params.release();
-        mc_bits.release(); qcount.release(); queue.release(); events.release(); jobs.release(); vis.release();
-        counters.release(); gm_tmp.release(); bilateral.release();
+        mc_bits.release(); counters.release(); gm_tmp.release(); bilateral.release();
         if (stream) (void)hipStreamDestroy(stream);
+        if (stream2) (void)hipStreamDestroy(stream2);
     }
 };
+
+/* both streams of a tracer idle */
+int sync_tracer(Tracer& t)
+{
+    HIP_TRY(hipStreamSynchronize(t.stream));
+    if (t.stream2) HIP_TRY(hipStreamSynchronize(t.stream2));
+    return ER_OK;
+}
 
 template<class T> struct Registry {
     std::map<int, T*> map;
@@ -287,26 +318,34 @@ int tracer_seed(Tracer& t, uint64_t seed)
 /* FrameBuffer::Resize (Source/framebuffer.h:45-68): buffers, zeroed estimates, new seeds */
 int tracer_resize(Tracer& t, int W, int H, int id)
 {
-    if (W == t.W && H == t.H && t.frame.ptr) return ER_OK;
+    if (W == t.W && H == t.H && t.set[0].frame.ptr) return ER_OK;
     if (W < 0 || H < 0) return set_error(ER_ERROR, "invalid film size %d x %d", W, H);
-    const size_t n = (size_t)W * H;
     int st;
-    if ((st = t.frame.alloc(n)) || (st = t.running.alloc(n)) || (st = t.display.alloc(n)) ||
-        (st = t.disp_tmp.alloc(n)) || (st = t.disp_filt.alloc(n)) || (st = t.seeds1.alloc(n)) ||
-        (st = t.seeds2.alloc(n)))
+    if ((st = sync_tracer(t))) return st;
+    const size_t n = (size_t)W * H;
+    if ((st = t.running.alloc(n)) || (st = t.display.alloc(n)) || (st = t.disp_tmp.alloc(n)) ||
+        (st = t.disp_filt.alloc(n)) || (st = t.seeds1.alloc(n)) || (st = t.seeds2.alloc(n)))
         return st;
     size_t rs = 0, re = 0, rj = 0;
     erd_pass_record_sizes(&rs, &re, &rj);
-    if ((st = t.queue.alloc(n * rs)) || (st = t.events.alloc(n * re)) || (st = t.jobs.alloc(2 * n * rj)) ||
-        (st = t.vis.alloc(2 * n)) || (st = t.qcount.alloc(4)))
-        return st;
+    for (PassSet& S : t.set) {
+        if ((st = S.frame.alloc(n)) || (st = S.queue.alloc(n * rs)) || (st = S.events.alloc(n * re)) ||
+            (st = S.jobs.alloc(2 * n * rj)) || (st = S.vis.alloc(2 * n)) || (st = S.qcount.alloc(4)))
+            return st;
+        if (!S.shaded) HIP_TRY(hipEventCreateWithFlags(&S.shaded, hipEventDisableTiming));
+        if (!S.posted) HIP_TRY(hipEventCreateWithFlags(&S.posted, hipEventDisableTiming));
+        S.pending = false;
+        if (n) {
+            HIP_TRY(hipMemset(S.frame.ptr, 0, n * sizeof(float4)));
+            HIP_TRY(hipMemset(S.qcount.ptr, 0, 4 * sizeof(uint32_t)));
+        }
+    }
+    t.cur = 0;
     t.W = W; t.H = H;
     if (n) {
-        HIP_TRY(hipMemset(t.frame.ptr, 0, n * sizeof(float4)));
         HIP_TRY(hipMemset(t.running.ptr, 0, n * sizeof(float4)));
         HIP_TRY(hipMemset(t.display.ptr, 0, n * sizeof(uchar4)));
         HIP_TRY(hipMemset(t.disp_filt.ptr, 0, n * sizeof(uchar4)));
-        HIP_TRY(hipMemset(t.qcount.ptr, 0, 4 * sizeof(uint32_t)));
         return tracer_seed(t, 0x45787052ull + (uint64_t)id);
     }
     return ER_OK;
@@ -588,23 +627,34 @@ bool use_bilateral(const Tracer& t)
 }
 
 /* one pass on the tracer's stream (no host synchronisation) */
-int enqueue_pass(Tracer& t, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
+/* One pass on the tracer's two streams (no host synchronisation). Timing events, when given:
+ * e[0] first stream before the primary kernel, e[1] after it, e[2] second stream after the
+ * combine kernel, e[3] after post. */
+int enqueue_pass(Tracer& t, hipEvent_t* e)
 {
     if (t.W == 0 || t.H == 0) return ER_OK;
+    const int b = 1 - t.cur;
+    PassSet& S = t.set[b];
     const int full = t.desc.settings.estimator == ER_ESTIMATOR_FULL;
     const int shade_blocks = std::max(1, std::min((t.W * t.H + 255) / 256, 2048));
-    if (e0) HIP_TRY(hipEventRecord(e0, t.stream));
+    /* the set was last used two passes ago: its post must have consumed the frame and queues */
+    if (S.pending) HIP_TRY(hipStreamWaitEvent(t.stream, S.posted, 0));
+    if (e) HIP_TRY(hipEventRecord(e[0], t.stream));
     const int split = !(full && t.desc.settings.sampler == ER_SAMPLER_WOODCOCK);
     const size_t lds_bits = (size_t)t.host_params.mc_words * 4;
-    HIP_TRY(erd_launch_trace(t.params.ptr, t.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, t.queue.ptr, t.qcount.ptr,
-                             t.events.ptr, t.jobs.ptr, t.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
-                             split, t.lds_bytes, lds_bits, shade_blocks, g_primary_blocks, t.stream));
-    if (e1) HIP_TRY(hipEventRecord(e1, t.stream));
-    HIP_TRY(erd_launch_post(t.params.ptr, t.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations,
-                            t.qcount.ptr, t.W, t.H, t.stream));
+    HIP_TRY(erd_launch_trace(t.params.ptr, S.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, S.queue.ptr, S.qcount.ptr,
+                             S.events.ptr, S.jobs.ptr, S.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
+                             split, t.lds_bytes, lds_bits, shade_blocks, g_primary_blocks, t.stream, t.stream2, S.shaded,
+                             e ? e[1] : nullptr));
+    if (e) HIP_TRY(hipEventRecord(e[2], t.stream2));
+    HIP_TRY(erd_launch_post(t.params.ptr, S.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations,
+                            S.qcount.ptr, t.W, t.H, t.stream2));
     if (use_bilateral(t))
-        HIP_TRY(erd_launch_bilateral(t.bilateral.ptr, t.display.ptr, t.disp_tmp.ptr, t.disp_filt.ptr, t.W, t.H, t.stream));
-    if (e2) HIP_TRY(hipEventRecord(e2, t.stream));
+        HIP_TRY(erd_launch_bilateral(t.bilateral.ptr, t.display.ptr, t.disp_tmp.ptr, t.disp_filt.ptr, t.W, t.H, t.stream2));
+    HIP_TRY(hipEventRecord(S.posted, t.stream2));
+    S.pending = true;
+    if (e) HIP_TRY(hipEventRecord(e[3], t.stream2));
+    t.cur = b;
     t.desc.no_iterations++;   /* core.cu:135 */
     return ER_OK;
 }
@@ -799,6 +849,7 @@ ER_API int er_bind_tracer(er_tracer_desc* d, int bind)
         if (fresh) {
             t = new Tracer();
             hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->stream2, hipStreamNonBlocking);
             if (e != hipSuccess) { delete t; t = nullptr; return set_error(ER_ERROR, "%s (hipStreamCreate)", hipGetErrorString(e)); }
             int st = t->counters.alloc(12);
             if (!st && hipMemset(t->counters.ptr, 0, 12 * sizeof(unsigned long long)) != hipSuccess)
@@ -895,8 +946,8 @@ ER_API int er_render_estimate(int tracer_id)
     Tracer* t = get_tracer(tracer_id, st);
     if (!t) return st;
     if ((st = build_params(*t))) return st;
-    if ((st = enqueue_pass(*t, nullptr, nullptr, nullptr))) return st;
-    HIP_TRY(hipStreamSynchronize(t->stream));
+    if ((st = enqueue_pass(*t, nullptr))) return st;
+    if ((st = sync_tracer(*t))) return st;
     return ER_OK;
 }
 
@@ -910,20 +961,23 @@ ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
     if ((st = build_params(*t))) return st;
     std::vector<hipEvent_t> ev;
     if (timing) {
-        ev.resize((size_t)passes * 3, nullptr);
+        ev.resize((size_t)passes * 4, nullptr);
         for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     }
-    for (int k = 0; k < passes && !st; ++k)
-        st = enqueue_pass(*t, timing ? ev[3 * k] : nullptr, timing ? ev[3 * k + 1] : nullptr, timing ? ev[3 * k + 2] : nullptr);
+    for (int k = 0; k < passes && !st; ++k) st = enqueue_pass(*t, timing ? &ev[4 * k] : nullptr);
     hipError_t se = hipStreamSynchronize(t->stream);
+    hipError_t se2 = hipStreamSynchronize(t->stream2);
+    if (se == hipSuccess) se = se2;
     if (timing && !st && se == hipSuccess) {
-        float total = 0, tr = 0, po = 0, ms = 0;
+        float total = 0, tr = 0, po = 0, pr = 0, ms = 0;
         for (int k = 0; k < passes; ++k) {
-            if (hipEventElapsedTime(&ms, ev[3 * k], ev[3 * k + 1]) == hipSuccess) tr += ms;
-            if (hipEventElapsedTime(&ms, ev[3 * k + 1], ev[3 * k + 2]) == hipSuccess) po += ms;
+            if (hipEventElapsedTime(&ms, ev[4 * k], ev[4 * k + 1]) == hipSuccess) pr += ms;
+            if (hipEventElapsedTime(&ms, ev[4 * k], ev[4 * k + 2]) == hipSuccess) tr += ms;
+            if (hipEventElapsedTime(&ms, ev[4 * k + 2], ev[4 * k + 3]) == hipSuccess) po += ms;
         }
-        if (passes > 0 && hipEventElapsedTime(&ms, ev[0], ev[3 * (passes - 1) + 2]) == hipSuccess) total = ms;
+        if (passes > 0 && hipEventElapsedTime(&ms, ev[0], ev[4 * (passes - 1) + 3]) == hipSuccess) total = ms;
         timing->total_ms = total; timing->trace_ms = tr; timing->post_ms = po; timing->passes = passes;
+        timing->primary_ms = pr;
     }
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (st) return st;
@@ -958,7 +1012,7 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         if (f && s.sampler == ER_SAMPLER_WOODCOCK) return set_error(ER_ERROR, "the Woodcock sampler is not batched");
         full = f;
         if ((st = build_params(*t))) return st;
-        HIP_TRY(hipStreamSynchronize(t->stream));
+        if ((st = sync_tracer(*t))) return st;
         if (items[k].passes > 0 && t->W > 0 && t->H > 0) jobs.push_back({items[k].passes, t});
     }
     if (jobs.empty()) return ER_OK;
@@ -972,11 +1026,12 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         Tracer& t = *jobs[k].second;
         erd::BatchJob& B = table[k];
         B.params = t.params.ptr;
-        B.frame = t.frame.ptr; B.running = t.running.ptr; B.display = t.display.ptr;
+        PassSet& S0 = t.set[0];
+        B.frame = S0.frame.ptr; B.running = t.running.ptr; B.display = t.display.ptr;
         B.seeds1 = t.seeds1.ptr; B.seeds2 = t.seeds2.ptr;
-        B.queue = (erd::ScatterRec*)t.queue.ptr; B.events = (erd::EventRec*)t.events.ptr;
-        B.jobs = (erd::ShadowJob*)t.jobs.ptr; B.vis = t.vis.ptr;
-        B.counts = t.qcount.ptr;
+        B.queue = (erd::ScatterRec*)S0.queue.ptr; B.events = (erd::EventRec*)S0.events.ptr;
+        B.jobs = (erd::ShadowJob*)S0.jobs.ptr; B.vis = S0.vis.ptr;
+        B.counts = S0.qcount.ptr;
         B.tiles_x = (t.W + 7) / 8;
         B.ntiles = B.tiles_x * ((t.H + 7) / 8);
         B.tile_base = tile_base;
@@ -1035,12 +1090,16 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
     for (auto& j : jobs) {
         Tracer& t = *j.second;
         t.desc.no_iterations += j.first;
+        t.cur = 0;
         /* the bilateral display filter only shapes the displayed image: once, after the last pass */
         if (use_bilateral(t))
             HIP_TRY(erd_launch_bilateral(t.bilateral.ptr, t.display.ptr, t.disp_tmp.ptr, t.disp_filt.ptr, t.W, t.H, t.stream));
     }
-    for (auto& j : jobs) HIP_TRY(hipStreamSynchronize(j.second->stream));
-    if (timing) { timing->total_ms = ms; timing->trace_ms = ms; timing->post_ms = 0.0f; timing->passes = rounds; }
+    for (auto& j : jobs) if ((st = sync_tracer(*j.second))) return st;
+    if (timing) {
+        timing->total_ms = ms; timing->trace_ms = ms; timing->post_ms = 0.0f; timing->passes = rounds;
+        timing->primary_ms = 0.0f;
+    }
     return ER_OK;
 }
 
@@ -1181,7 +1240,7 @@ ER_API int er_get_frame_xyza(int tracer_id, float* xyza, size_t size)
     int st;
     Tracer* t = get_tracer(tracer_id, st);
     if (!t) return st;
-    return copy_out(xyza, t->frame.ptr, (size_t)t->W * t->H * 16, size, t->stream);
+    return copy_out(xyza, t->set[t->cur].frame.ptr, (size_t)t->W * t->H * 16, size, t->stream);
 }
 
 ER_API int er_get_running_xyza(int tracer_id, float* xyza, size_t size)

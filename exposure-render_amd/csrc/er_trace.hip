@@ -458,28 +458,37 @@ template<int FULL, int COUNT, int LUT_LDS>
 static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, erd::ScatterRec* queue,
                        unsigned int* counts, erd::EventRec* events, erd::ShadowJob* jobs, unsigned char* vis,
                        unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks, int split,
-                       size_t lds, size_t lds_bits, hipStream_t st)
+                       size_t lds, size_t lds_bits, hipStream_t st, hipStream_t st2, hipEvent_t shaded,
+                       hipEvent_t after_primary)
 {
     hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS>), dim3(nblocks), dim3(256), lds, st, p, frame, s1, s2, queue,
                        counts, counters, tiles_x, ntiles);
+    if (after_primary) (void)hipEventRecord(after_primary, st);
     if (split) {
         hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1,
                            s2, (const erd::ScatterRec*)queue, counts, events, jobs, counters);
-        hipLaunchKernelGGL((erd::k_shadow<FULL, COUNT, LUT_LDS>), dim3(2 * shade_blocks), dim3(256), lds, st, p,
+        (void)hipEventRecord(shaded, st);
+        (void)hipStreamWaitEvent(st2, shaded, 0);
+        hipLaunchKernelGGL((erd::k_shadow<FULL, COUNT, LUT_LDS>), dim3(2 * shade_blocks), dim3(256), lds, st2, p,
                            (const unsigned int*)counts, (const erd::ShadowJob*)jobs, vis, counters);
-        hipLaunchKernelGGL(erd::k_finish, dim3(shade_blocks), dim3(256), 0, st, (const unsigned int*)counts,
+        hipLaunchKernelGGL(erd::k_finish, dim3(shade_blocks), dim3(256), 0, st2, (const unsigned int*)counts,
                            (const erd::EventRec*)events, (const unsigned char*)vis, frame);
     } else {
         hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS, 0>), dim3(shade_blocks), dim3(256), lds, st, p, frame, s1, s2,
                            (const erd::ScatterRec*)queue, counts, events, jobs, counters);
+        (void)hipEventRecord(shaded, st);
+        (void)hipStreamWaitEvent(st2, shaded, 0);
     }
 }
 
+/* One pass's trace kernels: primary and shading on st; shadow marches and combine on st2
+ * after the shading (event `shaded`). The caller runs post-processing on st2. */
 extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
                                        unsigned int* counts, void* events, void* jobs, unsigned char* vis,
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
-                                       int primary_blocks, hipStream_t st)
+                                       int primary_blocks, hipStream_t st, hipStream_t st2, hipEvent_t shaded,
+                                       hipEvent_t after_primary)
 {
     const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
     const int ntiles = tiles_x * tiles_y;
@@ -489,7 +498,7 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
     erd::EventRec* ev = (erd::EventRec*)events;
     erd::ShadowJob* jb = (erd::ShadowJob*)jobs;
 #define ER_ALL(F, C, L) launch_all<F, C, L>(p, frame, s1, s2, q, counts, ev, jb, vis, counters, tiles_x, ntiles, nblocks, \
-                                            shade_blocks, split, lds_bytes, lds_bits, st)
+                                            shade_blocks, split, lds_bytes, lds_bits, st, st2, shaded, after_primary)
     if (full) {
         if (count) { if (lut_lds) ER_ALL(1, 1, 1); else ER_ALL(1, 1, 0); }
         else { if (lut_lds) ER_ALL(1, 0, 1); else ER_ALL(1, 0, 0); }

@@ -141,7 +141,7 @@ class BitmapDesc(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_float), ("trace_ms", C.c_float), ("post_ms", C.c_float),
-                ("passes", C.c_int)]
+                ("passes", C.c_int), ("primary_ms", C.c_float)]
 
 
 class Counters(C.Structure):

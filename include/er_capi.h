@@ -251,9 +251,10 @@ typedef struct er_bitmap_desc {
 /* Per-stage timing of er_render_passes (hipEvent-measured on the tracer's stream). */
 typedef struct er_timing {
     float total_ms;        /* first launch to last completion */
-    float trace_ms;        /* sum over passes of the trace kernel */
+    float trace_ms;        /* sum over passes of primary start to combine end (passes overlap) */
     float post_ms;         /* sum over passes of filter + accumulate + tonemap (+ bilateral) */
     int   passes;
+    float primary_ms;      /* sum over passes of the primary-march kernel */
 } er_timing;
 
 /* Sample counters of one pass (er_set_counting): the unit counts of the algorithmic-bytes
