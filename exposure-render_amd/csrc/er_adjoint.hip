@@ -8,7 +8,8 @@
  * expected values on the same samples:
  *   - shadow test -> transmittance T = exp(-DS * dt * sum sigma_j) over every sample of the
  *     shadow segment (the probability that the reference's ScatterEventInVolume,
- *     raymarching.h:73-113, reports no event);
+ *     raymarching.h:73-113, reports no event), up to where the optical depth reaches
+ *     ADJ_TAU_CAP;
  *   - primary free flight -> the path's value times P_k(theta) / P_k(theta0), where
  *     P_k = exp(-tau_<k) - exp(-tau_<=k), tau_j = DS * sigma_j * dt, is the probability that
  *     SampleVolume (raymarching.h:30-71) stops at the sampled sample k (for a path that leaves
@@ -20,21 +21,26 @@
  * v linearly. Samples whose interpolated opacity is not positive have zero derivative, which
  * is what lets the replay skip empty macrocells exactly as the forward march does.
  *
- * Per path: march the primary ray (decisions), shade with both MIS halves marching their
- * shadow segments to the end (transmittance), then - knowing the path's value and the
- * adjoint of its pixel - replay the same marches once more, accumulating
- * dL/dtau_j * dtau_j/dnode into a per-workgroup LDS histogram of the node table. Workgroups
- * write their histograms to a partials array that a second kernel sums in FP64.
+ * Gradient mode (er_adjoint_lut) runs per pass the forward primary and shading kernels
+ * (er_trace.hip) and then the wavefront kernels below: shadow transmittances and their
+ * gradients, the smooth values and diffuse-colour gradients, a list of pixels with a non-zero
+ * adjoint, and the replay of their primary marches. Gradients go into per-workgroup LDS
+ * histograms of the node table; a last kernel sums the workgroups' histograms in FP64.
  *
- * Value mode (er_adjoint_value): the smooth estimator itself, evaluated with another node
- * table while the decisions keep the current one; it reads every sample (no skipping), since
- * a macrocell empty under one table need not be empty under the other.
+ * Value mode (er_adjoint_value, k_adj_value): the smooth estimator itself, evaluated with
+ * another node table while the decisions keep the current one, one thread per path; it reads
+ * every sample (no skipping), since a macrocell empty under one table need not be empty under
+ * the other.
  */
 #include "er_path.h"
 
 namespace erd {
 
 #define ADJ_NODES (ER_TF_LUT_NODES * 4)
+/* the smooth shadow test stops where the segment's optical depth reaches this value: what
+ * lies beyond is attenuated below exp(-12) = 6e-6 (the value and the gradient both stop there,
+ * so they stay each other's derivative; the expectation changes by at most that factor) */
+#define ADJ_TAU_CAP 12.0f
 /* dense samples per march iteration in the replay (loads in flight together) */
 #ifndef ER_BATCH_PRIMARY_ADJ
 #define ER_BATCH_PRIMARY_ADJ 4
@@ -46,9 +52,9 @@ namespace erd {
 struct AdjArgs {
     const float* lut_eval;      /* node table the values use (value mode) or the current one */
     const float* sigma_eval;    /* DS * opacity_eval(i), i in [0, lut_max] (value mode) */
-    const float4* adj;          /* grad mode: dL/d(frame XYZ) of this pass */
-    float4* value;              /* value mode: the smooth frame estimate */
-    float* partials;            /* grad mode: ADJ_NODES floats per workgroup */
+    const float4* adj;          /* unused by the value kernel */
+    float4* value;              /* the smooth frame estimate */
+    float* partials;            /* unused by the value kernel */
 };
 
 /* sums sigma over the visited samples; EVAL also sums the evaluation table's sigma */
@@ -123,7 +129,7 @@ __device__ __forceinline__ float transmittance(const DevParams& p, const Tables&
     float nearT, farT;
     if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return 1.0f;
     marched = true;
-    march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisSum<EVAL>>(p, T, r, false, rng, Tt, Pt, cnt, vs, true);
+    march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisSum<EVAL>>(p, T, r, false, rng, Tt, Pt, cnt, vs, ADJ_TAU_CAP / p.density_scale);
     const double dt = (double)((p.step_shadow) * p.minstep);
     return (float)exp(-(double)p.density_scale * dt * vs.se);
 }
@@ -133,19 +139,17 @@ __device__ __forceinline__ f3 lut_diffuse(const float* lut, float scale, float I
     return mk(tf_lut_eval(lut, scale, I, 1), tf_lut_eval(lut, scale, I, 2), tf_lut_eval(lut, scale, I, 3));
 }
 
-template<int MODE>   /* 0: gradient, 1: value */
-__global__ __launch_bounds__(256) void k_adjoint(const DevParams* __restrict__ pp, AdjArgs a,
+/* value mode: the smooth estimator of one pass with the evaluation table, every sample read */
+__global__ __launch_bounds__(256) void k_adj_value(const DevParams* __restrict__ pp, AdjArgs a,
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2)
 {
     extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
-    __shared__ float hist[ADJ_NODES];
     /* occupancy bits (value mode: all set) and the decision sigma table */
     const int nw = p.mc_words;
-    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = MODE ? 0xffffffffu : gmem(p.mc_bits)[k];
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = 0xffffffffu;
     float* lut = (float*)(lds + nw);
     for (int k = threadIdx.x; k <= p.lut_max; k += blockDim.x) lut[k] = gmem(p.sigma_lut)[k];
-    for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) hist[k] = 0.0f;
     __syncthreads();
     Tables T;
     T.bits = lds;
@@ -161,36 +165,14 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevParams* __restrict__ p
         rng.s0 = seeds1[i];
         rng.s1 = seeds2[i];
         Sample smp;
-        smp.comp = rng.u();
-        smp.dir0 = rng.u(); smp.dir1 = rng.u();
-        smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
-        smp.lightnum = rng.u();
-        smp.film0 = rng.u(); smp.film1 = rng.u();
-        smp.lens0 = rng.u(); smp.lens1 = rng.u();
-        const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
-        const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
-        const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
-        Ray r;
-        r.O = ld3(p.cam_pos);
-        r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
-        r.MinT = p.clip_near;
-        r.MaxT = p.clip_far;
-        if (p.aperture != 0.0f) {
-            float lx, ly;
-            concentric_disk(smp.lens0, smp.lens1, lx, ly);
-            lx = lx * p.aperture; ly = ly * p.aperture;
-            const f3 LI = add(scl(lx, U), scl(ly, V));
-            r.O = add(r.O, LI);
-            r.D = nrm(sub(scl(p.focal, r.D), LI));
-        }
-        const Rng rng_primary = rng;
-        VisSum<MODE> vp;
+        const Ray r = camera_ray(p, px, py, rng, smp);
+        VisSum<1> vp;
         vp.sig_eval = a.sigma_eval; vp.lut_max = p.lut_max;
         vp.sd = vp.se = 0.0; vp.last_d = vp.last_e = 0.0f;
         Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         float Tv = 0.0f;
         f3 Pv = mk(0.0f, 0.0f, 0.0f);
-        const bool vol = march_v<1, 0, ER_BATCH_PRIMARY_ADJ, VisSum<MODE>>(p, T, r, true, rng, Tv, Pv, cnt, vp, false);
+        const bool vol = march_v<1, 0, ER_BATCH_PRIMARY_ADJ, VisSum<1>>(p, T, r, true, rng, Tv, Pv, cnt, vp, -1.0f);
         float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
         const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
         int type = -1;
@@ -218,7 +200,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevParams* __restrict__ p
                 const DevLight& L = *Lp;
                 count = (float)nl;
                 const f3 Kd_dec = diffuse_at(p, Iev);
-                Kd = MODE ? lut_diffuse(a.lut_eval, p.tf_lut_scale, Iev) : Kd_dec;
+                Kd = lut_diffuse(a.lut_eval, p.tf_lut_scale, Iev);
                 /* light-sampling half (phase function: F = Kd / pi, pdf 1 / (4 pi)) */
                 f3 SP, SN; float su, sv;
                 sample_light_surface(p, L, smp.uvw0, smp.uvw1, smp.uvw2, SP, SN, su, sv);
@@ -229,7 +211,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevParams* __restrict__ p
                 const float PhasePdf = 0.0795774715459476678844f;
                 if (!black(Li) && !black(scl(ER_INV_PI_F, Kd_dec))) {
                     rng1 = rng;
-                    T1 = transmittance<MODE>(p, T, a, SEP, SP, rng, r1, rng1, m1);
+                    T1 = transmittance<1>(p, T, a, SEP, SP, rng, r1, rng1, m1);
                     const f3 d = sub(SEP, SP);
                     const float LightPdf = dot(d, d) / (absdot(SN, neg(Wi)) * L.area);
                     const float Weight = power_heuristic(LightPdf, PhasePdf);
@@ -249,7 +231,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevParams* __restrict__ p
                     const int idx2 = intersect_lights(p, rl, false, T2l, P2, N2, Le2);
                     if (idx2 >= 0 && idx2 == pos && !black(Le2)) {
                         rng2 = rng;
-                        T2 = transmittance<MODE>(p, T, a, SEP, P2, rng, r2s, rng2, m2);
+                        T2 = transmittance<1>(p, T, a, SEP, P2, rng, r2s, rng2, m2);
                         const f3 d = sub(SEP, P2);
                         const float LightPdf = dot(d, d) / (absdot(N2, neg(W2)) * L.area);
                         const float Weight = power_heuristic(PhasePdf, LightPdf);
@@ -265,88 +247,215 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevParams* __restrict__ p
 
         const double DS = (double)p.density_scale;
         const double dtp = (double)(p.step_primary * p.minstep);
-        if (MODE) {
-            /* P_k(theta) / P_k(theta0) of the sampled free flight */
-            double w = 1.0;
-            if (vol) {
-                const double d_before = (vp.se - (double)vp.last_e) - (vp.sd - (double)vp.last_d);
-                const double num = -expm1(-DS * dtp * (double)vp.last_e);
-                const double den = -expm1(-DS * dtp * (double)vp.last_d);
-                w = exp(-DS * dtp * d_before) * (den > 0.0 ? num / den : 1.0);
-            } else {
-                w = exp(-DS * dtp * (vp.se - vp.sd));
-            }
-            const float wf = (float)w;
-            a.value[i] = make_float4(wf * value.x, wf * value.y, wf * value.z, type >= 0 ? 1.0f : 0.0f);
+        /* P_k(theta) / P_k(theta0) of the sampled free flight */
+        double w = 1.0;
+        if (vol) {
+            const double d_before = (vp.se - (double)vp.last_e) - (vp.sd - (double)vp.last_d);
+            const double num = -expm1(-DS * dtp * (double)vp.last_e);
+            const double den = -expm1(-DS * dtp * (double)vp.last_d);
+            w = exp(-DS * dtp * d_before) * (den > 0.0 ? num / den : 1.0);
         } else {
-            const float4 A4 = a.adj[i];
-            const f3 A = mk(A4.x, A4.y, A4.z);
-            const float g_total = dot(A, value);
-            const float scale = p.tf_lut_scale;
-            VisGrad vg;
-            vg.hist = hist; vg.scale = scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f; vg.last_i = 0; vg.last_sigma = 0.0f;
-            /* primary free flight */
-            if (g_total != 0.0f) {
-                Rng rr = rng_primary;
-                vg.w = (float)(-(double)g_total * DS * DS * dtp);
-                float Tq; f3 Pq;
-                const bool v2 = march_v<1, 0, ER_BATCH_PRIMARY_ADJ, VisGrad>(p, T, r, true, rr, Tq, Pq, cnt, vg, false);
-                vg.flush();
-                if (v2 && vg.last_sigma > 0.0f) {
-                    /* the sampled sample k: replace -g by g exp(-tau) / (1 - exp(-tau)) */
-                    const double tau = DS * dtp * (double)vg.last_sigma;
-                    const double r_k = 1.0 / expm1(tau);
-                    vg.w = (float)(((double)g_total * r_k + (double)g_total) * DS * DS * dtp);
-                    vg.sample(vg.last_i, vg.last_sigma);
-                    vg.flush();
-                }
-            }
-            if (type == 0 && count > 0.0f) {
-                const double dts = (double)(p.step_shadow * p.minstep);
-                const float g1 = count * dot(A, scl(T1, mul(Kd, q1)));
-                const float g2 = count * dot(A, scl(T2, mul(Kd, q2)));
-                if (m1 && g1 != 0.0f) {
-                    Rng rr = rng1;
-                    vg.w = (float)(-(double)g1 * DS * DS * dts);
-                    float Tq; f3 Pq;
-                    march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisGrad>(p, T, r1, false, rr, Tq, Pq, cnt, vg, true);
-                    vg.flush();
-                }
-                if (m2 && g2 != 0.0f) {
-                    Rng rr = rng2;
-                    vg.w = (float)(-(double)g2 * DS * DS * dts);
-                    float Tq; f3 Pq;
-                    march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisGrad>(p, T, r2s, false, rr, Tq, Pq, cnt, vg, true);
-                    vg.flush();
-                }
-                /* diffuse colour at the event, channel c enters channel c linearly */
-                int b; float f;
-                tf_lut_coord(Iev, scale, b, f);
-                const f3 gk = scl(count, add(scl(T1, q1), scl(T2, q2)));
-                const float gc0 = A.x * gk.x, gc1 = A.y * gk.y, gc2 = A.z * gk.z;
-                const float* tl = p.tf_lut;
-                if (gc0 != 0.0f && tf_lut_raw(tl, b, f, 1) > 0.0f) {
-                    atomicAdd(&hist[b * 4 + 1], gc0 * (1.0f - f));
-                    atomicAdd(&hist[(b + 1) * 4 + 1], gc0 * f);
-                }
-                if (gc1 != 0.0f && tf_lut_raw(tl, b, f, 2) > 0.0f) {
-                    atomicAdd(&hist[b * 4 + 2], gc1 * (1.0f - f));
-                    atomicAdd(&hist[(b + 1) * 4 + 2], gc1 * f);
-                }
-                if (gc2 != 0.0f && tf_lut_raw(tl, b, f, 3) > 0.0f) {
-                    atomicAdd(&hist[b * 4 + 3], gc2 * (1.0f - f));
-                    atomicAdd(&hist[(b + 1) * 4 + 3], gc2 * f);
-                }
-            }
+            w = exp(-DS * dtp * (vp.se - vp.sd));
         }
+        const float wf = (float)w;
+        a.value[i] = make_float4(wf * value.x, wf * value.y, wf * value.z, type >= 0 ? 1.0f : 0.0f);
         seeds1[i] = rng.s0;
         seeds2[i] = rng.s1;
     }
-    if (!MODE) {
-        __syncthreads();
-        float* out = a.partials + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NODES;
-        for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) out[k] = hist[k];
+}
+
+/* ---- gradient mode as a wavefront over the forward kernels ------------------------------
+ * Per pass: the forward primary and shading kernels (same decisions, queues and deferred
+ * shadow jobs), then
+ *   k_adj_shadow   each shadow job: transmittance on the same samples (march to the optical
+ *                  depth cap), the job's shadowed value g = A . (count c T) from its event,
+ *                  and - when g != 0 - the same march again scattering -g dtau/dnode;
+ *   k_adj_finish   each event: the smooth value count (Ld0 + c1 T1 + c2 T2) into the frame,
+ *                  and the diffuse-colour gradient A_c count (c1_c T1 + c2_c T2) / Kd_c;
+ *   k_adj_list     every pixel with g_total = A . value != 0 into a list;
+ *   k_adj_primary  each listed pixel: its camera ray regenerated from the pass's start seeds
+ *                  and the primary march replayed, scattering -g_total dtau/dnode, and at the
+ *                  sampled scatter sample g_total / (exp(tau) - 1) more. */
+
+__device__ __forceinline__ Tables adj_stage(const DevParams& p, uint32_t* lds)
+{
+    const int nw = p.mc_words;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = gmem(p.mc_bits)[k];
+    float* lut = (float*)(lds + nw);
+    for (int k = threadIdx.x; k <= p.lut_max; k += blockDim.x) lut[k] = gmem(p.sigma_lut)[k];
+    Tables T;
+    T.bits = lds;
+    T.bits_glb = nullptr;
+    T.lut_lds = lut;
+    T.lut_glb = nullptr;
+    __syncthreads();
+    return T;
+}
+
+__device__ __forceinline__ void adj_flush_hist(const float* hist, float* __restrict__ partials)
+{
+    __syncthreads();
+    float* out = partials + (size_t)blockIdx.x * ADJ_NODES;
+    for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) out[k] = hist[k];
+}
+
+__global__ __launch_bounds__(256) void k_adj_shadow(const DevParams* __restrict__ pp, const unsigned int* __restrict__ counts,
+                                                   const ShadowJob* __restrict__ jobq, const EventRec* __restrict__ events,
+                                                   const float4* __restrict__ adj, float* __restrict__ tvis,
+                                                   float* __restrict__ partials)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ float hist[ADJ_NODES];
+    const DevParams& p = *pp;
+    for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) hist[k] = 0.0f;
+    const Tables T = adj_stage(p, lds);   /* ends with a barrier */
+    const unsigned int n = counts[1];
+    const double DS = (double)p.density_scale;
+    const double dts = (double)(p.step_shadow * p.minstep);
+    const float cap = ADJ_TAU_CAP / p.density_scale;
+    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const ShadowJob job = jobq[j];
+        Ray r;
+        r.O = mk(job.Ox, job.Oy, job.Oz);
+        r.D = mk(job.Dx, job.Dy, job.Dz);
+        r.MinT = 0.0f;
+        r.MaxT = job.MaxT;
+        Rng rng;
+        rng.s0 = job.s0;
+        rng.s1 = job.s1;
+        VisSum<0> vs;
+        vs.sig_eval = nullptr; vs.lut_max = p.lut_max;
+        vs.sd = vs.se = 0.0; vs.last_d = vs.last_e = 0.0f;
+        Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        float Tt; f3 Pt;
+        march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisSum<0>>(p, T, r, false, rng, Tt, Pt, cnt, vs, cap);
+        const float Tr = (float)exp(-DS * dts * vs.sd);
+        tvis[2 * job.event + job.slot] = Tr;
+        const EventRec& ev = events[job.event];
+        const f3 c = job.slot ? mk(ev.c2x, ev.c2y, ev.c2z) : mk(ev.c1x, ev.c1y, ev.c1z);
+        const float4 A4 = adj[ev.pixel];
+        const float g = ev.count * Tr * dot(mk(A4.x, A4.y, A4.z), c);
+        if (g != 0.0f) {
+            VisGrad vg;
+            vg.hist = hist; vg.scale = p.tf_lut_scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f;
+            vg.last_i = 0; vg.last_sigma = 0.0f;
+            vg.w = (float)(-(double)g * DS * DS * dts);
+            Rng rr;
+            rr.s0 = job.s0;
+            rr.s1 = job.s1;
+            march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisGrad>(p, T, r, false, rr, Tt, Pt, cnt, vg, cap);
+            vg.flush();
+        }
     }
+    adj_flush_hist(hist, partials);
+}
+
+__global__ __launch_bounds__(256) void k_adj_finish(const DevParams* __restrict__ pp, const unsigned int* __restrict__ counts,
+                                                   const EventRec* __restrict__ events, const ScatterRec* __restrict__ queue,
+                                                   const float* __restrict__ tvis, const float4* __restrict__ adj,
+                                                   float4* __restrict__ frame, float* __restrict__ partials)
+{
+    __shared__ float hist[ADJ_NODES];
+    const DevParams& p = *pp;
+    for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) hist[k] = 0.0f;
+    __syncthreads();
+    const unsigned int n = counts[0];
+    for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const EventRec ev = events[q];
+        const uint32_t s1 = ev.state & 3u, s2 = (ev.state >> 2) & 3u;
+        const float T1 = s1 == 1u ? 1.0f : (s1 == 2u ? tvis[2 * q] : 0.0f);
+        const float T2 = s2 == 1u ? 1.0f : (s2 == 2u ? tvis[2 * q + 1] : 0.0f);
+        const f3 c1 = mk(ev.c1x, ev.c1y, ev.c1z), c2 = mk(ev.c2x, ev.c2y, ev.c2z);
+        f3 Ld = mk(ev.Ld0x, ev.Ld0y, ev.Ld0z);
+        if (ev.count > 0.0f) Ld = scl(ev.count, add(Ld, add(scl(T1, c1), scl(T2, c2))));
+        frame[ev.pixel] = make_float4(Ld.x, Ld.y, Ld.z, 1.0f);
+        if (!(ev.count > 0.0f)) continue;
+        const float4 A4 = adj[ev.pixel];
+        const ScatterRec& rec = queue[q];
+        const float I = intensity(p, mk(rec.Px, rec.Py, rec.Pz));
+        int b; float f;
+        tf_lut_coord(I, p.tf_lut_scale, b, f);
+        const f3 gk = scl(ev.count, add(scl(T1, c1), scl(T2, c2)));   /* = Kd (.) dvalue/dKd */
+        const float A[3] = {A4.x, A4.y, A4.z}, G[3] = {gk.x, gk.y, gk.z};
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float kd = tf_lut_raw(p.tf_lut, b, f, ch + 1);
+            if (!(kd > 0.0f) || G[ch] == 0.0f || A[ch] == 0.0f) continue;
+            const float gc = A[ch] * (G[ch] / kd);
+            atomicAdd(&hist[b * 4 + ch + 1], gc * (1.0f - f));
+            atomicAdd(&hist[(b + 1) * 4 + ch + 1], gc * f);
+        }
+    }
+    adj_flush_hist(hist, partials);
+}
+
+/* pixels whose smooth value has a non-zero adjoint, with g_total, into a list */
+__global__ __launch_bounds__(256) void k_adj_list(const float4* __restrict__ frame, const float4* __restrict__ adj, int n,
+                                                 uint2* __restrict__ list, unsigned int* __restrict__ count)
+{
+    const int lane = threadIdx.x & 63;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        float g = 0.0f;
+        if (i < n) {
+            const float4 v = frame[i], A = adj[i];
+            g = A.x * v.x + A.y * v.y + A.z * v.z;
+        }
+        const bool has = g != 0.0f;
+        const unsigned long long m = __ballot(has);
+        if (!m) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned int slot = 0;
+        if (lane == leader) slot = atomicAdd(count, (unsigned int)__popcll(m));
+        slot = __shfl(slot, leader);
+        if (has) list[slot + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = make_uint2((unsigned int)i, __float_as_uint(g));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_primary(const DevParams* __restrict__ pp, const uint2* __restrict__ list,
+                                                    const unsigned int* __restrict__ count,
+                                                    const uint32_t* __restrict__ seeds1, const uint32_t* __restrict__ seeds2,
+                                                    float* __restrict__ partials)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ float hist[ADJ_NODES];
+    const DevParams& p = *pp;
+    for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) hist[k] = 0.0f;
+    const Tables T = adj_stage(p, lds);
+    const unsigned int n = *count;
+    const double DS = (double)p.density_scale;
+    const double dtp = (double)(p.step_primary * p.minstep);
+    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint2 e = list[j];
+        const int i = (int)e.x;
+        const float g = __uint_as_float(e.y);
+        Rng rng;
+        rng.s0 = seeds1[i];
+        rng.s1 = seeds2[i];
+        Sample smp;
+        const Ray r = camera_ray(p, i % p.film_w, i / p.film_w, rng, smp);
+        VisGrad vg;
+        vg.hist = hist; vg.scale = p.tf_lut_scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f;
+        vg.last_i = 0; vg.last_sigma = 0.0f;
+        vg.w = (float)(-(double)g * DS * DS * dtp);
+        Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        float Tq; f3 Pq;
+        const bool vol = march_v<1, 0, ER_BATCH_PRIMARY_ADJ, VisGrad>(p, T, r, true, rng, Tq, Pq, cnt, vg, -1.0f);
+        vg.flush();
+        if (vol && vg.last_sigma > 0.0f) {
+            const double tau = DS * dtp * (double)vg.last_sigma;
+            vg.w = (float)(((double)g / expm1(tau) + (double)g) * DS * DS * dtp);
+            vg.sample(vg.last_i, vg.last_sigma);
+            vg.flush();
+        }
+    }
+    adj_flush_hist(hist, partials);
+}
+
+__global__ void k_adj_reset(unsigned int* __restrict__ counts, unsigned int* __restrict__ lcount)
+{
+    if (threadIdx.x < 4) counts[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) *lcount = 0u;
 }
 
 /* dL/dframe of one pass from dL/drunning: the transpose of the 3x3 frame filter (weights
@@ -373,14 +482,20 @@ __global__ __launch_bounds__(256) void k_adj_filter_t(const float* __restrict__ 
     out[(size_t)y * W + x] = make_float4(weight * s0, weight * s1, weight * s2, 0.0f);
 }
 
-/* out[k] += sum over workgroups of partials[.][k], in FP64 */
+/* out[k] += sum over workgroups of partials[.][k], in FP64: one workgroup per entry */
 __global__ __launch_bounds__(256) void k_adj_reduce(const float* __restrict__ partials, int nblocks, double* __restrict__ out)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ADJ_NODES) return;
+    __shared__ double red[256];
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += (double)partials[(size_t)b * ADJ_NODES + k];
-    out[k] += s;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += (double)partials[(size_t)b * ADJ_NODES + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] += red[0];
 }
 
 } /* namespace erd */
@@ -394,8 +509,8 @@ extern "C" hipError_t erd_launch_adjoint(const erd::DevParams* p, int mode, cons
     erd::AdjArgs a;
     a.lut_eval = lut_eval; a.sigma_eval = sigma_eval; a.adj = adj; a.value = value; a.partials = partials;
     const dim3 grid((W + 15) / 16, (H + 15) / 16);
-    if (mode) hipLaunchKernelGGL(erd::k_adjoint<1>, grid, dim3(256), lds_bytes, st, p, a, s1, s2);
-    else hipLaunchKernelGGL(erd::k_adjoint<0>, grid, dim3(256), lds_bytes, st, p, a, s1, s2);
+    (void)mode;
+    hipLaunchKernelGGL(erd::k_adj_value, grid, dim3(256), lds_bytes, st, p, a, s1, s2);
     return hipGetLastError();
 }
 
@@ -407,6 +522,32 @@ extern "C" hipError_t erd_launch_adj_filter_t(const float* dldr, float4* out, in
 
 extern "C" hipError_t erd_launch_adj_reduce(const float* partials, int nblocks, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(erd::k_adj_reduce, dim3((ADJ_NODES + 255) / 256), dim3(256), 0, st, partials, nblocks, out);
+    hipLaunchKernelGGL(erd::k_adj_reduce, dim3(ADJ_NODES), dim3(256), 0, st, partials, nblocks, out);
+    return hipGetLastError();
+}
+
+/* gradient-mode kernels after the forward primary + shading kernels of one pass; partials gets
+ * 3 * blocks workgroup histograms */
+extern "C" hipError_t erd_launch_adj_wavefront(const erd::DevParams* p, unsigned int* counts, const void* jobs,
+                                               const void* events, const void* queue, const float4* adj, float* tvis,
+                                               float4* frame, uint2* list, unsigned int* lcount, const uint32_t* s1start,
+                                               const uint32_t* s2start, float* partials, int blocks, int n,
+                                               size_t lds_bytes, hipStream_t st)
+{
+    const size_t per = (size_t)blocks * ADJ_NODES;
+    hipLaunchKernelGGL(erd::k_adj_shadow, dim3(blocks), dim3(256), lds_bytes, st, p, (const unsigned int*)counts,
+                       (const erd::ShadowJob*)jobs, (const erd::EventRec*)events, adj, tvis, partials);
+    hipLaunchKernelGGL(erd::k_adj_finish, dim3(blocks), dim3(256), 0, st, p, (const unsigned int*)counts,
+                       (const erd::EventRec*)events, (const erd::ScatterRec*)queue, (const float*)tvis, adj, frame,
+                       partials + per);
+    hipLaunchKernelGGL(erd::k_adj_list, dim3(blocks), dim3(256), 0, st, (const float4*)frame, adj, n, list, lcount);
+    hipLaunchKernelGGL(erd::k_adj_primary, dim3(blocks), dim3(256), lds_bytes, st, p, (const uint2*)list,
+                       (const unsigned int*)lcount, s1start, s2start, partials + 2 * per);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t erd_launch_adj_reset(unsigned int* counts, unsigned int* lcount, hipStream_t st)
+{
+    hipLaunchKernelGGL(erd::k_adj_reset, dim3(1), dim3(64), 0, st, counts, lcount);
     return hipGetLastError();
 }

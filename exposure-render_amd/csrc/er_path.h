@@ -162,21 +162,22 @@ struct NoVisit {
 /* march_v with no observer: the forward march */
 template<int FULL, int COUNT, int K = 1, class Visit = NoVisit>
 __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, const Ray& r, bool primary, Rng& rng, float& Tout, f3& Pout,
-                        Cnt& cnt, Visit& vis, bool to_end);
+                        Cnt& cnt, Visit& vis, float S_fixed);
 
 template<int FULL, int COUNT, int K = 1>
 __device__ __forceinline__ bool march(const DevParams& p, const Tables& T, const Ray& r, bool primary, Rng& rng, float& Tout,
                                       f3& Pout, Cnt& cnt)
 {
     NoVisit nv;
-    return march_v<FULL, COUNT, K, NoVisit>(p, T, r, primary, rng, Tout, Pout, cnt, nv, false);
+    return march_v<FULL, COUNT, K, NoVisit>(p, T, r, primary, rng, Tout, Pout, cnt, nv, -1.0f);
 }
 
-/* to_end: the threshold is still drawn but never reached, so the march reads every sample up
- * to the end of the ray (the expected-transmittance form of a shadow test). */
+/* S_fixed >= 0: the threshold is still drawn (same draws) but S_fixed replaces it, so the march
+ * reads every sample until that optical depth / DensityScale or the end of the ray (the
+ * expected-transmittance form of a shadow test). */
 template<int FULL, int COUNT, int K, class Visit>
 __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, const Ray& r, bool primary, Rng& rng, float& Tout, f3& Pout,
-                        Cnt& cnt, Visit& vis, bool to_end)
+                        Cnt& cnt, Visit& vis, float S_fixed)
 {
     float nearT, farT;
     if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return false;
@@ -199,7 +200,7 @@ __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, con
     }
 
     float S = gmem(p.neglog)[rng.next_bits()] / p.density_scale;
-    if (to_end) S = __builtin_inff();
+    if (S_fixed >= 0.0f) S = S_fixed;
     const float StepSize = (primary ? p.step_primary : p.step_shadow) * p.minstep;
     vis.begin(StepSize);
     MinT += rng.u() * StepSize;
@@ -950,6 +951,35 @@ __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SE
     }
 }
 
+/* The 11 sample draws of one path (sample.h:258-262, in the reference's order) and its camera
+ * ray (singlescattering.h:29-50, with the thin lens of montecarlo.h:86-137). */
+__device__ __forceinline__ Ray camera_ray(const DevParams& p, int px, int py, Rng& rng, Sample& smp)
+{
+    smp.comp = rng.u();
+    smp.dir0 = rng.u(); smp.dir1 = rng.u();
+    smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
+    smp.lightnum = rng.u();
+    smp.film0 = rng.u(); smp.film1 = rng.u();
+    smp.lens0 = rng.u(); smp.lens1 = rng.u();
+    const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
+    const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
+    const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
+    Ray r;
+    r.O = ld3(p.cam_pos);
+    r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
+    r.MinT = p.clip_near;
+    r.MaxT = p.clip_far;
+    if (p.aperture != 0.0f) {
+        float lx, ly;
+        concentric_disk(smp.lens0, smp.lens1, lx, ly);
+        lx = lx * p.aperture; ly = ly * p.aperture;
+        const f3 LI = add(scl(lx, U), scl(ly, V));
+        r.O = add(r.O, LI);
+        r.D = nrm(sub(scl(p.focal, r.D), LI));
+    }
+    return r;
+}
+
 /* One scattering pixel handed from the primary kernel to the shading kernel (64 bytes). */
 struct ScatterRec {
     uint32_t pixel, s0, s1, pad;
@@ -979,11 +1009,10 @@ struct BatchJob {
     int passes, no_iter0;
 };
 
-/* device-side progress of a batch: the pass round and the shared primary-tile counter */
+/* device-side state of one buffer set of a batch: the shared primary-tile counter */
 struct BatchState {
-    unsigned int round;
     unsigned int tiles;
-    unsigned int pad[2];
+    unsigned int pad[3];
 };
 
 /* number of jobs still rendering in this round (a prefix of the table) */

@@ -95,9 +95,8 @@ __global__ __launch_bounds__(256) void k_post(const DevParams* __restrict__ pp, 
 }
 
 /* post-processing of a batch round: one workgroup per 16 x 16 tile of every active job */
-__global__ __launch_bounds__(256) void k_post_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+__global__ __launch_bounds__(256) void k_post_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
 {
-    const unsigned int round = st->round;
     const int na = batch_active(J, njobs, round);
     const int b = blockIdx.x;
     int job = -1;
@@ -301,9 +300,9 @@ extern "C" hipError_t erd_launch_gm_max(const erd::DevParams* p, unsigned int* o
     return hipGetLastError();
 }
 
-extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, const erd::BatchState* st, int total_tiles,
+extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, unsigned int round, int total_tiles,
                                             hipStream_t stream)
 {
-    hipLaunchKernelGGL(erd::k_post_b, dim3(total_tiles), dim3(256), 0, stream, J, njobs, st);
+    hipLaunchKernelGGL(erd::k_post_b, dim3(total_tiles), dim3(256), 0, stream, J, njobs, round);
     return hipGetLastError();
 }

@@ -51,11 +51,21 @@ extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lu
                                              hipStream_t st);
 extern "C" hipError_t erd_launch_gm_max(const DevParams* p, unsigned int* out, hipStream_t st);
 extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, erd::BatchState* st, int full,
-                                             int primary_blocks, int shade_blocks, hipStream_t stream);
-extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, const erd::BatchState* st, int total_tiles,
+                                             int primary_blocks, int shade_blocks, unsigned int round, hipStream_t stream,
+                                             hipStream_t stream2, hipEvent_t shaded);
+extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, unsigned int round, int total_tiles,
                                             hipStream_t stream);
 extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, erd::BatchState* st, hipStream_t stream);
 extern "C" size_t erd_adj_partials_per_block(void);
+extern "C" hipError_t erd_launch_primary_shade(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
+                                               unsigned int* counts, void* events, void* jobs, int W, int H, int lut_lds,
+                                               size_t lds_bytes, size_t lds_bits, int shade_blocks, int primary_blocks,
+                                               hipStream_t st);
+extern "C" hipError_t erd_launch_adj_wavefront(const DevParams* p, unsigned int* counts, const void* jobs, const void* events,
+                                               const void* queue, const float4* adj, float* tvis, float4* frame, uint2* list,
+                                               unsigned int* lcount, const uint32_t* s1start, const uint32_t* s2start,
+                                               float* partials, int blocks, int n, size_t lds_bytes, hipStream_t st);
+extern "C" hipError_t erd_launch_adj_reset(unsigned int* counts, unsigned int* lcount, hipStream_t st);
 extern "C" hipError_t erd_launch_adjoint(const DevParams* p, int mode, const float* lut_eval, const float* sigma_eval,
                                          const float4* adj, float4* value, float* partials, uint32_t* s1, uint32_t* s2,
                                          int W, int H, size_t lds_bytes, hipStream_t st);
@@ -166,7 +176,7 @@ int ensure_neglog_device()
 
 /* ---- resources --------------------------------------------------------------------------- */
 uint64_t g_scene_version = 1;
-hipStream_t g_batch_stream = nullptr;
+hipStream_t g_batch_stream = nullptr, g_batch_stream2 = nullptr;
 
 struct Volume {
     er_volume_desc desc{};
@@ -829,6 +839,7 @@ ER_API int er_shutdown(void)
     g_objects.clear(); g_bitmaps.clear();
     if (g_neglog_dev) { (void)hipFree(g_neglog_dev); g_neglog_dev = nullptr; }
     if (g_batch_stream) { (void)hipStreamDestroy(g_batch_stream); g_batch_stream = nullptr; }
+    if (g_batch_stream2) { (void)hipStreamDestroy(g_batch_stream2); g_batch_stream2 = nullptr; }
     g_scene_version++;
     return ER_OK;
 }
@@ -1020,53 +1031,89 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         return a.first > b.first;
     });
     const int nj = (int)jobs.size();
-    std::vector<erd::BatchJob> table(nj);
+    /* two job tables, one per buffer set: rounds alternate between the sets so that a round's
+     * shadow marches, combine and post (second stream) overlap the next round's primary and
+     * shading (first stream) */
+    std::vector<erd::BatchJob> table[2];
     int tile_base = 0, post_base = 0;
+    for (int b = 0; b < 2; ++b) table[b].resize(nj);
     for (int k = 0; k < nj; ++k) {
         Tracer& t = *jobs[k].second;
-        erd::BatchJob& B = table[k];
-        B.params = t.params.ptr;
-        PassSet& S0 = t.set[0];
-        B.frame = S0.frame.ptr; B.running = t.running.ptr; B.display = t.display.ptr;
-        B.seeds1 = t.seeds1.ptr; B.seeds2 = t.seeds2.ptr;
-        B.queue = (erd::ScatterRec*)S0.queue.ptr; B.events = (erd::EventRec*)S0.events.ptr;
-        B.jobs = (erd::ShadowJob*)S0.jobs.ptr; B.vis = S0.vis.ptr;
-        B.counts = S0.qcount.ptr;
-        B.tiles_x = (t.W + 7) / 8;
-        B.ntiles = B.tiles_x * ((t.H + 7) / 8);
-        B.tile_base = tile_base;
-        tile_base += B.ntiles;
-        B.post_tiles_x = (t.W + 15) / 16;
-        B.post_ntiles = B.post_tiles_x * ((t.H + 15) / 16);
-        B.post_base = post_base;
-        post_base += B.post_ntiles;
-        B.passes = jobs[k].first;
-        B.no_iter0 = t.desc.no_iterations;
+        for (int b = 0; b < 2; ++b) {
+            PassSet& S = t.set[b];
+            erd::BatchJob& B = table[b][k];
+            B.params = t.params.ptr;
+            B.frame = S.frame.ptr; B.running = t.running.ptr; B.display = t.display.ptr;
+            B.seeds1 = t.seeds1.ptr; B.seeds2 = t.seeds2.ptr;
+            B.queue = (erd::ScatterRec*)S.queue.ptr; B.events = (erd::EventRec*)S.events.ptr;
+            B.jobs = (erd::ShadowJob*)S.jobs.ptr; B.vis = S.vis.ptr;
+            B.counts = S.qcount.ptr;
+            B.tiles_x = (t.W + 7) / 8;
+            B.ntiles = B.tiles_x * ((t.H + 7) / 8);
+            B.tile_base = tile_base;
+            B.post_tiles_x = (t.W + 15) / 16;
+            B.post_ntiles = B.post_tiles_x * ((t.H + 15) / 16);
+            B.post_base = post_base;
+            B.passes = jobs[k].first;
+            B.no_iter0 = t.desc.no_iterations;
+        }
+        tile_base += table[0][k].ntiles;
+        post_base += table[0][k].post_ntiles;
     }
-    DevBuf<erd::BatchJob> dJ;
-    DevBuf<erd::BatchState> dS;
-    if ((st = dJ.alloc(nj)) || (st = dS.alloc(1))) return st;
-    HIP_TRY(hipMemcpy(dJ.ptr, table.data(), nj * sizeof(erd::BatchJob), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(dS.ptr, 0, sizeof(erd::BatchState)));
+    DevBuf<erd::BatchJob> dJ[2];
+    DevBuf<erd::BatchState> dS[2];
+    for (int b = 0; b < 2; ++b) {
+        if ((st = dJ[b].alloc(nj)) || (st = dS[b].alloc(1))) return st;
+        HIP_TRY(hipMemcpy(dJ[b].ptr, table[b].data(), nj * sizeof(erd::BatchJob), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(dS[b].ptr, 0, sizeof(erd::BatchState)));
+        for (int k = 0; k < nj; ++k) HIP_TRY(hipMemset(table[b][k].counts, 0, 4 * sizeof(uint32_t)));
+    }
     if (!g_batch_stream) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream, hipStreamNonBlocking));
-    hipStream_t bs = g_batch_stream;
+    if (!g_batch_stream2) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream2, hipStreamNonBlocking));
+    hipStream_t bs = g_batch_stream, bs2 = g_batch_stream2;
     const int shade_blocks = 2048;
-    /* one pass round as a graph: trace kernels, post, round end (no memset nodes) */
+    const int rounds = jobs[0].first;
+    /* all rounds as one graph (no memset nodes): per round, primary + shading on the first
+     * stream, then shadow + combine + post + queue reset on the second; a set is reused two
+     * rounds later, after its post */
+    std::vector<hipEvent_t> evs;
+    auto new_event = [&](hipEvent_t& e) -> hipError_t {
+        hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (r == hipSuccess) evs.push_back(e);
+        return r;
+    };
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    HIP_TRY(hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal));
-    hipError_t e1 = erd_launch_batch_trace(dJ.ptr, nj, dS.ptr, full, g_primary_blocks, shade_blocks, bs);
-    hipError_t e2 = erd_launch_batch_post(dJ.ptr, nj, dS.ptr, post_base, bs);
-    hipError_t e3 = erd_launch_round_end(dJ.ptr, nj, dS.ptr, bs);
-    hipError_t ec = hipStreamEndCapture(bs, &graph);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || ec != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return set_error(ER_ERROR, "batch graph capture failed: %s", hipGetErrorString(ec != hipSuccess ? ec : (e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3))));
+    hipError_t ec = hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal);
+    hipEvent_t fork = nullptr, posted[2] = {nullptr, nullptr};
+    if (ec == hipSuccess) ec = new_event(fork);
+    if (ec == hipSuccess) ec = hipEventRecord(fork, bs);
+    if (ec == hipSuccess) ec = hipStreamWaitEvent(bs2, fork, 0);
+    for (int r = 0; r < rounds && ec == hipSuccess; ++r) {
+        const int b = r & 1;
+        if (posted[b]) ec = hipStreamWaitEvent(bs, posted[b], 0);
+        hipEvent_t shaded = nullptr;
+        if (ec == hipSuccess) ec = new_event(shaded);
+        if (ec == hipSuccess)
+            ec = erd_launch_batch_trace(dJ[b].ptr, nj, dS[b].ptr, full, g_primary_blocks, shade_blocks, (unsigned int)r, bs,
+                                        bs2, shaded);
+        if (ec == hipSuccess) ec = erd_launch_batch_post(dJ[b].ptr, nj, (unsigned int)r, post_base, bs2);
+        if (ec == hipSuccess) ec = erd_launch_round_end(dJ[b].ptr, nj, dS[b].ptr, bs2);
+        if (ec == hipSuccess) ec = new_event(posted[b]);
+        if (ec == hipSuccess) ec = hipEventRecord(posted[b], bs2);
     }
-    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ei != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return set_error(ER_ERROR, "%s (hipGraphInstantiate)", hipGetErrorString(ei));
+    hipEvent_t join = nullptr;
+    if (ec == hipSuccess) ec = new_event(join);
+    if (ec == hipSuccess) ec = hipEventRecord(join, bs2);
+    if (ec == hipSuccess) ec = hipStreamWaitEvent(bs, join, 0);
+    hipError_t ee = hipStreamEndCapture(bs, &graph);
+    if (ec == hipSuccess) ec = ee;
+    if (ec == hipSuccess) ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (auto e : evs) (void)hipEventDestroy(e);
+    if (ec != hipSuccess) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_error(ER_ERROR, "batch graph: %s", hipGetErrorString(ec));
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (timing) {
@@ -1074,9 +1121,7 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         HIP_TRY(hipEventCreate(&t1));
         HIP_TRY(hipEventRecord(t0, bs));
     }
-    const int rounds = jobs[0].first;
-    hipError_t el = hipSuccess;
-    for (int r = 0; r < rounds && el == hipSuccess; ++r) el = hipGraphLaunch(exec, bs);
+    hipError_t el = hipGraphLaunch(exec, bs);
     if (timing && el == hipSuccess) el = hipEventRecord(t1, bs);
     const hipError_t es = hipStreamSynchronize(bs);
     (void)hipGraphExecDestroy(exec);
@@ -1090,7 +1135,8 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
     for (auto& j : jobs) {
         Tracer& t = *j.second;
         t.desc.no_iterations += j.first;
-        t.cur = 0;
+        t.cur = (j.first - 1) & 1;   /* the set of the tracer's last round */
+        t.set[0].pending = t.set[1].pending = false;
         /* the bilateral display filter only shapes the displayed image: once, after the last pass */
         if (use_bilateral(t))
             HIP_TRY(erd_launch_bilateral(t.bilateral.ptr, t.display.ptr, t.disp_tmp.ptr, t.disp_filt.ptr, t.W, t.H, t.stream));
@@ -1148,35 +1194,84 @@ ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t*
     if ((st = adjoint_setup(*t, seeds1, seeds2, count, passes))) return st;
     if (!dL_drunning_xyz || !dL_dlut) return set_error(ER_ERROR, "adjoint pointer is NULL");
     const size_t n = (size_t)t->W * t->H;
-    const int nblocks = ((t->W + 15) / 16) * ((t->H + 15) / 16);
+    const int blocks = 1024;
     const size_t per_block = erd_adj_partials_per_block();
+    size_t rs = 0, re = 0, rj = 0;
+    erd_pass_record_sizes(&rs, &re, &rj);
+    /* two buffer sets: pass k + 1's primary and shading kernels (first stream) overlap pass k's
+     * adjoint kernels (second stream), as the forward passes do */
+    struct AdjSet {
+        DevBuf<uint32_t> s1s, s2s, counts, lcount;
+        DevBuf<float> tvis;
+        DevBuf<float4> frame;
+        DevBuf<char> queue, events, jobs;
+        DevBuf<uint2> list;
+        hipEvent_t shaded = nullptr, done = nullptr;
+        bool pending = false;
+        ~AdjSet()
+        {
+            if (shaded) (void)hipEventDestroy(shaded);
+            if (done) (void)hipEventDestroy(done);
+        }
+    } S[2];
     DevBuf<uint32_t> d1, d2;
     DevBuf<float> dldr, partials;
     DevBuf<float4> adj;
     DevBuf<double> out;
     if ((st = d1.alloc(n)) || (st = d2.alloc(n)) || (st = dldr.alloc(n * 3)) || (st = adj.alloc(n)) ||
-        (st = partials.alloc((size_t)nblocks * per_block)) || (st = out.alloc(per_block)))
+        (st = partials.alloc(3 * (size_t)blocks * per_block)) || (st = out.alloc(per_block)))
         return st;
+    for (AdjSet& A : S) {
+        if ((st = A.s1s.alloc(n)) || (st = A.s2s.alloc(n)) || (st = A.counts.alloc(4)) || (st = A.lcount.alloc(1)) ||
+            (st = A.tvis.alloc(2 * n)) || (st = A.frame.alloc(n)) || (st = A.queue.alloc(n * rs)) ||
+            (st = A.events.alloc(n * re)) || (st = A.jobs.alloc(2 * n * rj)) || (st = A.list.alloc(n)))
+            return st;
+        HIP_TRY(hipEventCreateWithFlags(&A.shaded, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&A.done, hipEventDisableTiming));
+        HIP_TRY(hipMemset(A.counts.ptr, 0, 4 * sizeof(uint32_t)));
+        HIP_TRY(hipMemset(A.lcount.ptr, 0, sizeof(uint32_t)));
+    }
+    hipStream_t s = t->stream, s2 = t->stream2;
     HIP_TRY(hipMemcpy(d1.ptr, seeds1, n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d2.ptr, seeds2, n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dldr.ptr, dL_drunning_xyz, n * 12, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(out.ptr, 0, per_block * sizeof(double)));
-    const size_t lds = t->lds_bytes;
+    const int shade_blocks = std::max(1, std::min((int)((n + 255) / 256), 2048));
+    const size_t lds_bits = (size_t)t->host_params.mc_words * 4;
     float cur_w = -1.0f;
     /* the running estimate after N passes is the mean of the filtered frames of passes 2..N
      * (the first pass is discarded, Q1), or the first frame when N == 1 */
     for (int k = 1; k <= passes; ++k) {
+        AdjSet& A = S[k & 1];
         const float w = passes == 1 ? 1.0f : (k >= 2 ? 1.0f / (float)(passes - 1) : 0.0f);
-        if (w != cur_w) {
-            HIP_TRY(erd_launch_adj_filter_t(dldr.ptr, adj.ptr, t->W, t->H, w, t->stream));
-            cur_w = w;
+        if (A.pending) HIP_TRY(hipStreamWaitEvent(s, A.done, 0));
+        if (w != 0.0f) {
+            HIP_TRY(hipMemcpyAsync(A.s1s.ptr, d1.ptr, n * 4, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(hipMemcpyAsync(A.s2s.ptr, d2.ptr, n * 4, hipMemcpyDeviceToDevice, s));
         }
-        HIP_TRY(erd_launch_adjoint(t->params.ptr, 0, nullptr, nullptr, adj.ptr, nullptr, partials.ptr, d1.ptr, d2.ptr,
-                                   t->W, t->H, lds, t->stream));
-        if (w != 0.0f) HIP_TRY(erd_launch_adj_reduce(partials.ptr, nblocks, out.ptr, t->stream));
+        /* the forward pass's primary and shading kernels: decisions, queues, final seeds */
+        HIP_TRY(erd_launch_primary_shade(t->params.ptr, A.frame.ptr, d1.ptr, d2.ptr, A.queue.ptr, A.counts.ptr, A.events.ptr,
+                                         A.jobs.ptr, t->W, t->H, t->lut_lds, t->lds_bytes, lds_bits, shade_blocks,
+                                         g_primary_blocks, s));
+        HIP_TRY(hipEventRecord(A.shaded, s));
+        HIP_TRY(hipStreamWaitEvent(s2, A.shaded, 0));
+        if (w != 0.0f) {
+            if (w != cur_w) {
+                HIP_TRY(erd_launch_adj_filter_t(dldr.ptr, adj.ptr, t->W, t->H, w, s2));
+                cur_w = w;
+            }
+            HIP_TRY(erd_launch_adj_wavefront(t->params.ptr, A.counts.ptr, A.jobs.ptr, A.events.ptr, A.queue.ptr, adj.ptr,
+                                             A.tvis.ptr, A.frame.ptr, A.list.ptr, A.lcount.ptr, A.s1s.ptr, A.s2s.ptr,
+                                             partials.ptr, blocks, (int)n, t->lds_bytes, s2));
+            HIP_TRY(erd_launch_adj_reduce(partials.ptr, 3 * blocks, out.ptr, s2));
+        }
+        HIP_TRY(erd_launch_adj_reset(A.counts.ptr, A.lcount.ptr, s2));
+        HIP_TRY(hipEventRecord(A.done, s2));
+        A.pending = true;
     }
-    HIP_TRY(hipMemcpyAsync(dL_dlut, out.ptr, per_block * sizeof(double), hipMemcpyDeviceToHost, t->stream));
-    HIP_TRY(hipStreamSynchronize(t->stream));
+    HIP_TRY(hipMemcpyAsync(dL_dlut, out.ptr, per_block * sizeof(double), hipMemcpyDeviceToHost, s2));
+    HIP_TRY(hipStreamSynchronize(s2));
+    if ((st = sync_tracer(*t))) return st;
     return ER_OK;
 }
 

@@ -92,28 +92,7 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
         paths++;
         rng.s0 = seeds1[i];
         rng.s1 = seeds2[i];
-        smp.comp = rng.u();
-        smp.dir0 = rng.u(); smp.dir1 = rng.u();
-        smp.uvw0 = rng.u(); smp.uvw1 = rng.u(); smp.uvw2 = rng.u();
-        smp.lightnum = rng.u();
-        smp.film0 = rng.u(); smp.film1 = rng.u();
-        smp.lens0 = rng.u(); smp.lens1 = rng.u();
-        const float sx = p.screen00 + (p.invscreen0 * ((float)px + smp.film0 * 1.0f / (float)p.film_w));
-        const float sy = p.screen10 + (p.invscreen1 * ((float)py + smp.film1 * 1.0f / (float)p.film_h));
-        const f3 N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
-        Ray r;
-        r.O = ld3(p.cam_pos);
-        r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
-        r.MinT = p.clip_near;
-        r.MaxT = p.clip_far;
-        if (p.aperture != 0.0f) {
-            float lx, ly;
-            concentric_disk(smp.lens0, smp.lens1, lx, ly);
-            lx = lx * p.aperture; ly = ly * p.aperture;
-            const f3 LI = add(scl(lx, U), scl(ly, V));
-            r.O = add(r.O, LI);
-            r.D = nrm(sub(scl(p.focal, r.D), LI));
-        }
+        const Ray r = camera_ray(p, px, py, rng, smp);
         D = r.D;
         float Tv = 0.0f;
         const bool vol = march<FULL, COUNT, ER_BATCH_PRIMARY>(p, T, r, true, rng, Tv, Pv, cnt);
@@ -341,10 +320,10 @@ __device__ __forceinline__ Tables global_tables(const DevParams& p)
 /* Primary kernel of a batch round: persistent waves take 8 x 8 tiles from one counter over
  * the tiles of every active job; a wave finds its tile's job in the table. */
 template<int FULL>
-__global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ J, int njobs, BatchState* __restrict__ st)
+__global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ J, int njobs, BatchState* __restrict__ st,
+                                                  unsigned int round)
 {
     const int lane = threadIdx.x & 63;
-    const unsigned int round = st->round;
     const int na = batch_active(J, njobs, round);
     if (na == 0) return;
     const unsigned int total = (unsigned int)(J[na - 1].tile_base + J[na - 1].ntiles);
@@ -381,9 +360,9 @@ __device__ __forceinline__ bool batch_slot(const BatchJob* __restrict__ J, int n
 }
 
 template<int FULL>
-__global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+__global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
 {
-    const int na = batch_active(J, njobs, st->round);
+    const int na = batch_active(J, njobs, round);
     const int lane = threadIdx.x & 63;
     const unsigned int waves = gridDim.x * (blockDim.x >> 6);
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -398,9 +377,9 @@ __global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J,
 }
 
 template<int FULL>
-__global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+__global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
 {
-    const int na = batch_active(J, njobs, st->round);
+    const int na = batch_active(J, njobs, round);
     const int lane = threadIdx.x & 63;
     const unsigned int waves = gridDim.x * (blockDim.x >> 6);
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -415,9 +394,9 @@ __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J
     }
 }
 
-__global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J, int njobs, const BatchState* __restrict__ st)
+__global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
 {
-    const int na = batch_active(J, njobs, st->round);
+    const int na = batch_active(J, njobs, round);
     const int lane = threadIdx.x & 63;
     const unsigned int waves = gridDim.x * (blockDim.x >> 6);
     for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);; w += waves) {
@@ -429,19 +408,14 @@ __global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J
     }
 }
 
-/* end of a round: empty every job's queues and advance the round (one workgroup) */
+/* end of a round: empty the queues of this buffer set and its tile counter (one workgroup) */
 __global__ __launch_bounds__(64) void k_round_end(const BatchJob* __restrict__ J, int njobs, BatchState* __restrict__ st)
 {
-    const unsigned int round = st->round;
     for (int k = threadIdx.x; k < njobs; k += blockDim.x) {
         J[k].counts[0] = 0u;
         J[k].counts[1] = 0u;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        st->round = round + 1u;
-        st->tiles = 0u;
-    }
+    if (threadIdx.x == 0) st->tiles = 0u;
 }
 
 } /* namespace erd */
@@ -510,24 +484,53 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
     return hipGetLastError();
 }
 
+/* One batch round, split like a single pass: primary and shading on st; shadow marches and
+ * combine on st2 after the event `shaded`. */
 extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, erd::BatchState* st, int full,
-                                             int primary_blocks, int shade_blocks, hipStream_t stream)
+                                             int primary_blocks, int shade_blocks, unsigned int round, hipStream_t stream,
+                                             hipStream_t stream2, hipEvent_t shaded)
 {
     if (full) {
-        hipLaunchKernelGGL(erd::k_primary_b<1>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st);
-        hipLaunchKernelGGL(erd::k_shade_b<1>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
-        hipLaunchKernelGGL(erd::k_shadow_b<1>, dim3(2 * shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+        hipLaunchKernelGGL(erd::k_primary_b<1>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st, round);
+        hipLaunchKernelGGL(erd::k_shade_b<1>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, round);
     } else {
-        hipLaunchKernelGGL(erd::k_primary_b<0>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st);
-        hipLaunchKernelGGL(erd::k_shade_b<0>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
-        hipLaunchKernelGGL(erd::k_shadow_b<0>, dim3(2 * shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+        hipLaunchKernelGGL(erd::k_primary_b<0>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st, round);
+        hipLaunchKernelGGL(erd::k_shade_b<0>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, round);
     }
-    hipLaunchKernelGGL(erd::k_finish_b, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, (const erd::BatchState*)st);
+    hipError_t e = hipEventRecord(shaded, stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream2, shaded, 0);
+    if (e != hipSuccess) return e;
+    if (full) hipLaunchKernelGGL(erd::k_shadow_b<1>, dim3(2 * shade_blocks), dim3(256), 0, stream2, J, njobs, round);
+    else hipLaunchKernelGGL(erd::k_shadow_b<0>, dim3(2 * shade_blocks), dim3(256), 0, stream2, J, njobs, round);
+    hipLaunchKernelGGL(erd::k_finish_b, dim3(shade_blocks), dim3(256), 0, stream2, J, njobs, round);
     return hipGetLastError();
 }
 
 extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, erd::BatchState* st, hipStream_t stream)
 {
     hipLaunchKernelGGL(erd::k_round_end, dim3(1), dim3(64), 0, stream, J, njobs, st);
+    return hipGetLastError();
+}
+
+/* the primary and shading kernels of a full-estimator pass alone (the path-replay adjoint
+ * follows them with its own kernels) */
+extern "C" hipError_t erd_launch_primary_shade(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2,
+                                               void* queue, unsigned int* counts, void* events, void* jobs, int W, int H,
+                                               int lut_lds, size_t lds_bytes, size_t lds_bits, int shade_blocks,
+                                               int primary_blocks, hipStream_t st)
+{
+    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const int ntiles = tiles_x * tiles_y;
+    const int nblocks = std::max(1, std::min((ntiles + 3) / 4, primary_blocks));
+    erd::ScatterRec* q = (erd::ScatterRec*)queue;
+    if (lut_lds)
+        hipLaunchKernelGGL((erd::k_primary<1, 0, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
+                           (unsigned long long*)nullptr, tiles_x, ntiles);
+    else
+        hipLaunchKernelGGL((erd::k_primary<1, 0, 0>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
+                           (unsigned long long*)nullptr, tiles_x, ntiles);
+    hipLaunchKernelGGL((erd::k_shade<1, 0, 1, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1, s2,
+                       (const erd::ScatterRec*)q, counts, (erd::EventRec*)events, (erd::ShadowJob*)jobs,
+                       (unsigned long long*)nullptr);
     return hipGetLastError();
 }
