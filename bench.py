@@ -1,17 +1,24 @@
 """Benchmark: Mpaths/s and ms/frame of the single-scattering tracer on one MI355X.
 
-A step is one progressive pass (RenderEstimate: trace + 3x3 frame filter + running average +
-tonemap + bilateral display filter) over the whole film. The workload is the CT-like plain
-tracer scene: 256^3 uint16 phantom of 12 nested ellipsoids in 5 intensity bands with N(0, 20^2)
-noise, 1024 x 768 film, 2 plane lights + a uniform environment sphere light, hybrid
-BRDF/phase shading keyed on gradient magnitude, 8-node transfer function, 64 progressive
-passes. Inputs (volume, transfer functions, seeds) are resident on the device before timing.
+Headline (the JSON line's `value`): the CT-like plain tracer workload - 256^3 uint16 phantom
+of 12 nested ellipsoids in 5 intensity bands with N(0, 20^2) noise, 1024 x 768 film, 2 plane
+lights + a uniform environment sphere light, hybrid BRDF/phase shading keyed on gradient
+magnitude, 8-node transfer function, bilateral display filter, 64 progressive passes. A step
+is one progressive pass over the whole film (RenderEstimate: trace + 3x3 frame filter +
+running average + tonemap + bilateral). Inputs are resident on the device before timing;
+`value` = paths / wall time of the timed passes.
 
-Printed JSON line: value = paths / wall time of the K timed passes; roofline of the trace
-kernel (algorithmic bytes 16 B per trilinear sample + 64 B per path, exact sample counts from a
-counting replay of the same passes with the same seeds, divided by the HIP-event time of the
-trace kernel); RMSE of the tonemapped running estimate against the FP64 oracle's expectation
-(tests/golden/fp64_ct.npz); the CPU oracle timed on a bounded sample on the host cores.
+Also reported, each on its own workload:
+  roofline      the primary-march kernel (the dominant one): algorithmic bytes = 16 B per
+                trilinear sample (8 uint16 corners) of the primary march + 32 B per path
+                (seeds in/out, frame estimate), unit counts from a counting replay of the same
+                passes with the same seeds, over its HIP-event-timed average duration;
+  rmse          tonemapped running estimate vs the FP64 oracle's expectation (fixture);
+  cpu_baseline  the CPU oracle (the reference estimator restated, FP32, OpenMP) on the host;
+  small         the 64^3 blob workload (256 x 256, 16 passes): Mpaths/s, RMSE, GPU / CPU;
+  adjoint       the differentiation workload (128^3, 512 x 512, 32 passes): forward and
+                path-replay adjoint wall times;
+  batched       64 jobs of uneven size: one er_render_batch vs the same jobs one after another.
 """
 import argparse
 import json
@@ -51,7 +58,7 @@ def rmse_vs_golden(running, golden_path):
 
 
 def traffic_from_profile(path):
-    """HBM bytes per trace launch from the committed rocprofv3 --pmc summary, if present."""
+    """HBM bytes per primary-kernel launch from the committed rocprofv3 --pmc summary."""
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -59,12 +66,14 @@ def traffic_from_profile(path):
     return data.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(scene_builder, passes, threads):
+def cpu_rate(scene_builder, passes, threads):
+    """The CPU oracle over `passes` full passes of the scene: Mpaths/s and seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     sc = scene_builder()
     orc = O.Oracle(sc, nthreads=threads)
-    orc.compute_gm_max()
+    if sc.tracer.settings.shading_type not in (0, 1):
+        orc.compute_gm_max()
     W, H = sc.film
     s1, s2 = O.seeds_for(W, H, 99)
     t0 = time.perf_counter()
@@ -74,20 +83,151 @@ def cpu_baseline(scene_builder, passes, threads):
     return W * H * passes / dt / 1e6, dt
 
 
+def timed_passes(er, r, steps):
+    lib = er.load()
+    lib.er_synchronize()
+    t0 = time.perf_counter()
+    timing = r.render_passes(steps)
+    lib.er_synchronize()
+    return time.perf_counter() - t0, timing
+
+
+def bench_small(er, scenes, threads, no_cpu):
+    """The 64^3 blob workload: 16 passes; more if the RMSE gate needs them."""
+    sc = scenes.config_small()
+    r = er.Renderer(sc)
+    r.seed(4321)
+    r.render_passes(2)
+    r.rebind()
+    passes, wall = 0, 0.0
+    rm = None
+    for block in (16, 16, 32, 64, 128, 256, 512):
+        dt, _ = timed_passes(er, r, block)
+        wall += dt
+        passes += block
+        rm = rmse_vs_golden(r.running(), os.path.join(ROOT, "tests", "golden", "fp64_small.npz"))
+        if rm is None or rm["display"] <= 0.01:
+            break
+    W, H = r.W, r.H
+    value = W * H * passes / wall / 1e6
+    out = {"workload": "blobs_64c_256x256", "passes": passes, "mpaths_per_s": round(value, 2),
+           "ms_per_frame": round(wall * 1e3 / passes, 4), "rmse": rm}
+    if not no_cpu:
+        v, dt = cpu_rate(scenes.config_small, 4, threads)
+        out["cpu_mpaths_per_s"] = round(v, 4)
+        out["cpu_threads"] = threads
+        out["gpu_over_cpu"] = round(value / v, 1)
+    r.close()
+    return out
+
+
+def bench_adjoint(er, scenes):
+    """Differentiation workload: forward passes vs the path-replay adjoint of the same passes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    sc = scenes.config_adjoint()
+    r = er.Renderer(sc)
+    lib = er.load()
+    s1, s2 = O.seeds_for(r.W, r.H, 1)
+    out = {"workload": "blobs_128c_512x512_32spp_lut256x4", "passes": sc.passes}
+    for name, lut in (("initial_lut", scenes.initial_tf_lut()), ("target_lut", scenes.random_tf_lut(4))):
+        r.set_tf_lut(lut, sc.tf_range)
+        r.set_seeds(s1, s2)
+        r.render_passes(2)                      # warm-up
+        r.adjoint_lut(s1, s2, 2, np.ones((r.H, r.W, 3), np.float32))
+        r.set_tf_lut(lut, sc.tf_range)
+        r.set_seeds(s1, s2)
+        fwd, _ = timed_passes(er, r, sc.passes)
+        img = r.running()[..., :3]
+        g_img = (2.0 * img).astype(np.float32)  # dL/dimage of L = sum(image^2)
+        lib.er_synchronize()
+        t0 = time.perf_counter()
+        r.adjoint_lut(s1, s2, sc.passes, g_img)
+        adj = time.perf_counter() - t0
+        out[name] = {"forward_ms": round(fwd * 1e3, 3), "adjoint_ms": round(adj * 1e3, 3),
+                     "adjoint_over_forward": round(adj / fwd, 3)}
+    r.close()
+    return out
+
+
+def bench_batched(er, scenes, n_jobs):
+    """64 jobs: er_render_batch vs the same jobs rendered one after another, both with
+    RenderEstimate-style single passes and with er_render_passes. The results are bit-identical
+    (same seeds, same paths), so the RMSE is equal."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    lib = er.load()
+    lib.er_set_list_capacity(4096)
+    jobs = scenes.config_batch(n_jobs=n_jobs)
+    rs = [er.Renderer(sc) for sc in jobs]
+    passes = [sc.passes for sc in jobs]
+    seeds = [O.seeds_for(r.W, r.H, 40 + k) for k, r in enumerate(rs)]
+    for r in rs:
+        r.render_passes(1)
+
+    def reset():
+        for r, (s1, s2) in zip(rs, seeds):
+            r.rebind()
+            r.set_seeds(s1, s2)
+        lib.er_synchronize()
+
+    reset()
+    t0 = time.perf_counter()
+    for r, p in zip(rs, passes):
+        for _ in range(p):
+            r.render_estimate()
+    seq_est = time.perf_counter() - t0
+    ref = [r.frame() for r in rs]
+    reset()
+    t0 = time.perf_counter()
+    for r, p in zip(rs, passes):
+        r.render_passes(p)
+    lib.er_synchronize()
+    seq_passes = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    er.render_batch(rs, passes)
+    lib.er_synchronize()
+    bat = time.perf_counter() - t0
+    same = all(np.array_equal(a, r.frame()) for a, r in zip(ref, rs))
+    paths = sum(r.W * r.H * p for r, p in zip(rs, passes))
+    for r in rs:
+        r.close()
+    lib.er_set_list_capacity(32)
+    return {"workload": f"{n_jobs}_jobs_edges_32-256_films_128-1024_spp_8-128", "paths": int(paths),
+            "sequential_estimate_s": round(seq_est, 4), "sequential_passes_s": round(seq_passes, 4),
+            "batch_s": round(bat, 4), "batch_over_sequential_estimate": round(bat / seq_est, 4),
+            "batch_over_sequential_passes": round(bat / seq_passes, 4),
+            "batch_mpaths_per_s": round(paths / bat / 1e6, 2), "identical_frames": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-count", action="store_true", help="skip the counting replay (roofline)")
-    ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--cpu-passes", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy a device buffer of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the tracer has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        return
 
     import exposure_render as er
     from exposure_render import scenes
@@ -95,19 +235,12 @@ def main():
     sc = scenes.config_ct()
     r = er.Renderer(sc)
     r.seed(1234)
-    lib = er.load()
     W, H = r.W, r.H
     if args.warmup > 0:
         r.render_passes(args.warmup)
     s1, s2 = r.get_seeds()
-    n0 = r.no_iterations()
 
-    lib.er_synchronize()
-    t0 = time.perf_counter()
-    timing = r.render_passes(args.steps)
-    lib.er_synchronize()
-    wall = time.perf_counter() - t0
-
+    wall, timing = timed_passes(er, r, args.steps)
     paths = W * H * args.steps
     value = paths / wall / 1e6
     ms_per_step = wall * 1e3 / args.steps
@@ -124,24 +257,33 @@ def main():
         r.render_passes(args.steps)
         r.set_counting(False)
         counts = r.counters()
-        samples = counts["primary_samples"] + counts["shadow_samples"] + counts["shading_samples"]
-        alg_bytes = 16.0 * samples + 64.0 * counts["paths"]
-        per_launch = alg_bytes / args.steps
-        avg_ms = timing.trace_ms / args.steps
+        per_launch = (16.0 * counts["primary_samples"] + 32.0 * counts["paths"]) / args.steps
+        avg_ms = timing.primary_ms / args.steps
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic_from_profile(os.path.join(ROOT, "profiles", "trace_ct_traffic.json")),
-                    "kernel": "k_trace<FULL>", "avg_launch_ms": round(avg_ms, 4),
+                    "traffic": traffic_from_profile(os.path.join(ROOT, "profiles", "k_primary_ct_traffic.json")),
+                    "kernel": "k_primary<1,0,1>", "avg_launch_ms": round(avg_ms, 4),
                     "algorithmic_bytes_per_launch": int(per_launch)}
+        # what the march actually reads after exact empty-space skipping, and what HBM moved
+        fetched = (16.0 * counts["primary_fetches"] + 32.0 * counts["paths"]) / args.steps
+        roofline["fetched_bytes_per_launch"] = int(fetched)
+        roofline["fetched_GBs"] = round(fetched / (avg_ms * 1e-3) / 1e9, 1)
+        if roofline["traffic"]:
+            roofline["hbm_GBs"] = round(roofline["traffic"] / (avg_ms * 1e-3) / 1e9, 1)
+        samples = counts["primary_samples"] + counts["shadow_samples"] + counts["shading_samples"]
+        whole = (16.0 * samples + 64.0 * counts["paths"]) / args.steps
+        roofline["whole_pass"] = {"algorithmic_bytes_per_pass": int(whole),
+                                  "achieved_GBs": round(whole / (ms_per_step * 1e-3) / 1e9, 1)}
 
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v, dt = cpu_baseline(scenes.config_ct, args.cpu_passes, threads)
+        v, dt = cpu_rate(scenes.config_ct, args.cpu_passes, threads)
         cpu = {"value": round(v, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_passes} full passes of the same scene ({W}x{H} paths each) by the FP32 "
                          f"oracle restatement, {dt:.1f} s"}
+    r.close()
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
@@ -151,13 +293,17 @@ def main():
                    "volume": list(sc.voxels.shape[::-1]), "spp": args.steps, "estimator": "full",
                    "shading": "hybrid", "lights": "2 plane + environment sphere", "bilateral": True},
         "ms_per_frame": {"total": round(timing.total_ms / args.steps, 4),
-                         "trace": round(timing.trace_ms / args.steps, 4),
+                         "primary": round(timing.primary_ms / args.steps, 4),
+                         "trace_overlapping": round(timing.trace_ms / args.steps, 4),
                          "post": round(timing.post_ms / args.steps, 4)},
         "rmse": rm, "roofline": roofline, "cpu_baseline": cpu, "counts": counts,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
-    r.close()
+    if not args.no_extra:
+        out["small"] = bench_small(er, scenes, threads, args.no_cpu)
+        out["adjoint"] = bench_adjoint(er, scenes)
+        out["batched"] = bench_batched(er, scenes, args.batch_jobs)
     er.shutdown()
     print(json.dumps(out))
 
