@@ -66,6 +66,8 @@ extern "C" hipError_t erd_launch_adj_wavefront(const DevParams* p, unsigned int*
                                                unsigned int* lcount, const uint32_t* s1start, const uint32_t* s2start,
                                                float* partials, int blocks, int n, size_t lds_bytes, hipStream_t st);
 extern "C" hipError_t erd_launch_adj_reset(unsigned int* counts, unsigned int* lcount, hipStream_t st);
+extern "C" hipError_t erd_launch_autofocus(const DevParams* p, int u, int v, uint32_t s0, uint32_t s1, float* out,
+                                           hipStream_t st);
 extern "C" hipError_t erd_launch_adjoint(const DevParams* p, int mode, const float* lut_eval, const float* sigma_eval,
                                          const float4* adj, float4* value, float* partials, uint32_t* s1, uint32_t* s2,
                                          int W, int H, size_t lds_bytes, hipStream_t st);
@@ -1358,15 +1360,38 @@ ER_API int er_get_no_iterations(int tracer_id, int* n)
     return ER_OK;
 }
 
+ER_API int er_autofocus_seeds(int tracer_id, int film_u, int film_v, uint32_t* s0, uint32_t* s1)
+{
+    if (!s0 || !s1) return set_error(ER_ERROR, "output pointer is NULL");
+    uint64_t z = 0x4175746f466f6375ull ^ ((uint64_t)(uint32_t)tracer_id << 32) ^ ((uint64_t)(uint32_t)film_v << 16) ^
+                 (uint64_t)(uint32_t)film_u;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    *s0 = (uint32_t)z ? (uint32_t)z : 1u;
+    *s1 = (uint32_t)(z >> 32) ? (uint32_t)(z >> 32) : 1u;
+    return ER_OK;
+}
+
 ER_API int er_get_autofocus_distance(int tracer_id, int film_u, int film_v, float* distance)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mutex);
     int st;
     Tracer* t = get_tracer(tracer_id, st);
     if (!t) return st;
-    (void)film_u; (void)film_v;
     if (!distance) return set_error(ER_ERROR, "output pointer is NULL");
-    /* The reference entry point is a stub (core.cu:145-148): the distance is left unchanged. */
+    /* ComputeAutoFocusDistance (autofocus.cuh:26-75), which the reference entry point leaves
+     * commented out (core.cu:145-148). The reference seeds it with rand(); here the stream is
+     * a fixed function of the tracer ID and the film point, so the result is reproducible. */
+    if ((st = build_params(*t))) return st;
+    uint32_t s0, s1;
+    er_autofocus_seeds(tracer_id, film_u, film_v, &s0, &s1);
+    DevBuf<float> d;
+    if ((st = d.alloc(1))) return st;
+    HIP_TRY(erd_launch_autofocus(t->params.ptr, film_u, film_v, s0, s1, d.ptr, t->stream));
+    HIP_TRY(hipMemcpyAsync(distance, d.ptr, sizeof(float), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
     return ER_OK;
 }
 

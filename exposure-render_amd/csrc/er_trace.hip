@@ -303,6 +303,54 @@ __global__ __launch_bounds__(256) void k_finish(const unsigned int* __restrict__
         finish_item(events[q], q, vis, frame);
 }
 
+/* ComputeAutoFocusDistance (autofocus.cuh:26-63), one lane: 100 rays through film point
+ * (u, v), each jittered by 0.01 x a concentric-disk sample of the screen (Get2 in clang's
+ * order), marched with SampleVolume on one RNG stream; the mean distance of the rays' volume
+ * events, or the distance to the last event point (origin) when none. */
+template<int FULL>
+__device__ void autofocus(const DevParams& p, int u, int v, uint32_t s0, uint32_t s1, float* out)
+{
+    Tables T;
+    T.bits = nullptr; T.bits_glb = gmem(p.mc_bits); T.lut_lds = nullptr; T.lut_glb = gmem(p.sigma_lut);
+    Rng rng;
+    rng.s0 = s0;
+    rng.s1 = s1;
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const f3 O = ld3(p.cam_pos), N = ld3(p.cam_N), U = ld3(p.cam_U), V = ld3(p.cam_V);
+    float sum = 0.0f, w = 0.0f;
+    f3 lastP = mk(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < 100; ++i) {
+        float sx = p.screen00 + (p.invscreen0 * (float)u);
+        float sy = p.screen10 + (p.invscreen1 * (float)v);
+        const float d0 = rng.u(), d1 = rng.u();
+        float dx, dy;
+        concentric_disk(d0, d1, dx, dy);
+        sx += 0.01f * dx;
+        sy += 0.01f * dy;
+        Ray r;
+        r.O = O;
+        r.D = nrm(sub(add(N, scl(sx, U)), scl(sy, V)));
+        r.MinT = p.clip_near;
+        r.MaxT = p.clip_far;
+        float Tv; f3 Pv;
+        if (march<FULL, 0, 1>(p, T, r, true, rng, Tv, Pv, cnt)) {
+            sum += len(sub(Pv, O));
+            w += 1.0f;
+            lastP = Pv;
+        }
+    }
+    *out = sum <= 0.0f ? len(sub(lastP, O)) : sum / w;
+}
+
+__global__ __launch_bounds__(64) void k_autofocus(const DevParams* __restrict__ pp, int u, int v, uint32_t s0, uint32_t s1,
+                                                 float* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const DevParams& p = *pp;
+    if (p.estimator == ER_ESTIMATOR_FULL) autofocus<1>(p, u, v, s0, s1, out);
+    else autofocus<0>(p, u, v, s0, s1, out);
+}
+
 /* ---- batches: the same pass over many tracers per launch (er_render_batch) ------------ */
 
 /* tables of a batch job: occupancy bits and opacity table read from global memory (the
@@ -532,5 +580,12 @@ extern "C" hipError_t erd_launch_primary_shade(const erd::DevParams* p, float4* 
     hipLaunchKernelGGL((erd::k_shade<1, 0, 1, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1, s2,
                        (const erd::ScatterRec*)q, counts, (erd::EventRec*)events, (erd::ShadowJob*)jobs,
                        (unsigned long long*)nullptr);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t erd_launch_autofocus(const erd::DevParams* p, int u, int v, uint32_t s0, uint32_t s1, float* out,
+                                           hipStream_t st)
+{
+    hipLaunchKernelGGL(erd::k_autofocus, dim3(1), dim3(64), 0, st, p, u, v, s0, s1, out);
     return hipGetLastError();
 }

@@ -85,6 +85,16 @@ class Renderer:
         _check(self.lib, self.lib.er_get_estimate(self.id, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.nbytes))
         return out
 
+    def autofocus_distance(self, u, v):
+        d = C.c_float(-1.0)
+        _check(self.lib, self.lib.er_get_autofocus_distance(self.id, int(u), int(v), C.byref(d)))
+        return d.value
+
+    def autofocus_seeds(self, u, v):
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(self.lib, self.lib.er_autofocus_seeds(self.id, int(u), int(v), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def no_iterations(self):
         n = C.c_int(0)
         _check(self.lib, self.lib.er_get_no_iterations(self.id, C.byref(n)))

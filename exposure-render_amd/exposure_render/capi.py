@@ -205,6 +205,7 @@ SIGNATURES = {
     "er_set_tf_lut": (C.c_int, [C.c_int, _P(C.c_float), C.c_float]),
     "er_render_batch": (C.c_int, [_P(BatchItem), C.c_int, _P(Timing)]),
     "er_set_list_capacity": (C.c_int, [C.c_int]),
+    "er_autofocus_seeds": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(C.c_uint32), _P(C.c_uint32)]),
     "er_adjoint_lut": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
                                  _P(C.c_double)]),
     "er_adjoint_value": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),

@@ -79,7 +79,8 @@ extern "C" {
 #define ER_ESTIMATOR_FULL   1
 
 /* Free-flight samplers (new). RAYMARCH = Source/raymarching.h:30-113 (jittered fixed-step
- * tracking); WOODCOCK = delta tracking against per-brick majorants. */
+ * tracking); WOODCOCK = delta tracking against the global majorant DS^2 * max opacity (full
+ * estimator only). */
 #define ER_SAMPLER_RAYMARCH 0
 #define ER_SAMPLER_WOODCOCK 1
 
@@ -313,9 +314,16 @@ ER_API int er_render_estimate(int tracer_id);
  * size must be at least W*H*4. With the bilateral filter on, the filtered image. */
 ER_API int er_get_estimate(int tracer_id, uint8_t* rgba, size_t size);
 
-/* GetAutoFocusDistance (exposurerender.h:41; stub in core.cu:145-148): mean distance of
- * volume scatter points of 100 jittered rays through film point (u, v). */
+/* GetAutoFocusDistance (exposurerender.h:41): ComputeAutoFocusDistance (autofocus.cuh:26-75),
+ * which the reference leaves commented out (core.cu:145-148) - the mean distance from the
+ * camera of the volume events of 100 rays through film point (u, v), each jittered by 0.01 of
+ * a concentric-disk sample in screen space. The RNG stream is a fixed function of the tracer
+ * ID and the film point (the reference used rand()), exposed for checking through
+ * er_autofocus_seeds. */
 ER_API int er_get_autofocus_distance(int tracer_id, int film_u, int film_v, float* distance);
+/* the two RNG seeds er_get_autofocus_distance starts from (SplitMix64 of the tracer ID and
+ * film point, zero halves replaced by 1) */
+ER_API int er_autofocus_seeds(int tracer_id, int film_u, int film_v, uint32_t* s0, uint32_t* s1);
 
 /* GetNoIterations (exposurerender.h:42; stub in core.cu:150-153): passes since the last bind. */
 ER_API int er_get_no_iterations(int tracer_id, int* no_iterations);

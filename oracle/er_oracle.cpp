@@ -25,6 +25,8 @@
  *   camera (camera.h:61-96, singlescattering.h:29-50) pinned: closed-form pinhole rays
  *   shading (shader.h)                   pinned: Lambert/phase closed forms, Fresnel at normal incidence
  *   post (filterframeestimate.cuh, estimate.cuh, tonemap.cuh) pinned: closed forms
+ *   tf_lut_eval (node-table transfer function, no counterpart in the reference)
+ *                                        pinned: node values, linear interpolation, clamps
  *   full-mode hybrid shading, MIS, clipping, bilateral, Woodcock: own design from the reference's
  *   commented code; pinned by invariants and statistics (the light-sampling and BSDF-sampling
  *   strategies must agree in expectation), reference parity unpinned.
@@ -1141,6 +1143,46 @@ void* orc_create(const er_tracer_desc* t, const er_volume_desc* v, const er_ligh
 }
 
 void orc_destroy(void* h) { delete (Scene*)h; }
+
+/* ComputeAutoFocusDistance (autofocus.cuh:26-63): 100 rays through film point (u, v), each
+ * jittered by 0.01 x a concentric-disk sample of the screen (Get2, Q22 order: first draw ->
+ * first component), marched with SampleVolume; the mean distance of the rays' volume events,
+ * or the distance to the last event point when none (the reference's stale-event quirk is not
+ * kept: a ray without an event adds nothing). */
+float orc_autofocus(void* h, int u, int v, uint32_t s0, uint32_t s1)
+{
+    Scene& S = *(Scene*)h;
+    Rng r{s0, s1};
+    Est<float> est(S, r, nullptr);
+    const V3<float> O = from<float>(S.t.camera.pos), N = from<float>(S.camN), U = from<float>(S.camU),
+                    V = from<float>(S.camV);
+    float sum = 0.0f, w = 0.0f;
+    V3<float> lastP(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < 100; ++i) {
+        float sx = S.screen[0][0] + (S.invscreen[0] * (float)u);
+        float sy = S.screen[1][0] + (S.invscreen[1] * (float)v);
+        const float d0 = est.u(), d1 = est.u();
+        float dx, dy;
+        Est<float>::concentric_disk(d0, d1, dx, dy);
+        sx += 0.01f * dx;
+        sy += 0.01f * dy;
+        const Ray<float> ray(O, normalize((N + scale(sx, U)) - scale(sy, V)), S.t.camera.clip_near, S.t.camera.clip_far);
+        float T; V3<float> P;
+        if (est.march(ray, true, T, P)) {
+            sum += length(P - O);
+            w += 1.0f;
+            lastP = P;
+        }
+    }
+    return sum <= 0.0f ? length(lastP - O) : sum / w;
+}
+
+/* the node-table evaluation at intensity I, channel ch (float and double) */
+float orc_tf_lut_eval(const float* lut, float range, float I, int ch)
+{
+    std::vector<float> v(lut, lut + ER_TF_LUT_NODES * 4);
+    return tf_lut_eval(v, (float)(ER_TF_LUT_NODES - 1) / range, I, ch);
+}
 
 /* node-table transfer function; lut == NULL returns to the PLFs */
 void orc_set_tf_lut(void* h, const float* lut, float range)

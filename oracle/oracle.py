@@ -31,6 +31,8 @@ def load():
                                     _P(capi.TextureDesc), C.c_int, _P(capi.ClippingDesc), C.c_int]),
         "orc_destroy": (None, [C.c_void_p]),
         "orc_set_tf_lut": (None, [C.c_void_p, _P(C.c_float), C.c_float]),
+        "orc_tf_lut_eval": (C.c_float, [_P(C.c_float), C.c_float, C.c_float, C.c_int]),
+        "orc_autofocus": (C.c_float, [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32]),
         "orc_set_gm_max": (None, [C.c_void_p, C.c_float]),
         "orc_get_gm_max": (C.c_float, [C.c_void_p]),
         "orc_compute_gm_max": (C.c_float, [C.c_void_p, C.c_int]),
@@ -116,6 +118,9 @@ class Oracle:
             return
         self._lut = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
         self.lib.orc_set_tf_lut(self.h, _p(self._lut, C.c_float), float(range_))
+
+    def autofocus(self, u, v, s0, s1):
+        return self.lib.orc_autofocus(self.h, int(u), int(v), int(s0), int(s1))
 
     def compute_gm_max(self):
         return self.lib.orc_compute_gm_max(self.h, self.nthreads)
@@ -228,6 +233,11 @@ def gauss3x3_f64(img):
             tw[yd, xd] += w
     out[..., 3] = img[..., 3]
     return out / tw[..., None]
+
+
+def tf_lut_eval(lut, range_, intensity, ch):
+    lut = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
+    return load().orc_tf_lut_eval(_p(lut, C.c_float), float(range_), float(intensity), int(ch))
 
 
 def fmath(fn, x, y=None):

@@ -150,3 +150,19 @@ def test_node_table_transfer_function(fresh):
     """er_set_tf_lut: opacity and diffuse colour from a 256-node table, bit-for-bit decisions."""
     sc = scenes.config_adjoint(scale=0.25)
     run_parity(sc, passes=3, lut=scenes.random_tf_lut(7))
+
+
+def test_autofocus_distance(fresh):
+    """GetAutoFocusDistance (autofocus.cuh:26-75): the same 100 jittered rays and events as the
+    oracle from the same seeds, at the film centre and at a corner that misses the volume."""
+    sc = scenes.config_small(scale=0.5)
+    r = er.Renderer(sc)
+    orc = O.Oracle(sc)
+    for (u, v) in [(r.W // 2, r.H // 2), (r.W // 3, r.H // 2 + 5), (0, 0)]:
+        s0, s1 = r.autofocus_seeds(u, v)
+        d = r.autofocus_distance(u, v)
+        d_ref = orc.autofocus(u, v, s0, s1)
+        print(f"[autofocus] ({u},{v}) gpu {d:.7f} oracle {d_ref:.7f}", flush=True)
+        assert d == pytest.approx(d_ref, rel=1e-5, abs=1e-6)
+        assert d > 0.0
+    r.close()

@@ -380,3 +380,22 @@ def test_fp32_mean_matches_fp64_expectation():
     sel = e64[:, 3] > 0.5
     err = np.abs(m32[sel, :3] - e64[sel, :3]).mean()
     assert err < 0.05 * np.abs(e64[sel, :3]).mean()
+
+
+def test_node_table_transfer_function():
+    """Node n at intensity n * range / 255; linear between nodes; last node's value beyond the
+    range; negative interpolated values clamp to 0 (closed form, against numpy.interp)."""
+    rng = np.random.default_rng(3)
+    lut = rng.uniform(-0.2, 1.0, size=(256, 4)).astype(np.float32)
+    rng_i = 4095.0
+    nodes = np.arange(256) * (rng_i / 255.0)
+    for ch in range(4):
+        for I in [0.0, 1.0, 16.0, 17.0, 2047.0, 4094.0, 4095.0, 5000.0, 65535.0]:
+            got = O.tf_lut_eval(lut, rng_i, I, ch)
+            want = max(float(np.interp(min(I, rng_i), nodes, lut[:, ch].astype(np.float64))), 0.0)
+            assert abs(got - want) <= 2e-6 * max(1.0, abs(want)), (ch, I, got, want)
+    # exactly at a node: the node's value (when non-negative)
+    lut2 = np.abs(lut)
+    for n in [0, 1, 100, 254, 255]:
+        I = n * rng_i / 255.0
+        assert abs(O.tf_lut_eval(lut2, rng_i, I, 0) - lut2[n, 0]) <= 1e-6 * max(1.0, lut2[n, 0])
