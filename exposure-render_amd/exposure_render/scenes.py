@@ -141,6 +141,48 @@ def sphere_light(radius, multiplier, texture_id=0, center=(0.0, 0.0, 0.0), visib
     return L
 
 
+def shape_light(shape_type, pos, target, multiplier, texture_id=0, size=(0.0, 0.0, 0.0), inner=0.0, outer=0.0,
+                one_sided=0, visible=1, unit=capi.UNIT_POWER):
+    """A disk, ring or box light at `pos` facing `target` (Source/disk.h, ring.h, box.h)."""
+    L = capi.LightDesc()
+    L.id = -1
+    L.enabled = 1
+    L.visible = visible
+    L.texture_id = texture_id
+    L.multiplier = multiplier
+    L.unit = unit
+    tm, inv = look_at(pos, target)
+    set_matrix(L.shape.tm, tm)
+    set_matrix(L.shape.inv_tm, inv)
+    L.shape.one_sided = one_sided
+    L.shape.type = shape_type
+    for i in range(3):
+        L.shape.size[i] = size[i]
+    L.shape.inner_radius = inner
+    L.shape.outer_radius = outer
+    return L
+
+
+def checker_texture(c1, c2, repeat=(4.0, 4.0), output_level=1.0):
+    T = uniform_texture((1.0, 1.0, 1.0), output_level)
+    T.procedural.type = capi.PROCEDURAL_CHECKER
+    for i in range(3):
+        T.procedural.checker_color1[i] = c1[i]
+        T.procedural.checker_color2[i] = c2[i]
+    T.repeat[0], T.repeat[1] = repeat
+    return T
+
+
+def gradient_texture(nodes, output_level=1.0):
+    """A procedural gradient texture: nodes = [(position in [0, 1], (X, Y, Z)), ...]."""
+    T = uniform_texture((1.0, 1.0, 1.0), output_level)
+    T.procedural.type = capi.PROCEDURAL_GRADIENT
+    for x, c in nodes:
+        T.procedural.gradient.add(x, c)
+    T.repeat[0], T.repeat[1] = 1.0, 1.0
+    return T
+
+
 def uniform_texture(rgb=(1.0, 1.0, 1.0), output_level=1.0):
     T = capi.TextureDesc()
     T.id = -1

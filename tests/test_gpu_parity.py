@@ -166,3 +166,19 @@ def test_autofocus_distance(fresh):
         assert d == pytest.approx(d_ref, rel=1e-5, abs=1e-6)
         assert d > 0.0
     r.close()
+
+
+@pytest.mark.parametrize("estimator", [capi.ESTIMATOR_COMPAT, capi.ESTIMATOR_FULL])
+def test_light_shapes_and_textures(fresh, estimator):
+    """Disk (Lux, one-sided), ring and box lights (disk.h, ring.h, box.h:122-191) with checker
+    and gradient procedural textures (textures.h:64-111), next to the scene's plane light."""
+    sc = scenes.config_small(scale=0.5, estimator=estimator)
+    chk = sc.add_texture(scenes.checker_texture((1.0, 0.9, 0.8), (0.2, 0.3, 0.4)))
+    grd = sc.add_texture(scenes.gradient_texture([(0.0, (0.2, 0.2, 1.0)), (1.0, (1.0, 0.6, 0.2))]))
+    sc.add_light(scenes.shape_light(capi.SHAPE_DISK, (-0.9, 0.8, 0.7), (0.0, 0.0, 0.0), 40.0, texture_id=chk,
+                                    outer=0.3, one_sided=1, unit=capi.UNIT_LUX))
+    sc.add_light(scenes.shape_light(capi.SHAPE_RING, (0.2, -1.0, 0.8), (0.0, 0.0, 0.0), 60.0, texture_id=grd,
+                                    inner=0.1, outer=0.35))
+    sc.add_light(scenes.shape_light(capi.SHAPE_BOX, (0.0, 1.1, -0.9), (0.0, 0.0, 0.0), 30.0, texture_id=0,
+                                    size=(0.3, 0.2, 0.25)))
+    assert run_parity(sc, passes=3) > 0.99
