@@ -120,6 +120,7 @@ __device__ __forceinline__ float transmittance(const DevParams& p, const Tables&
     if (!p.shadows) return 1.0f;
     r = shadow_ray(p, P1, P2);
     if (intersects_any_light(p, r)) return 0.0f;
+    if (p.object_count > 0 && intersects_any_object(p, r)) return 0.0f;
     rng_at = rng;
     VisSum<EVAL> vs;
     vs.sig_eval = a.sigma_eval; vs.lut_max = p.lut_max;

@@ -59,6 +59,20 @@ struct DevTexture {
     int   gradient_index;            /* index into DevParams::tex_gradients, -1 if none */
 };
 
+/* a surface object (Source/object.h): its shape in the light layout, and the texture slots of
+ * its diffuse, specular and glossiness (objects.h:26-82, transport.h:141-149) */
+struct DevObject {
+    DevLight shape;
+    int   diffuse_tex, specular_tex, gloss_tex;
+    int   valid;
+};
+
+/* an RGBA8 bitmap (Source/erbitmap.h), row-major y * W + x */
+struct DevBitmap {
+    const uchar4* pixels;
+    int   w, h;
+};
+
 struct DevClip {
     float inv_tm[3][4];
     int   type, invert;
@@ -99,6 +113,9 @@ struct DevParams {
     DevTexture textures[ER_LIST_CAPACITY];
     int   clip_count;
     DevClip    clips[ER_LIST_CAPACITY];
+    int   object_count;
+    DevObject  objects[ER_LIST_CAPACITY];
+    DevBitmap  bitmaps[ER_LIST_CAPACITY];
     /* transfer functions (Source/ertracer.h:105-109) */
     er_plf      opacity;
     er_plf      glossiness;

@@ -410,6 +410,44 @@ __device__ __forceinline__ bool texture_needs_uv(const DevParams& p, int id)
     return T.valid && !(T.type == ER_TEXTURE_PROCEDURAL && T.proc_type == ER_PROCEDURAL_UNIFORM);
 }
 
+/* Buffer2D<ColorRGBAuc>::operator()(Vec2f, true) (buffer2d.h:227-254) and
+ * ColorXYZf::FromRGBAuc (color.h:195-211). In the reference the bilinear weights meet the
+ * colour type's operators: float * ColorRGBAuc converts the weight to unsigned char (a weight
+ * in (0, 1) becomes 0) and scales in place with unsigned-char wrap, and the sums wrap too; the
+ * compat estimator keeps that, so only a sample on an exact texel reads the texel. The full
+ * estimator interpolates the four texels in float. */
+__device__ __forceinline__ f3 bitmap_lookup(const DevParams& p, const DevBitmap& B, float u, float v)
+{
+    if (!B.pixels || B.w <= 0 || B.h <= 0) return mk(0.0f, 0.0f, 0.0f);
+    const float X = u * (float)B.w, Y = v * (float)B.h;
+    const int x0i = (int)floorf(X), x1i = (int)ceilf(X), y0i = (int)floorf(Y), y1i = (int)ceilf(Y);
+    const float du = X - (float)x0i, dv = Y - (float)y0i;
+    const int x0 = min(max(x0i, 0), B.w - 1), x1 = min(max(x1i, 0), B.w - 1);
+    const int y0 = min(max(y0i, 0), B.h - 1), y1 = min(max(y1i, 0), B.h - 1);
+    const uchar4 V0 = B.pixels[(size_t)y0 * B.w + x0], V1 = B.pixels[(size_t)y0 * B.w + x1];
+    const uchar4 V2 = B.pixels[(size_t)y1 * B.w + x0], V3 = B.pixels[(size_t)y1 * B.w + x1];
+    float rgb[3];
+    const unsigned char c0[3] = {V0.x, V0.y, V0.z}, c1[3] = {V1.x, V1.y, V1.z};
+    const unsigned char c2[3] = {V2.x, V2.y, V2.z}, c3[3] = {V3.x, V3.y, V3.z};
+    if (p.estimator == ER_ESTIMATOR_FULL) {
+        for (int k = 0; k < 3; ++k)
+            rgb[k] = (1.0f - dv) * ((1.0f - du) * (float)c0[k] + du * (float)c1[k]) +
+                     dv * ((1.0f - du) * (float)c2[k] + du * (float)c3[k]);
+    } else {
+        const unsigned char a = (unsigned char)(1.0f - dv), b = (unsigned char)(1.0f - du);
+        const unsigned char c = (unsigned char)du, d = (unsigned char)dv;
+        for (int k = 0; k < 3; ++k) {
+            const unsigned char i0 = (unsigned char)((unsigned char)(b * c0[k]) + (unsigned char)(c * c1[k]));
+            const unsigned char i1 = (unsigned char)((unsigned char)(b * c2[k]) + (unsigned char)(c * c3[k]));
+            rgb[k] = (float)(unsigned char)((unsigned char)(a * i0) + (unsigned char)(d * i1));
+        }
+    }
+    const float ONE_OVER_255 = 1.0f / 255.0f;
+    const float r = ONE_OVER_255 * rgb[0], g = ONE_OVER_255 * rgb[1], bl = ONE_OVER_255 * rgb[2];
+    return mk(0.412453f * r + 0.357580f * g + 0.180423f * bl, 0.212671f * r + 0.715160f * g + 0.072169f * bl,
+              0.019334f * r + 0.119193f * g + 0.950227f * bl);
+}
+
 __device__ f3 eval_texture(const DevParams& p, int id, float u, float v)
 {
     if (id < 0 || id >= ER_LIST_CAPACITY || !p.textures[id].valid) return mk(0.0f, 0.0f, 0.0f);
@@ -431,6 +469,8 @@ __device__ f3 eval_texture(const DevParams& p, int id, float u, float v)
         } else if (T.proc_type == ER_PROCEDURAL_GRADIENT && T.gradient_index >= 0) {
             L = ctf_eval(p.tex_gradients[T.gradient_index], v);
         }
+    } else if (T.type == ER_TEXTURE_BITMAP && T.bitmap_id >= 0 && T.bitmap_id < ER_LIST_CAPACITY) {
+        L = bitmap_lookup(p, p.bitmaps[T.bitmap_id], u, v);
     }
     return scl(T.output_level, L);
 }
@@ -628,6 +668,46 @@ __device__ bool intersects_any_light(const DevParams& p, const Ray& r)
     return false;
 }
 
+/* IntersectObjects (objects.h:46-66): the nearest object of the tracer's list; returns its
+ * list position, or -1 */
+__device__ int intersect_objects(const DevParams& p, const Ray& r, float& T, f3& P, f3& N, float& U, float& V)
+{
+    float best = 3.402823466e+38f;
+    int hit = -1;
+    for (int i = 0; i < p.object_count; ++i) {
+        const DevObject& O = p.objects[i];
+        if (!O.valid) continue;
+        const bool uv = texture_needs_uv(p, O.diffuse_tex) || texture_needs_uv(p, O.specular_tex) ||
+                        texture_needs_uv(p, O.gloss_tex);
+        Isect I;
+        isect_init(I);
+        intersect_shape(O.shape, xform_ray(O.shape.inv_tm, r), uv, I);
+        if (!I.Valid) continue;
+        const f3 Pw = xform_point(O.shape.tm, I.P);
+        const float t = len(sub(Pw, r.O));
+        if (t < best) {
+            best = t; hit = i; T = t; P = Pw;
+            N = xform_vector(O.shape.tm, I.N);
+            U = I.U; V = I.V;
+        }
+    }
+    return hit;
+}
+
+/* IntersectsObject (objects.h:73-82) */
+__device__ bool intersects_any_object(const DevParams& p, const Ray& r)
+{
+    for (int i = 0; i < p.object_count; ++i) {
+        const DevObject& O = p.objects[i];
+        if (!O.valid) continue;
+        Isect I;
+        isect_init(I);
+        intersect_shape(O.shape, xform_ray(O.shape.inv_tm, r), false, I);
+        if (I.Valid) return true;
+    }
+    return false;
+}
+
 /* the shadow ray of Visible (transport.h:50-53) */
 __device__ __forceinline__ Ray shadow_ray(const DevParams& p, f3 P1, f3 P2)
 {
@@ -656,6 +736,7 @@ __device__ int visible_defer(const DevParams& p, f3 P1, f3 P2, Rng& rng, Cnt& cn
     const Ray r = shadow_ray(p, P1, P2);
     if (COUNT) cnt.shadow_rays++;
     if (intersects_any_light(p, r)) return 0;
+    if (p.object_count > 0 && intersects_any_object(p, r)) return 0;
     float nearT, farT;
     if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return 1;
     job.Ox = r.O.x; job.Oy = r.O.y; job.Oz = r.O.z;
@@ -675,6 +756,7 @@ __device__ bool visible(const DevParams& p, const Tables& T, f3 P1, f3 P2, Rng& 
     const Ray r = shadow_ray(p, P1, P2);
     if (COUNT) cnt.shadow_rays++;
     if (intersects_any_light(p, r)) return false;
+    if (p.object_count > 0 && intersects_any_object(p, r)) return false;
     float Tt; f3 P;
     const Tables& Tb = T;
     if (march<FULL, COUNT>(p, Tb, r, false, rng, Tt, P, cnt)) return false;
@@ -785,10 +867,57 @@ __device__ f3 estimate_direct(const DevParams& p, const Tables& T, const DevLigh
     return Ld;
 }
 
+/* The shader of an event (transport.h:133-150). Volume event (obj < 0): BRDF in compat (Q9);
+ * in full mode the shading type picks BRDF or phase function, the stochastic types with one
+ * extra draw (design: scatterevent.h:155-212). Object event: BRDF with diffuse, specular and
+ * glossiness (its Y) from the object's textures at (U, V). */
+template<int FULL, int COUNT>
+__device__ __forceinline__ void event_shader(const DevParams& p, float I, int obj, float U, float V, f3 SEP, f3 SEN, f3 Wo,
+                                             Rng& rng, Cnt& cnt, Shader& sh)
+{
+    bool brdf = true;
+    if (obj >= 0) {
+        const DevObject& O = p.objects[obj];
+        sh.Kd = eval_texture(p, O.diffuse_tex, U, V);
+        sh.Ks = eval_texture(p, O.specular_tex, U, V);
+        sh.exponent = 1000000.0f * erm::powf_(eval_texture(p, O.gloss_tex, U, V).y, 7.0f);
+    } else {
+        if (FULL) {
+            const int type = p.shading_type;
+            if (type == ER_SHADING_PHASE_ONLY) {
+                brdf = false;
+            } else if (type != ER_SHADING_BRDF_ONLY) {
+                const float gm = gradient_magnitude(p, SEP);
+                if (COUNT) cnt.shading += 6;
+                const float NGM = gm * p.gm_inv;
+                if (type == ER_SHADING_THRESHOLD) {
+                    brdf = NGM > p.grad_threshold;
+                } else if (type == ER_SHADING_MODULATION) {
+                    const float PdfBrdf = 1.0f - erm::powf_(1.0f - NGM, 2.0f);
+                    brdf = rng.u() < PdfBrdf;
+                } else {
+                    const float e = 25.0f * erm::powf_(p.grad_factor, 3.0f) * NGM;
+                    const float base = 1.0f - erm::expf_(-e);
+                    const float PdfBrdf = p.opacity_modulated ? opacity_at(p, I) * base : base;
+                    brdf = rng.u() <= PdfBrdf;
+                }
+            }
+        }
+        sh.Kd = diffuse_at(p, I);
+        sh.Ks = ctf_eval(p.specular, I);
+        sh.exponent = 1000000.0f * erm::powf_(plf_eval(p.glossiness, I), 7.0f);   /* utilities.h:28-31 */
+    }
+    sh.brdf = brdf;
+    sh.Nn = nrm(SEN);
+    sh.Nu = nrm(crs(SEN, Wo));
+    sh.Nv = nrm(crs(SEN, sh.Nu));
+}
+
 /* UniformSampleOneLight (transport.h:115-156) with the full-mode shader selection
  * (design at scatterevent.h:155-212) */
 template<int FULL, int COUNT>
-__device__ f3 uniform_sample_one_light(const DevParams& p, const Tables& T, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt)
+__device__ f3 uniform_sample_one_light(const DevParams& p, const Tables& T, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt,
+                                       int obj = -1, float U = 0.0f, float V = 0.0f)
 {
     f3 Ld = mk(0.0f, 0.0f, 0.0f);
     const float I = intensity(p, SEP);
@@ -800,36 +929,8 @@ __device__ f3 uniform_sample_one_light(const DevParams& p, const Tables& T, cons
     const int LightID = p.light_ids[pos];
     const DevLight* L = light_by_id(p, LightID);
     if (LightID < 0 || !L) return Ld;
-    bool brdf = true;
-    if (FULL) {
-        const int type = p.shading_type;
-        if (type == ER_SHADING_PHASE_ONLY) {
-            brdf = false;
-        } else if (type != ER_SHADING_BRDF_ONLY) {
-            const float gm = gradient_magnitude(p, SEP);
-            if (COUNT) cnt.shading += 6;
-            const float NGM = gm * p.gm_inv;
-            if (type == ER_SHADING_THRESHOLD) {
-                brdf = NGM > p.grad_threshold;
-            } else if (type == ER_SHADING_MODULATION) {
-                const float PdfBrdf = 1.0f - erm::powf_(1.0f - NGM, 2.0f);
-                brdf = rng.u() < PdfBrdf;
-            } else {
-                const float e = 25.0f * erm::powf_(p.grad_factor, 3.0f) * NGM;
-                const float base = 1.0f - erm::expf_(-e);
-                const float PdfBrdf = p.opacity_modulated ? opacity_at(p, I) * base : base;
-                brdf = rng.u() <= PdfBrdf;
-            }
-        }
-    }
     Shader sh;
-    sh.brdf = brdf;
-    sh.Kd = diffuse_at(p, I);
-    sh.Ks = ctf_eval(p.specular, I);
-    sh.exponent = 1000000.0f * erm::powf_(plf_eval(p.glossiness, I), 7.0f);   /* utilities.h:28-31 */
-    sh.Nn = nrm(SEN);
-    sh.Nu = nrm(crs(SEN, Wo));
-    sh.Nv = nrm(crs(SEN, sh.Nu));
+    event_shader<FULL, COUNT>(p, I, obj, U, V, SEP, SEN, Wo, rng, cnt, sh);
     Ld = add(Ld, estimate_direct<FULL, COUNT>(p, T, *L, pos, smp, SEP, SEN, Wo, sh, rng, cnt));
     return scl((float)count, Ld);
 }
@@ -849,7 +950,8 @@ struct EventRec {
 
 template<int FULL, int COUNT>
 __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt,
-                            EventRec& ev, ShadowJob& job0, ShadowJob& job1, bool& has0, bool& has1)
+                            EventRec& ev, ShadowJob& job0, ShadowJob& job1, bool& has0, bool& has1, int obj = -1,
+                            float U = 0.0f, float V = 0.0f)
 {
     f3 Ld = mk(0.0f, 0.0f, 0.0f);
     const float I = intensity(p, SEP);
@@ -868,36 +970,8 @@ __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SE
     if (LightID < 0 || !Lp) return;
     const DevLight& L = *Lp;
     ev.count = (float)count;
-    bool brdf = true;
-    if (FULL) {
-        const int type = p.shading_type;
-        if (type == ER_SHADING_PHASE_ONLY) {
-            brdf = false;
-        } else if (type != ER_SHADING_BRDF_ONLY) {
-            const float gm = gradient_magnitude(p, SEP);
-            if (COUNT) cnt.shading += 6;
-            const float NGM = gm * p.gm_inv;
-            if (type == ER_SHADING_THRESHOLD) {
-                brdf = NGM > p.grad_threshold;
-            } else if (type == ER_SHADING_MODULATION) {
-                const float PdfBrdf = 1.0f - erm::powf_(1.0f - NGM, 2.0f);
-                brdf = rng.u() < PdfBrdf;
-            } else {
-                const float e = 25.0f * erm::powf_(p.grad_factor, 3.0f) * NGM;
-                const float base = 1.0f - erm::expf_(-e);
-                const float PdfBrdf = p.opacity_modulated ? opacity_at(p, I) * base : base;
-                brdf = rng.u() <= PdfBrdf;
-            }
-        }
-    }
     Shader sh;
-    sh.brdf = brdf;
-    sh.Kd = diffuse_at(p, I);
-    sh.Ks = ctf_eval(p.specular, I);
-    sh.exponent = 1000000.0f * erm::powf_(plf_eval(p.glossiness, I), 7.0f);
-    sh.Nn = nrm(SEN);
-    sh.Nu = nrm(crs(SEN, Wo));
-    sh.Nv = nrm(crs(SEN, sh.Nu));
+    event_shader<FULL, COUNT>(p, I, obj, U, V, SEP, SEN, Wo, rng, cnt, sh);
 
     /* light-sampling half */
     f3 SP, SN; float su, sv;
@@ -980,12 +1054,14 @@ __device__ __forceinline__ Ray camera_ray(const DevParams& p, int px, int py, Rn
     return r;
 }
 
-/* One scattering pixel handed from the primary kernel to the shading kernel (64 bytes). */
+/* One scattering pixel handed from the primary kernel to the shading kernel (96 bytes). */
 struct ScatterRec {
-    uint32_t pixel, s0, s1, pad;
+    uint32_t pixel, s0, s1;
+    int obj;                      /* object list position, or -1 for a volume event */
     float Px, Py, Pz, Wox, Woy, Woz;
     float comp, dir0, dir1, uvw0, uvw1, uvw2;
-    float lightnum, spare;
+    float lightnum, Nx, Ny, Nz;   /* N: object normal (object events) */
+    float U, V, pad0, pad1;       /* object surface coordinates */
 };
 
 /* One tracer of a batch (er_render_batch): its parameter block and per-pass buffers, its

@@ -276,7 +276,13 @@ struct LightObj { er_light_desc desc; };
 struct TextureObj { er_texture_desc desc; };
 struct ClipObj { er_clipping_desc desc; };
 struct ObjectObj { er_object_desc desc; };
-struct BitmapObj { er_bitmap_desc desc; std::vector<uint8_t> pixels; };
+struct BitmapObj {
+    er_bitmap_desc desc;
+    std::vector<uint8_t> pixels;
+    DevBuf<uchar4> dev;          /* device copy, refreshed when a bind brings new pixels */
+    bool dev_stale = true;
+    ~BitmapObj() { dev.release(); }
+};
 
 Registry<Tracer> g_tracers;
 Registry<Volume> g_volumes;
@@ -496,6 +502,19 @@ int build_params(Tracer& t)
     P.light_count = std::min(d.light_count, ER_MAX_INDICES);
     for (int i = 0; i < ER_MAX_INDICES; ++i) P.light_ids[i] = -1;
     std::map<int, int> light_slot, tex_slot;
+    /* texture ID -> slot; unknown IDs evaluate black (textures.h:64-65) */
+    auto texture_slot = [&](int id, int& slot) -> int {
+        slot = -1;
+        if (!g_textures.find(id)) return ER_OK;
+        auto tt = tex_slot.find(id);
+        if (tt == tex_slot.end()) {
+            const int ts = (int)tex_slot.size();
+            if (ts >= ER_LIST_CAPACITY) return set_error(ER_ERROR, "a tracer uses at most %d textures", ER_LIST_CAPACITY);
+            tt = tex_slot.emplace(id, ts).first;
+        }
+        slot = tt->second;
+        return ER_OK;
+    };
     for (int i = 0; i < P.light_count; ++i) {
         LightObj* lo = g_lights.find(d.light_ids[i]);
         if (!lo) continue;
@@ -514,22 +533,47 @@ int build_params(Tracer& t)
         D.area = shape_area(L.shape);
         D.inner_radius = L.shape.inner_radius; D.outer_radius = L.shape.outer_radius;
         D.multiplier = L.multiplier; D.valid = 1;
-        D.texture_id = -1;
-        if (g_textures.find(L.texture_id)) {
-            auto tt = tex_slot.find(L.texture_id);
-            if (tt == tex_slot.end()) {
-                const int ts = (int)tex_slot.size();
-                if (ts >= ER_LIST_CAPACITY) return set_error(ER_ERROR, "a tracer uses at most %d textures", ER_LIST_CAPACITY);
-                tt = tex_slot.emplace(L.texture_id, ts).first;
-            }
-            D.texture_id = tt->second;
+        if ((st = texture_slot(L.texture_id, D.texture_id))) return st;
+    }
+
+    /* Surface objects (objects.h:46-82). Compat: the first ObjectIDs.Count live objects in ID
+     * order, which is the device list the reference indexes with the loop counter (objects.h:52,77)
+     * (clamped to the live count, where the reference would read past its list); full: the
+     * objects the tracer's ObjectIDs name. Their textures join the texture slots. */
+    std::vector<const er_object_desc*> objs;
+    if (s.estimator == ER_ESTIMATOR_FULL) {
+        for (int k = 0; k < d.object_count && k < ER_MAX_INDICES; ++k)
+            if (ObjectObj* o = g_objects.find(d.object_ids[k])) objs.push_back(&o->desc);
+    } else {
+        for (auto& kv : g_objects.map) {
+            if ((int)objs.size() >= d.object_count) break;
+            objs.push_back(&kv.second->desc);
         }
     }
+    if ((int)objs.size() > ER_LIST_CAPACITY) return set_error(ER_ERROR, "a tracer uses at most %d objects", ER_LIST_CAPACITY);
+    P.object_count = (int)objs.size();
+    for (int k = 0; k < P.object_count; ++k) {
+        const er_object_desc& O = *objs[k];
+        erd::DevObject& D = P.objects[k];
+        erd::DevLight& S = D.shape;
+        mat34(S.tm, O.shape.tm);
+        mat34(S.inv_tm, O.shape.inv_tm);
+        S.type = O.shape.type; S.one_sided = O.shape.one_sided; S.visible = 1; S.unit = 0;
+        for (int i = 0; i < 3; ++i) S.size[i] = O.shape.size[i];
+        S.area = shape_area(O.shape);
+        S.inner_radius = O.shape.inner_radius; S.outer_radius = O.shape.outer_radius;
+        S.multiplier = 0.0f; S.valid = 1; S.texture_id = -1;
+        if ((st = texture_slot(O.diffuse_texture_id, D.diffuse_tex))) return st;
+        if ((st = texture_slot(O.specular_texture_id, D.specular_tex))) return st;
+        if ((st = texture_slot(O.glossiness_texture_id, D.gloss_tex))) return st;
+        D.valid = 1;
+    }
     int ngrad = 0;
+    std::map<int, int> bitmap_slot;
     for (auto& kv : tex_slot) {
         const er_texture_desc& T = g_textures.find(kv.first)->desc;
         erd::DevTexture& D = P.textures[kv.second];
-        D.type = T.type; D.proc_type = T.procedural.type; D.bitmap_id = T.bitmap_id; D.valid = 1;
+        D.type = T.type; D.proc_type = T.procedural.type; D.bitmap_id = -1; D.valid = 1;
         D.output_level = T.output_level;
         for (int i = 0; i < 2; ++i) { D.offset[i] = T.offset[i]; D.repeat[i] = T.repeat[i]; D.flip[i] = T.flip[i]; }
         for (int i = 0; i < 3; ++i) {
@@ -543,7 +587,29 @@ int build_params(Tracer& t)
             P.tex_gradients[ngrad] = T.procedural.gradient;
             D.gradient_index = ngrad++;
         }
-        if (T.type == ER_TEXTURE_BITMAP) return set_error(ER_ERROR, "bitmap textures are not supported by this build");
+        /* bitmaps (textures.h:99-105): uploaded once per set of pixels, compacted into slots */
+        BitmapObj* bo = T.type == ER_TEXTURE_BITMAP ? g_bitmaps.find(T.bitmap_id) : nullptr;
+        if (bo && !bo->pixels.empty()) {
+            auto bt = bitmap_slot.find(T.bitmap_id);
+            if (bt == bitmap_slot.end()) {
+                const int bs = (int)bitmap_slot.size();
+                if (bs >= ER_LIST_CAPACITY) return set_error(ER_ERROR, "a tracer uses at most %d bitmaps", ER_LIST_CAPACITY);
+                const size_t texels = bo->pixels.size() / 4;
+                if ((size_t)std::max(0, bo->desc.resolution[0]) * std::max(0, bo->desc.resolution[1]) != texels)
+                    return set_error(ER_ERROR, "bitmap %d: resolution does not match its pixels", T.bitmap_id);
+                if (bo->dev_stale || bo->dev.n != texels) {
+                    if ((st = bo->dev.alloc(texels))) return st;
+                    HIP_TRY(hipMemcpy(bo->dev.ptr, bo->pixels.data(), texels * sizeof(uchar4), hipMemcpyHostToDevice));
+                    bo->dev_stale = false;
+                }
+                erd::DevBitmap& B = P.bitmaps[bs];
+                B.pixels = bo->dev.ptr;
+                B.w = bo->desc.resolution[0];
+                B.h = bo->desc.resolution[1];
+                bt = bitmap_slot.emplace(T.bitmap_id, bs).first;
+            }
+            D.bitmap_id = bt->second;
+        }
     }
     P.clip_count = 0;
     if (s.estimator == ER_ESTIMATOR_FULL) {
@@ -557,7 +623,6 @@ int build_params(Tracer& t)
             D.outer_radius = c->desc.shape.outer_radius;
         }
     }
-    if (d.object_count > 0) return set_error(ER_ERROR, "surface objects are not supported by this build");
 
     P.opacity = d.opacity; P.glossiness = d.glossiness;
     P.diffuse = d.diffuse; P.specular = d.specular; P.emission = d.emission;
@@ -944,6 +1009,7 @@ ER_API int er_bind_bitmap(er_bitmap_desc* d, int bind)
         const size_t n = (size_t)std::max(0, desc.resolution[0]) * std::max(0, desc.resolution[1]) * 4;
         if ((fresh || desc.pixels_dirty) && desc.pixels && n) {
             o->pixels.assign(desc.pixels, desc.pixels + n);
+            o->dev_stale = true;
             desc.pixels_dirty = 0;
         }
         o->desc = desc;
@@ -1183,6 +1249,7 @@ static int adjoint_setup(Tracer& t, const uint32_t* s1, const uint32_t* s2, size
     if (count < (size_t)t.W * t.H) return set_error(ER_ERROR, "need %zu seeds, got %zu", (size_t)t.W * t.H, count);
     if ((st = build_params(t))) return st;
     if (!t.lut_lds) return set_error(ER_ERROR, "the adjoint needs the opacity table in LDS (volume maximum <= 8191)");
+    if (t.host_params.object_count > 0) return set_error(ER_ERROR, "the adjoint does not support surface objects");
     return ER_OK;
 }
 

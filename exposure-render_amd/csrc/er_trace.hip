@@ -87,7 +87,9 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
     bool scatter = false;
     Sample smp;
     Rng rng;
-    f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f);
+    f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f), Nobj = mk(0.0f, 0.0f, 0.0f);
+    int obj = -1;
+    float Uobj = 0.0f, Vobj = 0.0f;
     if (active) {
         paths++;
         rng.s0 = seeds1[i];
@@ -102,6 +104,11 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
         float Tn = 3.402823466e+38f;
         if (vol && Tv < Tn) { type = 0; Tn = Tv; }
         if (il >= 0 && Tl < Tn) { type = 1; Tn = Tl; }
+        if (p.object_count > 0) {
+            float To = 0.0f, Uo = 0.0f, Vo = 0.0f; f3 Po, No;
+            const int io = intersect_objects(p, r, To, Po, No, Uo, Vo);
+            if (io >= 0 && To < Tn) { type = 0; Tn = To; obj = io; Pv = Po; Nobj = No; Uobj = Uo; Vobj = Vo; }
+        }
         if (type == 0) {
             scatter = true;
             if (COUNT) cnt.scatter++;
@@ -123,12 +130,14 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
         if (scatter) {
             const unsigned int slot = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
             ScatterRec rec;
-            rec.pixel = (uint32_t)i; rec.s0 = rng.s0; rec.s1 = rng.s1; rec.pad = 0;
+            rec.pixel = (uint32_t)i; rec.s0 = rng.s0; rec.s1 = rng.s1; rec.obj = obj;
             rec.Px = Pv.x; rec.Py = Pv.y; rec.Pz = Pv.z;
             rec.Wox = -D.x; rec.Woy = -D.y; rec.Woz = -D.z;
             rec.comp = smp.comp; rec.dir0 = smp.dir0; rec.dir1 = smp.dir1;
             rec.uvw0 = smp.uvw0; rec.uvw1 = smp.uvw1; rec.uvw2 = smp.uvw2;
-            rec.lightnum = smp.lightnum; rec.spare = 0.0f;
+            rec.lightnum = smp.lightnum;
+            rec.Nx = Nobj.x; rec.Ny = Nobj.y; rec.Nz = Nobj.z;
+            rec.U = Uobj; rec.V = Vobj; rec.pad0 = 0.0f; rec.pad1 = 0.0f;
             queue[slot] = rec;
         }
     }
@@ -161,17 +170,19 @@ __device__ __forceinline__ void shade_chunk(const DevParams& p, const Tables& T,
         smp.lightnum = rec.lightnum;
         const f3 Pv = mk(rec.Px, rec.Py, rec.Pz);
         const f3 Wo = mk(rec.Wox, rec.Woy, rec.Woz);
-        const f3 Nv = normalized_gradient(p, Pv);
+        /* SE.N: the normalised gradient of a volume event (raymarching.h:70), the object's
+         * transformed shape normal of an object event (objects.h:36) */
+        const f3 Nv = rec.obj >= 0 ? mk(rec.Nx, rec.Ny, rec.Nz) : normalized_gradient(p, Pv);
         if (SPLIT) {
             EventRec ev;
             ev.pixel = rec.pixel;
             ev.c1x = ev.c1y = ev.c1z = ev.c2x = ev.c2y = ev.c2z = 0.0f;
             ev.pad = ev.pad2 = 0.0f;
-            shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1);
+            shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1, rec.obj, rec.U, rec.V);
             events[q] = ev;
         } else {
             f3 Lv = mk(0.0f, 0.0f, 0.0f);
-            Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, T, smp, Pv, Nv, Wo, rng, cnt));
+            Lv = add(Lv, uniform_sample_one_light<FULL, COUNT>(p, T, smp, Pv, Nv, Wo, rng, cnt, rec.obj, rec.U, rec.V));
             frame[rec.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
         }
         seeds1[rec.pixel] = rng.s0;

@@ -50,9 +50,24 @@ class Renderer:
         self.volume = cp(scene.volume)
         self.volume.voxels = scene.volume.voxels
         _check(lib, lib.er_bind_volume(C.byref(self.volume), 1))
+        self._pixels = list(scene.bitmaps)
+        self.bitmaps = []
+        for a in self._pixels:
+            B = capi.BitmapDesc()
+            lib.er_bitmap_desc_init(C.byref(B))
+            B.resolution[0], B.resolution[1] = a.shape[1], a.shape[0]
+            B.pixels = a.ctypes.data_as(C.POINTER(C.c_uint8))
+            B.pixels_dirty = 1
+            _check(lib, lib.er_bind_bitmap(C.byref(B), 1))
+            self.bitmaps.append(B)
         self.textures = [cp(T) for T in scene.textures]
         for T in self.textures:
+            if T.type == capi.TEXTURE_BITMAP:
+                T.bitmap_id = self.bitmaps[T.bitmap_id].id if 0 <= T.bitmap_id < len(self.bitmaps) else -1
             _check(lib, lib.er_bind_texture(C.byref(T), 1))
+
+        def tex_id(j):
+            return self.textures[j].id if 0 <= j < len(self.textures) else -1
         self.lights = [cp(L) for L in scene.lights]
         for L in self.lights:
             if 0 <= L.texture_id < len(self.textures):
@@ -61,6 +76,12 @@ class Renderer:
         self.clips = [cp(Cb) for Cb in scene.clips]
         for Cb in self.clips:
             _check(lib, lib.er_bind_clipping_object(C.byref(Cb), 1))
+        self.objects = [cp(O) for O in scene.objects]
+        for O in self.objects:
+            O.diffuse_texture_id = tex_id(O.diffuse_texture_id)
+            O.specular_texture_id = tex_id(O.specular_texture_id)
+            O.glossiness_texture_id = tex_id(O.glossiness_texture_id)
+            _check(lib, lib.er_bind_object(C.byref(O), 1))
         t = cp(scene.tracer)
         t.volume_id = self.volume.id
         for i in range(t.light_count):
@@ -69,6 +90,9 @@ class Renderer:
         for i in range(t.clipping_count):
             j = scene.tracer.clipping_ids[i]
             t.clipping_ids[i] = self.clips[j].id if 0 <= j < len(self.clips) else -1
+        for i in range(t.object_count):
+            j = scene.tracer.object_ids[i]
+            t.object_ids[i] = self.objects[j].id if 0 <= j < len(self.objects) else -1
         self.tracer = t
         _check(lib, lib.er_bind_tracer(C.byref(t), 1))
         self.id = t.id
@@ -193,6 +217,10 @@ class Renderer:
         lib.er_bind_tracer(C.byref(self.tracer), 0)
         for Cb in self.clips:
             lib.er_bind_clipping_object(C.byref(Cb), 0)
+        for O in self.objects:
+            lib.er_bind_object(C.byref(O), 0)
+        for B in self.bitmaps:
+            lib.er_bind_bitmap(C.byref(B), 0)
         for L in self.lights:
             lib.er_bind_light(C.byref(L), 0)
         for T in self.textures:

@@ -163,6 +163,38 @@ def shape_light(shape_type, pos, target, multiplier, texture_id=0, size=(0.0, 0.
     return L
 
 
+def object_desc(shape_type, tm, inv, size=(1.0, 1.0, 1.0), diffuse_tex=-1, specular_tex=-1,
+                glossiness_tex=-1, inner_radius=0.0, outer_radius=0.5, one_sided=0):
+    """A surface object (Source/erobject.h): a shape placed by tm / inv, shaded by the BRDF with
+    diffuse, specular and glossiness read from textures at the hit's UV (transport.h:141-149)."""
+    O = capi.ObjectDesc()
+    capi.load().er_object_desc_init(C.byref(O))
+    O.diffuse_texture_id, O.specular_texture_id, O.glossiness_texture_id = diffuse_tex, specular_tex, glossiness_tex
+    O.shape.type = shape_type
+    O.shape.one_sided = one_sided
+    for i in range(3):
+        O.shape.size[i] = size[i]
+    O.shape.inner_radius = inner_radius
+    O.shape.outer_radius = outer_radius
+    set_matrix(O.shape.tm, tm)
+    set_matrix(O.shape.inv_tm, inv)
+    return O
+
+
+def bitmap_texture(bitmap_index, repeat=(1.0, 1.0), offset=(0.0, 0.0), flip=(0, 0), output_level=1.0):
+    """A bitmap texture (textures.h:99-105) over the scene's bitmap at bitmap_index."""
+    T = capi.TextureDesc()
+    capi.load().er_texture_desc_init(C.byref(T))
+    T.type = capi.TEXTURE_BITMAP
+    T.bitmap_id = bitmap_index
+    T.output_level = output_level
+    for i in range(2):
+        T.repeat[i] = repeat[i]
+        T.offset[i] = offset[i]
+        T.flip[i] = flip[i]
+    return T
+
+
 def checker_texture(c1, c2, repeat=(4.0, 4.0), output_level=1.0):
     T = uniform_texture((1.0, 1.0, 1.0), output_level)
     T.procedural.type = capi.PROCEDURAL_CHECKER
@@ -231,6 +263,8 @@ class Scene:
         self.lights = []
         self.textures = []
         self.clips = []
+        self.objects = []
+        self.bitmaps = []       # (H, W, 4) uint8 arrays, RGBA
         self.name = "scene"
 
     @property
@@ -255,9 +289,25 @@ class Scene:
         self.tracer.clipping_count += 1
         return i
 
+    def add_object(self, O):
+        self.objects.append(O)
+        i = len(self.objects) - 1
+        self.tracer.object_ids[self.tracer.object_count] = i
+        self.tracer.object_count += 1
+        return i
+
+    def add_bitmap(self, rgba):
+        """An (H, W, 4) uint8 RGBA image; returns the index a bitmap texture names."""
+        a = np.ascontiguousarray(rgba, dtype=np.uint8)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("bitmap must be (H, W, 4) uint8")
+        self.bitmaps.append(a)
+        return len(self.bitmaps) - 1
+
     def array(self, kind):
-        lst = {"lights": self.lights, "textures": self.textures, "clips": self.clips}[kind]
-        typ = {"lights": capi.LightDesc, "textures": capi.TextureDesc, "clips": capi.ClippingDesc}[kind]
+        lst = {"lights": self.lights, "textures": self.textures, "clips": self.clips, "objects": self.objects}[kind]
+        typ = {"lights": capi.LightDesc, "textures": capi.TextureDesc, "clips": capi.ClippingDesc,
+               "objects": capi.ObjectDesc}[kind]
         arr = (typ * max(1, len(lst)))()
         for i, d in enumerate(lst):
             arr[i] = d

@@ -178,6 +178,9 @@ struct Scene {
     std::vector<er_light_desc> lights;     /* index = light ID */
     std::vector<er_texture_desc> textures; /* index = texture ID */
     std::vector<er_clipping_desc> clips;   /* index = clipping ID */
+    std::vector<er_object_desc> objects;   /* index = object ID, the reference's device list order */
+    struct Bitmap { int w = 0, h = 0; std::vector<uint8_t> rgba; };
+    std::vector<Bitmap> bitmaps;           /* index = bitmap ID */
     float gm_max = 0.0f;
     float maj_max = 0.0f;                  /* global majorant for Woodcock: DS^2 * max opacity */
     std::vector<float> tf_lut;             /* node table (er_set_tf_lut), empty: the PLFs */
@@ -650,6 +653,41 @@ template<class R> struct Est {
         ss.N = transform_vector(s.tm, ss.N);
     }
 
+    /* Buffer2D<ColorRGBAuc>::operator()(Vec2f, true) (buffer2d.h:227-254) then
+     * ColorXYZf::FromRGBAuc (color.h:195-211). COMPAT keeps the colour type's arithmetic:
+     * `float * ColorRGBAuc` binds the float weight to `const unsigned char&` (truncation, so
+     * a weight in (0, 1) is 0), the products and sums are stored back into unsigned char
+     * components (mod 256). FULL blends the four texels in float. */
+    V3<R> bitmap_lookup(const Scene::Bitmap& B, const R uv[2]) const
+    {
+        if (B.w <= 0 || B.h <= 0 || B.rgba.size() < (size_t)B.w * B.h * 4) return V3<R>();
+        const R X = uv[0] * R((float)B.w), Y = uv[1] * R((float)B.h);
+        const int cx0 = (int)std::floor(X), cx1 = (int)std::ceil(X), cy0 = (int)std::floor(Y), cy1 = (int)std::ceil(Y);
+        const R du = X - R((float)cx0), dv = Y - R((float)cy0);
+        const int x0 = std::min(std::max(cx0, 0), B.w - 1), x1 = std::min(std::max(cx1, 0), B.w - 1);
+        const int y0 = std::min(std::max(cy0, 0), B.h - 1), y1 = std::min(std::max(cy1, 0), B.h - 1);
+        const uint8_t* t00 = &B.rgba[((size_t)y0 * B.w + x0) * 4];
+        const uint8_t* t10 = &B.rgba[((size_t)y0 * B.w + x1) * 4];
+        const uint8_t* t01 = &B.rgba[((size_t)y1 * B.w + x0) * 4];
+        const uint8_t* t11 = &B.rgba[((size_t)y1 * B.w + x1) * 4];
+        R rgb[3];
+        for (int k = 0; k < 3; ++k) {
+            if (full) {
+                rgb[k] = (R(1) - dv) * ((R(1) - du) * R(t00[k]) + du * R(t10[k])) + dv * ((R(1) - du) * R(t01[k]) + du * R(t11[k]));
+            } else {
+                const uint8_t w_a = (uint8_t)(R(1) - dv), w_b = (uint8_t)(R(1) - du), w_c = (uint8_t)du, w_d = (uint8_t)dv;
+                const uint8_t top = (uint8_t)((uint8_t)(w_b * t00[k]) + (uint8_t)(w_c * t10[k]));
+                const uint8_t bot = (uint8_t)((uint8_t)(w_b * t01[k]) + (uint8_t)(w_c * t11[k]));
+                rgb[k] = R((float)(uint8_t)((uint8_t)(w_a * top) + (uint8_t)(w_d * bot)));
+            }
+        }
+        const R s255 = R(1.0f / 255.0f);
+        const R r = s255 * rgb[0], g = s255 * rgb[1], b = s255 * rgb[2];
+        return V3<R>(R(0.412453f) * r + R(0.357580f) * g + R(0.180423f) * b,
+                     R(0.212671f) * r + R(0.715160f) * g + R(0.072169f) * b,
+                     R(0.019334f) * r + R(0.119193f) * g + R(0.950227f) * b);
+    }
+
     /* EvaluateProcedural / EvaluateTexture (textures.h:26-111) */
     V3<R> eval_texture(int id, const R UVin[2]) const
     {
@@ -663,7 +701,9 @@ template<class R> struct Est {
         if (T.flip[0]) uv[0] = R(1) - uv[0];
         if (T.flip[1]) uv[1] = R(1) - uv[1];
         V3<R> L;
-        if (T.type == ER_TEXTURE_PROCEDURAL) {
+        if (T.type == ER_TEXTURE_BITMAP) {
+            if (T.bitmap_id >= 0 && T.bitmap_id < (int)S.bitmaps.size()) L = bitmap_lookup(S.bitmaps[T.bitmap_id], uv);
+        } else if (T.type == ER_TEXTURE_PROCEDURAL) {
             const er_procedural& p = T.procedural;
             switch (p.type) {
                 case ER_PROCEDURAL_UNIFORM: L = from<R>(p.uniform_color); break;
@@ -875,6 +915,51 @@ template<class R> struct Est {
         return false;
     }
 
+    /* The objects SampleRay and Visible test. COMPAT: gpObjects[i] for i < ObjectIDs.Count,
+     * the live objects in ID order (objects.h:50-52,75-77), clamped to the live count; FULL:
+     * the objects ObjectIDs names. */
+    int object_count() const
+    {
+        return full ? std::min(S.t.object_count, ER_MAX_INDICES) : std::min(S.t.object_count, (int)S.objects.size());
+    }
+    const er_object_desc* object(int i) const
+    {
+        const int id = full ? S.t.object_ids[i] : i;
+        return (id >= 0 && id < (int)S.objects.size()) ? &S.objects[id] : nullptr;
+    }
+    /* IntersectObjects (objects.h:26-66): nearest hit; SE.ObjectID = list position */
+    bool intersect_objects(const Ray<R>& r, R& T, V3<R>& P, V3<R>& N, R UV[2], int& idx) const
+    {
+        R best = R(FLT_MAX);
+        bool any = false;
+        for (int i = 0; i < object_count(); ++i) {
+            const er_object_desc* O = object(i);
+            if (!O) continue;
+            Isect<R> I;
+            intersect_shape(O->shape, transform_ray(O->shape.inv_tm, r), I);
+            if (!I.Valid) continue;
+            const V3<R> Pw = transform_point(O->shape.tm, I.P);
+            const R t = length(Pw - r.O);
+            if (t < best) {
+                best = t; T = t; P = Pw; N = transform_vector(O->shape.tm, I.N);
+                UV[0] = I.UV[0]; UV[1] = I.UV[1]; idx = i; any = true;
+            }
+        }
+        return any;
+    }
+    /* IntersectsObject (objects.h:68-82) */
+    bool intersects_any_object(const Ray<R>& r) const
+    {
+        for (int i = 0; i < object_count(); ++i) {
+            const er_object_desc* O = object(i);
+            if (!O) continue;
+            Isect<R> I;
+            intersect_shape(O->shape, transform_ray(O->shape.inv_tm, r), I);
+            if (I.Valid) return true;
+        }
+        return false;
+    }
+
     /* Visible (transport.h:46-56) -> Intersect (transport.h:30-44) */
     bool visible(const V3<R>& P1, const V3<R>& P2)
     {
@@ -883,6 +968,7 @@ template<class R> struct Est {
         const Ray<R> r(P1 + scale(R(RAY_EPS), W), W, R(0), std::min(length(P2 - P1) - R(RAY_EPS_2), R(S.t.settings.max_shadow_distance)));
         if (cnt) cnt[7]++;
         if (intersects_any_light(r)) return false;
+        if (intersects_any_object(r)) return false;
         R T; V3<R> P;
         if (march(r, false, T, P)) return false;
         return true;
@@ -938,7 +1024,8 @@ template<class R> struct Est {
 
     /* UniformSampleOneLight (transport.h:115-156) with the full-mode shader selection
      * (design: scatterevent.h:155-212) */
-    V3<R> uniform_sample_one_light(const Sample& smp, const V3<R>& SEP, const V3<R>& SEN, const V3<R>& Wo)
+    V3<R> uniform_sample_one_light(const Sample& smp, const V3<R>& SEP, const V3<R>& SEN, const V3<R>& Wo,
+                                   const er_object_desc* obj = nullptr, const R* UV = nullptr)
     {
         V3<R> Ld;
         const R I = get_intensity(SEP);
@@ -950,6 +1037,14 @@ template<class R> struct Est {
         const int LightID = S.t.light_ids[pos];
         const er_light_desc* L = light(LightID);
         if (LightID < 0 || !L) return Ld;
+        if (obj) {   /* object event (transport.h:141-149) */
+            const V3<R> Kd = eval_texture(obj->diffuse_texture_id, UV);
+            const V3<R> Ks = eval_texture(obj->specular_texture_id, UV);
+            const V3<R> G = eval_texture(obj->glossiness_texture_id, UV);
+            const Shader sh = make_shader(true, SEN, Wo, Kd, Ks, R(15.0f), R(1000000.0f) * M<R>::pow(G[1], R(7)));
+            Ld = Ld + estimate_direct(*L, pos, smp, SEP, SEN, Wo, sh);
+            return scale(R((float)count), Ld);
+        }
         bool brdf = true;
         if (full) {
             const int type = S.t.settings.shading_type;
@@ -1019,12 +1114,18 @@ template<class R> struct Est {
         R T = R(FLT_MAX);
         if (vol && Tv < T) { type = 0; T = Tv; }
         if (lig && Tl < T) { type = 1; T = Tl; }
+        R To = R(0), UVo[2] = {R(0), R(0)}; V3<R> Po, No; int io = -1;
+        if (intersect_objects(r, To, Po, No, UVo, io) && To < T) { type = 2; T = To; }
 
         V3<R> Lv;
         if (type == 0) {
             if (cnt) cnt[6]++;
             const V3<R> Nv = normalized_gradient(Pv);   /* raymarching.h:70 */
             Lv = Lv + uniform_sample_one_light(smp, Pv, Nv, -r.D);
+        }
+        if (type == 2) {
+            if (cnt) cnt[6]++;
+            Lv = Lv + uniform_sample_one_light(smp, Po, No, -r.D, object(io), UVo);
         }
         if (type == 1) Lv = Lv + Lel;
         out[0] = Lv[0]; out[1] = Lv[1]; out[2] = Lv[2];
@@ -1143,6 +1244,36 @@ void* orc_create(const er_tracer_desc* t, const er_volume_desc* v, const er_ligh
 }
 
 void orc_destroy(void* h) { delete (Scene*)h; }
+
+/* EvaluateTexture (textures.h:62-111) of texture `id` at (u, v), in FP32 */
+void orc_eval_texture(void* h, int id, float u, float v, float* xyz)
+{
+    Scene& S = *(Scene*)h;
+    Rng rng{1u, 1u};
+    Est<float> e(S, rng, nullptr);
+    const float uv[2] = {u, v};
+    const V3<float> L = e.eval_texture(id, uv);
+    for (int k = 0; k < 3; ++k) xyz[k] = L[k];
+}
+
+/* registered surface objects, index = object ID */
+void orc_set_objects(void* h, const er_object_desc* objs, int n)
+{
+    Scene* S = (Scene*)h;
+    S->objects.clear();
+    for (int i = 0; i < n; ++i) { er_object_desc O = objs[i]; O.shape.area = shape_area(O.shape); S->objects.push_back(O); }
+}
+
+/* a registered bitmap, index = bitmap ID: w * h RGBA8 texels, row-major */
+void orc_set_bitmap(void* h, int id, int w, int hgt, const uint8_t* rgba)
+{
+    Scene* S = (Scene*)h;
+    if (id < 0) return;
+    if ((int)S->bitmaps.size() <= id) S->bitmaps.resize(id + 1);
+    Scene::Bitmap& B = S->bitmaps[id];
+    B.w = w; B.h = hgt;
+    B.rgba.assign(rgba, rgba + (size_t)std::max(0, w) * std::max(0, hgt) * 4);
+}
 
 /* ComputeAutoFocusDistance (autofocus.cuh:26-63): 100 rays through film point (u, v), each
  * jittered by 0.01 x a concentric-disk sample of the screen (Get2, Q22 order: first draw ->

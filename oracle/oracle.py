@@ -30,6 +30,9 @@ def load():
         "orc_create": (C.c_void_p, [_P(capi.TracerDesc), _P(capi.VolumeDesc), _P(capi.LightDesc), C.c_int,
                                     _P(capi.TextureDesc), C.c_int, _P(capi.ClippingDesc), C.c_int]),
         "orc_destroy": (None, [C.c_void_p]),
+        "orc_set_objects": (None, [C.c_void_p, _P(capi.ObjectDesc), C.c_int]),
+        "orc_set_bitmap": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_uint8)]),
+        "orc_eval_texture": (None, [C.c_void_p, C.c_int, C.c_float, C.c_float, _P(C.c_float)]),
         "orc_set_tf_lut": (None, [C.c_void_p, _P(C.c_float), C.c_float]),
         "orc_tf_lut_eval": (C.c_float, [_P(C.c_float), C.c_float, C.c_float, C.c_int]),
         "orc_autofocus": (C.c_float, [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32]),
@@ -101,6 +104,10 @@ class Oracle:
         clips, nc = scene.array("clips")
         self._keep = (lights, tex, clips)
         self.h = self.lib.orc_create(C.byref(scene.tracer), C.byref(scene.volume), lights, nl, tex, nt, clips, nc)
+        objs, no = scene.array("objects")
+        self.lib.orc_set_objects(self.h, objs, no)
+        for i, a in enumerate(scene.bitmaps):
+            self.lib.orc_set_bitmap(self.h, i, a.shape[1], a.shape[0], _p(a, C.c_uint8))
         self.nthreads = nthreads
         self.W, self.H = scene.film
         if gm_max is not None:
@@ -156,6 +163,11 @@ class Oracle:
         out = np.zeros((n, 4), dtype=np.float64)
         self.lib.orc_expect_f64(self.h, _p(px, C.c_int), _p(py, C.c_int), n, spp, _p(s1, C.c_uint32),
                                 _p(s2, C.c_uint32), _p(out, C.c_double), self.nthreads)
+        return out
+
+    def eval_texture(self, tex, u, v):
+        out = np.zeros(3, dtype=np.float32)
+        self.lib.orc_eval_texture(self.h, int(tex), float(u), float(v), _p(out, C.c_float))
         return out
 
     def post(self, frame, running, no_iterations):

@@ -182,3 +182,31 @@ def test_light_shapes_and_textures(fresh, estimator):
     sc.add_light(scenes.shape_light(capi.SHAPE_BOX, (0.0, 1.1, -0.9), (0.0, 0.0, 0.0), 30.0, texture_id=0,
                                     size=(0.3, 0.2, 0.25)))
     assert run_parity(sc, passes=3) > 0.99
+
+
+@pytest.mark.parametrize("estimator", [capi.ESTIMATOR_COMPAT, capi.ESTIMATOR_FULL])
+def test_surface_objects_and_bitmaps(fresh, estimator):
+    """Surface objects (objects.h:26-82) as camera-ray candidates after the volume and the
+    lights, as shadow-ray occluders after the lights (transport.h:30-44), shaded by the BRDF
+    from textures at the hit (transport.h:141-149): a checker floor plane, a sphere with a
+    bitmap texture (textures.h:99-105) and a box with a glossiness bitmap."""
+    sc = scenes.config_small(scale=0.5, estimator=estimator)
+    rng = np.random.default_rng(21)
+    bmp = sc.add_bitmap(rng.integers(0, 256, size=(8, 16, 4), dtype=np.uint8))
+    gls = sc.add_bitmap(rng.integers(0, 256, size=(4, 4, 4), dtype=np.uint8))
+    chk = sc.add_texture(scenes.checker_texture((0.9, 0.8, 0.7), (0.2, 0.25, 0.3)))
+    btx = sc.add_texture(scenes.bitmap_texture(bmp, repeat=(2.0, 1.0)))
+    gtx = sc.add_texture(scenes.bitmap_texture(gls, flip=(1, 0)))
+    spc = sc.add_texture(scenes.uniform_texture((0.2, 0.2, 0.2)))
+    glu = sc.add_texture(scenes.uniform_texture((0.5, 0.5, 0.5)))
+    tm, inv = scenes.look_at((0.0, -0.6, 0.0), (0.0, 1.0, 0.0))
+    sc.add_object(scenes.object_desc(capi.SHAPE_PLANE, tm, inv, size=(3.0, 3.0, 0.0), diffuse_tex=chk,
+                                     specular_tex=spc, glossiness_tex=glu))
+    tm, inv = scenes.translate((0.55, 0.25, 0.1))
+    sc.add_object(scenes.object_desc(capi.SHAPE_SPHERE, tm, inv, outer_radius=0.2, diffuse_tex=btx,
+                                     specular_tex=spc, glossiness_tex=glu))
+    tm, inv = scenes.translate((-0.45, 0.1, 0.45))
+    sc.add_object(scenes.object_desc(capi.SHAPE_BOX, tm, inv, size=(0.25, 0.3, 0.2), diffuse_tex=chk,
+                                     specular_tex=spc, glossiness_tex=gtx))
+    sc.name = "small_objects"
+    assert run_parity(sc, passes=3) > 0.99

@@ -399,3 +399,183 @@ def test_node_table_transfer_function():
     for n in [0, 1, 100, 254, 255]:
         I = n * rng_i / 255.0
         assert abs(O.tf_lut_eval(lut2, rng_i, I, 0) - lut2[n, 0]) <= 1e-6 * max(1.0, lut2[n, 0])
+
+
+# ---- textures (textures.h:26-111, buffer2d.h:227-254, color.h:195-211) ---------------------
+
+# sRGB (D65) to CIE XYZ; a white texel maps to the D65 white point, the row sums
+_RGB2XYZ = np.array([[0.412453, 0.357580, 0.180423],
+                     [0.212671, 0.715160, 0.072169],
+                     [0.019334, 0.119193, 0.950227]])
+
+
+def _texture_oracle(textures, bitmaps, estimator):
+    sc = scenes.config_small(scale=0.125, estimator=estimator)
+    sc.textures = []
+    for T in textures:
+        sc.add_texture(T)
+    for b in bitmaps:
+        sc.add_bitmap(b)
+    return O.Oracle(sc)
+
+
+def test_procedural_textures_closed_form():
+    uni = scenes.uniform_texture((0.3, 0.6, 0.9), output_level=2.0)
+    chk = scenes.checker_texture((1.0, 0.0, 0.0), (0.0, 0.0, 1.0), repeat=(4.0, 4.0))
+    orc = _texture_oracle([uni, chk], [], capi.ESTIMATOR_COMPAT)
+    assert np.allclose(orc.eval_texture(0, 0.37, 0.81), [0.6, 1.2, 1.8])
+    rng = np.random.default_rng(7)
+    for u, v in rng.uniform(0.0, 1.0, size=(64, 2)):
+        # repeat, then V is offset by 1 - 0 before the wrap (textures.h:71-78)
+        a = math.floor(2.0 * ((4.0 * u) % 1.0))
+        b = math.floor(2.0 * ((4.0 * v + 1.0) % 1.0))
+        want = [1.0, 0.0, 0.0] if (a + b) % 2 == 0 else [0.0, 0.0, 1.0]
+        assert np.array_equal(orc.eval_texture(1, u, v), np.float32(want)), (u, v)
+    assert np.array_equal(orc.eval_texture(-1, 0.5, 0.5), np.zeros(3, np.float32))   # ID < 0: black
+
+
+def test_bitmap_texture_full_is_bilinear():
+    rng = np.random.default_rng(11)
+    bmp = rng.integers(0, 256, size=(4, 8, 4), dtype=np.uint8)     # H = 4, W = 8
+    T = scenes.bitmap_texture(0)
+    orc = _texture_oracle([T], [bmp], capi.ESTIMATOR_FULL)
+    H, W = bmp.shape[:2]
+    white = _texture_oracle([T], [np.full((2, 2, 4), 255, np.uint8)], capi.ESTIMATOR_FULL)
+    assert np.allclose(white.eval_texture(0, 0.3, 0.6), _RGB2XYZ.sum(axis=1), rtol=1e-5)
+    for u, v in rng.uniform(0.0, 0.999, size=(64, 2)):
+        X, Y = u * W, v * H
+        x0, y0 = int(math.floor(X)), int(math.floor(Y))
+        x1, y1 = min(x0 + 1, W - 1), min(y0 + 1, H - 1)
+        du, dv = X - x0, Y - y0
+        c = bmp[..., :3].astype(np.float64)
+        rgb = (1 - dv) * ((1 - du) * c[y0, x0] + du * c[y0, x1]) + dv * ((1 - du) * c[y1, x0] + du * c[y1, x1])
+        want = _RGB2XYZ @ (rgb / 255.0)
+        np.testing.assert_allclose(orc.eval_texture(0, u, v), want, rtol=2e-5, atol=2e-6)
+
+
+def test_bitmap_texture_compat_reads_exact_texels_only():
+    """The reference blends texels with `float * ColorRGBAuc`, which converts each bilinear
+    weight to unsigned char: exact texel coordinates read that texel, anything between texels
+    has weights 0 and reads black."""
+    rng = np.random.default_rng(12)
+    bmp = rng.integers(0, 256, size=(4, 8, 4), dtype=np.uint8)
+    orc = _texture_oracle([scenes.bitmap_texture(0)], [bmp], capi.ESTIMATOR_COMPAT)
+    H, W = bmp.shape[:2]
+    for j in range(H):
+        for i in range(W):
+            want = _RGB2XYZ @ (bmp[j, i, :3].astype(np.float64) / 255.0)
+            np.testing.assert_allclose(orc.eval_texture(0, i / W, j / H), want, rtol=2e-5, atol=2e-6)
+    for u, v in [(0.5 / W, 0.0), (0.0, 0.5 / H), (1.3 / W, 2.6 / H)]:
+        assert np.array_equal(orc.eval_texture(0, u, v), np.zeros(3, np.float32)), (u, v)
+
+
+# ---- surface objects (objects.h:26-82, transport.h:115-156) ---------------------------------
+
+LIGHT_POS, LIGHT_SIZE = (0.3, 0.1, 0.45), 0.8
+OCC_CENTER, OCC_RADIUS = (0.35, 0.15, 0.2), 0.15
+
+
+def _floor_scene(estimator, occluder):
+    """A transparent volume, a 2 x 2 floor object in z = 0 (diffuse 0.8 / 0.6 / 0.4, no
+    specular, glossiness 0), an invisible 0.8 x 0.8 plane light 0.45 above it (large and close,
+    so that the MIS weight matters) and, optionally, a disk object of radius 0.15 at z = 0.2
+    that shadows part of the light."""
+    sc = scenes.config_small(scale=0.25, estimator=estimator)
+    t = sc.tracer
+    t.opacity.init()
+    t.opacity.add(0.0, 0.0).add(70000.0, 0.0)
+    sc.textures, sc.lights = [], []
+    t.light_count = 0
+    white = sc.add_texture(scenes.uniform_texture((1.0, 1.0, 1.0)))
+    kd = sc.add_texture(scenes.uniform_texture((0.8, 0.6, 0.4)))
+    black = sc.add_texture(scenes.uniform_texture((0.0, 0.0, 0.0)))
+    sc.add_light(scenes.plane_light(LIGHT_POS, (0.25, 0.05, 0.0), (LIGHT_SIZE, LIGHT_SIZE), 5.0, texture_id=white,
+                                    visible=0))
+    tm, inv = scenes.translate((0.0, 0.0, 0.0))
+    sc.add_object(scenes.object_desc(capi.SHAPE_PLANE, tm, inv, size=(2.0, 2.0, 0.0), diffuse_tex=kd,
+                                     specular_tex=black, glossiness_tex=black))
+    if occluder:
+        tm, inv = scenes.translate(OCC_CENTER)
+        sc.add_object(scenes.object_desc(capi.SHAPE_DISK, tm, inv, outer_radius=OCC_RADIUS, diffuse_tex=kd,
+                                         specular_tex=black, glossiness_tex=black))
+    scenes.set_camera(t, (32, 32), (-0.6, -0.4, 1.5), (0.0, 0.0, 0.0), fov=30.0)
+    return sc
+
+
+def _floor_quadrature(sc, P, Wo, weighted, occluder, n=400):
+    """Direct light at floor point P by midpoint quadrature over the light's area, written from
+    the rendering equation: (Kd / pi) Le |cos_i| |cos_l| / d^2 dA, times the light-sampling MIS
+    weight LightPdf^2 / (LightPdf^2 + BsdfPdf^2) when `weighted` (the light-sampling half
+    alone), with the BSDF pdf Lambert + Blinn (exponent 0)."""
+    L = sc.lights[0]
+    tm = np.array([[L.shape.tm[i][j] for j in range(4)] for i in range(4)], dtype=np.float64)
+    s = (np.arange(n) + 0.5) / n
+    uu, vv = np.meshgrid(s, s, indexing="ij")
+    local = np.stack([(uu - 0.5) * LIGHT_SIZE, (vv - 0.5) * LIGHT_SIZE, np.zeros_like(uu), np.ones_like(uu)], axis=-1)
+    X = (local @ tm.T)[..., :3]
+    Nl = tm[:3, :3] @ np.array([0.0, 0.0, 1.0])
+    D = X - P
+    d2 = np.sum(D * D, axis=-1)
+    Wi = D / np.sqrt(d2)[..., None]
+    N = np.array([0.0, 0.0, 1.0])
+    cos_i = np.abs(Wi @ N)
+    cos_l = np.abs(Wi @ Nl)
+    A = LIGHT_SIZE * LIGHT_SIZE
+    f = np.array([0.8, 0.6, 0.4]) / math.pi
+    Le = 5.0     # multiplier x the uniform XYZ (1, 1, 1) texture
+    vis = np.ones_like(d2)
+    if occluder:
+        t = (OCC_CENTER[2] - P[2]) / Wi[..., 2]
+        q = P + t[..., None] * Wi
+        vis = (np.hypot(q[..., 0] - OCC_CENTER[0], q[..., 1] - OCC_CENTER[1]) > OCC_RADIUS).astype(np.float64)
+    integrand = cos_i * cos_l / d2 * vis
+    if weighted:
+        light_pdf = d2 / (cos_l * A)
+        # shading frame: Nn = N, Nu = norm(N x Wo), Nv = norm(N x Nu)
+        Nu = np.cross(N, Wo); Nu /= np.linalg.norm(Nu)
+        Nv = np.cross(N, Nu); Nv /= np.linalg.norm(Nv)
+        wo = np.array([Wo @ Nu, Wo @ Nv, Wo @ N])
+        wi = np.stack([Wi @ Nu, Wi @ Nv, Wi @ N], axis=-1)
+        lam = np.where(wi[..., 2] * wo[2] > 0, np.abs(wi[..., 2]) / math.pi, 0.0)
+        wh = wi + wo
+        wh /= np.linalg.norm(wh, axis=-1)[..., None]
+        wowh = wh @ wo
+        blinn = np.where((wowh > 0) & (wi[..., 2] * wo[2] > 0), 1.0 / (8.0 * math.pi * wowh), 0.0)
+        bsdf_pdf = lam + blinn
+        integrand = integrand * light_pdf ** 2 / (light_pdf ** 2 + bsdf_pdf ** 2)
+    return f * Le * A * integrand.mean()
+
+
+def _camera_hits(orc, W, H, px, py):
+    cam = orc.camera().astype(np.float64)
+    Nc, U, V = cam[0:3], cam[3:6], cam[6:9]
+    sx = cam[9] + cam[11] * (px + 0.5 / W)      # the mean of the film jitter
+    sy = cam[10] + cam[12] * (py + 0.5 / H)
+    d = Nc + sx * U - sy * V
+    return d / np.linalg.norm(d)
+
+
+@pytest.mark.parametrize("estimator", [capi.ESTIMATOR_COMPAT, capi.ESTIMATOR_FULL])
+@pytest.mark.parametrize("occluder", [False, True])
+def test_object_direct_light_quadrature(estimator, occluder):
+    """An object event is shaded by the BRDF from its textures at the hit (transport.h:141-149).
+    COMPAT (light sampling with its MIS weight) must equal the weighted quadrature, FULL (both
+    MIS halves) the unweighted rendering-equation integral; a shadowing object
+    (transport.h:36-37) removes the light it covers."""
+    sc = _floor_scene(estimator, occluder)
+    orc = O.Oracle(sc, nthreads=4)
+    W, H = sc.film
+    O_cam = np.array([sc.tracer.camera.pos[i] for i in range(3)], dtype=np.float64)
+    px = np.array([12, 16, 20, 16], dtype=np.int32)
+    py = np.array([16, 12, 16, 20], dtype=np.int32)
+    e = orc.expect_f64(px, py, 20000, seed=4)
+    for k in range(len(px)):
+        D = _camera_hits(orc, W, H, px[k], py[k])
+        P = O_cam + (-O_cam[2] / D[2]) * D
+        assert abs(P[0]) < 1 and abs(P[1]) < 1
+        if occluder:
+            q = O_cam + ((OCC_CENTER[2] - O_cam[2]) / D[2]) * D
+            assert np.hypot(q[0] - OCC_CENTER[0], q[1] - OCC_CENTER[1]) > OCC_RADIUS + 0.01   # the camera sees the floor
+        want = _floor_quadrature(sc, P, -D, estimator == capi.ESTIMATOR_COMPAT, occluder)
+        assert e[k, 3] == 1.0
+        np.testing.assert_allclose(e[k, :3], want, rtol=0.03, err_msg=f"pixel {px[k]},{py[k]}")
