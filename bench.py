@@ -152,8 +152,10 @@ def bench_adjoint(er, scenes):
 
 def bench_batched(er, scenes, n_jobs):
     """64 jobs: er_render_batch vs the same jobs rendered one after another, both with
-    RenderEstimate-style single passes and with er_render_passes. The results are bit-identical
-    (same seeds, same paths), so the RMSE is equal."""
+    RenderEstimate-style single passes and with er_render_passes. The batch runs with the brick
+    size, tile shape and workgroups per CU er_autotune_batch picks; the sequential runs use the
+    defaults. The results are bit-identical (same seeds, same paths, and no launch configuration
+    changes the image), so the RMSE is equal."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     lib = er.load()
@@ -184,6 +186,11 @@ def bench_batched(er, scenes, n_jobs):
         r.render_passes(p)
     lib.er_synchronize()
     seq_passes = time.perf_counter() - t0
+    # launch configuration for the batch from the autotuner (a one-time cost, reported apart)
+    reset()
+    t0 = time.perf_counter()
+    tune = er.autotune_batch(rs, rounds=4)
+    tune_s = time.perf_counter() - t0
     reset()
     t0 = time.perf_counter()
     er.render_batch(rs, passes)
@@ -198,7 +205,11 @@ def bench_batched(er, scenes, n_jobs):
             "sequential_estimate_s": round(seq_est, 4), "sequential_passes_s": round(seq_passes, 4),
             "batch_s": round(bat, 4), "batch_over_sequential_estimate": round(bat / seq_est, 4),
             "batch_over_sequential_passes": round(bat / seq_passes, 4),
-            "batch_mpaths_per_s": round(paths / bat / 1e6, 2), "identical_frames": bool(same)}
+            "batch_mpaths_per_s": round(paths / bat / 1e6, 2), "identical_frames": bool(same),
+            "autotune": {"brick": tune["brick"], "tile": [tune["tile_w"], tune["tile_h"]],
+                         "blocks_per_cu": tune["blocks_per_cu"], "trials": tune["trials"],
+                         "round_ms": round(tune["ms_per_pass"], 3),
+                         "default_round_ms": round(tune["default_ms_per_pass"], 3), "tune_s": round(tune_s, 3)}}
 
 
 def main():

@@ -88,6 +88,7 @@ struct DevParams {
     float screen00, screen10, invscreen0, invscreen1;
     float focal, aperture, clip_near, clip_far, exposure;
     int   film_w, film_h;
+    int   tile_shift;                /* primary tiles are 2^tile_shift x 2^(6 - tile_shift) pixels */
     /* settings */
     float step_primary, step_shadow, max_shadow, density_scale;
     float grad_threshold, grad_factor, gm_inv;

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -35,8 +36,8 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
                                        unsigned int* counts, void* events, void* jobs, unsigned char* vis,
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
-                                       int primary_blocks, hipStream_t st, hipStream_t st2, hipEvent_t shaded,
-                                       hipEvent_t after_primary);
+                                       int primary_blocks, int tile_shift, hipStream_t st, hipStream_t st2,
+                                       hipEvent_t shaded, hipEvent_t after_primary);
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
@@ -60,7 +61,7 @@ extern "C" size_t erd_adj_partials_per_block(void);
 extern "C" hipError_t erd_launch_primary_shade(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
                                                unsigned int* counts, void* events, void* jobs, int W, int H, int lut_lds,
                                                size_t lds_bytes, size_t lds_bits, int shade_blocks, int primary_blocks,
-                                               hipStream_t st);
+                                               int tile_shift, hipStream_t st);
 extern "C" hipError_t erd_launch_adj_wavefront(const DevParams* p, unsigned int* counts, const void* jobs, const void* events,
                                                const void* queue, const float4* adj, float* tvis, float4* frame, uint2* list,
                                                unsigned int* lcount, const uint32_t* s1start, const uint32_t* s2start,
@@ -180,15 +181,37 @@ int ensure_neglog_device()
 uint64_t g_scene_version = 1;
 hipStream_t g_batch_stream = nullptr, g_batch_stream2 = nullptr;
 
+/* Macrocells (bricks) of 2^shift cells a side: the min/max intensity of each, from which a
+ * tracer derives its occupancy bits. A volume keeps one grid per brick size in use. */
+struct MacroGrid {
+    int mc[3] = {0, 0, 0};
+    DevBuf<uint32_t> mm;
+};
+
 struct Volume {
     er_volume_desc desc{};
     DevBuf<uint16_t> vox;
     DevBuf<uint4> cells;
-    DevBuf<uint32_t> mm;
-    int mc[3] = {0, 0, 0};
-    int mc_shift = 3;
+    std::map<int, MacroGrid> grids;
+    int mc_shift = 3;               /* default brick: 8 cells, coarser for very large volumes */
     uint16_t vmax = 0;
+    uint64_t upload = 0;            /* upload serial number (autotuner cache key) */
+    ~Volume()
+    {
+        vox.release(); cells.release();
+        for (auto& g : grids) g.second.mm.release();
+    }
 };
+
+/* one occupancy bit per macrocell must fit the 16 KB the primary kernel stages in LDS */
+const size_t ER_MAX_MACROCELLS = 131072;
+
+bool brick_fits(const er_volume_desc& d, int shift)
+{
+    const int E = 1 << shift;
+    return (size_t)((d.resolution[0] + E - 1) / E) * ((d.resolution[1] + E - 1) / E) * ((d.resolution[2] + E - 1) / E) <=
+           ER_MAX_MACROCELLS;
+}
 
 /* The state of one pass in flight: its frame estimate and its work queues. A tracer keeps
  * two, so that a pass's shadow marches, combine and post kernels (second stream) overlap the
@@ -234,6 +257,10 @@ struct Tracer {
     std::vector<float> tf_lut;       /* node table (er_set_tf_lut), empty when the PLFs are used */
     float tf_scale = 0.0f;
     DevBuf<float> tf_lut_dev;
+    /* launch configuration (er_autotune / er_set_tune): brick edge 2^mc_shift (0: the
+     * volume's default), primary tiles 2^tile_shift pixels wide, persistent primary
+     * workgroups per CU (0: the library default) */
+    int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0;
     hipStream_t stream = nullptr;     /* primary + shading, uploads, readbacks */
     hipStream_t stream2 = nullptr;    /* shadow marches, combine, post */
     ~Tracer()
@@ -294,6 +321,7 @@ Registry<BitmapObj> g_bitmaps;
 bool g_device_ready = false;
 int g_list_capacity = ER_LIST_CAPACITY;   /* list.cuh:27 MaxSize; er_set_list_capacity */
 int g_primary_blocks = 2048;   /* persistent primary workgroups: 7 per CU (LDS and VGPR bound) */
+int g_cus = 256;
 
 int ensure_device()
 {
@@ -303,7 +331,10 @@ int ensure_device()
     if (e != hipSuccess || n <= 0) return set_error(ER_ERROR, "no HIP device available (%s)", hipGetErrorString(e));
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+    {
+        g_cus = cus;
         g_primary_blocks = cus * 7;
+    }
     g_device_ready = true;
     return ER_OK;
 }
@@ -425,18 +456,39 @@ int upload_volume(Volume& v)
     const size_t nc = (size_t)(rx + 1) * (ry + 1) * (rz + 1);
     if ((st = v.cells.alloc(nc))) return st;
     HIP_TRY(erd_launch_build_cells(v.vox.ptr, v.cells.ptr, rx, ry, rz, 0));
-    /* macrocell edge: 8 cells, coarser when one occupancy bit per macrocell would exceed 16 KB */
+    /* default macrocell edge: 8 cells, coarser when one occupancy bit per macrocell would
+     * exceed 16 KB */
     v.mc_shift = 3;
-    for (;;) {
-        const int E = 1 << v.mc_shift;
-        v.mc[0] = (rx + E - 1) / E; v.mc[1] = (ry + E - 1) / E; v.mc[2] = (rz + E - 1) / E;
-        if ((size_t)v.mc[0] * v.mc[1] * v.mc[2] <= 131072u || v.mc_shift >= 8) break;
-        v.mc_shift++;
-    }
-    if ((st = v.mm.alloc((size_t)v.mc[0] * v.mc[1] * v.mc[2]))) return st;
-    HIP_TRY(erd_launch_macro_minmax(v.vox.ptr, v.mm.ptr, rx, ry, rz, v.mc[0], v.mc[1], v.mc[2], v.mc_shift, 0));
+    while (!brick_fits(d, v.mc_shift) && v.mc_shift < 8) v.mc_shift++;
+    static uint64_t uploads = 0;
+    v.upload = ++uploads;
+    for (auto& g : v.grids) g.second.mm.release();
+    v.grids.clear();
     HIP_TRY(hipDeviceSynchronize());
     return ER_OK;
+}
+
+/* the volume's macrocell grid of 2^shift cells a side, built on first use */
+MacroGrid* volume_grid(Volume& v, int shift, int& st)
+{
+    st = ER_OK;
+    auto it = v.grids.find(shift);
+    if (it != v.grids.end()) return &it->second;
+    const er_volume_desc& d = v.desc;
+    const int E = 1 << shift;
+    MacroGrid& g = v.grids[shift];
+    for (int i = 0; i < 3; ++i) g.mc[i] = (d.resolution[i] + E - 1) / E;
+    if ((st = g.mm.alloc((size_t)g.mc[0] * g.mc[1] * g.mc[2]))) { v.grids.erase(shift); return nullptr; }
+    hipError_t e = erd_launch_macro_minmax(v.vox.ptr, g.mm.ptr, d.resolution[0], d.resolution[1], d.resolution[2], g.mc[0],
+                                           g.mc[1], g.mc[2], shift, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        g.mm.release();
+        v.grids.erase(shift);
+        st = set_error(ER_ERROR, "%s (macrocell grid)", hipGetErrorString(e));
+        return nullptr;
+    }
+    return &g;
 }
 
 bool needs_gm(const er_tracer_desc& d)
@@ -493,7 +545,11 @@ int build_params(Tracer& t)
     P.cstride_y = vd.resolution[0] + 1;
     P.cstride_z = (vd.resolution[0] + 1) * (vd.resolution[1] + 1);
     P.neglog = g_neglog_dev;
-    for (int i = 0; i < 3; ++i) P.mc_dim[i] = vol->mc[i];
+    const int mc_shift = (t.mc_shift > 0 && brick_fits(vd, t.mc_shift)) ? t.mc_shift : vol->mc_shift;
+    MacroGrid* grid = volume_grid(*vol, mc_shift, st);
+    if (!grid) return st;
+    for (int i = 0; i < 3; ++i) P.mc_dim[i] = grid->mc[i];
+    P.tile_shift = t.tile_shift;
 
     /* The tracer's lights, compacted into device slots in the order of its LightIDs, and the
      * textures they use, compacted likewise. The device indexes them by slot, so any number of
@@ -651,11 +707,11 @@ int build_params(Tracer& t)
     HIP_TRY(hipMemcpy(t.lut.ptr, lut.data(), lut_n * sizeof(float), hipMemcpyHostToDevice));
     P.sigma_lut = t.lut.ptr;
     P.lut_max = lut_n - 1;
-    const int nmc = vol->mc[0] * vol->mc[1] * vol->mc[2];
+    const int nmc = grid->mc[0] * grid->mc[1] * grid->mc[2];
     if ((st = t.mc_sigma.alloc(nmc))) return st;
-    HIP_TRY(erd_launch_macro_sigma(vol->mm.ptr, t.lut.ptr, P.lut_max, t.mc_sigma.ptr, nmc, t.stream));
+    HIP_TRY(erd_launch_macro_sigma(grid->mm.ptr, t.lut.ptr, P.lut_max, t.mc_sigma.ptr, nmc, t.stream));
     P.mc_sigma = t.mc_sigma.ptr;
-    P.mc_shift = vol->mc_shift;
+    P.mc_shift = mc_shift;
     P.mc_words = (nmc + 31) / 32;
     if ((st = t.mc_bits.alloc(P.mc_words))) return st;
     HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, t.mc_bits.ptr, nmc, P.mc_words, t.stream));
@@ -703,7 +759,8 @@ bool use_bilateral(const Tracer& t)
     return t.desc.settings.estimator == ER_ESTIMATOR_FULL && t.desc.settings.bilateral;
 }
 
-/* one pass on the tracer's stream (no host synchronisation) */
+int primary_blocks(const Tracer& t) { return t.blocks_per_cu > 0 ? g_cus * t.blocks_per_cu : g_primary_blocks; }
+
 /* One pass on the tracer's two streams (no host synchronisation). Timing events, when given:
  * e[0] first stream before the primary kernel, e[1] after it, e[2] second stream after the
  * combine kernel, e[3] after post. */
@@ -721,8 +778,8 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
     const size_t lds_bits = (size_t)t.host_params.mc_words * 4;
     HIP_TRY(erd_launch_trace(t.params.ptr, S.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, S.queue.ptr, S.qcount.ptr,
                              S.events.ptr, S.jobs.ptr, S.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
-                             split, t.lds_bytes, lds_bits, shade_blocks, g_primary_blocks, t.stream, t.stream2, S.shaded,
-                             e ? e[1] : nullptr));
+                             split, t.lds_bytes, lds_bits, shade_blocks, primary_blocks(t), t.tile_shift, t.stream,
+                             t.stream2, S.shaded, e ? e[1] : nullptr));
     if (e) HIP_TRY(hipEventRecord(e[2], t.stream2));
     HIP_TRY(erd_launch_post(t.params.ptr, S.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations,
                             S.qcount.ptr, t.W, t.H, t.stream2));
@@ -779,6 +836,149 @@ int bind_generic(R& reg, D* d, int bind, F make)
     g_scene_version++;
     return ER_OK;
 }
+
+/* ---- autotuner ------------------------------------------------------------------------ */
+
+/* What a tuned configuration depends on: the tracer's descriptor (film, camera, transfer
+ * functions, settings, lights, objects), its node table, and the volume's contents. */
+std::string tune_key(const Tracer& t)
+{
+    er_tracer_desc d = t.desc;
+    d.id = -1;
+    d.no_iterations = 0;
+    std::string k((const char*)&d, sizeof(d));
+    const Volume* v = g_volumes.find(t.desc.volume_id);
+    const uint64_t up = v ? v->upload : 0;
+    k.append((const char*)&up, sizeof(up));
+    k.append((const char*)t.tf_lut.data(), t.tf_lut.size() * sizeof(float));
+    k.append((const char*)&t.tf_scale, sizeof(float));
+    return k;
+}
+
+std::map<std::string, er_tune> g_tune_cache;
+
+void fill_tune(const Tracer& t, er_tune& r)
+{
+    const Volume* v = g_volumes.find(t.desc.volume_id);
+    const int shift = (t.mc_shift > 0 && v && brick_fits(v->desc, t.mc_shift)) ? t.mc_shift : (v ? v->mc_shift : 3);
+    r.brick = 1 << shift;
+    r.tile_w = 1 << t.tile_shift;
+    r.tile_h = 64 >> t.tile_shift;
+    r.blocks_per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : g_primary_blocks / std::max(1, g_cus);
+}
+
+int apply_tune(Tracer& t, const er_tune& r)
+{
+    int bs = 0, ts = 0;
+    while (bs < 8 && (1 << bs) != r.brick) ++bs;
+    while (ts < 7 && (1 << ts) != r.tile_w) ++ts;
+    if ((1 << bs) != r.brick || bs < 1 || bs > 6) return set_error(ER_ERROR, "brick must be 2, 4, ..., 64 cells");
+    if ((1 << ts) != r.tile_w || ts < 2 || ts > 5 || r.tile_w * r.tile_h != 64)
+        return set_error(ER_ERROR, "tile must be 4x16, 8x8, 16x4 or 32x2 pixels");
+    if (r.blocks_per_cu < 1 || r.blocks_per_cu > 16) return set_error(ER_ERROR, "blocks per CU must be in [1, 16]");
+    const Volume* v = g_volumes.find(t.desc.volume_id);
+    if (v && !brick_fits(v->desc, bs)) return set_error(ER_ERROR, "a %d-cell brick needs more than %zu macrocells", r.brick,
+                                                        ER_MAX_MACROCELLS);
+    t.mc_shift = bs;
+    t.tile_shift = ts;
+    t.blocks_per_cu = r.blocks_per_cu;
+    t.built_version = 0;
+    return ER_OK;
+}
+
+/* The per-tracer state a timing trial overwrites, saved and put back so that tuning leaves
+ * a render where it was. */
+struct Snapshot {
+    DevBuf<uint32_t> s1, s2;
+    DevBuf<float4> run;
+    DevBuf<uchar4> disp;
+    DevBuf<unsigned long long> cnt;
+    int no_iter = 0;
+    ~Snapshot() { s1.release(); s2.release(); run.release(); disp.release(); cnt.release(); }
+    int save(Tracer& t)
+    {
+        int st;
+        const size_t n = (size_t)t.W * t.H;
+        if ((st = sync_tracer(t))) return st;
+        if ((st = s1.alloc(n)) || (st = s2.alloc(n)) || (st = run.alloc(n)) || (st = disp.alloc(n)) || (st = cnt.alloc(12)))
+            return st;
+        HIP_TRY(hipMemcpy(s1.ptr, t.seeds1.ptr, n * 4, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(s2.ptr, t.seeds2.ptr, n * 4, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(run.ptr, t.running.ptr, n * sizeof(float4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(disp.ptr, t.display.ptr, n * sizeof(uchar4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(cnt.ptr, t.counters.ptr, 12 * sizeof(unsigned long long), hipMemcpyDeviceToDevice));
+        no_iter = t.desc.no_iterations;
+        return ER_OK;
+    }
+    /* the seeds only: every trial traces the same paths */
+    int rewind(Tracer& t)
+    {
+        const size_t n = (size_t)t.W * t.H;
+        HIP_TRY(hipMemcpyAsync(t.seeds1.ptr, s1.ptr, n * 4, hipMemcpyDeviceToDevice, t.stream));
+        HIP_TRY(hipMemcpyAsync(t.seeds2.ptr, s2.ptr, n * 4, hipMemcpyDeviceToDevice, t.stream));
+        return ER_OK;
+    }
+    int restore(Tracer& t)
+    {
+        int st;
+        const size_t n = (size_t)t.W * t.H;
+        if ((st = sync_tracer(t))) return st;
+        HIP_TRY(hipMemcpy(t.seeds1.ptr, s1.ptr, n * 4, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(t.seeds2.ptr, s2.ptr, n * 4, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(t.running.ptr, run.ptr, n * sizeof(float4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(t.display.ptr, disp.ptr, n * sizeof(uchar4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(t.counters.ptr, cnt.ptr, 12 * sizeof(unsigned long long), hipMemcpyDeviceToDevice));
+        t.desc.no_iterations = no_iter;
+        return ER_OK;
+    }
+};
+
+/* wall time of `passes` passes from the saved seeds (first stream start to last post end),
+ * after one warm-up pass */
+int time_passes(Tracer& t, Snapshot& snap, int passes, float& ms)
+{
+    int st;
+    if ((st = build_params(t))) return st;
+    if ((st = snap.rewind(t)) || (st = enqueue_pass(t, nullptr)) || (st = sync_tracer(t)) || (st = snap.rewind(t)))
+        return st;
+    std::vector<hipEvent_t> ev((size_t)passes * 4, nullptr);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    for (int k = 0; k < passes && !st; ++k) st = enqueue_pass(t, &ev[4 * k]);
+    const int sst = sync_tracer(t);
+    if (!st) st = sst;
+    if (!st) {
+        hipError_t e = hipEventElapsedTime(&ms, ev[0], ev[4 * (passes - 1) + 3]);
+        if (e != hipSuccess) st = set_error(ER_ERROR, "%s (hipEventElapsedTime)", hipGetErrorString(e));
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return st;
+}
+
+/* The candidates of one search axis around `cur`: 0 bricks that fit every volume, 1 tile
+ * shapes, 2 workgroups per CU. */
+std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::vector<const Volume*>& vols)
+{
+    std::vector<er_tune> c;
+    if (axis == 0) {
+        for (int b = 1; b <= 5; ++b) {
+            bool fits = true;
+            for (const Volume* v : vols) fits = fits && v && brick_fits(v->desc, b);
+            if (fits && (1 << b) != cur.brick) { er_tune x = cur; x.brick = 1 << b; c.push_back(x); }
+        }
+    } else if (axis == 1) {
+        for (int ts = 2; ts <= 5; ++ts)
+            if ((1 << ts) != cur.tile_w) { er_tune x = cur; x.tile_w = 1 << ts; x.tile_h = 64 >> ts; c.push_back(x); }
+    } else {
+        for (int b = 3; b <= 9; ++b)
+            if (b != cur.blocks_per_cu) { er_tune x = cur; x.blocks_per_cu = b; c.push_back(x); }
+    }
+    return c;
+}
+
+/* a candidate replaces the incumbent only when it is clearly faster: timing noise must not
+ * walk the search away from a good configuration */
+const float ER_TUNE_MARGIN = 0.97f;
+const float ER_TUNE_TRIAL_MS = 30.0f;   /* target length of one timing trial */
 
 } /* namespace */
 
@@ -1064,6 +1264,180 @@ ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
     return ER_OK;
 }
 
+/* Coordinate search over brick size, tile shape and persistent workgroups per CU. Each
+ * candidate renders the same passes from the same seeds (best of three trials of about
+ * 30 ms, or `passes` passes when given); a candidate must be 3% faster to be taken. The
+ * tracer's render state is restored afterwards. Results are cached by scene. */
+ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* tp = get_tracer(tracer_id, st);
+    if (!tp) return st;
+    Tracer& t = *tp;
+    if ((st = build_params(t))) return st;
+    const std::string key = tune_key(t);
+    auto cached = g_tune_cache.find(key);
+    if (cached != g_tune_cache.end() && !(flags & ER_TUNE_FORCE)) {
+        er_tune r = cached->second;
+        if ((st = apply_tune(t, r))) return st;
+        if ((st = build_params(t))) return st;
+        r.trials = 0;
+        if (out) *out = r;
+        return ER_OK;
+    }
+    if (t.W == 0 || t.H == 0) return set_error(ER_ERROR, "tracer %d has an empty film", tracer_id);
+    Snapshot snap;
+    if ((st = snap.save(t))) return st;
+    const int mc0 = t.mc_shift, ts0 = t.tile_shift, bpc0 = t.blocks_per_cu;
+    er_tune cur{};
+    fill_tune(t, cur);
+    int trials = 0;
+    auto measure = [&](const er_tune& c, float& best_ms) -> int {
+        int s;
+        if ((s = apply_tune(t, c))) return s;
+        best_ms = FLT_MAX;
+        for (int rep = 0; rep < 3; ++rep) {
+            float ms = 0.0f;
+            if ((s = time_passes(t, snap, passes, ms))) return s;
+            best_ms = std::min(best_ms, ms);
+        }
+        ++trials;
+        return ER_OK;
+    };
+    float cur_ms = 0.0f;
+    if (passes <= 0) {
+        /* size the trials from one probe pass of the starting configuration */
+        float probe = 0.0f;
+        passes = 2;
+        if (!(st = time_passes(t, snap, passes, probe)))
+            passes = std::max(2, std::min(64, (int)std::ceil(ER_TUNE_TRIAL_MS / std::max(probe / 2.0f, 1e-3f))));
+    }
+    if (!st) st = measure(cur, cur_ms);
+    const float default_ms = cur_ms;
+    std::vector<const Volume*> vols{g_volumes.find(t.desc.volume_id)};
+    for (int axis = 0; axis < 3 && !st; ++axis)
+        for (const er_tune& c : tune_candidates(axis, cur, vols)) {
+            float ms = 0.0f;
+            if ((st = measure(c, ms))) break;
+            if (ms < ER_TUNE_MARGIN * cur_ms) { cur = c; cur_ms = ms; }
+        }
+    if (!st) st = apply_tune(t, cur);
+    if (st) { t.mc_shift = mc0; t.tile_shift = ts0; t.blocks_per_cu = bpc0; t.built_version = 0; }
+    const int rst = snap.restore(t);
+    if (st) return st;
+    if (rst) return rst;
+    if ((st = build_params(t))) return st;
+    cur.ms_per_pass = cur_ms / (float)passes;
+    cur.default_ms_per_pass = default_ms / (float)passes;
+    cur.trials = trials;
+    g_tune_cache[key] = cur;
+    if (out) *out = cur;
+    return ER_OK;
+}
+
+/* One configuration for a whole batch: the same coordinate search, each candidate applied to
+ * every job (a brick is a candidate only if it fits every job's volume) and timed as `rounds`
+ * batch rounds from the jobs' saved seeds (best of three). Every job's render state is
+ * restored; the jobs keep the chosen configuration. ms_per_pass reports one round. */
+ER_API int er_autotune_batch(const er_batch_item* items, int n, int rounds, er_tune* out)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    if (n <= 0 || !items) return set_error(ER_ERROR, "empty batch");
+    if (rounds <= 0) rounds = 4;
+    std::vector<Tracer*> ts;
+    std::vector<const Volume*> vols;
+    for (int k = 0; k < n; ++k) {
+        Tracer* t = get_tracer(items[k].tracer_id, st);
+        if (!t) return st;
+        if ((st = build_params(*t))) return st;
+        ts.push_back(t);
+        vols.push_back(g_volumes.find(t->desc.volume_id));
+    }
+    std::vector<std::unique_ptr<Snapshot>> snaps;
+    std::vector<er_tune> orig(n);
+    for (int k = 0; k < n; ++k) {
+        snaps.emplace_back(new Snapshot());
+        if ((st = snaps[k]->save(*ts[k]))) return st;
+        fill_tune(*ts[k], orig[k]);
+    }
+    std::vector<er_batch_item> trial(items, items + n);
+    for (auto& it : trial) it.passes = rounds;
+    int trials = 0;
+    auto apply_all = [&](const er_tune& c) -> int {
+        int s;
+        for (int k = 0; k < n; ++k) {
+            er_tune x = c;
+            if (vols[k] && !brick_fits(vols[k]->desc, (int)std::log2((double)c.brick))) x.brick = orig[k].brick;
+            if ((s = apply_tune(*ts[k], x))) return s;
+        }
+        return ER_OK;
+    };
+    auto measure = [&](const er_tune& c, float& best_ms) -> int {
+        int s;
+        if ((s = apply_all(c))) return s;
+        best_ms = FLT_MAX;
+        for (int rep = 0; rep < 4; ++rep) {   /* the first is a warm-up */
+            for (int k = 0; k < n; ++k)
+                if ((s = snaps[k]->rewind(*ts[k])) || (s = sync_tracer(*ts[k]))) return s;
+            er_timing tm{};
+            if ((s = er_render_batch(trial.data(), n, &tm))) return s;
+            if (rep > 0) best_ms = std::min(best_ms, tm.total_ms);
+        }
+        ++trials;
+        return ER_OK;
+    };
+    er_tune cur = orig[0];
+    cur.brick = orig[0].brick;
+    float cur_ms = 0.0f;
+    st = measure(cur, cur_ms);
+    const float default_ms = cur_ms;
+    for (int axis = 0; axis < 3 && !st; ++axis)
+        for (const er_tune& c : tune_candidates(axis, cur, vols)) {
+            float ms = 0.0f;
+            if ((st = measure(c, ms))) break;
+            if (ms < ER_TUNE_MARGIN * cur_ms) { cur = c; cur_ms = ms; }
+        }
+    if (!st) st = apply_all(cur);
+    if (st) for (int k = 0; k < n; ++k) (void)apply_tune(*ts[k], orig[k]);
+    for (int k = 0; k < n; ++k) {
+        const int rst = snaps[k]->restore(*ts[k]);
+        if (!st) st = rst;
+    }
+    if (st) return st;
+    cur.ms_per_pass = cur_ms / (float)rounds;
+    cur.default_ms_per_pass = default_ms / (float)rounds;
+    cur.trials = trials;
+    if (out) *out = cur;
+    return ER_OK;
+}
+
+ER_API int er_set_tune(int tracer_id, const er_tune* tune)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if (!tune) return set_error(ER_ERROR, "tune is NULL");
+    if ((st = sync_tracer(*t))) return st;
+    return apply_tune(*t, *tune);
+}
+
+ER_API int er_get_tune(int tracer_id, er_tune* tune)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    if (!tune) return set_error(ER_ERROR, "tune is NULL");
+    er_tune r{};
+    fill_tune(*t, r);
+    *tune = r;
+    return ER_OK;
+}
+
+
 ER_API int er_set_list_capacity(int capacity)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mutex);
@@ -1116,8 +1490,8 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
             B.queue = (erd::ScatterRec*)S.queue.ptr; B.events = (erd::EventRec*)S.events.ptr;
             B.jobs = (erd::ShadowJob*)S.jobs.ptr; B.vis = S.vis.ptr;
             B.counts = S.qcount.ptr;
-            B.tiles_x = (t.W + 7) / 8;
-            B.ntiles = B.tiles_x * ((t.H + 7) / 8);
+            B.tiles_x = (t.W + (1 << t.tile_shift) - 1) >> t.tile_shift;
+            B.ntiles = B.tiles_x * ((t.H + (64 >> t.tile_shift) - 1) / (64 >> t.tile_shift));
             B.tile_base = tile_base;
             B.post_tiles_x = (t.W + 15) / 16;
             B.post_ntiles = B.post_tiles_x * ((t.H + 15) / 16);
@@ -1140,6 +1514,8 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
     if (!g_batch_stream2) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream2, hipStreamNonBlocking));
     hipStream_t bs = g_batch_stream, bs2 = g_batch_stream2;
     const int shade_blocks = 2048;
+    int batch_blocks = 0;   /* persistent primary workgroups: the largest any job asks for */
+    for (auto& j : jobs) batch_blocks = std::max(batch_blocks, primary_blocks(*j.second));
     const int rounds = jobs[0].first;
     /* all rounds as one graph (no memset nodes): per round, primary + shading on the first
      * stream, then shadow + combine + post + queue reset on the second; a set is reused two
@@ -1163,7 +1539,7 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         hipEvent_t shaded = nullptr;
         if (ec == hipSuccess) ec = new_event(shaded);
         if (ec == hipSuccess)
-            ec = erd_launch_batch_trace(dJ[b].ptr, nj, dS[b].ptr, full, g_primary_blocks, shade_blocks, (unsigned int)r, bs,
+            ec = erd_launch_batch_trace(dJ[b].ptr, nj, dS[b].ptr, full, batch_blocks, shade_blocks, (unsigned int)r, bs,
                                         bs2, shaded);
         if (ec == hipSuccess) ec = erd_launch_batch_post(dJ[b].ptr, nj, (unsigned int)r, post_base, bs2);
         if (ec == hipSuccess) ec = erd_launch_round_end(dJ[b].ptr, nj, dS[b].ptr, bs2);
@@ -1321,7 +1697,7 @@ ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t*
         /* the forward pass's primary and shading kernels: decisions, queues, final seeds */
         HIP_TRY(erd_launch_primary_shade(t->params.ptr, A.frame.ptr, d1.ptr, d2.ptr, A.queue.ptr, A.counts.ptr, A.events.ptr,
                                          A.jobs.ptr, t->W, t->H, t->lut_lds, t->lds_bytes, lds_bits, shade_blocks,
-                                         g_primary_blocks, s));
+                                         primary_blocks(*t), t->tile_shift, s));
         HIP_TRY(hipEventRecord(A.shaded, s));
         HIP_TRY(hipStreamWaitEvent(s2, A.shaded, 0));
         if (w != 0.0f) {

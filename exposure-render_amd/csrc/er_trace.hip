@@ -80,8 +80,9 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
                                              uint32_t* __restrict__ seeds2, ScatterRec* __restrict__ queue,
                                              unsigned int* __restrict__ qcount, Cnt& cnt, int& paths)
 {
-    const int px = (int)(tile % (unsigned int)tiles_x) * 8 + (lane & 7);
-    const int py = (int)(tile / (unsigned int)tiles_x) * 8 + (lane >> 3);
+    const int ts = p.tile_shift;
+    const int px = (int)(tile % (unsigned int)tiles_x) * (1 << ts) + (lane & ((1 << ts) - 1));
+    const int py = (int)(tile / (unsigned int)tiles_x) * (64 >> ts) + (lane >> ts);
     const bool active = px < p.film_w && py < p.film_h;
     const size_t i = (size_t)py * p.film_w + px;
     bool scatter = false;
@@ -244,7 +245,8 @@ __device__ __forceinline__ void finish_item(const EventRec& ev, unsigned int q, 
     frame[ev.pixel] = make_float4(Lv.x, Lv.y, Lv.z, 1.0f);
 }
 
-/* Primary kernel: persistent waves take 8 x 8 pixel tiles from an atomic counter (counts[2]),
+/* Primary kernel: persistent waves take 64-pixel tiles (8 x 8 by default, DevParams::tile_shift)
+ * from an atomic counter (counts[2]),
  * so the cost of a tile - from a few samples for background pixels to thousands behind
  * low-density tissue - is balanced over the whole device. */
 template<int FULL, int COUNT, int LUT_LDS>
@@ -520,10 +522,11 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
                                        unsigned int* counts, void* events, void* jobs, unsigned char* vis,
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
-                                       int primary_blocks, hipStream_t st, hipStream_t st2, hipEvent_t shaded,
-                                       hipEvent_t after_primary)
+                                       int primary_blocks, int tile_shift, hipStream_t st, hipStream_t st2,
+                                       hipEvent_t shaded, hipEvent_t after_primary)
 {
-    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const int tw = 1 << tile_shift, th = 64 >> tile_shift;
+    const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
     const int ntiles = tiles_x * tiles_y;
     /* persistent workgroups: enough to fill every CU, never more than there are tiles */
     const int nblocks = std::max(1, std::min((ntiles + 3) / 4, primary_blocks));
@@ -576,9 +579,10 @@ extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, er
 extern "C" hipError_t erd_launch_primary_shade(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2,
                                                void* queue, unsigned int* counts, void* events, void* jobs, int W, int H,
                                                int lut_lds, size_t lds_bytes, size_t lds_bits, int shade_blocks,
-                                               int primary_blocks, hipStream_t st)
+                                               int primary_blocks, int tile_shift, hipStream_t st)
 {
-    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const int tw = 1 << tile_shift, th = 64 >> tile_shift;
+    const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
     const int ntiles = tiles_x * tiles_y;
     const int nblocks = std::max(1, std::min((ntiles + 3) / 4, primary_blocks));
     erd::ScatterRec* q = (erd::ScatterRec*)queue;

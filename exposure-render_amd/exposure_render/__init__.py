@@ -130,6 +130,22 @@ class Renderer:
         _check(self.lib, self.lib.er_render_passes(self.id, passes, C.byref(t)))
         return t
 
+    def autotune(self, passes=6, force=False):
+        """Pick the fastest brick size, tile shape and workgroups per CU for this scene
+        (er_autotune); the render state is left as it was. Returns the chosen er_tune."""
+        t = capi.Tune()
+        _check(self.lib, self.lib.er_autotune(self.id, int(passes), capi.TUNE_FORCE if force else 0, C.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, brick, tile_w, blocks_per_cu):
+        t = capi.Tune(brick=brick, tile_w=tile_w, tile_h=64 // tile_w, blocks_per_cu=blocks_per_cu)
+        _check(self.lib, self.lib.er_set_tune(self.id, C.byref(t)))
+
+    def get_tune(self):
+        t = capi.Tune()
+        _check(self.lib, self.lib.er_get_tune(self.id, C.byref(t)))
+        return t.as_dict()
+
     def seed(self, seed):
         _check(self.lib, self.lib.er_seed_tracer(self.id, int(seed)))
 
@@ -226,6 +242,20 @@ class Renderer:
         for T in self.textures:
             lib.er_bind_texture(C.byref(T), 0)
         lib.er_bind_volume(C.byref(self.volume), 0)
+
+
+def autotune_batch(renderers, rounds=4):
+    """er_autotune_batch: one brick size, tile shape and workgroups-per-CU setting for the
+    whole batch, timed as `rounds` batch rounds; the jobs' render states are restored and the
+    jobs keep the chosen configuration. Returns the er_tune as a dict."""
+    lib = load()
+    items = (capi.BatchItem * len(renderers))()
+    for k, r in enumerate(renderers):
+        items[k].tracer_id = r.id
+        items[k].passes = rounds
+    t = capi.Tune()
+    _check(lib, lib.er_autotune_batch(items, len(renderers), int(rounds), C.byref(t)))
+    return t.as_dict()
 
 
 def render_batch(renderers, passes):

@@ -169,6 +169,17 @@ class BatchItem(C.Structure):
     _fields_ = [("tracer_id", C.c_int), ("passes", C.c_int)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("brick", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("blocks_per_cu", C.c_int),
+                ("ms_per_pass", C.c_float), ("default_ms_per_pass", C.c_float), ("trials", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+TUNE_FORCE = 1
+
+
 SIGNATURES = {
     "er_abi_version": (C.c_int, []),
     "er_abi_sizeof": (C.c_int, [_P(C.c_int), C.c_int]),
@@ -205,6 +216,10 @@ SIGNATURES = {
     "er_set_tf_lut": (C.c_int, [C.c_int, _P(C.c_float), C.c_float]),
     "er_render_batch": (C.c_int, [_P(BatchItem), C.c_int, _P(Timing)]),
     "er_set_list_capacity": (C.c_int, [C.c_int]),
+    "er_autotune": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(Tune)]),
+    "er_set_tune": (C.c_int, [C.c_int, _P(Tune)]),
+    "er_autotune_batch": (C.c_int, [_P(BatchItem), C.c_int, C.c_int, _P(Tune)]),
+    "er_get_tune": (C.c_int, [C.c_int, _P(Tune)]),
     "er_autofocus_seeds": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(C.c_uint32), _P(C.c_uint32)]),
     "er_adjoint_lut": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
                                  _P(C.c_double)]),

@@ -367,6 +367,32 @@ ER_API int er_neglog_table_entry(uint32_t m, float* value);
  * ER_LIST_CAPACITY distinct lights and textures. */
 ER_API int er_set_list_capacity(int capacity);
 
+/* ---- launch autotuner (new; no counterpart in the reference) --------------------------- */
+
+/* A tracer's launch configuration. None of it changes the image: paths are per pixel and the
+ * empty-space skipping is exact, so every configuration renders the same bits. */
+typedef struct er_tune {
+    int   brick;                  /* macrocell (brick) edge in cells: 2, 4, 8, 16, 32 */
+    int   tile_w, tile_h;         /* primary tile, 64 pixels: 4x16, 8x8, 16x4 or 32x2 */
+    int   blocks_per_cu;          /* persistent 256-lane primary workgroups per CU */
+    float ms_per_pass;            /* er_autotune: the chosen configuration's time per pass */
+    float default_ms_per_pass;    /* er_autotune: the starting configuration's time per pass */
+    int   trials;                 /* er_autotune: configurations timed (0: taken from the cache) */
+} er_tune;
+
+#define ER_TUNE_FORCE 1           /* er_autotune flag: measure even if the scene is cached */
+
+/* Times `passes` passes (default 6) of each candidate from the tracer's current seeds and
+ * keeps the fastest: bricks, then tile shapes, then workgroups per CU. The tracer's seeds,
+ * running estimate, display, counters and pass count are restored afterwards. Results are
+ * cached per scene (descriptor, node table and volume contents). */
+ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out);
+/* One configuration for a batch (er_render_batch): the same search with each candidate applied
+ * to every job and timed as `rounds` batch rounds (default 4); every job's render state is
+ * restored and the jobs keep the chosen configuration. */
+ER_API int er_set_tune(int tracer_id, const er_tune* tune);
+ER_API int er_get_tune(int tracer_id, er_tune* tune);
+
 /* ---- batches (new; no counterpart in the reference) ----------------------------------- */
 
 typedef struct er_batch_item {

@@ -154,6 +154,75 @@ static int render()
     GetNoIterations(T.ID, n);
     CHECK(n == PASSES, "GetNoIterations = %d", n);
 
+    /* ---- a bitmap-textured floor object, full estimator (ObjectIDs name the objects) ---- */
+    const int BW = 8, BH = 4;
+    std::vector<ColorRGBAuc> pix((size_t)BW * BH);
+    for (size_t i = 0; i < pix.size(); ++i)
+        for (int c = 0; c < 4; ++c) pix[i].D[c] = (unsigned char)((i * 37 + c * 91) & 255);
+    /* a 3 x 3 plane at y = -0.6 facing +y: local x -> x, y -> -z, z -> y */
+    const float ftm[4][4] = {{1, 0, 0, 0}, {0, 0, 1, -0.6f}, {0, -1, 0, 0}, {0, 0, 0, 1}};
+    const float finv[4][4] = {{1, 0, 0, 0}, {0, 0, -1, 0}, {0, 1, 0, 0.6f}, {0, 0, 0, 1}};
+
+    ErBitmap B;
+    B.BindPixels(Vec2i(BW, BH), pix.data());
+    BindBitmap(B);
+    CHECK(B.ID >= 0 && !B.Pixels.Dirty, "BindBitmap assigns an ID (%d) and clears Dirty", B.ID);
+    ErTexture BT;
+    BT.Type = Enums::Bitmap;
+    BT.BitmapID = B.ID;
+    BT.Repeat = Vec2f(2.0f, 1.0f);
+    BindTexture(BT);
+    ErObject Fl;
+    Fl.DiffuseTextureID = BT.ID;
+    Fl.SpecularTextureID = Tex.ID;
+    Fl.Shape.Type = Enums::Plane;
+    Fl.Shape.Size = Vec3f(3.0f, 3.0f, 0.0f);
+    std::memcpy(Fl.Shape.TM.NN, ftm, sizeof(ftm));
+    std::memcpy(Fl.Shape.InvTM.NN, finv, sizeof(finv));
+    BindObject(Fl);
+    CHECK(Fl.ID >= 0, "BindObject assigns an ID (%d)", Fl.ID);
+    T.RenderSettings.Estimator = Enums::Full;
+    T.ObjectIDs[0] = Fl.ID;
+    T.ObjectIDs.Count = 1;
+    BindTracer(T);
+
+    er_bitmap_desc bd;
+    er_bitmap_desc_init(&bd);
+    bd.resolution[0] = BW; bd.resolution[1] = BH;
+    bd.pixels = reinterpret_cast<const uint8_t*>(pix.data());
+    bd.pixels_dirty = 1;
+    CHECK(er_bind_bitmap(&bd, 1) == ER_OK && bd.id >= 0 && bd.id != B.ID, "C-ABI bitmap bound (%d)", bd.id);
+    er_texture_desc td;
+    er_texture_desc_init(&td);
+    td.type = ER_TEXTURE_BITMAP;
+    td.bitmap_id = bd.id;
+    td.repeat[0] = 2.0f; td.repeat[1] = 1.0f;
+    CHECK(er_bind_texture(&td, 1) == ER_OK, "C-ABI bitmap texture bound (%d)", td.id);
+    er_object_desc od;
+    er_object_desc_init(&od);
+    od.diffuse_texture_id = td.id;
+    od.specular_texture_id = Tex.ID;
+    od.shape.type = ER_SHAPE_PLANE;
+    od.shape.size[0] = od.shape.size[1] = 3.0f;
+    std::memcpy(od.shape.tm, ftm, sizeof(ftm));
+    std::memcpy(od.shape.inv_tm, finv, sizeof(finv));
+    CHECK(er_bind_object(&od, 1) == ER_OK && od.id >= 0, "C-ABI object bound (%d)", od.id);
+    d.settings.estimator = ER_ESTIMATOR_FULL;
+    d.object_ids[0] = od.id;
+    d.object_count = 1;
+    CHECK(er_bind_tracer(&d, 1) == ER_OK, "C-ABI tracer rebound with the object");
+
+    er_seed_tracer(T.ID, 78);
+    er_seed_tracer(d.id, 78);
+    for (int k = 0; k < PASSES; ++k) RenderEstimate(T.ID);
+    for (int k = 0; k < PASSES; ++k) er_render_estimate(d.id);
+    GetEstimate(T.ID, a.data());
+    er_get_estimate(d.id, b.data(), b.size());
+    size_t nz2 = 0;
+    for (unsigned char x : a) nz2 += x != 0;
+    CHECK(a == b, "objects and bitmaps: C++ API and C-ABI images are identical (%zu non-zero bytes)", nz2);
+    CHECK(nz2 > nz, "the floor object adds coverage (%zu > %zu)", nz2, nz);
+
     /* rebind with the same ID updates in place and restarts accumulation */
     const int id0 = T.ID;
     T.Camera.Exposure = 2.0f;

@@ -1,0 +1,157 @@
+"""The launch autotuner (er_autotune / er_set_tune).
+
+No launch configuration may change the image: paths are independent per pixel, and the
+empty-space skipping is exact for every brick size. So every configuration must render
+bit-identical frames, running estimates, RNG states and reference sample counts, and tuning
+must leave a render exactly where it was.
+"""
+import numpy as np
+import pytest
+
+import exposure_render as er
+from exposure_render import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+COUNTERS = ["paths", "primary_samples", "shadow_samples", "shading_samples", "scatter_events", "shadow_rays"]
+
+
+def _render(sc, passes, tune=None, seed=9):
+    r = er.Renderer(sc, seed=seed)
+    if tune is not None:
+        r.set_tune(*tune)
+    r.set_counting(True)
+    r.reset_counters()
+    for _ in range(passes):
+        r.render_estimate()
+    out = (r.frame(), r.running(), r.get_seeds(), r.get_estimate(), r.counters(), r.get_tune())
+    r.close()
+    return out
+
+
+def _same(a, b, what):
+    fa, ra, (sa1, sa2), da, ca, _ = a
+    fb, rb, (sb1, sb2), db, cb, tb = b
+    assert np.array_equal(fa, fb), f"frame differs with {what} {tb}"
+    assert np.array_equal(ra, rb), f"running estimate differs with {what} {tb}"
+    assert np.array_equal(sa1, sb1) and np.array_equal(sa2, sb2), f"RNG state differs with {what} {tb}"
+    assert np.array_equal(da, db), f"display differs with {what} {tb}"
+    for k in COUNTERS:
+        assert ca[k] == cb[k], (what, k, ca[k], cb[k])
+
+
+@pytest.mark.parametrize("estimator", [capi.ESTIMATOR_COMPAT, capi.ESTIMATOR_FULL])
+def test_every_configuration_renders_the_same_image(fresh, estimator):
+    sc = scenes.config_small(scale=0.5, estimator=estimator)
+    print(f"[tune] small estimator={estimator}", flush=True)
+    base = _render(sc, 3)
+    assert base[5]["brick"] == 8 and base[5]["tile_w"] == 8
+    for tune in [(2, 8, 7), (4, 4, 7), (16, 16, 5), (32, 32, 3), (8, 8, 9), (4, 32, 4)]:
+        _same(base, _render(sc, 3, tune), "tune")
+
+
+def test_ct_brick_sizes_render_the_same_image(fresh):
+    sc = scenes.config_ct(scale=0.25)
+    print("[tune] ct 64^3 bricks", flush=True)
+    base = _render(sc, 2)
+    for tune in [(2, 8, 7), (4, 16, 6), (16, 4, 8)]:
+        _same(base, _render(sc, 2, tune), "tune")
+
+
+def test_autotune_restores_the_render_and_caches(fresh):
+    sc = scenes.config_small(scale=0.5)
+    print("[tune] autotune restore", flush=True)
+    ref = er.Renderer(sc, seed=3)
+    for _ in range(2):
+        ref.render_estimate()
+    want = (ref.running().copy(), ref.get_seeds(), ref.get_estimate().copy())
+    ref.close()
+
+    r = er.Renderer(sc, seed=3)
+    r.set_counting(True)
+    r.reset_counters()
+    r.render_estimate()
+    before = (r.running().copy(), r.get_seeds(), r.no_iterations(), r.counters())
+    t = r.autotune(passes=2)
+    assert t["trials"] >= 10 and t["ms_per_pass"] > 0 and t["ms_per_pass"] <= t["default_ms_per_pass"]
+    assert t["brick"] in (2, 4, 8, 16, 32) and t["tile_w"] * t["tile_h"] == 64 and 3 <= t["blocks_per_cu"] <= 9
+    after = (r.running(), r.get_seeds(), r.no_iterations(), r.counters())
+    assert np.array_equal(before[0], after[0])
+    assert np.array_equal(before[1][0], after[1][0]) and np.array_equal(before[1][1], after[1][1])
+    assert before[2] == after[2] == 1
+    assert before[3] == after[3]
+    assert r.get_tune()["brick"] == t["brick"] and r.get_tune()["tile_w"] == t["tile_w"]
+    # the render continues as if no tuning had happened
+    r.render_estimate()
+    assert np.array_equal(r.running(), want[0])
+    assert np.array_equal(r.get_seeds()[0], want[1][0])
+    assert np.array_equal(r.get_estimate(), want[2])
+    # a second call takes the cached result
+    t2 = r.autotune(passes=2)
+    assert t2["trials"] == 0 and (t2["brick"], t2["tile_w"], t2["blocks_per_cu"]) == (t["brick"], t["tile_w"],
+                                                                                        t["blocks_per_cu"])
+    r.close()
+
+
+def test_set_tune_rejects_invalid_configurations(fresh):
+    sc = scenes.config_small(scale=0.25)
+    r = er.Renderer(sc, seed=1)
+    for bad in [(3, 8, 7), (8, 6, 7), (8, 64, 7), (8, 8, 0), (8, 8, 17), (128, 8, 7)]:
+        with pytest.raises(er.ExposureRenderError):
+            r.set_tune(*bad)
+    r.close()
+
+
+def test_batch_with_tuned_jobs_matches_sequential(fresh):
+    """Jobs of one batch with different tiles and bricks (the batch kernels read each job's
+    configuration) render what the same jobs render one after another."""
+    lib = capi.load()
+    lib.er_set_list_capacity(64)
+    tunes = [(2, 4, 7), (16, 32, 7), (4, 16, 7)]
+    scs = [scenes.config_small(scale=0.25 + 0.125 * k, seed=k + 1) for k in range(3)]
+    seq = []
+    for sc, tune in zip(scs, tunes):
+        r = er.Renderer(sc, seed=5)
+        r.set_tune(*tune)
+        r.render_passes(3)
+        seq.append((r.running().copy(), r.get_estimate().copy()))
+        r.close()
+    rs = []
+    for sc, tune in zip(scs, tunes):
+        r = er.Renderer(sc, seed=5)
+        r.set_tune(*tune)
+        rs.append(r)
+    er.render_batch(rs, [3, 3, 3])
+    for r, (run, disp) in zip(rs, seq):
+        assert np.array_equal(r.running(), run)
+        assert np.array_equal(r.get_estimate(), disp)
+        r.close()
+    lib.er_set_list_capacity(32)
+
+
+def test_autotune_batch_restores_and_matches_sequential(fresh):
+    lib = capi.load()
+    lib.er_set_list_capacity(64)
+    scs = scenes.config_batch(n_jobs=4, scale=0.1, seed=6)
+    passes = [min(sc.passes, 6) for sc in scs]
+    seq = []
+    for sc, n in zip(scs, passes):
+        r = er.Renderer(sc, seed=8)
+        r.render_passes(n)
+        seq.append((r.running().copy(), r.get_estimate().copy(), r.get_seeds()))
+        r.close()
+    rs = [er.Renderer(sc, seed=8) for sc in scs]
+    t = er.autotune_batch(rs, rounds=2)
+    print(f"[tune] batch {t}", flush=True)
+    assert t["trials"] >= 10 and t["tile_w"] * t["tile_h"] == 64
+    for r in rs:
+        assert r.no_iterations() == 0
+        assert r.get_tune()["tile_w"] == t["tile_w"] and r.get_tune()["blocks_per_cu"] == t["blocks_per_cu"]
+    er.render_batch(rs, passes)
+    for r, (run, disp, (s1, s2)) in zip(rs, seq):
+        assert np.array_equal(r.running(), run)
+        assert np.array_equal(r.get_estimate(), disp)
+        g1, g2 = r.get_seeds()
+        assert np.array_equal(g1, s1) and np.array_equal(g2, s2)
+        r.close()
+    lib.er_set_list_capacity(32)
