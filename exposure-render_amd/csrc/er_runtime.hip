@@ -91,6 +91,17 @@ int set_error(int status, const char* fmt, ...)
     return status;
 }
 
+} /* namespace */
+
+/* the error message of the library's other translation units (not exported) */
+extern "C" __attribute__((visibility("hidden"))) int er_set_error_message(int status, const char* message)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    return set_error(status, "%s", message);
+}
+
+namespace {
+
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \

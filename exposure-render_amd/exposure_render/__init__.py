@@ -7,6 +7,7 @@ progressive passes, read the estimate back. Every call fails loudly when the nat
 missing or returns a non-zero status; there is no fallback implementation.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -242,6 +243,32 @@ class Renderer:
         for T in self.textures:
             lib.er_bind_texture(C.byref(T), 0)
         lib.er_bind_volume(C.byref(self.volume), 0)
+
+
+def load_mhd(path):
+    """A MetaImage volume (er_mhd_read) as (voxels[z, y, x] uint16, spacing (x, y, z), offset,
+    scale), with u16 = (file value - offset) * scale."""
+    lib = load()
+    res = (C.c_int * 3)()
+    sp = (C.c_float * 3)()
+    et = C.c_int()
+    b = os.fsencode(path)
+    _check(lib, lib.er_mhd_info(b, res, sp, C.byref(et)))
+    vox = np.empty((res[2], res[1], res[0]), dtype=np.uint16)
+    off, sc = C.c_float(), C.c_float()
+    _check(lib, lib.er_mhd_read(b, vox.ctypes.data_as(C.POINTER(C.c_uint16)), vox.size, C.byref(off), C.byref(sc)))
+    return vox, (sp[0], sp[1], sp[2]), off.value, sc.value
+
+
+def load_raw(path, resolution, element_type, msb=False, header_bytes=0):
+    """A headerless raw volume (er_raw_read) of resolution (x, y, z): (voxels, offset, scale)."""
+    lib = load()
+    res = (C.c_int * 3)(*resolution)
+    vox = np.empty((resolution[2], resolution[1], resolution[0]), dtype=np.uint16)
+    off, sc = C.c_float(), C.c_float()
+    _check(lib, lib.er_raw_read(os.fsencode(path), res, int(element_type), 1 if msb else 0, int(header_bytes),
+                                vox.ctypes.data_as(C.POINTER(C.c_uint16)), vox.size, C.byref(off), C.byref(sc)))
+    return vox, off.value, sc.value
 
 
 def autotune_batch(renderers, rounds=4):

@@ -179,6 +179,9 @@ class Tune(C.Structure):
 
 TUNE_FORCE = 1
 
+(ELEMENT_UINT8, ELEMENT_INT8, ELEMENT_UINT16, ELEMENT_INT16, ELEMENT_UINT32, ELEMENT_INT32, ELEMENT_FLOAT32,
+ ELEMENT_FLOAT64) = range(8)
+
 
 SIGNATURES = {
     "er_abi_version": (C.c_int, []),
@@ -219,6 +222,10 @@ SIGNATURES = {
     "er_autotune": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(Tune)]),
     "er_set_tune": (C.c_int, [C.c_int, _P(Tune)]),
     "er_autotune_batch": (C.c_int, [_P(BatchItem), C.c_int, C.c_int, _P(Tune)]),
+    "er_mhd_info": (C.c_int, [C.c_char_p, _P(C.c_int), _P(C.c_float), _P(C.c_int)]),
+    "er_mhd_read": (C.c_int, [C.c_char_p, _P(C.c_uint16), C.c_size_t, _P(C.c_float), _P(C.c_float)]),
+    "er_raw_read": (C.c_int, [C.c_char_p, _P(C.c_int), C.c_int, C.c_int, C.c_longlong, _P(C.c_uint16), C.c_size_t,
+                              _P(C.c_float), _P(C.c_float)]),
     "er_get_tune": (C.c_int, [C.c_int, _P(Tune)]),
     "er_autofocus_seeds": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(C.c_uint32), _P(C.c_uint32)]),
     "er_adjoint_lut": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),

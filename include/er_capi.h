@@ -439,6 +439,30 @@ ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t*
 ER_API int er_adjoint_value(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
                             const float* eval_lut, float* running_xyz);
 
+/* ---- volume ingestion (new: the reference leaves loading to its absent application) ---- */
+
+/* element types of a volume file */
+#define ER_ELEMENT_UINT8    0
+#define ER_ELEMENT_INT8     1
+#define ER_ELEMENT_UINT16   2
+#define ER_ELEMENT_INT16    3
+#define ER_ELEMENT_UINT32   4
+#define ER_ELEMENT_INT32    5
+#define ER_ELEMENT_FLOAT32  6
+#define ER_ELEMENT_FLOAT64  7
+
+/* MetaImage header (.mhd): resolution (x, y, z), spacing and element type. The data may be
+ * LOCAL or in a separate raw file; compressed data and file lists are not supported. */
+ER_API int er_mhd_info(const char* path, int resolution[3], float spacing[3], int* element_type);
+/* Reads a MetaImage volume into `voxels` (x fastest, what er_volume_desc.voxels takes).
+ * uint8 / uint16 values are kept; other integer types are shifted by their minimum, floating
+ * point is rescaled from [min, max] to [0, 65535]; u16 = (value - offset) * scale, clamped. */
+ER_API int er_mhd_read(const char* path, uint16_t* voxels, size_t count, float* offset, float* scale);
+/* The same for a headerless raw file of resolution[0] * [1] * [2] elements after
+ * header_bytes bytes (-1: the data is the end of the file); msb: big-endian elements. */
+ER_API int er_raw_read(const char* path, const int resolution[3], int element_type, int msb, long long header_bytes,
+                       uint16_t* voxels, size_t count, float* offset, float* scale);
+
 /* ---- synthetic phantoms (seeded, deterministic) --------------------------------------- */
 
 /* Isotropic Gaussian blobs plus uniform noise in [-noise, noise]; clamp to [0,65535], round.
