@@ -1079,17 +1079,12 @@ struct BatchJob {
     EventRec* events;
     ShadowJob* jobs;
     unsigned char* vis;
-    unsigned int* counts;       /* [0] scatter queue, [1] shadow jobs */
+    unsigned int* counts;       /* [0] scatter queue, [1] shadow jobs, [2] primary tiles claimed */
     int tiles_x, ntiles, tile_base;
     int post_tiles_x, post_ntiles, post_base;
     int passes, no_iter0;
 };
 
-/* device-side state of one buffer set of a batch: the shared primary-tile counter */
-struct BatchState {
-    unsigned int tiles;
-    unsigned int pad[3];
-};
 
 /* number of jobs still rendering in this round (a prefix of the table) */
 __device__ __forceinline__ int batch_active(const BatchJob* __restrict__ J, int n, unsigned int round)

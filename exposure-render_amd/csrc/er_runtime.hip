@@ -51,12 +51,12 @@ extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm,
 extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lut, int lut_max, float* out, int n,
                                              hipStream_t st);
 extern "C" hipError_t erd_launch_gm_max(const DevParams* p, unsigned int* out, hipStream_t st);
-extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, erd::BatchState* st, int full,
-                                             int primary_blocks, int shade_blocks, unsigned int round, hipStream_t stream,
+extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, int full, int primary_blocks,
+                                             int lds_words, int shade_blocks, unsigned int round, hipStream_t stream,
                                              hipStream_t stream2, hipEvent_t shaded);
 extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, unsigned int round, int total_tiles,
                                             hipStream_t stream);
-extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, erd::BatchState* st, hipStream_t stream);
+extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, hipStream_t stream);
 extern "C" size_t erd_adj_partials_per_block(void);
 extern "C" hipError_t erd_launch_primary_shade(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
                                                unsigned int* counts, void* events, void* jobs, int W, int H, int lut_lds,
@@ -1514,11 +1514,9 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         post_base += table[0][k].post_ntiles;
     }
     DevBuf<erd::BatchJob> dJ[2];
-    DevBuf<erd::BatchState> dS[2];
     for (int b = 0; b < 2; ++b) {
-        if ((st = dJ[b].alloc(nj)) || (st = dS[b].alloc(1))) return st;
+        if ((st = dJ[b].alloc(nj))) return st;
         HIP_TRY(hipMemcpy(dJ[b].ptr, table[b].data(), nj * sizeof(erd::BatchJob), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemset(dS[b].ptr, 0, sizeof(erd::BatchState)));
         for (int k = 0; k < nj; ++k) HIP_TRY(hipMemset(table[b][k].counts, 0, 4 * sizeof(uint32_t)));
     }
     if (!g_batch_stream) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream, hipStreamNonBlocking));
@@ -1526,7 +1524,18 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
     hipStream_t bs = g_batch_stream, bs2 = g_batch_stream2;
     const int shade_blocks = 2048;
     int batch_blocks = 0;   /* persistent primary workgroups: the largest any job asks for */
-    for (auto& j : jobs) batch_blocks = std::max(batch_blocks, primary_blocks(*j.second));
+    /* LDS per primary / shadow workgroup: the largest job's macrocell bits plus opacity table,
+     * as a single pass stages them; when some job's opacity table is too large for LDS, every
+     * job reads its tables from global memory (lds_words = 0) */
+    int lds_words = 0;
+    bool all_lds = true;
+    for (auto& j : jobs) {
+        const Tracer& t = *j.second;
+        batch_blocks = std::max(batch_blocks, primary_blocks(t));
+        lds_words = std::max(lds_words, (int)(t.lds_bytes / 4));
+        all_lds = all_lds && t.lut_lds;
+    }
+    if (!all_lds) lds_words = 0;
     const int rounds = jobs[0].first;
     /* all rounds as one graph (no memset nodes): per round, primary + shading on the first
      * stream, then shadow + combine + post + queue reset on the second; a set is reused two
@@ -1550,10 +1559,10 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         hipEvent_t shaded = nullptr;
         if (ec == hipSuccess) ec = new_event(shaded);
         if (ec == hipSuccess)
-            ec = erd_launch_batch_trace(dJ[b].ptr, nj, dS[b].ptr, full, batch_blocks, shade_blocks, (unsigned int)r, bs,
+            ec = erd_launch_batch_trace(dJ[b].ptr, nj, full, batch_blocks, lds_words, shade_blocks, (unsigned int)r, bs,
                                         bs2, shaded);
         if (ec == hipSuccess) ec = erd_launch_batch_post(dJ[b].ptr, nj, (unsigned int)r, post_base, bs2);
-        if (ec == hipSuccess) ec = erd_launch_round_end(dJ[b].ptr, nj, dS[b].ptr, bs2);
+        if (ec == hipSuccess) ec = erd_launch_round_end(dJ[b].ptr, nj, bs2);
         if (ec == hipSuccess) ec = new_event(posted[b]);
         if (ec == hipSuccess) ec = hipEventRecord(posted[b], bs2);
     }

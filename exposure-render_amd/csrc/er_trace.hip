@@ -378,30 +378,78 @@ __device__ __forceinline__ Tables global_tables(const DevParams& p)
     return T;
 }
 
-/* Primary kernel of a batch round: persistent waves take 8 x 8 tiles from one counter over
- * the tiles of every active job; a wave finds its tile's job in the table. */
-template<int FULL>
-__global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ J, int njobs, BatchState* __restrict__ st,
-                                                  unsigned int round)
+/* A job's tables for a batch kernel: LDS = 1 copies the macrocell bits and the opacity table
+ * into the workgroup's LDS (the host launches this form only when every job's tables fit),
+ * LDS = 0 reads both from global memory. Called by the whole workgroup after every wave has
+ * finished with the previous job's tables. */
+template<int LDS>
+__device__ __forceinline__ Tables stage_job_tables(const DevParams& p, uint32_t* lds)
 {
+    if (!LDS) return global_tables(p);
+    const int nw = p.mc_words;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = gmem(p.mc_bits)[k];
+    float* lut = (float*)(lds + nw);
+    for (int k = threadIdx.x; k <= p.lut_max; k += blockDim.x) lut[k] = gmem(p.sigma_lut)[k];
+    __syncthreads();
+    Tables T;
+    T.bits = lds;
+    T.bits_glb = nullptr;
+    T.lut_lds = lut;
+    T.lut_glb = nullptr;
+    return T;
+}
+
+/* The next job after j (cyclically) whose work counter counts[which] is still below its item
+ * count (ntiles for tiles, counts[1] for shadow jobs); -1 when every item is claimed. Counters
+ * only grow, so a stale read can only send a workgroup to a job that has just run dry. */
+__device__ __forceinline__ int next_job_with_work(const BatchJob* __restrict__ J, int na, int j, int which)
+{
+    for (int k = 1; k <= na; ++k) {
+        const int c = (j + k) % na;
+        const unsigned int claimed = __hip_atomic_load(&J[c].counts[which], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int items = which == 2 ? (unsigned int)J[c].ntiles
+                                              : __hip_atomic_load(&J[c].counts[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (claimed < items) return c;
+    }
+    return -1;
+}
+
+/* Primary kernel of a batch round. Workgroups go job by job, so that the job's tables can sit
+ * in LDS as in a single pass: a workgroup starts at the job that holds its proportional share
+ * of the round's tiles, its waves take that job's 64-pixel tiles from the job's own counter
+ * (counts[2]), and when the job has no tiles left the workgroup moves on to the next job that
+ * has (a counter only grows, so a workgroup leaves only when every tile is claimed). */
+template<int FULL, int LDS>
+__global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ int next_job;
     const int lane = threadIdx.x & 63;
     const int na = batch_active(J, njobs, round);
     if (na == 0) return;
     const unsigned int total = (unsigned int)(J[na - 1].tile_base + J[na - 1].ntiles);
+    const unsigned int pos = (unsigned int)(((unsigned long long)blockIdx.x * total) / gridDim.x);
+    int j = 0;
+    while (j < na - 1 && pos >= (unsigned int)(J[j].tile_base + J[j].ntiles)) ++j;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int paths = 0;
-    int j = 0;
     for (;;) {
-        unsigned int tile = 0;
-        if (lane == 0) tile = atomicAdd(&st->tiles, 1u);
-        tile = __shfl(tile, 0);
-        if (tile >= total) break;
-        while (tile >= (unsigned int)(J[j].tile_base + J[j].ntiles)) ++j;   /* tiles arrive in increasing order */
         const BatchJob& B = J[j];
         const DevParams& p = *B.params;
-        const Tables T = global_tables(p);
-        primary_tile<FULL, 0>(p, T, tile - (unsigned int)B.tile_base, B.tiles_x, lane, B.frame, B.seeds1, B.seeds2,
-                              B.queue, &B.counts[0], cnt, paths);
+        const Tables T = stage_job_tables<LDS>(p, lds);
+        for (;;) {
+            unsigned int tile = 0;
+            if (lane == 0) tile = atomicAdd(&B.counts[2], 1u);
+            tile = __shfl(tile, 0);
+            if (tile >= (unsigned int)B.ntiles) break;
+            primary_tile<FULL, 0>(p, T, tile, B.tiles_x, lane, B.frame, B.seeds1, B.seeds2, B.queue, &B.counts[0], cnt,
+                                  paths);
+        }
+        __syncthreads();   /* every wave is done with this job's tables */
+        if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 2);
+        __syncthreads();
+        j = next_job;
+        if (j < 0) break;
     }
 }
 
@@ -437,21 +485,45 @@ __global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J,
     }
 }
 
-template<int FULL>
+/* Shadow marches of a batch round, job by job like k_primary_b: a workgroup starts at the job
+ * holding its proportional share of the round's shadow jobs, its waves claim 64 of that job's
+ * shadow jobs at a time (counts[3]), then it moves on to the next job with shadow jobs left. */
+template<int FULL, int LDS>
 __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
 {
+    extern __shared__ uint32_t lds[];
+    __shared__ int next_job;
     const int na = batch_active(J, njobs, round);
+    if (na == 0) return;
     const int lane = threadIdx.x & 63;
-    const unsigned int waves = gridDim.x * (blockDim.x >> 6);
+    unsigned int total = 0;
+    for (int k = 0; k < na; ++k) total += J[k].counts[1];
+    if (total == 0) return;
+    const unsigned int pos = (unsigned int)(((unsigned long long)blockIdx.x * total) / gridDim.x);
+    int j = 0;
+    for (unsigned int base = 0; j < na - 1; ++j) {
+        base += J[j].counts[1];
+        if (pos < base) break;
+    }
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);; w += waves) {
-        int job; unsigned int j0, n;
-        if (!batch_slot(J, na, 1, w, job, j0, n)) break;
-        const BatchJob& B = J[job];
+    for (;;) {
+        const BatchJob& B = J[j];
         const DevParams& p = *B.params;
-        const Tables T = global_tables(p);
-        const unsigned int jq = j0 + lane;
-        if (jq < n) shadow_item<FULL, 0>(p, T, B.jobs[jq], B.vis, cnt);
+        const Tables T = stage_job_tables<LDS>(p, lds);
+        const unsigned int n = B.counts[1];
+        for (;;) {
+            unsigned int first = 0;
+            if (lane == 0) first = atomicAdd(&B.counts[3], 64u);
+            first = __shfl(first, 0);
+            if (first >= n) break;
+            const unsigned int jq = first + lane;
+            if (jq < n) shadow_item<FULL, 0>(p, T, B.jobs[jq], B.vis, cnt);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 3);
+        __syncthreads();
+        j = next_job;
+        if (j < 0) break;
     }
 }
 
@@ -470,13 +542,14 @@ __global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J
 }
 
 /* end of a round: empty the queues of this buffer set and its tile counter (one workgroup) */
-__global__ __launch_bounds__(64) void k_round_end(const BatchJob* __restrict__ J, int njobs, BatchState* __restrict__ st)
+__global__ __launch_bounds__(64) void k_round_end(const BatchJob* __restrict__ J, int njobs)
 {
     for (int k = threadIdx.x; k < njobs; k += blockDim.x) {
         J[k].counts[0] = 0u;
         J[k].counts[1] = 0u;
+        J[k].counts[2] = 0u;
+        J[k].counts[3] = 0u;
     }
-    if (threadIdx.x == 0) st->tiles = 0u;
 }
 
 } /* namespace erd */
@@ -548,29 +621,37 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
 
 /* One batch round, split like a single pass: primary and shading on st; shadow marches and
  * combine on st2 after the event `shaded`. */
-extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, erd::BatchState* st, int full,
-                                             int primary_blocks, int shade_blocks, unsigned int round, hipStream_t stream,
+extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, int full, int primary_blocks,
+                                             int lds_words, int shade_blocks, unsigned int round, hipStream_t stream,
                                              hipStream_t stream2, hipEvent_t shaded)
 {
+    /* lds_words > 0: every job's tables fit in that many words of LDS */
+    const size_t lds = (size_t)lds_words * 4;
+#define ER_PRIMARY_B(F, L) \
+    hipLaunchKernelGGL((erd::k_primary_b<F, L>), dim3(primary_blocks), dim3(256), lds, stream, J, njobs, round)
     if (full) {
-        hipLaunchKernelGGL(erd::k_primary_b<1>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st, round);
+        if (lds_words > 0) ER_PRIMARY_B(1, 1); else ER_PRIMARY_B(1, 0);
         hipLaunchKernelGGL(erd::k_shade_b<1>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, round);
     } else {
-        hipLaunchKernelGGL(erd::k_primary_b<0>, dim3(primary_blocks), dim3(256), 0, stream, J, njobs, st, round);
+        if (lds_words > 0) ER_PRIMARY_B(0, 1); else ER_PRIMARY_B(0, 0);
         hipLaunchKernelGGL(erd::k_shade_b<0>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, round);
     }
+#undef ER_PRIMARY_B
     hipError_t e = hipEventRecord(shaded, stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(stream2, shaded, 0);
     if (e != hipSuccess) return e;
-    if (full) hipLaunchKernelGGL(erd::k_shadow_b<1>, dim3(2 * shade_blocks), dim3(256), 0, stream2, J, njobs, round);
-    else hipLaunchKernelGGL(erd::k_shadow_b<0>, dim3(2 * shade_blocks), dim3(256), 0, stream2, J, njobs, round);
+#define ER_SHADOW_B(F, L) \
+    hipLaunchKernelGGL((erd::k_shadow_b<F, L>), dim3(primary_blocks), dim3(256), lds, stream2, J, njobs, round)
+    if (full) { if (lds_words > 0) ER_SHADOW_B(1, 1); else ER_SHADOW_B(1, 0); }
+    else { if (lds_words > 0) ER_SHADOW_B(0, 1); else ER_SHADOW_B(0, 0); }
+#undef ER_SHADOW_B
     hipLaunchKernelGGL(erd::k_finish_b, dim3(shade_blocks), dim3(256), 0, stream2, J, njobs, round);
     return hipGetLastError();
 }
 
-extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, erd::BatchState* st, hipStream_t stream)
+extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, hipStream_t stream)
 {
-    hipLaunchKernelGGL(erd::k_round_end, dim3(1), dim3(64), 0, stream, J, njobs, st);
+    hipLaunchKernelGGL(erd::k_round_end, dim3(1), dim3(64), 0, stream, J, njobs);
     return hipGetLastError();
 }
 
