@@ -223,6 +223,8 @@ def main():
     ap.add_argument("--cpu-passes", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched section (its own JSON line; for kernel profiles of the batch)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy a device buffer of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -230,6 +232,11 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the tracer has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.batched_only:
+        import exposure_render as er
+        from exposure_render import scenes
+        print(json.dumps({"batched": bench_batched(er, scenes, args.batch_jobs)}))
+        return
     if args.copy_probe:
         import torch
         a = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda").fill_(1.0)

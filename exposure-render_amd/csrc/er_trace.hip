@@ -440,7 +440,7 @@ __global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ 
         for (;;) {
             unsigned int tile = 0;
             if (lane == 0) tile = atomicAdd(&B.counts[2], 1u);
-            tile = __shfl(tile, 0);
+            tile = __builtin_amdgcn_readfirstlane(tile);
             if (tile >= (unsigned int)B.ntiles) break;
             primary_tile<FULL, 0>(p, T, tile, B.tiles_x, lane, B.frame, B.seeds1, B.seeds2, B.queue, &B.counts[0], cnt,
                                   paths);
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ 
         __syncthreads();   /* every wave is done with this job's tables */
         if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 2);
         __syncthreads();
-        j = next_job;
+        j = __builtin_amdgcn_readfirstlane(next_job);   /* uniform: the job's fields load as scalars */
         if (j < 0) break;
     }
 }
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J,
     const int lane = threadIdx.x & 63;
     const unsigned int waves = gridDim.x * (blockDim.x >> 6);
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);; w += waves) {
+    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);; w += waves) {
         int job; unsigned int q0, n;
         if (!batch_slot(J, na, 0, w, job, q0, n)) break;
         const BatchJob& B = J[job];
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J
         for (;;) {
             unsigned int first = 0;
             if (lane == 0) first = atomicAdd(&B.counts[3], 64u);
-            first = __shfl(first, 0);
+            first = __builtin_amdgcn_readfirstlane(first);
             if (first >= n) break;
             const unsigned int jq = first + lane;
             if (jq < n) shadow_item<FULL, 0>(p, T, B.jobs[jq], B.vis, cnt);
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J
         __syncthreads();
         if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 3);
         __syncthreads();
-        j = next_job;
+        j = __builtin_amdgcn_readfirstlane(next_job);   /* uniform: the job's fields load as scalars */
         if (j < 0) break;
     }
 }
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J
     const int na = batch_active(J, njobs, round);
     const int lane = threadIdx.x & 63;
     const unsigned int waves = gridDim.x * (blockDim.x >> 6);
-    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);; w += waves) {
+    for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);; w += waves) {
         int job; unsigned int q0, n;
         if (!batch_slot(J, na, 0, w, job, q0, n)) break;
         const BatchJob& B = J[job];
