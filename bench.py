@@ -16,6 +16,7 @@ Also reported, each on its own workload:
   rmse          tonemapped running estimate vs the FP64 oracle's expectation (fixture);
   cpu_baseline  the CPU oracle (the reference estimator restated, FP32, OpenMP) on the host;
   small         the 64^3 blob workload (256 x 256, 16 passes): Mpaths/s, RMSE, GPU / CPU;
+  large         the 512 x 512 x 460 clipped volume at 1920 x 1080, 256 passes: Mpaths/s;
   adjoint       the differentiation workload (128^3, 512 x 512, 32 passes): forward and
                 path-replay adjoint wall times;
   batched       64 jobs of uneven size: one er_render_batch vs the same jobs one after another.
@@ -117,6 +118,22 @@ def bench_small(er, scenes, threads, no_cpu):
         out["cpu_mpaths_per_s"] = round(v, 4)
         out["cpu_threads"] = threads
         out["gpu_over_cpu"] = round(value / v, 1)
+    r.close()
+    return out
+
+
+def bench_large(er, scenes):
+    """The large anisotropic workload (512 x 512 x 460, 0.7 x 0.7 x 1.0 spacing, clipping box,
+    1920 x 1080, 3 plane lights, hybrid shading): throughput over its 256 passes. No RMSE
+    gate here: an FP64 reference at this size is beyond a bounded CPU run."""
+    sc = scenes.config_large()
+    r = er.Renderer(sc)
+    r.seed(99)
+    r.render_passes(2)
+    r.rebind()
+    dt, t = timed_passes(er, r, sc.passes)
+    out = {"workload": "tubes_512x512x460_1920x1080_clip", "passes": sc.passes,
+           "mpaths_per_s": round(r.W * r.H * sc.passes / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / sc.passes, 4)}
     r.close()
     return out
 
@@ -320,6 +337,7 @@ def main():
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     if not args.no_extra:
         out["small"] = bench_small(er, scenes, threads, args.no_cpu)
+        out["large"] = bench_large(er, scenes)
         out["adjoint"] = bench_adjoint(er, scenes)
         out["batched"] = bench_batched(er, scenes, args.batch_jobs)
     er.shutdown()

@@ -172,6 +172,148 @@ __device__ __forceinline__ bool march(const DevParams& p, const Tables& T, const
     return march_v<FULL, COUNT, K, NoVisit>(p, T, r, primary, rng, Tout, Pout, cnt, nv, -1.0f);
 }
 
+/* The state of a ray-march between iterations of its loop (raymarching.h:55-68). */
+struct MarchState {
+    f3 O, D;          /* the ray */
+    float MinT, MaxT; /* next sample position, end of the volume segment */
+    float Sum, S;     /* optical depth so far / DensityScale, threshold */
+};
+
+/* Set-up of the fixed-step march (raymarching.h:38-53) after the box test: the threshold
+ * S = -log(u) / DensityScale, then the jitter of the first sample (two draws). S_fixed >= 0
+ * replaces the threshold after it is drawn. False when the march cannot advance (zero-size
+ * volume, Q2, or a step below the spacing of floats at MaxT): the reference would not
+ * terminate; it is reported as no event. */
+__device__ __forceinline__ bool march_begin(const DevParams& p, const Ray& r, float nearT, float farT, float StepSize, Rng& rng,
+                                            float S_fixed, MarchState& m)
+{
+    m.O = r.O;
+    m.D = r.D;
+    m.MinT = fmaxf(nearT, r.MinT);
+    m.MaxT = fminf(farT, r.MaxT);
+    m.S = gmem(p.neglog)[rng.next_bits()] / p.density_scale;
+    if (S_fixed >= 0.0f) m.S = S_fixed;
+    m.MinT += rng.u() * StepSize;
+    m.Sum = 0.0f;
+    return StepSize > 0.0f && m.MaxT + StepSize > m.MaxT;
+}
+
+/* One iteration of the march loop: up to K consecutive samples in an occupied macrocell, or
+ * the skip over the rest of an empty one. Returns 1 on an event (Pout = the sample, m.MinT one
+ * step beyond it, Q13), -1 when the ray leaves the segment first (the primary march ends at
+ * t >= tmax, the shadow march at t > tmax, Q14), 0 to go on. */
+template<int FULL, int COUNT, int K, class Visit>
+__device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, MarchState& m, bool primary, float StepSize,
+                                          Cnt& cnt, Visit& vis, f3& Pout)
+{
+    const int sh = p.mc_shift;
+    const f3 Ps = add(m.O, scl(m.MinT, m.D));
+    if (primary ? (m.MinT >= m.MaxT) : (m.MinT > m.MaxT)) return -1;
+    if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
+    const Lattice l = to_lattice(p, Ps);
+    const int mxi = mc_axis(l.Lx, sh, p.mc_dim[0]);
+    const int myi = mc_axis(l.Ly, sh, p.mc_dim[1]);
+    const int mzi = mc_axis(l.Lz, sh, p.mc_dim[2]);
+    const int mc = (mzi * p.mc_dim[1] + myi) * p.mc_dim[0] + mxi;
+    if (K == 1 && mc_occupied(T, mc)) {
+        float SigmaT = 0.0f;
+        if (!(FULL && p.clip_count > 0 && clipped(p, Ps))) {
+            if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
+            const int iv = (int)trilinear_cell(p, l);
+            SigmaT = T.lut(min(iv, p.lut_max));
+            vis.sample(iv, SigmaT);
+        }
+        m.Sum += SigmaT * StepSize;
+        m.MinT += StepSize;
+        if (!(m.Sum < m.S)) { Pout = Ps; return 1; }
+        return 0;
+    }
+    if (K > 1 && mc_occupied(T, mc)) {
+        /* K consecutive samples at once: their positions follow from the same sequence of
+         * MinT additions and do not depend on the values, so all K cell loads are issued
+         * before the first value is used. The values are then consumed in order, with the
+         * loop's own end and threshold tests between them; loads past the point where the
+         * loop stops are discarded (their addresses are clamped into the cell array). A
+         * sample in an empty macrocell has sigma exactly 0 and is consumed without a fetch. */
+        CellFetch f[K];
+        bool live[K];
+        float tk[K];
+        int kn = K;
+        tk[0] = m.MinT;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            f3 Pk = Ps;
+            Lattice lk = l;
+            bool occ = true;
+            if (k > 0) {
+                tk[k] = tk[k - 1] + StepSize;
+                if (kn == K && (primary ? (tk[k] >= m.MaxT) : (tk[k] > m.MaxT))) kn = k;
+                Pk = add(m.O, scl(tk[k], m.D));
+                lk = to_lattice(p, Pk);
+                occ = mc_occupied(T, (mc_axis(lk.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(lk.Ly, sh, p.mc_dim[1])) * p.mc_dim[0] +
+                                         mc_axis(lk.Lx, sh, p.mc_dim[0]));
+            }
+            live[k] = occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
+            f[k] = cell_fetch(p, lk);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k > 0) {
+                if (k >= kn) return -1;
+                if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
+            }
+            float SigmaT = 0.0f;
+            if (live[k]) {
+                if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
+                const int iv = (int)cell_value(f[k]);
+                SigmaT = T.lut(min(iv, p.lut_max));
+                vis.sample(iv, SigmaT);
+            }
+            m.Sum += SigmaT * StepSize;
+            m.MinT += StepSize;
+            if (!(m.Sum < m.S)) { Pout = add(m.O, scl(tk[k], m.D)); return 1; }
+        }
+        return 0;
+    }
+    /* Empty macrocell: this sample adds exactly 0. Skip the following samples that stay in
+     * the same macrocell without computing them, keeping the exact sequence of MinT
+     * additions. Each operation of P = O + t D, (P - Min) * InvSize * Res is monotone in t,
+     * so if the first and the last skipped sample lie in this macrocell's lattice box,
+     * every sample between them does too; the last one is checked with the same
+     * expression as above, and the skip is undone if it left the box. The exit estimate
+     * uses the lattice-space direction and hardware reciprocals (approximate: the skip is
+     * verified). */
+    const float Ldx = (m.D.x * p.invsize[0]) * p.resf[0];
+    const float Ldy = (m.D.y * p.invsize[1]) * p.resf[1];
+    const float Ldz = (m.D.z * p.invsize[2]) * p.resf[2];
+    const float iLdx = __builtin_amdgcn_rcpf(Ldx), iLdy = __builtin_amdgcn_rcpf(Ldy), iLdz = __builtin_amdgcn_rcpf(Ldz);
+    const float margin = 1.0e-3f * __builtin_amdgcn_rcpf(fmaxf(fabsf(Ldx), fmaxf(fabsf(Ldy), fabsf(Ldz))));
+    m.MinT += StepSize;
+    const float big = 3.0e38f;
+    const float lo_x = mxi == 0 ? -big : (float)(mxi << sh), hi_x = mxi == p.mc_dim[0] - 1 ? big : (float)((mxi + 1) << sh);
+    const float lo_y = myi == 0 ? -big : (float)(myi << sh), hi_y = myi == p.mc_dim[1] - 1 ? big : (float)((myi + 1) << sh);
+    const float lo_z = mzi == 0 ? -big : (float)(mzi << sh), hi_z = mzi == p.mc_dim[2] - 1 ? big : (float)((mzi + 1) << sh);
+    const float tx = Ldx > 0.0f ? (hi_x - l.Lx) * iLdx : (Ldx < 0.0f ? (lo_x - l.Lx) * iLdx : big);
+    const float ty = Ldy > 0.0f ? (hi_y - l.Ly) * iLdy : (Ldy < 0.0f ? (lo_y - l.Ly) * iLdy : big);
+    const float tz = Ldz > 0.0f ? (hi_z - l.Lz) * iLdz : (Ldz < 0.0f ? (lo_z - l.Lz) * iLdz : big);
+    const float dt_exit = fminf(tx, fminf(ty, tz)) - margin;
+    const float t_lim = fminf((m.MinT - StepSize) + dt_exit, m.MaxT);
+    const float t_first = m.MinT;
+    float t_prev = m.MinT;
+    uint32_t n = skip_additions(m.MinT, StepSize, t_lim, t_prev);
+    if (n > 0) {
+        const Lattice lp = to_lattice(p, add(m.O, scl(t_prev, m.D)));
+        if (mc_axis(lp.Lx, sh, p.mc_dim[0]) != mxi || mc_axis(lp.Ly, sh, p.mc_dim[1]) != myi ||
+            mc_axis(lp.Lz, sh, p.mc_dim[2]) != mzi) {
+            m.MinT = t_first;
+            n = 0;
+            if (COUNT) cnt.undone++;
+        }
+    }
+    if (COUNT) { if (primary) cnt.primary += n; else cnt.shadow += n; cnt.skipped += n; }
+    return 0;
+}
+
 /* S_fixed >= 0: the threshold is still drawn (same draws) but S_fixed replaces it, so the march
  * reads every sample until that optical depth / DensityScale or the end of the ray (the
  * expected-transmittance form of a shadow test). */
@@ -181,14 +323,12 @@ __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, con
 {
     float nearT, farT;
     if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return false;
-    float MinT = fmaxf(nearT, r.MinT);
-    const float MaxT = fminf(farT, r.MaxT);
-
     if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
         /* delta tracking against the global majorant DS^2 * max opacity */
         const float mu = p.woodcock_majorant;
         if (!(mu > 0.0f)) return false;
-        float t = MinT;
+        float t = fmaxf(nearT, r.MinT);
+        const float MaxT = fminf(farT, r.MaxT);
         for (;;) {
             t += gmem(p.neglog)[rng.next_bits()] / mu;
             if (primary ? (t >= MaxT) : (t > MaxT)) return false;
@@ -198,129 +338,16 @@ __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, con
             if (rng.u() * mu < sigma) { Tout = t; Pout = Ps; return true; }
         }
     }
-
-    float S = gmem(p.neglog)[rng.next_bits()] / p.density_scale;
-    if (S_fixed >= 0.0f) S = S_fixed;
     const float StepSize = (primary ? p.step_primary : p.step_shadow) * p.minstep;
     vis.begin(StepSize);
-    MinT += rng.u() * StepSize;
-    /* a march that could not advance (zero-size volume, Q2, or a step below the spacing of
-     * floats at MaxT) would not terminate in the reference; it is reported as no event */
-    if (!(StepSize > 0.0f) || !(MaxT + StepSize > MaxT)) return false;
-    /* lattice-space direction, used only to estimate where the ray leaves a macrocell */
-    const float Ldx = (r.D.x * p.invsize[0]) * p.resf[0];
-    const float Ldy = (r.D.y * p.invsize[1]) * p.resf[1];
-    const float Ldz = (r.D.z * p.invsize[2]) * p.resf[2];
-    const float Lmax = fmaxf(fabsf(Ldx), fmaxf(fabsf(Ldy), fabsf(Ldz)));
-    /* reciprocals for the exit estimate only (hardware approximation: the skip is verified) */
-    const float iLdx = __builtin_amdgcn_rcpf(Ldx), iLdy = __builtin_amdgcn_rcpf(Ldy), iLdz = __builtin_amdgcn_rcpf(Ldz);
-    const float margin = 1.0e-3f * __builtin_amdgcn_rcpf(Lmax);
-    const int sh = p.mc_shift;
-    float Sum = 0.0f;
-    f3 Ps = mk(0.0f, 0.0f, 0.0f);
-    while (Sum < S) {
-        Ps = add(r.O, scl(MinT, r.D));
-        if (primary ? (MinT >= MaxT) : (MinT > MaxT)) return false;
-        if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
-        const Lattice l = to_lattice(p, Ps);
-        const int mxi = mc_axis(l.Lx, sh, p.mc_dim[0]);
-        const int myi = mc_axis(l.Ly, sh, p.mc_dim[1]);
-        const int mzi = mc_axis(l.Lz, sh, p.mc_dim[2]);
-        const int mc = (mzi * p.mc_dim[1] + myi) * p.mc_dim[0] + mxi;
-        if (K == 1 && mc_occupied(T, mc)) {
-            float SigmaT = 0.0f;
-            if (!(FULL && p.clip_count > 0 && clipped(p, Ps))) {
-                if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-                const int iv = (int)trilinear_cell(p, l);
-                SigmaT = T.lut(min(iv, p.lut_max));
-                vis.sample(iv, SigmaT);
-            }
-            Sum += SigmaT * StepSize;
-            MinT += StepSize;
-            continue;
-        }
-        if (K > 1 && mc_occupied(T, mc)) {
-            /* K consecutive samples at once: their positions follow from the same sequence of
-             * MinT additions and do not depend on the values, so all K cell loads are issued
-             * before the first value is used. The values are then consumed in order, with the
-             * loop's own end and threshold tests between them; loads past the point where the
-             * loop stops are discarded (their addresses are clamped into the cell array). A
-             * sample in an empty macrocell has sigma exactly 0 and is consumed without a fetch. */
-            CellFetch f[K];
-            bool live[K];
-            float tk[K];
-            int kn = K;
-            tk[0] = MinT;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                f3 Pk = Ps;
-                Lattice lk = l;
-                bool occ = true;
-                if (k > 0) {
-                    tk[k] = tk[k - 1] + StepSize;
-                    if (kn == K && (primary ? (tk[k] >= MaxT) : (tk[k] > MaxT))) kn = k;
-                    Pk = add(r.O, scl(tk[k], r.D));
-                    lk = to_lattice(p, Pk);
-                    occ = mc_occupied(T, (mc_axis(lk.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(lk.Ly, sh, p.mc_dim[1])) * p.mc_dim[0] +
-                                             mc_axis(lk.Lx, sh, p.mc_dim[0]));
-                }
-                live[k] = occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
-                f[k] = cell_fetch(p, lk);
-            }
-            bool done = false;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (k > 0) {
-                    if (k >= kn) return false;
-                    if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
-                }
-                float SigmaT = 0.0f;
-                if (live[k]) {
-                    if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-                    const int iv = (int)cell_value(f[k]);
-                    SigmaT = T.lut(min(iv, p.lut_max));
-                    vis.sample(iv, SigmaT);
-                }
-                Sum += SigmaT * StepSize;
-                MinT += StepSize;
-                if (!(Sum < S)) { Ps = add(r.O, scl(tk[k], r.D)); done = true; break; }
-            }
-            if (done) break;
-            continue;
-        }
-        /* Empty macrocell: this sample adds exactly 0. Skip the following samples that stay in
-         * the same macrocell without computing them, keeping the exact sequence of MinT
-         * additions. Each operation of P = O + t D, (P - Min) * InvSize * Res is monotone in t,
-         * so if the first and the last skipped sample lie in this macrocell's lattice box,
-         * every sample between them does too; the last one is checked with the same
-         * expression as above, and the skip is undone if it left the box. */
-        MinT += StepSize;
-        const float big = 3.0e38f;
-        const float lo_x = mxi == 0 ? -big : (float)(mxi << sh), hi_x = mxi == p.mc_dim[0] - 1 ? big : (float)((mxi + 1) << sh);
-        const float lo_y = myi == 0 ? -big : (float)(myi << sh), hi_y = myi == p.mc_dim[1] - 1 ? big : (float)((myi + 1) << sh);
-        const float lo_z = mzi == 0 ? -big : (float)(mzi << sh), hi_z = mzi == p.mc_dim[2] - 1 ? big : (float)((mzi + 1) << sh);
-        const float tx = Ldx > 0.0f ? (hi_x - l.Lx) * iLdx : (Ldx < 0.0f ? (lo_x - l.Lx) * iLdx : big);
-        const float ty = Ldy > 0.0f ? (hi_y - l.Ly) * iLdy : (Ldy < 0.0f ? (lo_y - l.Ly) * iLdy : big);
-        const float tz = Ldz > 0.0f ? (hi_z - l.Lz) * iLdz : (Ldz < 0.0f ? (lo_z - l.Lz) * iLdz : big);
-        const float dt_exit = fminf(tx, fminf(ty, tz)) - margin;
-        const float t_lim = fminf((MinT - StepSize) + dt_exit, MaxT);
-        const float t_first = MinT;
-        float t_prev = MinT;
-        uint32_t n = skip_additions(MinT, StepSize, t_lim, t_prev);
-        if (n > 0) {
-            const Lattice lp = to_lattice(p, add(r.O, scl(t_prev, r.D)));
-            if (mc_axis(lp.Lx, sh, p.mc_dim[0]) != mxi || mc_axis(lp.Ly, sh, p.mc_dim[1]) != myi ||
-                mc_axis(lp.Lz, sh, p.mc_dim[2]) != mzi) {
-                MinT = t_first;
-                n = 0;
-                if (COUNT) cnt.undone++;
-            }
-        }
-        if (COUNT) { if (primary) cnt.primary += n; else cnt.shadow += n; cnt.skipped += n; }
+    MarchState m;
+    if (!march_begin(p, r, nearT, farT, StepSize, rng, S_fixed, m)) return false;
+    if (!(m.Sum < m.S)) { Tout = m.MinT; Pout = mk(0.0f, 0.0f, 0.0f); return true; }
+    for (;;) {
+        const int st = march_step<FULL, COUNT, K, Visit>(p, T, m, primary, StepSize, cnt, vis, Pout);
+        if (st > 0) { Tout = m.MinT; return true; }
+        if (st < 0) return false;
     }
-    Tout = MinT;
-    Pout = Ps;
-    return true;
 }
 
 /* ---- shapes (plane.h:41-88, disk.h, ring.h, box.h:51-83, sphere.h:84-140) ------------ */

@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ p
     for (;;) {
         unsigned int tile = 0;
         if (lane == 0) tile = atomicAdd(&counts[2], 1u);
-        tile = __shfl(tile, 0);
+        tile = __builtin_amdgcn_readfirstlane(tile);
         if (tile >= (unsigned int)ntiles) break;
         primary_tile<FULL, COUNT>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths);
     }
