@@ -289,10 +289,19 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
     const float iLdx = __builtin_amdgcn_rcpf(Ldx), iLdy = __builtin_amdgcn_rcpf(Ldy), iLdz = __builtin_amdgcn_rcpf(Ldz);
     const float margin = 1.0e-3f * __builtin_amdgcn_rcpf(fmaxf(fabsf(Ldx), fmaxf(fabsf(Ldy), fabsf(Ldz))));
     m.MinT += StepSize;
+    /* the box to skip: the macrocell, or the whole group of 4 x 4 x 4 macrocells around it when
+     * the group is empty (the same argument holds for any lattice-aligned box) */
+    int bsh = sh, bx = mxi, by = myi, bz = mzi, nx = p.mc_dim[0], ny = p.mc_dim[1], nz = p.mc_dim[2];
+    {
+        const int gx = mxi >> 2, gy = myi >> 2, gz = mzi >> 2;
+        if (!mc_occupied(T, (p.mc2_base << 5) + (gz * p.mc2_dim[1] + gy) * p.mc2_dim[0] + gx)) {
+            bsh = sh + 2; bx = gx; by = gy; bz = gz; nx = p.mc2_dim[0]; ny = p.mc2_dim[1]; nz = p.mc2_dim[2];
+        }
+    }
     const float big = 3.0e38f;
-    const float lo_x = mxi == 0 ? -big : (float)(mxi << sh), hi_x = mxi == p.mc_dim[0] - 1 ? big : (float)((mxi + 1) << sh);
-    const float lo_y = myi == 0 ? -big : (float)(myi << sh), hi_y = myi == p.mc_dim[1] - 1 ? big : (float)((myi + 1) << sh);
-    const float lo_z = mzi == 0 ? -big : (float)(mzi << sh), hi_z = mzi == p.mc_dim[2] - 1 ? big : (float)((mzi + 1) << sh);
+    const float lo_x = bx == 0 ? -big : (float)(bx << bsh), hi_x = bx == nx - 1 ? big : (float)((bx + 1) << bsh);
+    const float lo_y = by == 0 ? -big : (float)(by << bsh), hi_y = by == ny - 1 ? big : (float)((by + 1) << bsh);
+    const float lo_z = bz == 0 ? -big : (float)(bz << bsh), hi_z = bz == nz - 1 ? big : (float)((bz + 1) << bsh);
     const float tx = Ldx > 0.0f ? (hi_x - l.Lx) * iLdx : (Ldx < 0.0f ? (lo_x - l.Lx) * iLdx : big);
     const float ty = Ldy > 0.0f ? (hi_y - l.Ly) * iLdy : (Ldy < 0.0f ? (lo_y - l.Ly) * iLdy : big);
     const float tz = Ldz > 0.0f ? (hi_z - l.Lz) * iLdz : (Ldz < 0.0f ? (lo_z - l.Lz) * iLdz : big);
@@ -303,8 +312,7 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
     uint32_t n = skip_additions(m.MinT, StepSize, t_lim, t_prev);
     if (n > 0) {
         const Lattice lp = to_lattice(p, add(m.O, scl(t_prev, m.D)));
-        if (mc_axis(lp.Lx, sh, p.mc_dim[0]) != mxi || mc_axis(lp.Ly, sh, p.mc_dim[1]) != myi ||
-            mc_axis(lp.Lz, sh, p.mc_dim[2]) != mzi) {
+        if (mc_axis(lp.Lx, bsh, nx) != bx || mc_axis(lp.Ly, bsh, ny) != by || mc_axis(lp.Lz, bsh, nz) != bz) {
             m.MinT = t_first;
             n = 0;
             if (COUNT) cnt.undone++;

@@ -214,6 +214,30 @@ __global__ __launch_bounds__(256) void k_macro_sigma(const uint32_t* __restrict_
 }
 
 /* occupancy bits: bit m of the mask <=> macrocell m can hold a sample with sigma > 0 */
+/* group bits: bit g set iff any macrocell of group g (4 x 4 x 4 macrocells, clipped to the
+ * grid) has its bit set; the group bits start at word `base` of the same array */
+__global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits, int mx, int my, int mz, int gx, int gy,
+                                                   int gz, int base, int words)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint32_t v = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int g = w * 32 + b;
+        if (g >= gx * gy * gz) break;
+        const int x = g % gx, y = (g / gx) % gy, z = g / (gx * gy);
+        bool any = false;
+        for (int k = 4 * z; k < min(4 * z + 4, mz) && !any; ++k)
+            for (int j = 4 * y; j < min(4 * y + 4, my) && !any; ++j)
+                for (int i = 4 * x; i < min(4 * x + 4, mx) && !any; ++i) {
+                    const int m = (k * my + j) * mx + i;
+                    any = (bits[m >> 5] >> (m & 31)) & 1u;
+                }
+        if (any) v |= 1u << b;
+    }
+    bits[base + w] = v;
+}
+
 __global__ __launch_bounds__(256) void k_macro_bits(const float* __restrict__ mc_sigma, uint32_t* __restrict__ bits,
                                                    int n, int words)
 {
@@ -285,6 +309,14 @@ extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm,
 extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, uint32_t* bits, int n, int words, hipStream_t st)
 {
     hipLaunchKernelGGL(erd::k_macro_bits, dim3((words + 255) / 256), dim3(256), 0, st, mc_sigma, bits, n, words);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, int mx, int my, int mz, int gx, int gy, int gz, int base, int words,
+                                            hipStream_t st)
+{
+    hipLaunchKernelGGL(erd::k_group_bits, dim3((words + 255) / 256), dim3(256), 0, st, bits, mx, my, mz, gx, gy, gz, base,
+                       words);
     return hipGetLastError();
 }
 

@@ -42,6 +42,8 @@ extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* jo
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
 extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, uint32_t* bits, int n, int words, hipStream_t st);
+extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, int mx, int my, int mz, int gx, int gy, int gz, int base, int words,
+                                            hipStream_t st);
 extern "C" hipError_t erd_launch_bilateral(const void* bp, const uchar4* in, uchar4* tmp, uchar4* out, int W, int H,
                                            hipStream_t st);
 extern "C" size_t erd_bilateral_params_size(void);
@@ -723,9 +725,17 @@ int build_params(Tracer& t)
     HIP_TRY(erd_launch_macro_sigma(grid->mm.ptr, t.lut.ptr, P.lut_max, t.mc_sigma.ptr, nmc, t.stream));
     P.mc_sigma = t.mc_sigma.ptr;
     P.mc_shift = mc_shift;
-    P.mc_words = (nmc + 31) / 32;
+    /* occupancy bits: one per macrocell, then one per group of 4 x 4 x 4 macrocells */
+    const int mc_words = (nmc + 31) / 32;
+    for (int i = 0; i < 3; ++i) P.mc2_dim[i] = (grid->mc[i] + 3) / 4;
+    const int ng = P.mc2_dim[0] * P.mc2_dim[1] * P.mc2_dim[2];
+    const int g_words = (ng + 31) / 32;
+    P.mc2_base = mc_words;
+    P.mc_words = mc_words + g_words;
     if ((st = t.mc_bits.alloc(P.mc_words))) return st;
-    HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, t.mc_bits.ptr, nmc, P.mc_words, t.stream));
+    HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, t.mc_bits.ptr, nmc, mc_words, t.stream));
+    HIP_TRY(erd_launch_group_bits(t.mc_bits.ptr, grid->mc[0], grid->mc[1], grid->mc[2], P.mc2_dim[0], P.mc2_dim[1],
+                                  P.mc2_dim[2], mc_words, g_words, t.stream));
     P.mc_bits = t.mc_bits.ptr;
     t.lut_lds = (size_t)lut_n * sizeof(float) <= 32768u;
     t.lds_bytes = (size_t)P.mc_words * 4 + (t.lut_lds ? (size_t)lut_n * sizeof(float) : 0);
