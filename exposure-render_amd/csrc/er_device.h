@@ -31,6 +31,9 @@
 #include "er_fmath.h"
 
 #define ER_NEGLOG_BITS 23          /* -log(u) table over u = m * 2^-23 */
+#ifndef ER_GROUP_SHIFT
+#define ER_GROUP_SHIFT 2           /* skip groups of 2^ER_GROUP_SHIFT macrocells a side */
+#endif
 
 namespace erd {
 

@@ -289,13 +289,13 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
     const float iLdx = __builtin_amdgcn_rcpf(Ldx), iLdy = __builtin_amdgcn_rcpf(Ldy), iLdz = __builtin_amdgcn_rcpf(Ldz);
     const float margin = 1.0e-3f * __builtin_amdgcn_rcpf(fmaxf(fabsf(Ldx), fmaxf(fabsf(Ldy), fabsf(Ldz))));
     m.MinT += StepSize;
-    /* the box to skip: the macrocell, or the whole group of 4 x 4 x 4 macrocells around it when
-     * the group is empty (the same argument holds for any lattice-aligned box) */
+    /* the box to skip: the macrocell, or the whole group of macrocells around it when the group
+     * is empty (the same argument holds for any lattice-aligned box) */
     int bsh = sh, bx = mxi, by = myi, bz = mzi, nx = p.mc_dim[0], ny = p.mc_dim[1], nz = p.mc_dim[2];
     {
-        const int gx = mxi >> 2, gy = myi >> 2, gz = mzi >> 2;
+        const int gx = mxi >> ER_GROUP_SHIFT, gy = myi >> ER_GROUP_SHIFT, gz = mzi >> ER_GROUP_SHIFT;
         if (!mc_occupied(T, (p.mc2_base << 5) + (gz * p.mc2_dim[1] + gy) * p.mc2_dim[0] + gx)) {
-            bsh = sh + 2; bx = gx; by = gy; bz = gz; nx = p.mc2_dim[0]; ny = p.mc2_dim[1]; nz = p.mc2_dim[2];
+            bsh = sh + ER_GROUP_SHIFT; bx = gx; by = gy; bz = gz; nx = p.mc2_dim[0]; ny = p.mc2_dim[1]; nz = p.mc2_dim[2];
         }
     }
     const float big = 3.0e38f;

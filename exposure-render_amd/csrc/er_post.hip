@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_macro_sigma(const uint32_t* __restrict_
 }
 
 /* occupancy bits: bit m of the mask <=> macrocell m can hold a sample with sigma > 0 */
-/* group bits: bit g set iff any macrocell of group g (4 x 4 x 4 macrocells, clipped to the
+/* group bits: bit g set iff any macrocell of group g (2^ER_GROUP_SHIFT macrocells a side, clipped to the
  * grid) has its bit set; the group bits start at word `base` of the same array */
 __global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits, int mx, int my, int mz, int gx, int gy,
                                                    int gz, int base, int words)
@@ -227,9 +227,10 @@ __global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits,
         if (g >= gx * gy * gz) break;
         const int x = g % gx, y = (g / gx) % gy, z = g / (gx * gy);
         bool any = false;
-        for (int k = 4 * z; k < min(4 * z + 4, mz) && !any; ++k)
-            for (int j = 4 * y; j < min(4 * y + 4, my) && !any; ++j)
-                for (int i = 4 * x; i < min(4 * x + 4, mx) && !any; ++i) {
+        const int E = 1 << ER_GROUP_SHIFT;
+        for (int k = E * z; k < min(E * z + E, mz) && !any; ++k)
+            for (int j = E * y; j < min(E * y + E, my) && !any; ++j)
+                for (int i = E * x; i < min(E * x + E, mx) && !any; ++i) {
                     const int m = (k * my + j) * mx + i;
                     any = (bits[m >> 5] >> (m & 31)) & 1u;
                 }

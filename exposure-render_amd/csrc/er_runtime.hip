@@ -725,9 +725,9 @@ int build_params(Tracer& t)
     HIP_TRY(erd_launch_macro_sigma(grid->mm.ptr, t.lut.ptr, P.lut_max, t.mc_sigma.ptr, nmc, t.stream));
     P.mc_sigma = t.mc_sigma.ptr;
     P.mc_shift = mc_shift;
-    /* occupancy bits: one per macrocell, then one per group of 4 x 4 x 4 macrocells */
+    /* occupancy bits: one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a side */
     const int mc_words = (nmc + 31) / 32;
-    for (int i = 0; i < 3; ++i) P.mc2_dim[i] = (grid->mc[i] + 3) / 4;
+    for (int i = 0; i < 3; ++i) P.mc2_dim[i] = (grid->mc[i] + (1 << ER_GROUP_SHIFT) - 1) >> ER_GROUP_SHIFT;
     const int ng = P.mc2_dim[0] * P.mc2_dim[1] * P.mc2_dim[2];
     const int g_words = (ng + 31) / 32;
     P.mc2_base = mc_words;
