@@ -55,6 +55,8 @@ __device__ __forceinline__ bool clipped(const DevParams& p, f3 P)
 }
 
 /* dense samples per march iteration (loads in flight together) in the primary and shadow kernels */
+#define ER_BATCH_CONST_JOBS 128   /* jobs per batch graph (parameter blocks in constant memory) */
+
 #ifndef ER_BATCH_PRIMARY
 #define ER_BATCH_PRIMARY 6
 #endif

@@ -59,6 +59,7 @@ extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, 
 extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, unsigned int round, int total_tiles,
                                             hipStream_t stream);
 extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, hipStream_t stream);
+extern "C" hipError_t erd_batch_params_upload(const erd::DevParams* host, int n);
 extern "C" size_t erd_adj_partials_per_block(void);
 extern "C" hipError_t erd_launch_primary_shade(const DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, void* queue,
                                                unsigned int* counts, void* events, void* jobs, int W, int H, int lut_lds,
@@ -1467,32 +1468,10 @@ ER_API int er_set_list_capacity(int capacity)
     return ER_OK;
 }
 
-ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
+/* One graph of batch rounds over jobs sorted by pass count (at most ER_BATCH_CONST_JOBS). */
+static int render_batch_chunk(std::vector<std::pair<int, Tracer*>>& jobs, int full, er_timing* timing)
 {
-    std::lock_guard<std::recursive_mutex> lk(g_mutex);
     int st;
-    if (n <= 0 || !items) return set_error(ER_ERROR, "empty batch");
-    std::vector<std::pair<int, Tracer*>> jobs;   /* (passes, tracer) */
-    int full = -1;
-    for (int k = 0; k < n; ++k) {
-        Tracer* t = get_tracer(items[k].tracer_id, st);
-        if (!t) return st;
-        if (items[k].passes < 0) return set_error(ER_ERROR, "negative pass count");
-        for (auto& j : jobs)
-            if (j.second == t) return set_error(ER_ERROR, "tracer %d appears twice in the batch", items[k].tracer_id);
-        const er_render_settings& s = t->desc.settings;
-        const int f = s.estimator == ER_ESTIMATOR_FULL;
-        if (full >= 0 && f != full) return set_error(ER_ERROR, "all tracers of a batch must use the same estimator mode");
-        if (f && s.sampler == ER_SAMPLER_WOODCOCK) return set_error(ER_ERROR, "the Woodcock sampler is not batched");
-        full = f;
-        if ((st = build_params(*t))) return st;
-        if ((st = sync_tracer(*t))) return st;
-        if (items[k].passes > 0 && t->W > 0 && t->H > 0) jobs.push_back({items[k].passes, t});
-    }
-    if (jobs.empty()) return ER_OK;
-    std::stable_sort(jobs.begin(), jobs.end(), [](const std::pair<int, Tracer*>& a, const std::pair<int, Tracer*>& b) {
-        return a.first > b.first;
-    });
     const int nj = (int)jobs.size();
     /* two job tables, one per buffer set: rounds alternate between the sets so that a round's
      * shadow marches, combine and post (second stream) overlap the next round's primary and
@@ -1528,6 +1507,12 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         if ((st = dJ[b].alloc(nj))) return st;
         HIP_TRY(hipMemcpy(dJ[b].ptr, table[b].data(), nj * sizeof(erd::BatchJob), hipMemcpyHostToDevice));
         for (int k = 0; k < nj; ++k) HIP_TRY(hipMemset(table[b][k].counts, 0, 4 * sizeof(uint32_t)));
+    }
+    /* the jobs' parameter blocks, in table order, for the batch kernels */
+    {
+        std::vector<erd::DevParams> hp(nj);
+        for (int k = 0; k < nj; ++k) hp[k] = jobs[k].second->host_params;
+        HIP_TRY(erd_batch_params_upload(hp.data(), nj));
     }
     if (!g_batch_stream) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream, hipStreamNonBlocking));
     if (!g_batch_stream2) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream2, hipStreamNonBlocking));
@@ -1620,6 +1605,47 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         timing->total_ms = ms; timing->trace_ms = ms; timing->post_ms = 0.0f; timing->passes = rounds;
         timing->primary_ms = 0.0f;
     }
+    return ER_OK;
+}
+
+ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    if (n <= 0 || !items) return set_error(ER_ERROR, "empty batch");
+    std::vector<std::pair<int, Tracer*>> jobs;   /* (passes, tracer) */
+    int full = -1;
+    for (int k = 0; k < n; ++k) {
+        Tracer* t = get_tracer(items[k].tracer_id, st);
+        if (!t) return st;
+        if (items[k].passes < 0) return set_error(ER_ERROR, "negative pass count");
+        for (auto& j : jobs)
+            if (j.second == t) return set_error(ER_ERROR, "tracer %d appears twice in the batch", items[k].tracer_id);
+        const er_render_settings& s = t->desc.settings;
+        const int f = s.estimator == ER_ESTIMATOR_FULL;
+        if (full >= 0 && f != full) return set_error(ER_ERROR, "all tracers of a batch must use the same estimator mode");
+        if (f && s.sampler == ER_SAMPLER_WOODCOCK) return set_error(ER_ERROR, "the Woodcock sampler is not batched");
+        full = f;
+        if ((st = build_params(*t))) return st;
+        if ((st = sync_tracer(*t))) return st;
+        if (items[k].passes > 0 && t->W > 0 && t->H > 0) jobs.push_back({items[k].passes, t});
+    }
+    if (jobs.empty()) return ER_OK;
+    std::stable_sort(jobs.begin(), jobs.end(), [](const std::pair<int, Tracer*>& a, const std::pair<int, Tracer*>& b) {
+        return a.first > b.first;
+    });
+    /* one graph per ER_BATCH_CONST_JOBS jobs of similar pass counts */
+    er_timing sum{};
+    for (size_t off = 0; off < jobs.size(); off += ER_BATCH_CONST_JOBS) {
+        std::vector<std::pair<int, Tracer*>> chunk(jobs.begin() + off,
+                                                   jobs.begin() + std::min(jobs.size(), off + ER_BATCH_CONST_JOBS));
+        er_timing tm{};
+        if ((st = render_batch_chunk(chunk, full, timing ? &tm : nullptr))) return st;
+        sum.total_ms += tm.total_ms;
+        sum.trace_ms += tm.trace_ms;
+        sum.passes = std::max(sum.passes, tm.passes);
+    }
+    if (timing) *timing = sum;
     return ER_OK;
 }
 

@@ -378,6 +378,13 @@ __device__ __forceinline__ Tables global_tables(const DevParams& p)
     return T;
 }
 
+/* The parameter blocks of a batch's jobs, in job-table order. Read from the constant address
+ * space, their fields are fetched with scalar loads, as the single-pass kernels fetch their
+ * kernel-argument block; through the job table's pointers the compiler would have to use
+ * vector loads (it cannot rule out that the kernel's own stores alias the block), which took
+ * 40 more VGPRs in the primary kernel. er_render_batch runs larger batches in chunks. */
+__constant__ DevParams c_batch_params[ER_BATCH_CONST_JOBS];
+
 /* A job's tables for a batch kernel: LDS = 1 copies the macrocell bits and the opacity table
  * into the workgroup's LDS (the host launches this form only when every job's tables fit),
  * LDS = 0 reads both from global memory. Called by the whole workgroup after every wave has
@@ -435,7 +442,7 @@ __global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ 
     int paths = 0;
     for (;;) {
         const BatchJob& B = J[j];
-        const DevParams& p = *B.params;
+        const DevParams& p = c_batch_params[j];
         const Tables T = stage_job_tables<LDS>(p, lds);
         for (;;) {
             unsigned int tile = 0;
@@ -478,8 +485,11 @@ __global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J,
     for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);; w += waves) {
         int job; unsigned int q0, n;
         if (!batch_slot(J, na, 0, w, job, q0, n)) break;
+        job = __builtin_amdgcn_readfirstlane(job);   /* uniform (the counts are vector loads) */
+        q0 = __builtin_amdgcn_readfirstlane(q0);
+        n = __builtin_amdgcn_readfirstlane(n);
         const BatchJob& B = J[job];
-        const DevParams& p = *B.params;
+        const DevParams& p = c_batch_params[job];
         const Tables T = global_tables(p);
         shade_chunk<FULL, 0, 1>(p, T, q0, n, lane, B.frame, B.seeds1, B.seeds2, B.queue, B.events, B.jobs, &B.counts[1], cnt);
     }
@@ -505,12 +515,13 @@ __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J
         base += J[j].counts[1];
         if (pos < base) break;
     }
+    j = __builtin_amdgcn_readfirstlane(j);   /* uniform (the counts are vector loads) */
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (;;) {
         const BatchJob& B = J[j];
-        const DevParams& p = *B.params;
+        const DevParams& p = c_batch_params[j];
         const Tables T = stage_job_tables<LDS>(p, lds);
-        const unsigned int n = B.counts[1];
+        const unsigned int n = __builtin_amdgcn_readfirstlane(B.counts[1]);
         for (;;) {
             unsigned int first = 0;
             if (lane == 0) first = atomicAdd(&B.counts[3], 64u);
@@ -647,6 +658,14 @@ extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, 
 #undef ER_SHADOW_B
     hipLaunchKernelGGL(erd::k_finish_b, dim3(shade_blocks), dim3(256), 0, stream2, J, njobs, round);
     return hipGetLastError();
+}
+
+/* the jobs' parameter blocks for the batch kernels (at most ER_BATCH_CONST_JOBS) */
+extern "C" hipError_t erd_batch_params_upload(const erd::DevParams* host, int n)
+{
+    if (n < 0 || n > ER_BATCH_CONST_JOBS) return hipErrorInvalidValue;
+    return hipMemcpyToSymbol(HIP_SYMBOL(erd::c_batch_params), host, (size_t)n * sizeof(erd::DevParams), 0,
+                             hipMemcpyHostToDevice);
 }
 
 extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, hipStream_t stream)
