@@ -579,3 +579,79 @@ def test_object_direct_light_quadrature(estimator, occluder):
         want = _floor_quadrature(sc, P, -D, estimator == capi.ESTIMATOR_COMPAT, occluder)
         assert e[k, 3] == 1.0
         np.testing.assert_allclose(e[k, :3], want, rtol=0.03, err_msg=f"pixel {px[k]},{py[k]}")
+
+
+def _round_light_scene(estimator, shape, inner, outer):
+    """The floor scene of _floor_scene with a disk or ring light (Source/disk.h, ring.h) of the
+    given radii in place of the plane light."""
+    sc = _floor_scene(estimator, False)
+    sc.lights = []
+    sc.tracer.light_count = 0
+    sc.add_light(scenes.shape_light(shape, LIGHT_POS, (0.25, 0.05, 0.0), 5.0, texture_id=0, inner=inner, outer=outer,
+                                    visible=0))
+    return sc
+
+
+def _round_light_quadrature(sc, P, Wo, full, inner, outer, n=600):
+    """Direct light at floor point P from a disk (inner = 0) or ring light, by midpoint
+    quadrature in polar coordinates over the light. The light sample is not area-uniform on a
+    ring: SampleUnitRing (ring.h:59-67) takes r = a + (1 - a) sqrt(u) with a = inner / outer,
+    an area density of (rho - a) / (pi rho (1 - a)^2) on the unit ring, while the estimator
+    divides by the uniform density 1 / Area. The light-sampling half therefore carries the
+    factor Area * density; the BSDF half (full mode) does not. Compat: the light half with its
+    MIS weight; full: both halves with the mixture pdf 0.5 (Lambert + Blinn)."""
+    L = sc.lights[0]
+    tm = np.array([[L.shape.tm[i][j] for j in range(4)] for i in range(4)], dtype=np.float64)
+    a = inner / outer
+    rr = a + (1.0 - a) * (np.arange(n) + 0.5) / n            # rho in [a, 1]
+    th = 2.0 * math.pi * (np.arange(n) + 0.5) / n
+    R, TH = np.meshgrid(rr, th, indexing="ij")
+    dA = (1.0 - a) / n * (2.0 * math.pi / n) * R * outer * outer
+    local = np.stack([R * outer * np.cos(TH), R * outer * np.sin(TH), np.zeros_like(R), np.ones_like(R)], axis=-1)
+    X = (local @ tm.T)[..., :3]
+    Nl = tm[:3, :3] @ np.array([0.0, 0.0, 1.0])
+    D = X - P
+    d2 = np.sum(D * D, axis=-1)
+    Wi = D / np.sqrt(d2)[..., None]
+    N = np.array([0.0, 0.0, 1.0])
+    cos_i, cos_l = np.abs(Wi @ N), np.abs(Wi @ Nl)
+    area = math.pi * (outer * outer - inner * inner)
+    Nu = np.cross(N, Wo); Nu /= np.linalg.norm(Nu)
+    Nv = np.cross(N, Nu); Nv /= np.linalg.norm(Nv)
+    wo = np.array([Wo @ Nu, Wo @ Nv, Wo @ N])
+    wi = np.stack([Wi @ Nu, Wi @ Nv, Wi @ N], axis=-1)
+    same = wi[..., 2] * wo[2] > 0
+    lam = np.where(same, np.abs(wi[..., 2]) / math.pi, 0.0)
+    wh = wi + wo
+    wh /= np.linalg.norm(wh, axis=-1)[..., None]
+    wowh = wh @ wo
+    blinn = np.where((wowh > 0) & same, 1.0 / (8.0 * math.pi * wowh), 0.0)
+    bsdf = 0.5 * (lam + blinn) if full else lam + blinn
+    light_pdf = d2 / (cos_l * area)
+    w_light = light_pdf ** 2 / (light_pdf ** 2 + bsdf ** 2)
+    sample_weight = (1.0 + a) * (R - a) / (R * (1.0 - a)) if a > 0 else np.ones_like(R)   # Area x density
+    weight = sample_weight * w_light + (bsdf ** 2 / (bsdf ** 2 + light_pdf ** 2) if full else 0.0)
+    g = cos_i * cos_l / d2 * weight
+    f = np.array([0.8, 0.6, 0.4]) / math.pi
+    return f * 5.0 * float(np.sum(g * dA))
+
+
+@pytest.mark.parametrize("estimator", [capi.ESTIMATOR_COMPAT, capi.ESTIMATOR_FULL])
+@pytest.mark.parametrize("shape,inner,outer", [(capi.SHAPE_DISK, 0.0, 0.45), (capi.SHAPE_RING, 0.2, 0.5)])
+def test_round_light_direct_light_quadrature(estimator, shape, inner, outer):
+    """Disk and ring lights (disk.h, ring.h): their sampling, area and intersection, against a
+    quadrature that states the ring's non-uniform sample density explicitly. A ring sampled as
+    if it were area-uniform would be off by several per cent here."""
+    sc = _round_light_scene(estimator, shape, inner, outer)
+    orc = O.Oracle(sc, nthreads=4)
+    W, H = sc.film
+    O_cam = np.array([sc.tracer.camera.pos[i] for i in range(3)], dtype=np.float64)
+    px = np.array([12, 16, 20, 16], dtype=np.int32)
+    py = np.array([16, 12, 16, 20], dtype=np.int32)
+    e = orc.expect_f64(px, py, 20000, seed=6)
+    for k in range(len(px)):
+        D = _camera_hits(orc, W, H, px[k], py[k])
+        P = O_cam + (-O_cam[2] / D[2]) * D
+        want = _round_light_quadrature(sc, P, -D, estimator == capi.ESTIMATOR_FULL, inner, outer)
+        assert e[k, 3] == 1.0
+        np.testing.assert_allclose(e[k, :3], want, rtol=0.03, err_msg=f"pixel {px[k]},{py[k]}")
