@@ -655,3 +655,59 @@ def test_round_light_direct_light_quadrature(estimator, shape, inner, outer):
         want = _round_light_quadrature(sc, P, -D, estimator == capi.ESTIMATOR_FULL, inner, outer)
         assert e[k, 3] == 1.0
         np.testing.assert_allclose(e[k, :3], want, rtol=0.03, err_msg=f"pixel {px[k]},{py[k]}")
+
+
+def test_environment_sphere_light():
+    """A uniform two-sided sphere light of radius 10 around the scene (sphere.h:152-170) over a
+    Lambertian floor object. Full mode, both MIS halves: the radiance is exactly Kd * Le (the
+    cosine integral of a uniform environment, pi, times Kd / pi). Compat, light sampling with
+    its MIS weight: a quadrature over the sphere."""
+    Le = 2.0
+    for estimator in (capi.ESTIMATOR_FULL, capi.ESTIMATOR_COMPAT):
+        sc = _floor_scene(estimator, False)
+        sc.lights = []
+        sc.tracer.light_count = 0
+        sc.add_light(scenes.sphere_light(10.0, Le, texture_id=0, visible=0))
+        orc = O.Oracle(sc, nthreads=4)
+        W, H = sc.film
+        O_cam = np.array([sc.tracer.camera.pos[i] for i in range(3)], dtype=np.float64)
+        px = np.array([14, 18], dtype=np.int32)
+        py = np.array([16, 15], dtype=np.int32)
+        e = orc.expect_f64(px, py, 40000, seed=8)
+        kd = np.array([0.8, 0.6, 0.4])
+        for k in range(len(px)):
+            assert e[k, 3] == 1.0
+            if estimator == capi.ESTIMATOR_FULL:
+                np.testing.assert_allclose(e[k, :3], kd * Le, rtol=0.03)
+                continue
+            D = _camera_hits(orc, W, H, px[k], py[k])
+            P = O_cam + (-O_cam[2] / D[2]) * D
+            Wo = -D
+            n = 800
+            th = math.pi * (np.arange(n) + 0.5) / n
+            ph = 2.0 * math.pi * (np.arange(n) + 0.5) / n
+            TH, PH = np.meshgrid(th, ph, indexing="ij")
+            Rs = 10.0
+            X = Rs * np.stack([np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)], axis=-1)
+            dA = Rs * Rs * np.sin(TH) * (math.pi / n) * (2.0 * math.pi / n)
+            Dv = X - P
+            d2 = np.sum(Dv * Dv, axis=-1)
+            Wi = Dv / np.sqrt(d2)[..., None]
+            Nl = X / Rs
+            N = np.array([0.0, 0.0, 1.0])
+            cos_i, cos_l = np.abs(Wi @ N), np.abs(np.sum(Wi * Nl, axis=-1))
+            Nu = np.cross(N, Wo); Nu /= np.linalg.norm(Nu)
+            Nv = np.cross(N, Nu); Nv /= np.linalg.norm(Nv)
+            wo = np.array([Wo @ Nu, Wo @ Nv, Wo @ N])
+            wi = np.stack([Wi @ Nu, Wi @ Nv, Wi @ N], axis=-1)
+            same = wi[..., 2] * wo[2] > 0
+            lam = np.where(same, np.abs(wi[..., 2]) / math.pi, 0.0)
+            wh = wi + wo
+            wh /= np.linalg.norm(wh, axis=-1)[..., None]
+            wowh = wh @ wo
+            blinn = np.where((wowh > 0) & same, 1.0 / (8.0 * math.pi * wowh), 0.0)
+            bsdf = lam + blinn
+            light_pdf = d2 / (cos_l * 4.0 * math.pi * Rs * Rs)
+            w = np.where(bsdf > 0, light_pdf ** 2 / (light_pdf ** 2 + bsdf ** 2), 0.0)
+            want = kd / math.pi * Le * float(np.sum(cos_i * cos_l / d2 * w * dA))
+            np.testing.assert_allclose(e[k, :3], want, rtol=0.03)
