@@ -711,3 +711,74 @@ def test_environment_sphere_light():
             w = np.where(bsdf > 0, light_pdf ** 2 / (light_pdf ** 2 + bsdf ** 2), 0.0)
             want = kd / math.pi * Le * float(np.sum(cos_i * cos_l / d2 * w * dA))
             np.testing.assert_allclose(e[k, :3], want, rtol=0.03)
+
+
+def test_box_light_direct_light_quadrature():
+    """A box light (box.h:122-191): SampleUnitBox picks each of the six faces with probability
+    1/6 whatever its area, so the light-sampling half carries Area / (6 * face area) per face;
+    faces turned away from the shaded point are hidden by the box itself (the shadow test meets
+    the box first). Compat and full, against a quadrature over the faces."""
+    size = np.array([0.5, 0.3, 0.4])
+    for estimator in (capi.ESTIMATOR_COMPAT, capi.ESTIMATOR_FULL):
+        full = estimator == capi.ESTIMATOR_FULL
+        sc = _floor_scene(estimator, False)
+        sc.lights = []
+        sc.tracer.light_count = 0
+        sc.add_light(scenes.shape_light(capi.SHAPE_BOX, LIGHT_POS, (0.25, 0.05, 0.0), 5.0, texture_id=0,
+                                        size=tuple(size), visible=0))
+        orc = O.Oracle(sc, nthreads=4)
+        W, H = sc.film
+        O_cam = np.array([sc.tracer.camera.pos[i] for i in range(3)], dtype=np.float64)
+        L = sc.lights[0]
+        tm = np.array([[L.shape.tm[i][j] for j in range(4)] for i in range(4)], dtype=np.float64)
+        px = np.array([12, 16, 20], dtype=np.int32)
+        py = np.array([16, 12, 16], dtype=np.int32)
+        e = orc.expect_f64(px, py, 20000, seed=9)
+        area = 2.0 * (size[0] * size[1] + size[0] * size[2] + size[1] * size[2])
+        n = 300
+        s = (np.arange(n) + 0.5) / n - 0.5
+        A_, B_ = np.meshgrid(s, s, indexing="ij")
+        for k in range(len(px)):
+            D = _camera_hits(orc, W, H, px[k], py[k])
+            P = O_cam + (-O_cam[2] / D[2]) * D
+            Wo = -D
+            N = np.array([0.0, 0.0, 1.0])
+            Nu = np.cross(N, Wo); Nu /= np.linalg.norm(Nu)
+            Nv = np.cross(N, Nu); Nv /= np.linalg.norm(Nv)
+            wo = np.array([Wo @ Nu, Wo @ Nv, Wo @ N])
+            total = 0.0
+            for axis in range(3):
+                for sign in (1.0, -1.0):
+                    u, v = [a for a in range(3) if a != axis]
+                    local = np.zeros(A_.shape + (4,))
+                    local[..., axis] = sign * 0.5 * size[axis]
+                    local[..., u] = A_ * size[u]
+                    local[..., v] = B_ * size[v]
+                    local[..., 3] = 1.0
+                    X = (local @ tm.T)[..., :3]
+                    nl = np.zeros(3); nl[axis] = sign
+                    Nl = tm[:3, :3] @ nl
+                    face_area = size[u] * size[v]
+                    Dv = X - P
+                    d2 = np.sum(Dv * Dv, axis=-1)
+                    Wi = Dv / np.sqrt(d2)[..., None]
+                    facing = (Wi @ Nl) < 0            # the face looks at P
+                    cos_i, cos_l = np.abs(Wi @ N), np.abs(Wi @ Nl)
+                    wi = np.stack([Wi @ Nu, Wi @ Nv, Wi @ N], axis=-1)
+                    same = wi[..., 2] * wo[2] > 0
+                    lam = np.where(same, np.abs(wi[..., 2]) / math.pi, 0.0)
+                    wh = wi + wo
+                    wh /= np.linalg.norm(wh, axis=-1)[..., None]
+                    wowh = wh @ wo
+                    blinn = np.where((wowh > 0) & same, 1.0 / (8.0 * math.pi * wowh), 0.0)
+                    bsdf = 0.5 * (lam + blinn) if full else lam + blinn
+                    lp = d2 / (cos_l * area)
+                    wl = np.where(bsdf > 0, lp ** 2 / (lp ** 2 + bsdf ** 2), 0.0)
+                    weight = area / (6.0 * face_area) * wl
+                    if full:
+                        weight = weight + np.where(bsdf > 0, bsdf ** 2 / (bsdf ** 2 + lp ** 2), 0.0)
+                    g = np.where(facing, cos_i * cos_l / d2 * weight, 0.0)
+                    total += float(np.sum(g)) * face_area / (n * n)
+            want = np.array([0.8, 0.6, 0.4]) / math.pi * 5.0 * total
+            assert e[k, 3] == 1.0
+            np.testing.assert_allclose(e[k, :3], want, rtol=0.03, err_msg=f"estimator {estimator} pixel {px[k]},{py[k]}")
