@@ -782,3 +782,79 @@ def test_box_light_direct_light_quadrature():
             want = np.array([0.8, 0.6, 0.4]) / math.pi * 5.0 * total
             assert e[k, 3] == 1.0
             np.testing.assert_allclose(e[k, :3], want, rtol=0.03, err_msg=f"estimator {estimator} pixel {px[k]},{py[k]}")
+
+
+def _slab_chord(o, d, lo, hi):
+    """Length of the ray o + t d (t >= 0) inside the box [lo, hi]."""
+    t0, t1 = 0.0, np.inf
+    for i in range(3):
+        if abs(d[i]) < 1e-12:
+            if o[i] < lo[i] or o[i] > hi[i]:
+                return 0.0
+            continue
+        a, b = (lo[i] - o[i]) / d[i], (hi[i] - o[i]) / d[i]
+        t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+    return max(0.0, t1 - t0)
+
+
+@pytest.mark.parametrize("clip", [False, True])
+def test_woodcock_free_flight_and_clipping_closed_form(clip):
+    """Delta tracking (full mode) in a homogeneous cube with the majorant equal to the
+    extinction sigma = DS^2 * opacity (Q3) samples exact exponential free flights: a camera
+    ray's scatter probability is 1 - exp(-sigma L), L its chord through the cube. A clipping
+    box (full mode) removes the density inside it: L loses the chord through the clip box."""
+    vox = np.full((16, 16, 16), 1000, dtype=np.uint16)
+    sc = scenes.Scene(vox)
+    t = sc.tracer
+    t.opacity.add(0.0, 0.5).add(70000.0, 0.5)
+    t.settings.density_scale = 2.0
+    t.settings.estimator = capi.ESTIMATOR_FULL
+    t.settings.shading_type = capi.SHADING_PHASE_ONLY
+    t.settings.sampler = capi.SAMPLER_WOODCOCK
+    scenes.set_camera(t, (24, 24), (1.2, 0.9, 1.6), (0.0, 0.0, 0.0), fov=30.0)
+    clip_lo, clip_hi = np.array([0.0, -1.0, -1.0]), np.array([0.4, 1.0, 1.0])
+    if clip:
+        sc.add_clip(scenes.clipping_box(tuple((clip_lo + clip_hi) / 2), tuple(clip_hi - clip_lo)))
+    orc = O.Oracle(sc, nthreads=4)
+    W, H = sc.film
+    sigma = 2.0 * 2.0 * 0.5
+    O_cam = np.array([t.camera.pos[i] for i in range(3)], dtype=np.float64)
+    px = np.array([4, 8, 12, 16, 20, 12], dtype=np.int32)
+    py = np.array([12, 6, 12, 18, 12, 2], dtype=np.int32)
+    e = orc.expect_f64(px, py, 20000, seed=10)
+    lo, hi = np.full(3, -0.5), np.full(3, 0.5)
+    for k in range(len(px)):
+        D = _camera_hits(orc, W, H, px[k], py[k])
+        L = _slab_chord(O_cam, D, lo, hi)
+        if clip:
+            L -= _slab_chord(O_cam, D, np.maximum(lo, clip_lo), np.minimum(hi, clip_hi))
+        want = 1.0 - math.exp(-sigma * L)
+        assert abs(e[k, 3] - want) < 0.012, (px[k], py[k], e[k, 3], want, L)
+
+
+def test_shading_type_selection_limits():
+    """Full-mode shader selection (design: scatterevent.h:155-212) at its limits: hybrid with
+    gradient factor 0 and threshold above every normalised gradient always choose the phase
+    function; threshold below 0 always the BRDF. Each must give the expectation of the
+    corresponding fixed shading type (the selection's extra draw changes the streams, not the
+    expectation)."""
+    def expect(shading, factor=1.0, threshold=0.5):
+        sc = scenes.config_small(scale=0.25, estimator=capi.ESTIMATOR_FULL)
+        sc.tracer.settings.shading_type = shading
+        sc.tracer.settings.gradient_factor = factor
+        sc.tracer.settings.gradient_threshold = threshold
+        orc = O.Oracle(sc, nthreads=4)
+        orc.compute_gm_max()
+        W, H = sc.film
+        px = np.array([W // 2 - 4, W // 2, W // 2 + 3, W // 2], dtype=np.int32)
+        py = np.array([H // 2, H // 2 - 3, H // 2 + 2, H // 2 + 5], dtype=np.int32)
+        return orc.expect_f64(px, py, 20000, seed=11)
+
+    phase = expect(capi.SHADING_PHASE_ONLY)
+    brdf = expect(capi.SHADING_BRDF_ONLY)
+    assert np.abs(phase[:, :3] - brdf[:, :3]).max() > 0.1 * np.abs(phase[:, :3]).max()   # the two differ
+    for got, want in ((expect(capi.SHADING_HYBRID, factor=0.0), phase),
+                      (expect(capi.SHADING_THRESHOLD, threshold=2.0), phase),
+                      (expect(capi.SHADING_THRESHOLD, threshold=-1.0), brdf)):
+        np.testing.assert_allclose(got[:, :3], want[:, :3], rtol=0.04, atol=0.02 * np.abs(want[:, :3]).max())
+        assert np.array_equal(got[:, 3] > 0, want[:, 3] > 0)
