@@ -258,18 +258,25 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
             live[k] = occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
             f[k] = cell_fetch(p, lk);
         }
+        /* every sample's intensity and sigma first, so that the K table reads are in flight
+         * together; then the samples are consumed in order */
+        int iv[K];
+        float sig[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            iv[k] = (int)cell_value(f[k]);
+            sig[k] = live[k] ? T.lut(min(iv[k], p.lut_max)) : 0.0f;
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (k > 0) {
                 if (k >= kn) return -1;
                 if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; cnt.iters++; }
             }
-            float SigmaT = 0.0f;
+            const float SigmaT = sig[k];
             if (live[k]) {
                 if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
-                const int iv = (int)cell_value(f[k]);
-                SigmaT = T.lut(min(iv, p.lut_max));
-                vis.sample(iv, SigmaT);
+                vis.sample(iv[k], SigmaT);
             }
             m.Sum += SigmaT * StepSize;
             m.MinT += StepSize;
