@@ -67,6 +67,34 @@ def traffic_from_profile(path):
     return data.get("hbm_bytes_per_launch")
 
 
+def copy_rate_gbs(nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (bytes read + written per second): the HBM rate a plain
+    streaming kernel reaches on this GPU, reported beside the published peak. It runs in a
+    child process: torch's HIP runtime does not see the GPU in a process where the tracer
+    library's runtime already holds it."""
+    import subprocess
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-rate", str(nbytes), str(reps)],
+                         capture_output=True, text=True, timeout=300, check=True).stdout
+    return float(out.strip().splitlines()[-1])
+
+
+def _copy_rate_child(nbytes, reps):
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    secs = t0.elapsed_time(t1) * 1e-3 / reps
+    print(2.0 * nbytes / secs / 1e9, flush=True)
+
+
 def cpu_rate(scene_builder, passes, threads):
     """The CPU oracle over `passes` full passes of the scene: Mpaths/s and seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -230,6 +258,9 @@ def bench_batched(er, scenes, n_jobs):
 
 
 def main():
+    if len(sys.argv) == 4 and sys.argv[1] == "--copy-rate":
+        _copy_rate_child(int(sys.argv[2]), int(sys.argv[3]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
@@ -308,6 +339,9 @@ def main():
             roofline["hbm_GBs"] = round(roofline["traffic"] / (avg_ms * 1e-3) / 1e9, 1)
         samples = counts["primary_samples"] + counts["shadow_samples"] + counts["shading_samples"]
         whole = (16.0 * samples + 64.0 * counts["paths"]) / args.steps
+        copy = copy_rate_gbs()
+        roofline["measured_copy_GBs"] = round(copy, 1)
+        roofline["frac_of_measured_copy"] = round(achieved / copy, 4)
         roofline["whole_pass"] = {"algorithmic_bytes_per_pass": int(whole),
                                   "achieved_GBs": round(whole / (ms_per_step * 1e-3) / 1e9, 1)}
 

@@ -41,7 +41,8 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
-extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, uint32_t* bits, int n, int words, hipStream_t st);
+extern "C" hipError_t erd_launch_macro_bits(const erd::DevParams* pp, const float* mc_sigma, uint32_t* bits, int n, int words,
+                                            double edge_ext, hipStream_t st);
 extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, int mx, int my, int mz, int gx, int gy, int gz, int base, int words,
                                             hipStream_t st);
 extern "C" hipError_t erd_launch_bilateral(const void* bp, const uchar4* in, uchar4* tmp, uchar4* out, int W, int H,
@@ -511,6 +512,35 @@ bool needs_gm(const er_tracer_desc& d)
     return d.settings.estimator == ER_ESTIMATOR_FULL && t != ER_SHADING_BRDF_ONLY && t != ER_SHADING_PHASE_ONLY;
 }
 
+/* A bound on the magnitudes in the ray arithmetic of every march: the rays start at the camera
+ * (within the aperture), at scatter points inside the volume or on a surface object, and end
+ * inside the volume's box. The rounding of a sample position against that box stays far below
+ * 1e-5 of it; the clip-aware occupancy bits widen the edge macrocells by that much. */
+double ray_origin_bound(const erd::DevParams& P, const std::vector<const er_object_desc*>& objs)
+{
+    double box = 0.0, diag2 = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        box = std::max(box, std::max(std::fabs((double)P.minp[i]), std::fabs((double)P.maxp[i])));
+        diag2 += ((double)P.maxp[i] - P.minp[i]) * ((double)P.maxp[i] - P.minp[i]);
+    }
+    double origin = std::sqrt((double)P.cam_pos[0] * P.cam_pos[0] + (double)P.cam_pos[1] * P.cam_pos[1] +
+                              (double)P.cam_pos[2] * P.cam_pos[2]) + std::fabs((double)P.aperture);
+    for (const er_object_desc* o : objs) {
+        const er_shape& S = o->shape;
+        const double ext = std::max({std::fabs((double)S.size[0]), std::fabs((double)S.size[1]),
+                                     std::fabs((double)S.size[2]), std::fabs((double)S.outer_radius)});
+        double b = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            double lin = 0.0;
+            for (int j = 0; j < 3; ++j) lin += std::fabs((double)S.tm[i][j]);
+            b += std::fabs((double)S.tm[i][3]) + lin * ext;
+        }
+        origin = std::max(origin, b);
+    }
+    const double r = 2.0 * origin + 4.0 * box + std::sqrt(diag2);
+    return r == r ? r : 1.0e300;
+}
+
 /* Rebuild the tracer's DevParams (and the LUT / macrocell maxima / gm_max that depend on it). */
 int build_params(Tracer& t)
 {
@@ -734,9 +764,6 @@ int build_params(Tracer& t)
     P.mc2_base = mc_words;
     P.mc_words = mc_words + g_words;
     if ((st = t.mc_bits.alloc(P.mc_words))) return st;
-    HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, t.mc_bits.ptr, nmc, mc_words, t.stream));
-    HIP_TRY(erd_launch_group_bits(t.mc_bits.ptr, grid->mc[0], grid->mc[1], grid->mc[2], P.mc2_dim[0], P.mc2_dim[1],
-                                  P.mc2_dim[2], mc_words, g_words, t.stream));
     P.mc_bits = t.mc_bits.ptr;
     t.lut_lds = (size_t)lut_n * sizeof(float) <= 32768u;
     t.lds_bytes = (size_t)P.mc_words * 4 + (t.lut_lds ? (size_t)lut_n * sizeof(float) : 0);
@@ -745,6 +772,12 @@ int build_params(Tracer& t)
     P.gm_inv = 0.0f;
     t.gm_max = 0.0f;
     HIP_TRY(hipMemcpyAsync(t.params.ptr, &P, sizeof(P), hipMemcpyHostToDevice, t.stream));
+    /* macrocells wholly clipped away count as empty (the bits kernel reads the clipping objects
+     * from the parameters just uploaded) */
+    HIP_TRY(erd_launch_macro_bits(t.params.ptr, t.mc_sigma.ptr, t.mc_bits.ptr, nmc, mc_words,
+                                  ray_origin_bound(P, objs) * 1.0e-5, t.stream));
+    HIP_TRY(erd_launch_group_bits(t.mc_bits.ptr, grid->mc[0], grid->mc[1], grid->mc[2], P.mc2_dim[0], P.mc2_dim[1],
+                                  P.mc2_dim[2], mc_words, g_words, t.stream));
     if (needs_gm(d)) {
         if ((st = t.gm_tmp.alloc(1))) return st;
         HIP_TRY(hipMemsetAsync(t.gm_tmp.ptr, 0, sizeof(unsigned int), t.stream));

@@ -230,16 +230,22 @@ def uniform_texture(rgb=(1.0, 1.0, 1.0), output_level=1.0):
 
 
 def clipping_box(center, size, invert=0):
+    return clipping_shape(capi.SHAPE_BOX, *translate(center), size=size, invert=invert)
+
+
+def clipping_shape(shape_type, tm, inv, size=(1.0, 1.0, 1.0), outer=0.0, invert=0):
+    """A clipping object (ErClippingObject): samples inside the shape (outside, if inverted)
+    have zero opacity in the full estimator."""
     Cb = capi.ClippingDesc()
     Cb.id = -1
     Cb.enabled = 1
     Cb.invert = invert
-    tm, inv = translate(center)
     set_matrix(Cb.shape.tm, tm)
     set_matrix(Cb.shape.inv_tm, inv)
-    Cb.shape.type = capi.SHAPE_BOX
+    Cb.shape.type = shape_type
     for i in range(3):
         Cb.shape.size[i] = size[i]
+    Cb.shape.outer_radius = outer
     return Cb
 
 

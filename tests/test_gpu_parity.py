@@ -97,6 +97,65 @@ def test_large_clipping_anisotropic(fresh):
     run_parity(sc, passes=2)
 
 
+def _rot(axis, angle):
+    c, s = np.cos(angle), np.sin(angle)
+    i, j = [(1, 2), (2, 0), (0, 1)][axis]
+    R = np.eye(3)
+    R[i, i], R[i, j], R[j, i], R[j, j] = c, -s, s, c
+    return R
+
+
+@pytest.mark.parametrize("case", ["sphere", "cylinder_rotated", "box_inverted", "sphere_inverted",
+                                  "cylinder_inverted", "plane", "plane_inverted", "two_boxes"])
+def test_clipping_shapes(fresh, case):
+    """Every clipping shape, plain and inverted, rotated: macrocells wholly clipped away are
+    skipped like empty ones, and the samples, decisions and radiance stay those of the oracle,
+    which tests every sample."""
+    sc = scenes.config_ct(scale=0.125, bilateral=False)
+    sc.name = f"ct_clip_{case}"
+    R = _rot(1, 0.6) @ _rot(0, 0.3)
+    if case == "sphere":
+        sc.add_clip(scenes.clipping_shape(capi.SHAPE_SPHERE, *scenes.translate((0.2, 0.1, 0.0)), outer=0.35))
+    elif case == "cylinder_rotated":
+        sc.add_clip(scenes.clipping_shape(capi.SHAPE_CYLINDER, *scenes._rigid(R, (0.1, 0.0, -0.1)),
+                                          size=(0.0, 0.8, 0.0), outer=0.3))
+    elif case == "box_inverted":
+        sc.add_clip(scenes.clipping_shape(capi.SHAPE_BOX, *scenes._rigid(R, (0.05, 0.0, 0.0)),
+                                          size=(0.5, 0.6, 0.55), invert=1))
+    elif case == "sphere_inverted":
+        sc.add_clip(scenes.clipping_shape(capi.SHAPE_SPHERE, *scenes.translate((0.0, -0.1, 0.1)), outer=0.3, invert=1))
+    elif case == "cylinder_inverted":
+        sc.add_clip(scenes.clipping_shape(capi.SHAPE_CYLINDER, *scenes._rigid(R, (0.0, 0.0, 0.0)),
+                                          size=(0.0, 0.5, 0.0), outer=0.25, invert=1))
+    elif case == "plane":
+        sc.add_clip(scenes.clipping_shape(capi.SHAPE_PLANE, *scenes._rigid(R, (0.0, 0.05, 0.0))))
+    elif case == "plane_inverted":
+        sc.add_clip(scenes.clipping_shape(capi.SHAPE_PLANE, *scenes._rigid(R, (0.0, 0.05, 0.0)), invert=1))
+    else:
+        sc.add_clip(scenes.clipping_box((0.3, 0.0, 0.0), (0.4, 1.1, 1.1)))
+        sc.add_clip(scenes.clipping_box((-0.3, 0.0, 0.0), (0.3, 0.4, 1.1)))
+    run_parity(sc, passes=2)
+
+
+def test_fully_clipped_volume_is_skipped(fresh):
+    """A clipping box around the whole volume: every macrocell is clipped away, so the march
+    skips through it in a few iterations per ray while counting every sample the reference
+    takes, and nothing scatters."""
+    sc = scenes.config_ct(scale=0.125, bilateral=False)
+    sc.add_clip(scenes.clipping_box((0.0, 0.0, 0.0), (1.2, 1.2, 1.2)))
+    run_parity(sc, passes=1)
+    r = er.Renderer(sc)
+    r.set_counting(True)
+    r.reset_counters()
+    r.render_estimate()
+    c = r.counters()
+    print("[fully clipped]", c, flush=True)
+    assert c["scatter_events"] == 0 and c["primary_fetches"] == 0
+    assert c["primary_samples"] > 50 * c["paths"]
+    assert c["march_iterations"] <= 8 * c["paths"]
+    r.close()
+
+
 def test_shading_modes(fresh):
     for mode in (capi.SHADING_BRDF_ONLY, capi.SHADING_MODULATION, capi.SHADING_THRESHOLD,
                  capi.SHADING_GRADIENT_MAGNITUDE):
