@@ -6,7 +6,9 @@ lights + a uniform environment sphere light, hybrid BRDF/phase shading keyed on 
 magnitude, 8-node transfer function, bilateral display filter, 64 progressive passes. A step
 is one progressive pass over the whole film (RenderEstimate: trace + 3x3 frame filter +
 running average + tonemap + bilateral). Inputs are resident on the device before timing;
-`value` = paths / wall time of the timed passes.
+`value` = paths / wall time of the timed passes. Each workload is first autotuned
+(er_autotune: brick, tile, workgroups and shadow waves per CU; outside the timed region,
+which it leaves where it was; `--no-tune` keeps the defaults); `autotune` reports the choice.
 
 Also reported, each on its own workload:
   roofline      the primary-march kernel (the dominant one): algorithmic bytes = 16 B per
@@ -121,11 +123,23 @@ def timed_passes(er, r, steps):
     return time.perf_counter() - t0, timing
 
 
-def bench_small(er, scenes, threads, no_cpu):
+def tune_summary(r):
+    """er_autotune on the scene (outside the timed region; the render state is restored):
+    brick, tile, primary workgroups and shadow waves per CU."""
+    t0 = time.perf_counter()
+    t = r.autotune(passes=0)
+    return {"brick": t["brick"], "tile": [t["tile_w"], t["tile_h"]], "blocks_per_cu": t["blocks_per_cu"],
+            "shadow_waves_per_cu": t["shadow_waves_per_cu"], "trials": t["trials"],
+            "ms_per_pass": round(t["ms_per_pass"], 4), "default_ms_per_pass": round(t["default_ms_per_pass"], 4),
+            "tune_s": round(time.perf_counter() - t0, 3)}
+
+
+def bench_small(er, scenes, threads, no_cpu, tune=True):
     """The 64^3 blob workload: 16 passes; more if the RMSE gate needs them."""
     sc = scenes.config_small()
     r = er.Renderer(sc)
     r.seed(4321)
+    tn = tune_summary(r) if tune else None
     r.render_passes(2)
     r.rebind()
     passes, wall = 0, 0.0
@@ -140,7 +154,7 @@ def bench_small(er, scenes, threads, no_cpu):
     W, H = r.W, r.H
     value = W * H * passes / wall / 1e6
     out = {"workload": "blobs_64c_256x256", "passes": passes, "mpaths_per_s": round(value, 2),
-           "ms_per_frame": round(wall * 1e3 / passes, 4), "rmse": rm}
+           "ms_per_frame": round(wall * 1e3 / passes, 4), "rmse": rm, "autotune": tn}
     if not no_cpu:
         v, dt = cpu_rate(scenes.config_small, 4, threads)
         out["cpu_mpaths_per_s"] = round(v, 4)
@@ -150,18 +164,20 @@ def bench_small(er, scenes, threads, no_cpu):
     return out
 
 
-def bench_large(er, scenes):
+def bench_large(er, scenes, tune=True):
     """The large anisotropic workload (512 x 512 x 460, 0.7 x 0.7 x 1.0 spacing, clipping box,
     1920 x 1080, 3 plane lights, hybrid shading): throughput over its 256 passes. No RMSE
     gate here: an FP64 reference at this size is beyond a bounded CPU run."""
     sc = scenes.config_large()
     r = er.Renderer(sc)
     r.seed(99)
+    tn = tune_summary(r) if tune else None
     r.render_passes(2)
     r.rebind()
     dt, t = timed_passes(er, r, sc.passes)
     out = {"workload": "tubes_512x512x460_1920x1080_clip", "passes": sc.passes,
-           "mpaths_per_s": round(r.W * r.H * sc.passes / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / sc.passes, 4)}
+           "mpaths_per_s": round(r.W * r.H * sc.passes / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / sc.passes, 4),
+           "autotune": tn}
     r.close()
     return out
 
@@ -268,6 +284,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-count", action="store_true", help="skip the counting replay (roofline)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="render with the default launch configuration instead of the autotuned one")
     ap.add_argument("--cpu-passes", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -302,6 +320,7 @@ def main():
     r = er.Renderer(sc)
     r.seed(1234)
     W, H = r.W, r.H
+    tune = None if args.no_tune else tune_summary(r)
     if args.warmup > 0:
         r.render_passes(args.warmup)
     s1, s2 = r.get_seeds()
@@ -365,13 +384,13 @@ def main():
                          "primary": round(timing.primary_ms / args.steps, 4),
                          "trace_overlapping": round(timing.trace_ms / args.steps, 4),
                          "post": round(timing.post_ms / args.steps, 4)},
-        "rmse": rm, "roofline": roofline, "cpu_baseline": cpu, "counts": counts,
+        "rmse": rm, "roofline": roofline, "cpu_baseline": cpu, "counts": counts, "autotune": tune,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     if not args.no_extra:
-        out["small"] = bench_small(er, scenes, threads, args.no_cpu)
-        out["large"] = bench_large(er, scenes)
+        out["small"] = bench_small(er, scenes, threads, args.no_cpu, not args.no_tune)
+        out["large"] = bench_large(er, scenes, not args.no_tune)
         out["adjoint"] = bench_adjoint(er, scenes)
         out["batched"] = bench_batched(er, scenes, args.batch_jobs)
     er.shutdown()

@@ -36,8 +36,8 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
                                        unsigned int* counts, void* events, void* jobs, unsigned char* vis,
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
-                                       int primary_blocks, int tile_shift, hipStream_t st, hipStream_t st2,
-                                       hipEvent_t shaded, hipEvent_t after_primary);
+                                       int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
+                                       hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary);
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
@@ -275,7 +275,7 @@ struct Tracer {
     /* launch configuration (er_autotune / er_set_tune): brick edge 2^mc_shift (0: the
      * volume's default), primary tiles 2^tile_shift pixels wide, persistent primary
      * workgroups per CU (0: the library default) */
-    int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0;
+    int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0, shadow_waves_per_cu = 4;
     hipStream_t stream = nullptr;     /* primary + shading, uploads, readbacks */
     hipStream_t stream2 = nullptr;    /* shadow marches, combine, post */
     ~Tracer()
@@ -815,6 +815,7 @@ bool use_bilateral(const Tracer& t)
 }
 
 int primary_blocks(const Tracer& t) { return t.blocks_per_cu > 0 ? g_cus * t.blocks_per_cu : g_primary_blocks; }
+int shadow_blocks(const Tracer& t) { return std::max(1, g_cus * t.shadow_waves_per_cu / 4); }
 
 /* One pass on the tracer's two streams (no host synchronisation). Timing events, when given:
  * e[0] first stream before the primary kernel, e[1] after it, e[2] second stream after the
@@ -833,7 +834,7 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
     const size_t lds_bits = (size_t)t.host_params.mc_words * 4;
     HIP_TRY(erd_launch_trace(t.params.ptr, S.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, S.queue.ptr, S.qcount.ptr,
                              S.events.ptr, S.jobs.ptr, S.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
-                             split, t.lds_bytes, lds_bits, shade_blocks, primary_blocks(t), t.tile_shift, t.stream,
+                             split, t.lds_bytes, lds_bits, shade_blocks, primary_blocks(t), shadow_blocks(t), t.tile_shift, t.stream,
                              t.stream2, S.shaded, e ? e[1] : nullptr));
     if (e) HIP_TRY(hipEventRecord(e[2], t.stream2));
     HIP_TRY(erd_launch_post(t.params.ptr, S.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations,
@@ -920,6 +921,7 @@ void fill_tune(const Tracer& t, er_tune& r)
     r.tile_w = 1 << t.tile_shift;
     r.tile_h = 64 >> t.tile_shift;
     r.blocks_per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : g_primary_blocks / std::max(1, g_cus);
+    r.shadow_waves_per_cu = t.shadow_waves_per_cu;
 }
 
 int apply_tune(Tracer& t, const er_tune& r)
@@ -931,12 +933,15 @@ int apply_tune(Tracer& t, const er_tune& r)
     if ((1 << ts) != r.tile_w || ts < 2 || ts > 5 || r.tile_w * r.tile_h != 64)
         return set_error(ER_ERROR, "tile must be 4x16, 8x8, 16x4 or 32x2 pixels");
     if (r.blocks_per_cu < 1 || r.blocks_per_cu > 16) return set_error(ER_ERROR, "blocks per CU must be in [1, 16]");
+    if (r.shadow_waves_per_cu < 1 || r.shadow_waves_per_cu > 64)
+        return set_error(ER_ERROR, "shadow waves per CU must be in [1, 64]");
     const Volume* v = g_volumes.find(t.desc.volume_id);
     if (v && !brick_fits(v->desc, bs)) return set_error(ER_ERROR, "a %d-cell brick needs more than %zu macrocells", r.brick,
                                                         ER_MAX_MACROCELLS);
     t.mc_shift = bs;
     t.tile_shift = ts;
     t.blocks_per_cu = r.blocks_per_cu;
+    t.shadow_waves_per_cu = r.shadow_waves_per_cu;
     t.built_version = 0;
     return ER_OK;
 }
@@ -1023,9 +1028,12 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
     } else if (axis == 1) {
         for (int ts = 2; ts <= 5; ++ts)
             if ((1 << ts) != cur.tile_w) { er_tune x = cur; x.tile_w = 1 << ts; x.tile_h = 64 >> ts; c.push_back(x); }
-    } else {
+    } else if (axis == 2) {
         for (int b = 3; b <= 9; ++b)
             if (b != cur.blocks_per_cu) { er_tune x = cur; x.blocks_per_cu = b; c.push_back(x); }
+    } else {
+        for (int w = 2; w <= 8; w *= 2)
+            if (w != cur.shadow_waves_per_cu) { er_tune x = cur; x.shadow_waves_per_cu = w; c.push_back(x); }
     }
     return c;
 }
@@ -1344,7 +1352,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     if (t.W == 0 || t.H == 0) return set_error(ER_ERROR, "tracer %d has an empty film", tracer_id);
     Snapshot snap;
     if ((st = snap.save(t))) return st;
-    const int mc0 = t.mc_shift, ts0 = t.tile_shift, bpc0 = t.blocks_per_cu;
+    const int mc0 = t.mc_shift, ts0 = t.tile_shift, bpc0 = t.blocks_per_cu, sw0 = t.shadow_waves_per_cu;
     er_tune cur{};
     fill_tune(t, cur);
     int trials = 0;
@@ -1371,14 +1379,17 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     if (!st) st = measure(cur, cur_ms);
     const float default_ms = cur_ms;
     std::vector<const Volume*> vols{g_volumes.find(t.desc.volume_id)};
-    for (int axis = 0; axis < 3 && !st; ++axis)
+    for (int axis = 0; axis < 4 && !st; ++axis)
         for (const er_tune& c : tune_candidates(axis, cur, vols)) {
             float ms = 0.0f;
             if ((st = measure(c, ms))) break;
             if (ms < ER_TUNE_MARGIN * cur_ms) { cur = c; cur_ms = ms; }
         }
     if (!st) st = apply_tune(t, cur);
-    if (st) { t.mc_shift = mc0; t.tile_shift = ts0; t.blocks_per_cu = bpc0; t.built_version = 0; }
+    if (st) {
+        t.mc_shift = mc0; t.tile_shift = ts0; t.blocks_per_cu = bpc0; t.shadow_waves_per_cu = sw0;
+        t.built_version = 0;
+    }
     const int rst = snap.restore(t);
     if (st) return st;
     if (rst) return rst;

@@ -292,19 +292,83 @@ __global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp,
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
-/* Shadow-march kernel over the shadow-job queue (counts[1] entries). */
+#ifndef ER_SHADOW_REFILL
+#define ER_SHADOW_REFILL 16
+#endif
+
+/* Shadow-march kernel over the shadow-job queue (counts[1] entries), one march per lane. The
+ * marches of one queue differ in length by orders of magnitude (a ray through empty space ends
+ * after a few skips, one through tissue takes hundreds of samples), so a wave does not wait for
+ * its longest march. The grid is persistent (by default one workgroup per CU: fewer waves leave
+ * the CUs to the primary kernel of the next pass, which runs beside it); wave w first takes jobs
+ * [64 w, 64 w + 64), and whenever at least ER_SHADOW_REFILL of its lanes are idle they take the
+ * next jobs beyond the static chunks from a counter (counts[3]); the wave steps all its live
+ * marches one iteration at a time. The job order does not matter: each march is independent and
+ * writes only its own visibility byte. */
 template<int FULL, int COUNT, int LUT_LDS>
-__global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp, const unsigned int* __restrict__ counts,
+__global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp, unsigned int* __restrict__ counts,
                                                const ShadowJob* __restrict__ jobq, unsigned char* __restrict__ vis,
                                                unsigned long long* __restrict__ counters)
 {
     extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
-    const Tables T = stage_tables<LUT_LDS>(p, lds);
     const unsigned int n = counts[1];
+    const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned int static_end = ((gridDim.x * blockDim.x) >> 6) * 64u;
+    if (blockIdx.x * blockDim.x >= n) return;   /* no static chunk in the workgroup (workgroup-uniform) */
+    const Tables T = stage_tables<LUT_LDS>(p, lds);
+    const int lane = threadIdx.x & 63;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-        shadow_item<FULL, COUNT>(p, T, jobq[j], vis, cnt);
+    const float StepSize = p.step_shadow * p.minstep;
+    NoVisit nv;
+    MarchState m;
+    unsigned int slot = 0;
+    bool busy = false, drained = static_end >= n;
+    unsigned int j = wave * 64u + (unsigned int)lane;   /* the job this lane starts next, >= n: none */
+    for (;;) {
+        if (j < n) {
+            const ShadowJob job = jobq[j];
+            slot = 2u * job.event + job.slot;
+            Ray r;
+            r.O = mk(job.Ox, job.Oy, job.Oz);
+            r.D = mk(job.Dx, job.Dy, job.Dz);
+            r.MinT = 0.0f;
+            r.MaxT = job.MaxT;
+            Rng rng;
+            rng.s0 = job.s0;
+            rng.s1 = job.s1;
+            float nearT, farT;
+            if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT) ||
+                !march_begin(p, r, nearT, farT, StepSize, rng, -1.0f, m)) {
+                vis[slot] = 1;
+            } else if (!(m.Sum < m.S)) {
+                vis[slot] = 0;
+            } else {
+                busy = true;
+            }
+            j = n;
+        }
+        if (busy) {
+            f3 P;
+            const int st = march_step<FULL, COUNT, ER_BATCH_SHADOW, NoVisit>(p, T, m, false, StepSize, cnt, nv, P);
+            if (st != 0) {
+                vis[slot] = st > 0 ? 0 : 1;
+                busy = false;
+            }
+        }
+        const unsigned long long idle = __ballot(!busy);
+        const int nidle = __popcll(idle);
+        if (drained) {
+            if (nidle == 64) break;
+        } else if (nidle >= ER_SHADOW_REFILL) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(&counts[3], (unsigned int)nidle);
+            base = static_end + __shfl(base, leader);
+            if (base + (unsigned int)nidle >= n) drained = true;
+            if (!busy) j = base + (unsigned int)__popcll(idle & ((1ull << lane) - 1ull));
+        }
+    }
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
@@ -576,9 +640,9 @@ extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* jo
 template<int FULL, int COUNT, int LUT_LDS>
 static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uint32_t* s2, erd::ScatterRec* queue,
                        unsigned int* counts, erd::EventRec* events, erd::ShadowJob* jobs, unsigned char* vis,
-                       unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks, int split,
-                       size_t lds, size_t lds_bits, hipStream_t st, hipStream_t st2, hipEvent_t shaded,
-                       hipEvent_t after_primary)
+                       unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks,
+                       int shadow_blocks, int split, size_t lds, size_t lds_bits, hipStream_t st, hipStream_t st2,
+                       hipEvent_t shaded, hipEvent_t after_primary)
 {
     hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS>), dim3(nblocks), dim3(256), lds, st, p, frame, s1, s2, queue,
                        counts, counters, tiles_x, ntiles);
@@ -588,8 +652,8 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
                            s2, (const erd::ScatterRec*)queue, counts, events, jobs, counters);
         (void)hipEventRecord(shaded, st);
         (void)hipStreamWaitEvent(st2, shaded, 0);
-        hipLaunchKernelGGL((erd::k_shadow<FULL, COUNT, LUT_LDS>), dim3(2 * shade_blocks), dim3(256), lds, st2, p,
-                           (const unsigned int*)counts, (const erd::ShadowJob*)jobs, vis, counters);
+        hipLaunchKernelGGL((erd::k_shadow<FULL, COUNT, LUT_LDS>), dim3(shadow_blocks), dim3(256), lds, st2, p, counts,
+                           (const erd::ShadowJob*)jobs, vis, counters);
         hipLaunchKernelGGL(erd::k_finish, dim3(shade_blocks), dim3(256), 0, st2, (const unsigned int*)counts,
                            (const erd::EventRec*)events, (const unsigned char*)vis, frame);
     } else {
@@ -606,8 +670,8 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
                                        unsigned int* counts, void* events, void* jobs, unsigned char* vis,
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
-                                       int primary_blocks, int tile_shift, hipStream_t st, hipStream_t st2,
-                                       hipEvent_t shaded, hipEvent_t after_primary)
+                                       int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
+                                       hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary)
 {
     const int tw = 1 << tile_shift, th = 64 >> tile_shift;
     const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
@@ -618,7 +682,8 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
     erd::EventRec* ev = (erd::EventRec*)events;
     erd::ShadowJob* jb = (erd::ShadowJob*)jobs;
 #define ER_ALL(F, C, L) launch_all<F, C, L>(p, frame, s1, s2, q, counts, ev, jb, vis, counters, tiles_x, ntiles, nblocks, \
-                                            shade_blocks, split, lds_bytes, lds_bits, st, st2, shaded, after_primary)
+                                            shade_blocks, std::max(1, shadow_blocks), split, lds_bytes, lds_bits, st, st2, \
+                                            shaded, after_primary)
     if (full) {
         if (count) { if (lut_lds) ER_ALL(1, 1, 1); else ER_ALL(1, 1, 0); }
         else { if (lut_lds) ER_ALL(1, 0, 1); else ER_ALL(1, 0, 0); }

@@ -96,10 +96,30 @@ def test_autotune_restores_the_render_and_caches(fresh):
 def test_set_tune_rejects_invalid_configurations(fresh):
     sc = scenes.config_small(scale=0.25)
     r = er.Renderer(sc, seed=1)
-    for bad in [(3, 8, 7), (8, 6, 7), (8, 64, 7), (8, 8, 0), (8, 8, 17), (128, 8, 7)]:
+    for bad in [(3, 8, 7), (8, 6, 7), (8, 64, 7), (8, 8, 0), (8, 8, 17), (128, 8, 7), (8, 8, 7, 0), (8, 8, 7, 65)]:
         with pytest.raises(er.ExposureRenderError):
             r.set_tune(*bad)
     r.close()
+
+
+def test_shadow_grid_does_not_change_the_image(fresh):
+    """The shadow marches are independent and each writes its own visibility: any number of
+    persistent shadow waves (which changes which lane marches which ray, and when) renders the
+    same image bit for bit."""
+    sc = scenes.config_ct(scale=0.25)
+    out = []
+    for waves in (1, 2, 4, 8, 64):
+        print(f"[tune] shadow waves per CU {waves}", flush=True)
+        r = er.Renderer(sc, seed=9)
+        r.set_tune(8, 8, 7, waves)
+        assert r.get_tune()["shadow_waves_per_cu"] == waves
+        r.set_counting(True)
+        r.reset_counters()
+        r.render_passes(3)
+        out.append((r.running().copy(), r.get_estimate().copy(), r.counters()))
+        r.close()
+    for run, disp, cnt in out[1:]:
+        assert np.array_equal(run, out[0][0]) and np.array_equal(disp, out[0][1]) and cnt == out[0][2]
 
 
 def test_batch_with_tuned_jobs_matches_sequential(fresh):
