@@ -206,25 +206,6 @@ __device__ __forceinline__ void shade_chunk(const DevParams& p, const Tables& T,
     }
 }
 
-/* ScatterEventInVolume (raymarching.h:73-113) for one deferred shadow ray; writes the
- * visibility of the event's contribution. */
-template<int FULL, int COUNT>
-__device__ __forceinline__ void shadow_item(const DevParams& p, const Tables& T, const ShadowJob& job,
-                                            unsigned char* __restrict__ vis, Cnt& cnt)
-{
-    Ray r;
-    r.O = mk(job.Ox, job.Oy, job.Oz);
-    r.D = mk(job.Dx, job.Dy, job.Dz);
-    r.MinT = 0.0f;
-    r.MaxT = job.MaxT;
-    Rng rng;
-    rng.s0 = job.s0;
-    rng.s1 = job.s1;
-    float Tt; f3 P;
-    const bool hit = march<FULL, COUNT, ER_BATCH_SHADOW>(p, T, r, false, rng, Tt, P, cnt);
-    vis[2 * job.event + job.slot] = hit ? 0 : 1;
-}
-
 /* The event's radiance from its contributions and their visibility (the tail of
  * UniformSampleOneLight, transport.h:152-155, and SingleScattering, singlescattering.h:96-99). */
 __device__ __forceinline__ void finish_item(const EventRec& ev, unsigned int q, const unsigned char* __restrict__ vis,
@@ -296,35 +277,24 @@ __global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp,
 #define ER_SHADOW_REFILL 16
 #endif
 
-/* Shadow-march kernel over the shadow-job queue (counts[1] entries), one march per lane. The
- * marches of one queue differ in length by orders of magnitude (a ray through empty space ends
- * after a few skips, one through tissue takes hundreds of samples), so a wave does not wait for
- * its longest march. The grid is persistent (by default one workgroup per CU: fewer waves leave
- * the CUs to the primary kernel of the next pass, which runs beside it); wave w first takes jobs
- * [64 w, 64 w + 64), and whenever at least ER_SHADOW_REFILL of its lanes are idle they take the
- * next jobs beyond the static chunks from a counter (counts[3]); the wave steps all its live
- * marches one iteration at a time. The job order does not matter: each march is independent and
- * writes only its own visibility byte. */
-template<int FULL, int COUNT, int LUT_LDS>
-__global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp, unsigned int* __restrict__ counts,
-                                               const ShadowJob* __restrict__ jobq, unsigned char* __restrict__ vis,
-                                               unsigned long long* __restrict__ counters)
+/* The shadow marches of one queue of n jobs (jobq), one march per lane. The marches differ in
+ * length by orders of magnitude (a ray through empty space ends after a few skips, one through
+ * tissue takes hundreds of samples), so a wave does not wait for its longest march: the lane
+ * starts job j (none if j >= n), and whenever at least ER_SHADOW_REFILL lanes are idle they
+ * claim the next jobs from *claim (job claim_base + the counter's value) with one atomic and
+ * start them, and the wave steps all its live marches one iteration at a time. `drained`:
+ * nothing is left to claim. The job order does not matter: each march is independent and
+ * writes only its own visibility byte. Every lane of the wave must call this. */
+template<int FULL, int COUNT>
+__device__ __forceinline__ void shadow_marches(const DevParams& p, const Tables& T, const ShadowJob* __restrict__ jobq,
+                                               unsigned char* __restrict__ vis, unsigned int* claim, unsigned int n,
+                                               unsigned int j, unsigned int claim_base, bool drained, int lane, Cnt& cnt)
 {
-    extern __shared__ uint32_t lds[];
-    const DevParams& p = *pp;
-    const unsigned int n = counts[1];
-    const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const unsigned int static_end = ((gridDim.x * blockDim.x) >> 6) * 64u;
-    if (blockIdx.x * blockDim.x >= n) return;   /* no static chunk in the workgroup (workgroup-uniform) */
-    const Tables T = stage_tables<LUT_LDS>(p, lds);
-    const int lane = threadIdx.x & 63;
-    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const float StepSize = p.step_shadow * p.minstep;
     NoVisit nv;
     MarchState m;
     unsigned int slot = 0;
-    bool busy = false, drained = static_end >= n;
-    unsigned int j = wave * 64u + (unsigned int)lane;   /* the job this lane starts next, >= n: none */
+    bool busy = false;
     for (;;) {
         if (j < n) {
             const ShadowJob job = jobq[j];
@@ -363,12 +333,34 @@ __global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp
         } else if (nidle >= ER_SHADOW_REFILL) {
             const int leader = __ffsll((long long)idle) - 1;
             unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(&counts[3], (unsigned int)nidle);
-            base = static_end + __shfl(base, leader);
+            if (lane == leader) base = atomicAdd(claim, (unsigned int)nidle);
+            base = claim_base + __shfl(base, leader);
             if (base + (unsigned int)nidle >= n) drained = true;
             if (!busy) j = base + (unsigned int)__popcll(idle & ((1ull << lane) - 1ull));
         }
     }
+}
+
+/* Shadow-march kernel over the shadow-job queue (counts[1] entries). The grid is persistent
+ * (its size, in waves per CU, is a launch parameter: fewer waves leave the CUs to the primary
+ * kernel of the next pass, which runs beside it); wave w first takes jobs [64 w, 64 w + 64),
+ * and idle lanes then claim the jobs beyond the static chunks from counts[3]. */
+template<int FULL, int COUNT, int LUT_LDS>
+__global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp, unsigned int* __restrict__ counts,
+                                               const ShadowJob* __restrict__ jobq, unsigned char* __restrict__ vis,
+                                               unsigned long long* __restrict__ counters)
+{
+    extern __shared__ uint32_t lds[];
+    const DevParams& p = *pp;
+    const unsigned int n = counts[1];
+    const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned int static_end = ((gridDim.x * blockDim.x) >> 6) * 64u;
+    if (blockIdx.x * blockDim.x >= n) return;   /* no static chunk in the workgroup (workgroup-uniform) */
+    const Tables T = stage_tables<LUT_LDS>(p, lds);
+    const int lane = threadIdx.x & 63;
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    shadow_marches<FULL, COUNT>(p, T, jobq, vis, &counts[3], n, wave * 64u + (unsigned int)lane, static_end,
+                                static_end >= n, lane, cnt);
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
@@ -560,8 +552,9 @@ __global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J,
 }
 
 /* Shadow marches of a batch round, job by job like k_primary_b: a workgroup starts at the job
- * holding its proportional share of the round's shadow jobs, its waves claim 64 of that job's
- * shadow jobs at a time (counts[3]), then it moves on to the next job with shadow jobs left. */
+ * holding its proportional share of the round's shadow jobs, its waves' idle lanes claim that
+ * job's shadow jobs (counts[3], shadow_marches), then it moves on to the next job with shadow
+ * jobs left. */
 template<int FULL, int LDS>
 __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
 {
@@ -586,14 +579,7 @@ __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J
         const DevParams& p = c_batch_params[j];
         const Tables T = stage_job_tables<LDS>(p, lds);
         const unsigned int n = __builtin_amdgcn_readfirstlane(B.counts[1]);
-        for (;;) {
-            unsigned int first = 0;
-            if (lane == 0) first = atomicAdd(&B.counts[3], 64u);
-            first = __builtin_amdgcn_readfirstlane(first);
-            if (first >= n) break;
-            const unsigned int jq = first + lane;
-            if (jq < n) shadow_item<FULL, 0>(p, T, B.jobs[jq], B.vis, cnt);
-        }
+        shadow_marches<FULL, 0>(p, T, B.jobs, B.vis, &B.counts[3], n, n, 0u, false, lane, cnt);
         __syncthreads();
         if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 3);
         __syncthreads();
