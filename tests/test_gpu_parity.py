@@ -137,6 +137,28 @@ def test_clipping_shapes(fresh, case):
     run_parity(sc, passes=2)
 
 
+@pytest.mark.parametrize("seed", [11, 12, 13, 14])
+def test_clipping_random_affine(fresh, seed):
+    """Random clipping objects with scaled and sheared (non-rigid) transforms, plain and
+    inverted, one to three at once: the clip-aware occupancy bits never skip a sample the
+    oracle keeps."""
+    rng = np.random.default_rng(seed)
+    sc = scenes.config_ct(scale=0.125, bilateral=False)
+    sc.name = f"ct_clip_random_{seed}"
+    for _ in range(int(rng.integers(1, 4))):
+        A = np.eye(3) + rng.uniform(-0.3, 0.3, size=(3, 3))
+        A = A @ _rot(int(rng.integers(0, 3)), float(rng.uniform(0, np.pi)))
+        tm = np.eye(4)
+        tm[:3, :3] = A
+        tm[:3, 3] = rng.uniform(-0.3, 0.3, size=3)
+        inv = np.linalg.inv(tm)
+        shape = [capi.SHAPE_BOX, capi.SHAPE_SPHERE, capi.SHAPE_CYLINDER, capi.SHAPE_PLANE][int(rng.integers(0, 4))]
+        size = tuple(rng.uniform(0.2, 0.9, size=3))
+        sc.add_clip(scenes.clipping_shape(shape, tm, inv, size=size, outer=float(rng.uniform(0.15, 0.45)),
+                                          invert=int(rng.integers(0, 2))))
+    run_parity(sc, passes=1)
+
+
 def test_fully_clipped_volume_is_skipped(fresh):
     """A clipping box around the whole volume: every macrocell is clipped away, so the march
     skips through it in a few iterations per ray while counting every sample the reference
