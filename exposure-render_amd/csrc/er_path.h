@@ -63,6 +63,10 @@ __device__ __forceinline__ bool clipped(const DevParams& p, f3 P)
 #ifndef ER_BATCH_SHADOW
 #define ER_BATCH_SHADOW 8
 #endif
+/* idle lanes of a shadow-march wave that claim new jobs together */
+#ifndef ER_SHADOW_REFILL
+#define ER_SHADOW_REFILL 16
+#endif
 
 /* ---- free-flight sampling --------------------------------------------------------------- */
 

@@ -273,10 +273,6 @@ __global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp,
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
-#ifndef ER_SHADOW_REFILL
-#define ER_SHADOW_REFILL 16
-#endif
-
 /* The shadow marches of one queue of n jobs (jobq), one march per lane. The marches differ in
  * length by orders of magnitude (a ray through empty space ends after a few skips, one through
  * tissue takes hundreds of samples), so a wave does not wait for its longest march: the lane
