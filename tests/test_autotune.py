@@ -106,7 +106,7 @@ def test_shadow_grid_does_not_change_the_image(fresh):
     """The shadow marches are independent and each writes its own visibility: any number of
     persistent shadow waves (which changes which lane marches which ray, and when) renders the
     same image bit for bit."""
-    sc = scenes.config_ct(scale=0.25)
+    sc = scenes.config_ct(scale=0.5)   # enough shadow jobs to outrun the static chunks at 1-2 waves per CU
     out = []
     for waves in (1, 2, 4, 8, 64):
         print(f"[tune] shadow waves per CU {waves}", flush=True)
@@ -118,6 +118,7 @@ def test_shadow_grid_does_not_change_the_image(fresh):
         r.render_passes(3)
         out.append((r.running().copy(), r.get_estimate().copy(), r.counters()))
         r.close()
+    print(f"[tune] shadow rays per pass {out[0][2]['shadow_rays'] // 3}", flush=True)
     for run, disp, cnt in out[1:]:
         assert np.array_equal(run, out[0][0]) and np.array_equal(disp, out[0][1]) and cnt == out[0][2]
 
