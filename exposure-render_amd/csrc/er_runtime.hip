@@ -218,14 +218,18 @@ struct Volume {
     }
 };
 
-/* one occupancy bit per macrocell must fit the 16 KB the primary kernel stages in LDS */
-const size_t ER_MAX_MACROCELLS = 131072;
-
+/* The occupancy bits (one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a
+ * side) must fit the 32 KB the march kernels stage in LDS; the opacity table takes at most
+ * 32 KB more, so a workgroup's tables stay within 64 KB. */
+const size_t ER_MAX_BITS_BYTES = 32768;
 bool brick_fits(const er_volume_desc& d, int shift)
 {
-    const int E = 1 << shift;
-    return (size_t)((d.resolution[0] + E - 1) / E) * ((d.resolution[1] + E - 1) / E) * ((d.resolution[2] + E - 1) / E) <=
-           ER_MAX_MACROCELLS;
+    const size_t E = (size_t)1 << shift, G = (size_t)1 << ER_GROUP_SHIFT;
+    size_t mc[3];
+    for (int i = 0; i < 3; ++i) mc[i] = ((size_t)std::max(d.resolution[i], 1) + E - 1) / E;
+    const size_t n = mc[0] * mc[1] * mc[2];
+    const size_t g = ((mc[0] + G - 1) / G) * ((mc[1] + G - 1) / G) * ((mc[2] + G - 1) / G);
+    return 4 * ((n + 31) / 32 + (g + 31) / 32) <= ER_MAX_BITS_BYTES;
 }
 
 /* The state of one pass in flight: its frame estimate and its work queues. A tracer keeps
@@ -471,8 +475,8 @@ int upload_volume(Volume& v)
     const size_t nc = (size_t)(rx + 1) * (ry + 1) * (rz + 1);
     if ((st = v.cells.alloc(nc))) return st;
     HIP_TRY(erd_launch_build_cells(v.vox.ptr, v.cells.ptr, rx, ry, rz, 0));
-    /* default macrocell edge: 8 cells, coarser when one occupancy bit per macrocell would
-     * exceed 16 KB */
+    /* default macrocell edge: 8 cells, coarser when the occupancy bits would exceed their
+     * 32 KB of LDS */
     v.mc_shift = 3;
     while (!brick_fits(d, v.mc_shift) && v.mc_shift < 8) v.mc_shift++;
     static uint64_t uploads = 0;
@@ -936,8 +940,8 @@ int apply_tune(Tracer& t, const er_tune& r)
     if (r.shadow_waves_per_cu < 1 || r.shadow_waves_per_cu > 64)
         return set_error(ER_ERROR, "shadow waves per CU must be in [1, 64]");
     const Volume* v = g_volumes.find(t.desc.volume_id);
-    if (v && !brick_fits(v->desc, bs)) return set_error(ER_ERROR, "a %d-cell brick needs more than %zu macrocells", r.brick,
-                                                        ER_MAX_MACROCELLS);
+    if (v && !brick_fits(v->desc, bs))
+        return set_error(ER_ERROR, "a %d-cell brick needs more than %zu bytes of occupancy bits", r.brick, ER_MAX_BITS_BYTES);
     t.mc_shift = bs;
     t.tile_shift = ts;
     t.blocks_per_cu = r.blocks_per_cu;

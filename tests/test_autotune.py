@@ -58,6 +58,19 @@ def test_ct_brick_sizes_render_the_same_image(fresh):
         _same(base, _render(sc, 2, tune), "tune")
 
 
+def test_large_volume_bricks_render_the_same_image(fresh):
+    """The full 512 x 512 x 460 clipped volume (1.9 GB of cell records) at 960 x 540: the
+    default 8-cell brick (its occupancy bits take 29.5 KB of LDS) and the 16- and 32-cell
+    bricks render the same passes bit for bit, with the reference sample counts."""
+    sc = scenes.config_large()
+    sc.tracer.camera.film_size[0], sc.tracer.camera.film_size[1] = 960, 540
+    print("[tune] large 512x512x460 bricks", flush=True)
+    base = _render(sc, 2)
+    assert base[5]["brick"] == 8
+    for tune in [(16, 8, 7), (32, 16, 5)]:
+        _same(base, _render(sc, 2, tune), "tune")
+
+
 def test_autotune_restores_the_render_and_caches(fresh):
     sc = scenes.config_small(scale=0.5)
     print("[tune] autotune restore", flush=True)
