@@ -219,17 +219,20 @@ struct Volume {
 };
 
 /* The occupancy bits (one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a
- * side) must fit the 32 KB the march kernels stage in LDS; the opacity table takes at most
- * 32 KB more, so a workgroup's tables stay within 64 KB. */
-const size_t ER_MAX_BITS_BYTES = 32768;
-bool brick_fits(const er_volume_desc& d, int shift)
+ * side) and, when it takes at most 32 KB, the opacity table (one float per intensity up to the
+ * volume's maximum) are staged in LDS by the march kernels; together they must fit 52 KB, so
+ * that three march workgroups (the primary kernel's register-bound occupancy) share a CU's
+ * 160 KB. */
+const size_t ER_MAX_TABLES_BYTES = 53248;
+bool brick_fits(const er_volume_desc& d, uint16_t vmax, int shift)
 {
     const size_t E = (size_t)1 << shift, G = (size_t)1 << ER_GROUP_SHIFT;
     size_t mc[3];
     for (int i = 0; i < 3; ++i) mc[i] = ((size_t)std::max(d.resolution[i], 1) + E - 1) / E;
     const size_t n = mc[0] * mc[1] * mc[2];
     const size_t g = ((mc[0] + G - 1) / G) * ((mc[1] + G - 1) / G) * ((mc[2] + G - 1) / G);
-    return 4 * ((n + 31) / 32 + (g + 31) / 32) <= ER_MAX_BITS_BYTES;
+    const size_t lut = ((size_t)vmax + 1) * sizeof(float);
+    return 4 * ((n + 31) / 32 + (g + 31) / 32) + (lut <= 32768 ? lut : 0) <= ER_MAX_TABLES_BYTES;
 }
 
 /* The state of one pass in flight: its frame estimate and its work queues. A tracer keeps
@@ -475,10 +478,10 @@ int upload_volume(Volume& v)
     const size_t nc = (size_t)(rx + 1) * (ry + 1) * (rz + 1);
     if ((st = v.cells.alloc(nc))) return st;
     HIP_TRY(erd_launch_build_cells(v.vox.ptr, v.cells.ptr, rx, ry, rz, 0));
-    /* default macrocell edge: 8 cells, coarser when the occupancy bits would exceed their
-     * 32 KB of LDS */
+    /* default macrocell edge: 8 cells, coarser when the occupancy bits and the opacity table
+     * would not fit their 52 KB of LDS */
     v.mc_shift = 3;
-    while (!brick_fits(d, v.mc_shift) && v.mc_shift < 8) v.mc_shift++;
+    while (!brick_fits(d, v.vmax, v.mc_shift) && v.mc_shift < 8) v.mc_shift++;
     static uint64_t uploads = 0;
     v.upload = ++uploads;
     for (auto& g : v.grids) g.second.mm.release();
@@ -593,7 +596,7 @@ int build_params(Tracer& t)
     P.cstride_y = vd.resolution[0] + 1;
     P.cstride_z = (vd.resolution[0] + 1) * (vd.resolution[1] + 1);
     P.neglog = g_neglog_dev;
-    const int mc_shift = (t.mc_shift > 0 && brick_fits(vd, t.mc_shift)) ? t.mc_shift : vol->mc_shift;
+    const int mc_shift = (t.mc_shift > 0 && brick_fits(vd, vol->vmax, t.mc_shift)) ? t.mc_shift : vol->mc_shift;
     MacroGrid* grid = volume_grid(*vol, mc_shift, st);
     if (!grid) return st;
     for (int i = 0; i < 3; ++i) P.mc_dim[i] = grid->mc[i];
@@ -920,7 +923,7 @@ std::map<std::string, er_tune> g_tune_cache;
 void fill_tune(const Tracer& t, er_tune& r)
 {
     const Volume* v = g_volumes.find(t.desc.volume_id);
-    const int shift = (t.mc_shift > 0 && v && brick_fits(v->desc, t.mc_shift)) ? t.mc_shift : (v ? v->mc_shift : 3);
+    const int shift = (t.mc_shift > 0 && v && brick_fits(v->desc, v->vmax, t.mc_shift)) ? t.mc_shift : (v ? v->mc_shift : 3);
     r.brick = 1 << shift;
     r.tile_w = 1 << t.tile_shift;
     r.tile_h = 64 >> t.tile_shift;
@@ -940,8 +943,9 @@ int apply_tune(Tracer& t, const er_tune& r)
     if (r.shadow_waves_per_cu < 1 || r.shadow_waves_per_cu > 64)
         return set_error(ER_ERROR, "shadow waves per CU must be in [1, 64]");
     const Volume* v = g_volumes.find(t.desc.volume_id);
-    if (v && !brick_fits(v->desc, bs))
-        return set_error(ER_ERROR, "a %d-cell brick needs more than %zu bytes of occupancy bits", r.brick, ER_MAX_BITS_BYTES);
+    if (v && !brick_fits(v->desc, v->vmax, bs))
+        return set_error(ER_ERROR, "a %d-cell brick's occupancy bits and opacity table exceed %zu bytes of LDS", r.brick,
+                         ER_MAX_TABLES_BYTES);
     t.mc_shift = bs;
     t.tile_shift = ts;
     t.blocks_per_cu = r.blocks_per_cu;
@@ -1026,7 +1030,7 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
     if (axis == 0) {
         for (int b = 1; b <= 5; ++b) {
             bool fits = true;
-            for (const Volume* v : vols) fits = fits && v && brick_fits(v->desc, b);
+            for (const Volume* v : vols) fits = fits && v && brick_fits(v->desc, v->vmax, b);
             if (fits && (1 << b) != cur.brick) { er_tune x = cur; x.brick = 1 << b; c.push_back(x); }
         }
     } else if (axis == 1) {
@@ -1439,7 +1443,7 @@ ER_API int er_autotune_batch(const er_batch_item* items, int n, int rounds, er_t
         int s;
         for (int k = 0; k < n; ++k) {
             er_tune x = c;
-            if (vols[k] && !brick_fits(vols[k]->desc, (int)std::log2((double)c.brick))) x.brick = orig[k].brick;
+            if (vols[k] && !brick_fits(vols[k]->desc, vols[k]->vmax, (int)std::log2((double)c.brick))) x.brick = orig[k].brick;
             if ((s = apply_tune(*ts[k], x))) return s;
         }
         return ER_OK;
