@@ -1060,12 +1060,13 @@ ER_API int er_abi_version(void) { return 1; }
 
 ER_API int er_abi_sizeof(int* sizes, int n)
 {
-    const int s[11] = {(int)sizeof(er_tracer_desc), (int)sizeof(er_volume_desc), (int)sizeof(er_light_desc),
+    const int s[13] = {(int)sizeof(er_tracer_desc), (int)sizeof(er_volume_desc), (int)sizeof(er_light_desc),
                        (int)sizeof(er_texture_desc), (int)sizeof(er_clipping_desc), (int)sizeof(er_object_desc),
                        (int)sizeof(er_bitmap_desc), (int)sizeof(er_plf), (int)sizeof(er_shape),
-                       (int)sizeof(er_timing), (int)sizeof(er_counters)};
+                       (int)sizeof(er_timing), (int)sizeof(er_counters), (int)sizeof(er_batch_item),
+                       (int)sizeof(er_tune)};
     int k = 0;
-    for (; sizes && k < n && k < 11; ++k) sizes[k] = s[k];
+    for (; sizes && k < n && k < 13; ++k) sizes[k] = s[k];
     return k;
 }
 

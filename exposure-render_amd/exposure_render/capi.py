@@ -156,8 +156,6 @@ class Counters(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
-ABI_STRUCTS = [TracerDesc, VolumeDesc, LightDesc, TextureDesc, ClippingDesc, ObjectDesc,
-               BitmapDesc, Plf, Shape, Timing, Counters]
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libexposurerender_amd.so")
@@ -178,6 +176,9 @@ class Tune(C.Structure):
 
 
 TUNE_FORCE = 1
+
+ABI_STRUCTS = [TracerDesc, VolumeDesc, LightDesc, TextureDesc, ClippingDesc, ObjectDesc,
+               BitmapDesc, Plf, Shape, Timing, Counters, BatchItem, Tune]
 
 (ELEMENT_UINT8, ELEMENT_INT8, ELEMENT_UINT16, ELEMENT_INT16, ELEMENT_UINT32, ELEMENT_INT32, ELEMENT_FLOAT32,
  ELEMENT_FLOAT64) = range(8)

@@ -279,7 +279,7 @@ typedef struct er_counters {
 
 ER_API int  er_abi_version(void);
 /* sizeof of each descriptor, in the order tracer, volume, light, texture, clipping, object,
- * bitmap, plf, shape, timing, counters; returns how many it wrote. */
+ * bitmap, plf, shape, timing, counters, batch item, tune; returns how many it wrote. */
 ER_API int  er_abi_sizeof(int* sizes, int n);
 ER_API void er_plf_init(er_plf* plf);
 ER_API void er_plf_add_node(er_plf* plf, float position, float value);  /* plf.h:50-66 */
