@@ -60,6 +60,10 @@ __device__ __forceinline__ bool clipped(const DevParams& p, f3 P)
 #ifndef ER_BATCH_PRIMARY
 #define ER_BATCH_PRIMARY 6
 #endif
+/* a primary wave finishes its marches together (coop_march) once this few lanes still march */
+#ifndef ER_COOP_LANES
+#define ER_COOP_LANES 4
+#endif
 #ifndef ER_BATCH_SHADOW
 #define ER_BATCH_SHADOW 8
 #endif
@@ -368,6 +372,79 @@ __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, con
         const int st = march_step<FULL, COUNT, K, Visit>(p, T, m, primary, StepSize, cnt, vis, Pout);
         if (st > 0) { Tout = m.MinT; return true; }
         if (st < 0) return false;
+    }
+}
+
+/* The rest of one primary march (SampleVolume's loop, raymarching.h:55-68) run by the whole
+ * wave on one ray: m is the same in every lane. A march whose wave has only a few lanes still
+ * marching would otherwise spend one memory round trip per ER_BATCH_PRIMARY samples; here each
+ * round trip reads 64 consecutive samples, lane k taking sample k. Lane k finds t_k with the
+ * same k float additions of StepSize as the sequential loop, and the optical depth is then
+ * accumulated in sample order (one addition per sample, broadcast from its lane), with the
+ * loop's own end test before each sample and threshold test after it, so every decision,
+ * position and sum is the sequential march's. Empty macrocells at the start of a block are
+ * skipped by march_step's exact skip. Returns 1 on an event (Pout = the sample, m.MinT one step
+ * beyond it), -1 when the ray leaves the volume first; the same in every lane. Counts go to
+ * lane 0's cnt. */
+template<int FULL, int COUNT>
+__device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane, Cnt& cnt, f3& Pout)
+{
+    NoVisit nv;
+    for (;;) {
+        if (m.MinT >= m.MaxT) return -1;
+        const f3 P0 = add(m.O, scl(m.MinT, m.D));
+        const Lattice l0 = to_lattice(p, P0);
+        const int sh = p.mc_shift;
+        const int mc0 = (mc_axis(l0.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(l0.Ly, sh, p.mc_dim[1])) * p.mc_dim[0] +
+                        mc_axis(l0.Lx, sh, p.mc_dim[0]);
+        if (!mc_occupied(T, mc0)) {
+            /* the sequential march would skip: the same skip in every lane */
+            Cnt c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            f3 P;
+            (void)march_step<FULL, COUNT, 1, NoVisit>(p, T, m, true, StepSize, c, nv, P);
+            if (COUNT && lane == 0) { cnt.primary += c.primary; cnt.iters += c.iters; cnt.skipped += c.skipped; cnt.undone += c.undone; }
+            continue;
+        }
+        float tk = m.MinT;
+        for (int s = 0; s < lane; ++s) tk += StepSize;
+        const bool inseg = tk < m.MaxT;
+        const f3 Pk = add(m.O, scl(tk, m.D));
+        const Lattice lk = to_lattice(p, Pk);
+        const bool occ = mc_occupied(T, (mc_axis(lk.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(lk.Ly, sh, p.mc_dim[1])) *
+                                            p.mc_dim[0] + mc_axis(lk.Lx, sh, p.mc_dim[0]));
+        const bool live = inseg && occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
+        float sigma = 0.0f;
+        if (live) sigma = T.lut(min((int)trilinear_cell(p, lk), p.lut_max));
+        const float c = sigma * StepSize;
+        /* samples 0 .. kexit-1 lie before the end of the segment (t_k grows with k) */
+        const unsigned long long segm = __ballot(inseg);
+        const int kexit = segm == ~0ull ? 64 : __ffsll((long long)~segm) - 1;
+        float Sum = m.Sum;
+        int kend = 64, term = 0;
+        for (int k = 0; k < 64; ++k) {
+            if (k >= kexit) { term = -1; kend = k; break; }
+            Sum = Sum + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), k));
+            if (!(Sum < m.S)) { term = 1; kend = k; break; }
+        }
+        const unsigned long long lm = __ballot(live);
+        if (COUNT && lane == 0) {
+            const int ns = term == 1 ? kend + 1 : kend;
+            const unsigned long long within = ns >= 64 ? ~0ull : ((1ull << ns) - 1ull);
+            cnt.primary += (uint32_t)ns;
+            cnt.iters += (uint32_t)ns;
+            cnt.pfetch += (uint32_t)__popcll(lm & within);
+        }
+        if (term == 1) {
+            Pout = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(Pk.x), kend)),
+                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Pk.y), kend)),
+                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Pk.z), kend)));
+            m.MinT = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tk), kend)) + StepSize;
+            m.Sum = Sum;
+            return 1;
+        }
+        if (term == -1) return -1;
+        m.Sum = Sum;
+        m.MinT = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tk), 63)) + StepSize;
     }
 }
 

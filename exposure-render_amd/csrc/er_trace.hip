@@ -91,14 +91,60 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
     f3 Pv = mk(0.0f, 0.0f, 0.0f), D = mk(0.0f, 0.0f, 1.0f), Nobj = mk(0.0f, 0.0f, 0.0f);
     int obj = -1;
     float Uobj = 0.0f, Vobj = 0.0f;
+    Ray r;
+    bool vol = false, marching = false;
+    float Tv = 0.0f;
+    MarchState m;
+    const float StepSize = p.step_primary * p.minstep;
     if (active) {
         paths++;
         rng.s0 = seeds1[i];
         rng.s1 = seeds2[i];
-        const Ray r = camera_ray(p, px, py, rng, smp);
+        r = camera_ray(p, px, py, rng, smp);
         D = r.D;
-        float Tv = 0.0f;
-        const bool vol = march<FULL, COUNT, ER_BATCH_PRIMARY>(p, T, r, true, rng, Tv, Pv, cnt);
+        if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
+            vol = march<FULL, COUNT, ER_BATCH_PRIMARY>(p, T, r, true, rng, Tv, Pv, cnt);
+        } else {
+            /* march_v's set-up; its loop follows, wave by wave */
+            float nearT, farT;
+            if (intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT) &&
+                march_begin(p, r, nearT, farT, StepSize, rng, -1.0f, m)) {
+                if (!(m.Sum < m.S)) { vol = true; Tv = m.MinT; Pv = mk(0.0f, 0.0f, 0.0f); }
+                else marching = true;
+            }
+        }
+    }
+    /* The march loop. Once at most ER_COOP_LANES lanes of the wave are still marching, their
+     * marches are finished one at a time by the whole wave (coop_march): a tile's cost is set
+     * by its longest ray, and a lone ray through kilo-samples of tissue would otherwise keep
+     * its wave for hundreds of microseconds at a few samples per memory round trip. */
+    NoVisit nv;
+    for (;;) {
+        const int nm = __popcll(__ballot(marching));
+        if (nm == 0 || nm <= ER_COOP_LANES) break;
+        if (marching) {
+            const int st = march_step<FULL, COUNT, ER_BATCH_PRIMARY, NoVisit>(p, T, m, true, StepSize, cnt, nv, Pv);
+            if (st != 0) { marching = false; vol = st > 0; Tv = m.MinT; }
+        }
+    }
+    for (unsigned long long rest = __ballot(marching); rest; rest &= rest - 1ull) {
+        const int L = __ffsll((long long)rest) - 1;
+        MarchState w;
+        w.O = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.O.x), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.O.y), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.O.z), L)));
+        w.D = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.D.x), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.D.y), L)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.D.z), L)));
+        w.MinT = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.MinT), L));
+        w.MaxT = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.MaxT), L));
+        w.Sum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.Sum), L));
+        w.S = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.S), L));
+        f3 P = mk(0.0f, 0.0f, 0.0f);
+        const int st = coop_march<FULL, COUNT>(p, T, w, StepSize, lane, cnt, P);
+        if (lane == L) { marching = false; vol = st > 0; Tv = w.MinT; if (st > 0) Pv = P; }
+    }
+    if (active) {
         float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
         const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
         int type = -1;
