@@ -385,9 +385,11 @@ __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, con
  * position and sum is the sequential march's. Empty macrocells at the start of a block are
  * skipped by march_step's exact skip. Returns 1 on an event (Pout = the sample, m.MinT one step
  * beyond it), -1 when the ray leaves the volume first; the same in every lane. Counts go to
- * lane 0's cnt. */
+ * lane 0's cnt; steps counts the iterations the per-lane loop would have taken (a skip, or
+ * ER_BATCH_PRIMARY samples). */
 template<int FULL, int COUNT>
-__device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane, Cnt& cnt, f3& Pout)
+__device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane, Cnt& cnt, f3& Pout,
+                          int& steps)
 {
     NoVisit nv;
     for (;;) {
@@ -403,6 +405,7 @@ __device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, fl
             f3 P;
             (void)march_step<FULL, COUNT, 1, NoVisit>(p, T, m, true, StepSize, c, nv, P);
             if (COUNT && lane == 0) { cnt.primary += c.primary; cnt.iters += c.iters; cnt.skipped += c.skipped; cnt.undone += c.undone; }
+            ++steps;
             continue;
         }
         float tk = m.MinT;
@@ -427,8 +430,9 @@ __device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, fl
             if (!(Sum < m.S)) { term = 1; kend = k; break; }
         }
         const unsigned long long lm = __ballot(live);
+        const int ns = term == 1 ? kend + 1 : kend;
+        steps += (ns + ER_BATCH_PRIMARY - 1) / ER_BATCH_PRIMARY;   /* the sequential loop's iterations */
         if (COUNT && lane == 0) {
-            const int ns = term == 1 ? kend + 1 : kend;
             const unsigned long long within = ns >= 64 ? ~0ull : ((1ull << ns) - 1ull);
             cnt.primary += (uint32_t)ns;
             cnt.iters += (uint32_t)ns;

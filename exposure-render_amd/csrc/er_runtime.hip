@@ -37,7 +37,8 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
                                        int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
-                                       hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary);
+                                       hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary,
+                                       const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap);
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
@@ -282,7 +283,11 @@ struct Tracer {
     /* launch configuration (er_autotune / er_set_tune): brick edge 2^mc_shift (0: the
      * volume's default), primary tiles 2^tile_shift pixels wide, persistent primary
      * workgroups per CU (0: the library default) */
-    int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0, shadow_waves_per_cu = 4;
+    int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0, shadow_waves_per_cu = 4, heavy_tiles = 0;
+    /* primary tiles: each pass records its tiles' march steps and orders the next pass's tiles
+     * most expensive first; the order is valid for the tile shape and count it was built for */
+    DevBuf<uint32_t> tile_cost, tile_order;
+    int order_shift = -1, order_tiles = 0;
     hipStream_t stream = nullptr;     /* primary + shading, uploads, readbacks */
     hipStream_t stream2 = nullptr;    /* shadow marches, combine, post */
     ~Tracer()
@@ -291,6 +296,7 @@ struct Tracer {
         running.release(); display.release(); disp_tmp.release(); disp_filt.release();
         seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); params.release();
         mc_bits.release(); counters.release(); gm_tmp.release(); bilateral.release();
+        tile_cost.release(); tile_order.release();
         if (stream) (void)hipStreamDestroy(stream);
         if (stream2) (void)hipStreamDestroy(stream2);
     }
@@ -839,10 +845,21 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
     if (e) HIP_TRY(hipEventRecord(e[0], t.stream));
     const int split = !(full && t.desc.settings.sampler == ER_SAMPLER_WOODCOCK);
     const size_t lds_bits = (size_t)t.host_params.mc_words * 4;
+    const int ntiles = ((t.W + (1 << t.tile_shift) - 1) >> t.tile_shift) * ((t.H + (64 >> t.tile_shift) - 1) / (64 >> t.tile_shift));
+    if (t.tile_cost.n < (size_t)ntiles) {
+        /* every pass writes the cost of each of its tiles before the order kernel reads them */
+        int st;
+        if ((st = sync_tracer(t)) || (st = t.tile_cost.alloc(ntiles)) || (st = t.tile_order.alloc(ntiles + 1))) return st;
+        t.order_tiles = 0;
+    }
+    const bool order_valid = t.order_shift == t.tile_shift && t.order_tiles == ntiles;
     HIP_TRY(erd_launch_trace(t.params.ptr, S.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, S.queue.ptr, S.qcount.ptr,
                              S.events.ptr, S.jobs.ptr, S.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
                              split, t.lds_bytes, lds_bits, shade_blocks, primary_blocks(t), shadow_blocks(t), t.tile_shift, t.stream,
-                             t.stream2, S.shaded, e ? e[1] : nullptr));
+                             t.stream2, S.shaded, e ? e[1] : nullptr, order_valid ? t.tile_order.ptr : nullptr,
+                             t.tile_cost.ptr, t.tile_order.ptr, t.heavy_tiles));
+    t.order_shift = t.tile_shift;
+    t.order_tiles = ntiles;
     if (e) HIP_TRY(hipEventRecord(e[2], t.stream2));
     HIP_TRY(erd_launch_post(t.params.ptr, S.frame.ptr, t.running.ptr, t.display.ptr, t.desc.no_iterations,
                             S.qcount.ptr, t.W, t.H, t.stream2));
@@ -929,6 +946,7 @@ void fill_tune(const Tracer& t, er_tune& r)
     r.tile_h = 64 >> t.tile_shift;
     r.blocks_per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : g_primary_blocks / std::max(1, g_cus);
     r.shadow_waves_per_cu = t.shadow_waves_per_cu;
+    r.heavy_tiles = t.heavy_tiles;
 }
 
 int apply_tune(Tracer& t, const er_tune& r)
@@ -942,6 +960,7 @@ int apply_tune(Tracer& t, const er_tune& r)
     if (r.blocks_per_cu < 1 || r.blocks_per_cu > 16) return set_error(ER_ERROR, "blocks per CU must be in [1, 16]");
     if (r.shadow_waves_per_cu < 1 || r.shadow_waves_per_cu > 64)
         return set_error(ER_ERROR, "shadow waves per CU must be in [1, 64]");
+    if (r.heavy_tiles < 0 || r.heavy_tiles > 4096) return set_error(ER_ERROR, "heavy tiles must be in [0, 4096]");
     const Volume* v = g_volumes.find(t.desc.volume_id);
     if (v && !brick_fits(v->desc, v->vmax, bs))
         return set_error(ER_ERROR, "a %d-cell brick's occupancy bits and opacity table exceed %zu bytes of LDS", r.brick,
@@ -950,6 +969,7 @@ int apply_tune(Tracer& t, const er_tune& r)
     t.tile_shift = ts;
     t.blocks_per_cu = r.blocks_per_cu;
     t.shadow_waves_per_cu = r.shadow_waves_per_cu;
+    t.heavy_tiles = r.heavy_tiles;
     t.built_version = 0;
     return ER_OK;
 }
@@ -1039,9 +1059,12 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
     } else if (axis == 2) {
         for (int b = 3; b <= 9; ++b)
             if (b != cur.blocks_per_cu) { er_tune x = cur; x.blocks_per_cu = b; c.push_back(x); }
-    } else {
+    } else if (axis == 3) {
         for (int w = 2; w <= 8; w *= 2)
             if (w != cur.shadow_waves_per_cu) { er_tune x = cur; x.shadow_waves_per_cu = w; c.push_back(x); }
+    } else {
+        for (int h : {0, 256})
+            if (h != cur.heavy_tiles) { er_tune x = cur; x.heavy_tiles = h; c.push_back(x); }
     }
     return c;
 }
@@ -1361,7 +1384,8 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     if (t.W == 0 || t.H == 0) return set_error(ER_ERROR, "tracer %d has an empty film", tracer_id);
     Snapshot snap;
     if ((st = snap.save(t))) return st;
-    const int mc0 = t.mc_shift, ts0 = t.tile_shift, bpc0 = t.blocks_per_cu, sw0 = t.shadow_waves_per_cu;
+    const int mc0 = t.mc_shift, ts0 = t.tile_shift, bpc0 = t.blocks_per_cu, sw0 = t.shadow_waves_per_cu,
+              ht0 = t.heavy_tiles;
     er_tune cur{};
     fill_tune(t, cur);
     int trials = 0;
@@ -1388,7 +1412,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     if (!st) st = measure(cur, cur_ms);
     const float default_ms = cur_ms;
     std::vector<const Volume*> vols{g_volumes.find(t.desc.volume_id)};
-    for (int axis = 0; axis < 4 && !st; ++axis)
+    for (int axis = 0; axis < 5 && !st; ++axis)
         for (const er_tune& c : tune_candidates(axis, cur, vols)) {
             float ms = 0.0f;
             if ((st = measure(c, ms))) break;
@@ -1396,7 +1420,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
         }
     if (!st) st = apply_tune(t, cur);
     if (st) {
-        t.mc_shift = mc0; t.tile_shift = ts0; t.blocks_per_cu = bpc0; t.shadow_waves_per_cu = sw0;
+        t.mc_shift = mc0; t.tile_shift = ts0; t.blocks_per_cu = bpc0; t.shadow_waves_per_cu = sw0; t.heavy_tiles = ht0;
         t.built_version = 0;
     }
     const int rst = snap.restore(t);

@@ -78,11 +78,15 @@ template<int FULL, int COUNT>
 __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T, unsigned int tile, int tiles_x, int lane,
                                              float4* __restrict__ frame, uint32_t* __restrict__ seeds1,
                                              uint32_t* __restrict__ seeds2, ScatterRec* __restrict__ queue,
-                                             unsigned int* __restrict__ qcount, Cnt& cnt, int& paths)
+                                             unsigned int* __restrict__ qcount, Cnt& cnt, int& paths, int& steps, int ray)
 {
     const int ts = p.tile_shift;
-    const int px = (int)(tile % (unsigned int)tiles_x) * (1 << ts) + (lane & ((1 << ts) - 1));
-    const int py = (int)(tile / (unsigned int)tiles_x) * (64 >> ts) + (lane >> ts);
+    /* ray >= 0: the whole wave computes that one pixel of the tile (every lane the same) and
+     * marches it with coop_march; lane 0 writes the results */
+    const int pl = ray >= 0 ? ray : lane;
+    const bool owner = ray < 0 || lane == 0;
+    const int px = (int)(tile % (unsigned int)tiles_x) * (1 << ts) + (pl & ((1 << ts) - 1));
+    const int py = (int)(tile / (unsigned int)tiles_x) * (64 >> ts) + (pl >> ts);
     const bool active = px < p.film_w && py < p.film_h;
     const size_t i = (size_t)py * p.film_w + px;
     bool scatter = false;
@@ -97,7 +101,7 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
     MarchState m;
     const float StepSize = p.step_primary * p.minstep;
     if (active) {
-        paths++;
+        if (owner) paths++;
         rng.s0 = seeds1[i];
         rng.s1 = seeds2[i];
         r = camera_ray(p, px, py, rng, smp);
@@ -115,19 +119,23 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
         }
     }
     /* The march loop. Once at most ER_COOP_LANES lanes of the wave are still marching, their
-     * marches are finished one at a time by the whole wave (coop_march): a tile's cost is set
-     * by its longest ray, and a lone ray through kilo-samples of tissue would otherwise keep
-     * its wave for hundreds of microseconds at a few samples per memory round trip. */
+     * marches are finished one at a time by the whole wave (coop_march): a lone ray through
+     * kilo-samples of tissue would otherwise keep its wave for hundreds of microseconds.
+     * it: the lane's march iterations (coop_march counts its equivalent), for the tile cost. */
     NoVisit nv;
+    int it = 0;
     for (;;) {
         const int nm = __popcll(__ballot(marching));
-        if (nm == 0 || nm <= ER_COOP_LANES) break;
+        if (nm == 0 || nm <= ER_COOP_LANES || ray >= 0) break;
         if (marching) {
+            ++it;
             const int st = march_step<FULL, COUNT, ER_BATCH_PRIMARY, NoVisit>(p, T, m, true, StepSize, cnt, nv, Pv);
             if (st != 0) { marching = false; vol = st > 0; Tv = m.MinT; }
         }
     }
-    for (unsigned long long rest = __ballot(marching); rest; rest &= rest - 1ull) {
+    unsigned long long rest = __ballot(marching);
+    if (ray >= 0) rest &= 1ull;
+    for (; rest; rest &= rest - 1ull) {
         const int L = __ffsll((long long)rest) - 1;
         MarchState w;
         w.O = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.O.x), L)),
@@ -141,9 +149,17 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
         w.Sum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.Sum), L));
         w.S = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m.S), L));
         f3 P = mk(0.0f, 0.0f, 0.0f);
-        const int st = coop_march<FULL, COUNT>(p, T, w, StepSize, lane, cnt, P);
-        if (lane == L) { marching = false; vol = st > 0; Tv = w.MinT; if (st > 0) Pv = P; }
+        int csteps = 0;
+        const int st = coop_march<FULL, COUNT>(p, T, w, StepSize, lane, cnt, P, csteps);
+        if (lane == L || ray >= 0) {
+            marching = false; vol = st > 0; Tv = w.MinT;
+            if (st > 0) Pv = P;
+            it += csteps;
+        }
     }
+    /* the tile's cost: its longest march */
+    for (int o = 32; o > 0; o >>= 1) it = max(it, __shfl_xor(it, o));
+    steps = it;
     if (active) {
         float Tl = 0.0f; f3 Pl, Nl, Lel = mk(0.0f, 0.0f, 0.0f);
         const int il = intersect_lights(p, r, true, Tl, Pl, Nl, Lel);
@@ -157,9 +173,9 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
             if (io >= 0 && To < Tn) { type = 0; Tn = To; obj = io; Pv = Po; Nobj = No; Uobj = Uo; Vobj = Vo; }
         }
         if (type == 0) {
-            scatter = true;
-            if (COUNT) cnt.scatter++;
-        } else {
+            scatter = owner;
+            if (COUNT && owner) cnt.scatter++;
+        } else if (owner) {
             f3 Lv = mk(0.0f, 0.0f, 0.0f);
             if (type == 1) Lv = add(Lv, Lel);
             frame[i] = make_float4(Lv.x, Lv.y, Lv.z, type >= 0 ? 1.0f : 0.0f);
@@ -280,7 +296,8 @@ template<int FULL, int COUNT, int LUT_LDS>
 __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
                                                 ScatterRec* __restrict__ queue, unsigned int* __restrict__ counts,
-                                                unsigned long long* __restrict__ counters, int tiles_x, int ntiles)
+                                                unsigned long long* __restrict__ counters, int tiles_x, int ntiles,
+                                                const uint32_t* __restrict__ order, uint32_t* __restrict__ cost)
 {
     extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
@@ -288,14 +305,90 @@ __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ p
     const int lane = threadIdx.x & 63;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int paths = 0;
+    /* claims [0, 64 hc): the rays of the hc heaviest tiles one by one (order[0 .. hc), 64 claims
+     * each, every ray marched by a whole wave); then the other tiles in their order */
+    unsigned int hc = order ? __builtin_amdgcn_readfirstlane(order[ntiles]) : 0u;
+    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) hc = 0u;
+    const unsigned int nclaims = hc * 64u + ((unsigned int)ntiles - hc);
     for (;;) {
-        unsigned int tile = 0;
-        if (lane == 0) tile = atomicAdd(&counts[2], 1u);
-        tile = __builtin_amdgcn_readfirstlane(tile);
-        if (tile >= (unsigned int)ntiles) break;
-        primary_tile<FULL, COUNT>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths);
+        unsigned int claim = 0;
+        if (lane == 0) claim = atomicAdd(&counts[2], 1u);
+        claim = __builtin_amdgcn_readfirstlane(claim);
+        if (claim >= nclaims) break;
+        const bool heavy = claim < hc * 64u;
+        const unsigned int idx = heavy ? (claim >> 6) : hc + (claim - hc * 64u);
+        const unsigned int tile = order ? __builtin_amdgcn_readfirstlane(order[idx]) : idx;
+        int steps = 0;
+        primary_tile<FULL, COUNT>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths, steps,
+                                  heavy ? (int)(claim & 63u) : -1);
+        if (cost && lane == 0) {
+            if (heavy) atomicMax(&cost[tile], (uint32_t)steps);
+            else cost[tile] = (uint32_t)steps;
+        }
     }
     flush_counters<COUNT>(cnt, true, paths, counters);
+}
+
+/* The claim order of the next pass's tiles: most expensive first by this pass's cost per tile
+ * (the march iterations of its longest ray; a progressive render's tiles cost about the same
+ * from pass to pass), so that the slowest tiles start at once instead of deciding when the
+ * kernel ends. Eight classes by cost relative to the maximum (above 1/2, 1/4, ..., 1/128, the
+ * rest), stable within a class (scan order keeps neighbouring tiles together). order[n] = hc:
+ * the tiles of the first class (at most heavy_cap and n / 16), whose rays the next pass
+ * marches one wave per ray; their costs are cleared for its per-ray maxima. One workgroup;
+ * order[0 .. n) is a permutation of 0 .. n-1 whatever the costs hold. */
+__global__ __launch_bounds__(1024) void k_tile_order(uint32_t* __restrict__ cost, uint32_t* __restrict__ order, int n,
+                                                    int heavy_cap)
+{
+    __shared__ uint32_t smax, base[8], wcount[16][8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) smax = 0u;
+    if (tid < 8) base[tid] = 0u;
+    __syncthreads();
+    uint32_t mx = 0u;
+    for (int i = tid; i < n; i += blockDim.x) mx = max(mx, cost[i]);
+    atomicMax(&smax, mx);
+    __syncthreads();
+    const uint64_t cmax = smax;
+    auto cls = [cmax](uint32_t c) {
+        int b = 0;
+        while (b < 7 && ((uint64_t)c << (b + 1)) <= cmax) ++b;
+        return b;
+    };
+    for (int i = tid; i < n; i += blockDim.x) atomicAdd(&base[cls(cost[i])], 1u);
+    __syncthreads();
+    __shared__ uint32_t heavy;
+    if (tid == 0) {
+        heavy = min(base[0], min((uint32_t)max(heavy_cap, 0), (uint32_t)n / 16u));
+        uint32_t acc = 0u;
+        for (int b = 0; b < 8; ++b) { const uint32_t c = base[b]; base[b] = acc; acc += c; }
+    }
+    __syncthreads();
+    for (int chunk = 0; chunk < n; chunk += blockDim.x) {
+        const int i = chunk + tid;
+        const int b = i < n ? cls(cost[i]) : 8;
+        uint32_t rank = 0u;
+        for (int bb = 0; bb < 8; ++bb) {
+            const unsigned long long m = __ballot(b == bb);
+            if (lane == 0) wcount[wave][bb] = (uint32_t)__popcll(m);
+            if (b == bb) rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        }
+        __syncthreads();
+        if (b < 8) {
+            uint32_t off = base[b] + rank;
+            for (int w = 0; w < wave; ++w) off += wcount[w][b];
+            if (off < (uint32_t)n) order[off] = (uint32_t)i;
+        }
+        __syncthreads();
+        if (tid < 8) {
+            uint32_t total = 0u;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += wcount[w][tid];
+            base[tid] += total;
+        }
+        __syncthreads();
+    }
+    for (uint32_t k = tid; k < heavy; k += blockDim.x) cost[order[k]] = 0u;
+    if (tid == 0) order[n] = heavy;
 }
 
 /* Shading kernel over the scatter queue (counts[0] entries), wave-aligned grid stride. */
@@ -547,8 +640,9 @@ __global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ 
             if (lane == 0) tile = atomicAdd(&B.counts[2], 1u);
             tile = __builtin_amdgcn_readfirstlane(tile);
             if (tile >= (unsigned int)B.ntiles) break;
+            int steps = 0;
             primary_tile<FULL, 0>(p, T, tile, B.tiles_x, lane, B.frame, B.seeds1, B.seeds2, B.queue, &B.counts[0], cnt,
-                                  paths);
+                                  paths, steps, -1);
         }
         __syncthreads();   /* every wave is done with this job's tables */
         if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 2);
@@ -670,10 +764,11 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
                        unsigned int* counts, erd::EventRec* events, erd::ShadowJob* jobs, unsigned char* vis,
                        unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks,
                        int shadow_blocks, int split, size_t lds, size_t lds_bits, hipStream_t st, hipStream_t st2,
-                       hipEvent_t shaded, hipEvent_t after_primary)
+                       hipEvent_t shaded, hipEvent_t after_primary, const uint32_t* order_in, uint32_t* cost,
+                       uint32_t* order_out, int heavy_cap)
 {
     hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS>), dim3(nblocks), dim3(256), lds, st, p, frame, s1, s2, queue,
-                       counts, counters, tiles_x, ntiles);
+                       counts, counters, tiles_x, ntiles, order_in, cost);
     if (after_primary) (void)hipEventRecord(after_primary, st);
     if (split) {
         hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1,
@@ -690,6 +785,10 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
         (void)hipEventRecord(shaded, st);
         (void)hipStreamWaitEvent(st2, shaded, 0);
     }
+    /* the next pass's tile order from this pass's tile costs (after the primary kernel has
+     * read the current order; stream order keeps the next primary kernel behind it) */
+    if (cost && order_out)
+        hipLaunchKernelGGL(erd::k_tile_order, dim3(1), dim3(1024), 0, st, cost, order_out, ntiles, heavy_cap);
 }
 
 /* One pass's trace kernels: primary and shading on st; shadow marches and combine on st2
@@ -699,7 +798,8 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
                                        unsigned long long* counters, int W, int H, int full, int count, int lut_lds,
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
                                        int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
-                                       hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary)
+                                       hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary,
+                                       const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap)
 {
     const int tw = 1 << tile_shift, th = 64 >> tile_shift;
     const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
@@ -711,7 +811,7 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
     erd::ShadowJob* jb = (erd::ShadowJob*)jobs;
 #define ER_ALL(F, C, L) launch_all<F, C, L>(p, frame, s1, s2, q, counts, ev, jb, vis, counters, tiles_x, ntiles, nblocks, \
                                             shade_blocks, std::max(1, shadow_blocks), split, lds_bytes, lds_bits, st, st2, \
-                                            shaded, after_primary)
+                                            shaded, after_primary, order_in, cost, order_out, heavy_cap)
     if (full) {
         if (count) { if (lut_lds) ER_ALL(1, 1, 1); else ER_ALL(1, 1, 0); }
         else { if (lut_lds) ER_ALL(1, 0, 1); else ER_ALL(1, 0, 0); }
@@ -781,10 +881,10 @@ extern "C" hipError_t erd_launch_primary_shade(const erd::DevParams* p, float4* 
     erd::ScatterRec* q = (erd::ScatterRec*)queue;
     if (lut_lds)
         hipLaunchKernelGGL((erd::k_primary<1, 0, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
-                           (unsigned long long*)nullptr, tiles_x, ntiles);
+                           (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     else
         hipLaunchKernelGGL((erd::k_primary<1, 0, 0>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
-                           (unsigned long long*)nullptr, tiles_x, ntiles);
+                           (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     hipLaunchKernelGGL((erd::k_shade<1, 0, 1, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1, s2,
                        (const erd::ScatterRec*)q, counts, (erd::EventRec*)events, (erd::ShadowJob*)jobs,
                        (unsigned long long*)nullptr);

@@ -376,6 +376,8 @@ typedef struct er_tune {
     int   tile_w, tile_h;         /* primary tile, 64 pixels: 4x16, 8x8, 16x4 or 32x2 */
     int   blocks_per_cu;          /* persistent 256-lane primary workgroups per CU */
     int   shadow_waves_per_cu;    /* persistent 64-lane shadow-march waves per CU (default 4) */
+    int   heavy_tiles;            /* most expensive tiles of a pass whose rays the next pass marches
+                                     one whole wave per ray (default 0) */
     float ms_per_pass;            /* er_autotune: the chosen configuration's time per pass */
     float default_ms_per_pass;    /* er_autotune: the starting configuration's time per pass */
     int   trials;                 /* er_autotune: configurations timed (0: taken from the cache) */
@@ -384,7 +386,8 @@ typedef struct er_tune {
 #define ER_TUNE_FORCE 1           /* er_autotune flag: measure even if the scene is cached */
 
 /* Times `passes` passes (default 6) of each candidate from the tracer's current seeds and
- * keeps the fastest: bricks, then tile shapes, then workgroups per CU, then shadow waves per CU. The tracer's seeds,
+ * keeps the fastest: bricks, then tile shapes, then workgroups per CU, then shadow waves per CU,
+ * then heavy tiles. The tracer's seeds,
  * running estimate, display, counters and pass count are restored afterwards. Results are
  * cached per scene (descriptor, node table and volume contents). */
 ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out);

@@ -109,7 +109,8 @@ def test_autotune_restores_the_render_and_caches(fresh):
 def test_set_tune_rejects_invalid_configurations(fresh):
     sc = scenes.config_small(scale=0.25)
     r = er.Renderer(sc, seed=1)
-    for bad in [(3, 8, 7), (8, 6, 7), (8, 64, 7), (8, 8, 0), (8, 8, 17), (128, 8, 7), (8, 8, 7, 0), (8, 8, 7, 65)]:
+    for bad in [(3, 8, 7), (8, 6, 7), (8, 64, 7), (8, 8, 0), (8, 8, 17), (128, 8, 7), (8, 8, 7, 0), (8, 8, 7, 65),
+                (8, 8, 7, 4, -1), (8, 8, 7, 4, 4097)]:
         with pytest.raises(er.ExposureRenderError):
             r.set_tune(*bad)
     r.close()
@@ -133,7 +134,35 @@ def test_shadow_grid_does_not_change_the_image(fresh):
         r.close()
     print(f"[tune] shadow rays per pass {out[0][2]['shadow_rays'] // 3}", flush=True)
     for run, disp, cnt in out[1:]:
-        assert np.array_equal(run, out[0][0]) and np.array_equal(disp, out[0][1]) and cnt == out[0][2]
+        assert np.array_equal(run, out[0][0]) and np.array_equal(disp, out[0][1])
+        for k in COUNTERS:
+            assert cnt[k] == out[0][2][k], (k, cnt[k], out[0][2][k])
+
+
+@pytest.mark.parametrize("scale", [0.5, 1.0])
+def test_tile_order_and_heavy_rays_do_not_change_the_image(fresh, scale):
+    """From its second pass on, a pass claims its tiles most expensive first (by the previous
+    pass's costs), and with heavy_tiles > 0 marches the rays of the most expensive tiles one
+    whole wave per ray (coop_march). Neither changes any pixel, RNG state or sample count."""
+    sc = scenes.config_ct(scale=scale)
+    out = []
+    for heavy in (0, 16, 256, 4096):
+        print(f"[tune] heavy tiles {heavy} scale {scale}", flush=True)
+        r = er.Renderer(sc, seed=9)
+        r.set_tune(8, 8, 7, 4, heavy)
+        assert r.get_tune()["heavy_tiles"] == heavy
+        r.set_counting(True)
+        r.reset_counters()
+        r.render_passes(4)
+        out.append((r.running().copy(), r.get_estimate().copy(), r.get_seeds(), r.counters()))
+        r.close()
+    for run, disp, (s1, s2), cnt in out[1:]:
+        assert np.array_equal(run, out[0][0]) and np.array_equal(disp, out[0][1])
+        assert np.array_equal(s1, out[0][2][0]) and np.array_equal(s2, out[0][2][1])
+        # the reference's sample counts (the march's own iteration and skip tallies differ: a
+        # whole-wave march reads the empty samples inside a block instead of skipping them)
+        for k in COUNTERS:
+            assert cnt[k] == out[0][3][k], (k, cnt[k], out[0][3][k])
 
 
 def test_batch_with_tuned_jobs_matches_sequential(fresh):
