@@ -286,7 +286,11 @@ __device__ __forceinline__ CellFetch cell_fetch(const DevParams& p, const Lattic
     const int cy = min(max(vy, -1), p.res[1] - 1) + 1;
     const int cz = min(max(vz, -1), p.res[2] - 1) + 1;
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4 v = gmem((const u32x4*)p.cells)[(size_t)cz * (size_t)p.cstride_z + (size_t)(cy * p.cstride_y + cx)];
+    /* a cell index is below 2^32 (the record array stays below 64 GB) and each factor below
+     * 2^24: full-rate 24-bit multiplies */
+    const unsigned idx = __umul24((unsigned)cz, (unsigned)p.cstride_z) + __umul24((unsigned)cy, (unsigned)p.cstride_y) +
+                         (unsigned)cx;
+    const u32x4 v = gmem((const u32x4*)p.cells)[idx];
     f.c = make_uint4(v.x, v.y, v.z, v.w);
     return f;
 }

@@ -97,13 +97,21 @@ __device__ __forceinline__ int mc_axis(float L, int shift, int dim)
     return min(max(((int)floorf(L)) >> shift, 0), dim - 1);
 }
 
+/* macrocell index of a lattice point (x fastest); the grid has fewer than 2^24 macrocells a
+ * side, so the products use the full-rate 24-bit multiply */
+__device__ __forceinline__ int mc_index(const DevParams& p, const Lattice& l, int sh)
+{
+    return (int)(__umul24(__umul24((unsigned)mc_axis(l.Lz, sh, p.mc_dim[2]), (unsigned)p.mc_dim[1]) +
+                          (unsigned)mc_axis(l.Ly, sh, p.mc_dim[1]), (unsigned)p.mc_dim[0]) +
+                 (unsigned)mc_axis(l.Lx, sh, p.mc_dim[0]));
+}
+
 /* sigma of one sample (Woodcock path) */
 template<int FULL, int COUNT>
 __device__ __forceinline__ float sample_sigma(const DevParams& p, const Tables& T, f3 Ps, bool primary, Cnt& cnt)
 {
     const Lattice l = to_lattice(p, Ps);
-    const int m = (mc_axis(l.Lz, p.mc_shift, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(l.Ly, p.mc_shift, p.mc_dim[1])) * p.mc_dim[0] +
-                  mc_axis(l.Lx, p.mc_shift, p.mc_dim[0]);
+    const int m = mc_index(p, l, p.mc_shift);
     if (!mc_occupied(T, m)) return 0.0f;
     if (FULL && p.clip_count > 0 && clipped(p, Ps)) return 0.0f;
     if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
@@ -224,7 +232,8 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
     const int mxi = mc_axis(l.Lx, sh, p.mc_dim[0]);
     const int myi = mc_axis(l.Ly, sh, p.mc_dim[1]);
     const int mzi = mc_axis(l.Lz, sh, p.mc_dim[2]);
-    const int mc = (mzi * p.mc_dim[1] + myi) * p.mc_dim[0] + mxi;
+    const int mc = (int)(__umul24(__umul24((unsigned)mzi, (unsigned)p.mc_dim[1]) + (unsigned)myi, (unsigned)p.mc_dim[0]) +
+                         (unsigned)mxi);
     if (K == 1 && mc_occupied(T, mc)) {
         float SigmaT = 0.0f;
         if (!(FULL && p.clip_count > 0 && clipped(p, Ps))) {
@@ -260,8 +269,7 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
                 if (kn == K && (primary ? (tk[k] >= m.MaxT) : (tk[k] > m.MaxT))) kn = k;
                 Pk = add(m.O, scl(tk[k], m.D));
                 lk = to_lattice(p, Pk);
-                occ = mc_occupied(T, (mc_axis(lk.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(lk.Ly, sh, p.mc_dim[1])) * p.mc_dim[0] +
-                                         mc_axis(lk.Lx, sh, p.mc_dim[0]));
+                occ = mc_occupied(T, mc_index(p, lk, sh));
             }
             live[k] = occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
             f[k] = cell_fetch(p, lk);
@@ -397,8 +405,7 @@ __device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, fl
         const f3 P0 = add(m.O, scl(m.MinT, m.D));
         const Lattice l0 = to_lattice(p, P0);
         const int sh = p.mc_shift;
-        const int mc0 = (mc_axis(l0.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(l0.Ly, sh, p.mc_dim[1])) * p.mc_dim[0] +
-                        mc_axis(l0.Lx, sh, p.mc_dim[0]);
+        const int mc0 = mc_index(p, l0, sh);
         if (!mc_occupied(T, mc0)) {
             /* the sequential march would skip: the same skip in every lane */
             Cnt c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -413,8 +420,7 @@ __device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, fl
         const bool inseg = tk < m.MaxT;
         const f3 Pk = add(m.O, scl(tk, m.D));
         const Lattice lk = to_lattice(p, Pk);
-        const bool occ = mc_occupied(T, (mc_axis(lk.Lz, sh, p.mc_dim[2]) * p.mc_dim[1] + mc_axis(lk.Ly, sh, p.mc_dim[1])) *
-                                            p.mc_dim[0] + mc_axis(lk.Lx, sh, p.mc_dim[0]));
+        const bool occ = mc_occupied(T, mc_index(p, lk, sh));
         const bool live = inseg && occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
         float sigma = 0.0f;
         if (live) sigma = T.lut(min((int)trilinear_cell(p, lk), p.lut_max));

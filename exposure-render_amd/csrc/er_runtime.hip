@@ -474,6 +474,10 @@ int upload_volume(Volume& v)
     const int rx = d.resolution[0], ry = d.resolution[1], rz = d.resolution[2];
     const size_t n = (size_t)rx * ry * rz;
     if (rx <= 0 || ry <= 0 || rz <= 0) return set_error(ER_ERROR, "invalid volume resolution %d x %d x %d", rx, ry, rz);
+    /* the cell records are indexed with 24-bit factors and a 32-bit index (er_device.h) */
+    if ((size_t)(rx + 1) * (ry + 1) >= ((size_t)1 << 24) || (size_t)(rx + 1) * (ry + 1) * (rz + 1) >= ((size_t)1 << 32))
+        return set_error(ER_ERROR, "volume %d x %d x %d too large: (Rx+1)(Ry+1) must stay below 2^24 and the cell "
+                                   "count below 2^32", rx, ry, rz);
     if (!d.voxels) return set_error(ER_ERROR, "volume voxels pointer is NULL");
     int st;
     if ((st = v.vox.alloc(n))) return st;
