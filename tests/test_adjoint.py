@@ -133,7 +133,7 @@ def test_autograd_adam_reduces_loss(fresh):
         opt.step()
         with torch.no_grad():
             lut.clamp_(min=0.0)
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     print(f"[adjoint-adam] loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
     assert losses[-1] < 0.5 * losses[0]
     r.close()
