@@ -289,9 +289,10 @@ __device__ __forceinline__ void finish_item(const EventRec& ev, unsigned int q, 
 }
 
 /* Primary kernel: persistent waves take 64-pixel tiles (8 x 8 by default, DevParams::tile_shift)
- * from an atomic counter (counts[2]),
- * so the cost of a tile - from a few samples for background pixels to thousands behind
- * low-density tissue - is balanced over the whole device. */
+ * from an atomic counter (counts[2]), so the cost of a tile - from a few samples for background
+ * pixels to thousands behind low-density tissue - is balanced over the whole device. With an
+ * order (k_tile_order, from the previous pass) the tiles are claimed most expensive first, and
+ * each tile's cost is recorded for the next order. */
 template<int FULL, int COUNT, int LUT_LDS>
 __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ p
     int paths = 0;
     /* claims [0, 64 hc): the rays of the hc heaviest tiles one by one (order[0 .. hc), 64 claims
      * each, every ray marched by a whole wave); then the other tiles in their order */
-    unsigned int hc = order ? __builtin_amdgcn_readfirstlane(order[ntiles]) : 0u;
+    unsigned int hc = order ? min(__builtin_amdgcn_readfirstlane(order[ntiles]), (unsigned int)ntiles) : 0u;
     if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) hc = 0u;
     const unsigned int nclaims = hc * 64u + ((unsigned int)ntiles - hc);
     for (;;) {
