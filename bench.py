@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched section (its own JSON line; for kernel profiles of the batch)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the large section (its own JSON line; for kernel profiles of that workload)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy a device buffer of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -302,6 +304,11 @@ def main():
         import exposure_render as er
         from exposure_render import scenes
         print(json.dumps({"batched": bench_batched(er, scenes, args.batch_jobs)}))
+        return
+    if args.large_only:
+        import exposure_render as er
+        from exposure_render import scenes
+        print(json.dumps({"large": bench_large(er, scenes, not args.no_tune)}))
         return
     if args.copy_probe:
         import torch

@@ -857,11 +857,17 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
         t.order_tiles = 0;
     }
     const bool order_valid = t.order_shift == t.tile_shift && t.order_tiles == ntiles;
+    /* tile costs change slowly from pass to pass: the order is rebuilt every fourth pass (and
+     * whenever it does not fit the tile shape) */
+#ifndef ER_REORDER_MASK
+#define ER_REORDER_MASK 3
+#endif
+    const bool reorder = !order_valid || (t.desc.no_iterations & ER_REORDER_MASK) == 0;
     HIP_TRY(erd_launch_trace(t.params.ptr, S.frame.ptr, t.seeds1.ptr, t.seeds2.ptr, S.queue.ptr, S.qcount.ptr,
                              S.events.ptr, S.jobs.ptr, S.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
                              split, t.lds_bytes, lds_bits, shade_blocks, primary_blocks(t), shadow_blocks(t), t.tile_shift, t.stream,
                              t.stream2, S.shaded, e ? e[1] : nullptr, order_valid ? t.tile_order.ptr : nullptr,
-                             t.tile_cost.ptr, t.tile_order.ptr, t.heavy_tiles));
+                             t.tile_cost.ptr, reorder ? t.tile_order.ptr : nullptr, t.heavy_tiles));
     t.order_shift = t.tile_shift;
     t.order_tiles = ntiles;
     if (e) HIP_TRY(hipEventRecord(e[2], t.stream2));

@@ -348,7 +348,8 @@ __global__ __launch_bounds__(1024) void k_tile_order(uint32_t* __restrict__ cost
     __syncthreads();
     uint32_t mx = 0u;
     for (int i = tid; i < n; i += blockDim.x) mx = max(mx, cost[i]);
-    atomicMax(&smax, mx);
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+    if (lane == 0) atomicMax(&smax, mx);
     __syncthreads();
     const uint64_t cmax = smax;
     auto cls = [cmax](uint32_t c) {
@@ -356,7 +357,15 @@ __global__ __launch_bounds__(1024) void k_tile_order(uint32_t* __restrict__ cost
         while (b < 7 && ((uint64_t)c << (b + 1)) <= cmax) ++b;
         return b;
     };
-    for (int i = tid; i < n; i += blockDim.x) atomicAdd(&base[cls(cost[i])], 1u);
+    /* class counts: one ballot per class and wave instead of an LDS atomic per tile */
+    for (int chunk = 0; chunk < n; chunk += blockDim.x) {
+        const int i = chunk + tid;
+        const int b = i < n ? cls(cost[i]) : 8;
+        for (int bb = 0; bb < 8; ++bb) {
+            const unsigned long long m = __ballot(b == bb);
+            if (lane == 0 && m) atomicAdd(&base[bb], (uint32_t)__popcll(m));
+        }
+    }
     __syncthreads();
     __shared__ uint32_t heavy;
     if (tid == 0) {
