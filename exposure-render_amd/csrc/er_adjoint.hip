@@ -43,10 +43,10 @@ namespace erd {
 #define ADJ_TAU_CAP 12.0f
 /* dense samples per march iteration in the replay (loads in flight together) */
 #ifndef ER_BATCH_PRIMARY_ADJ
-#define ER_BATCH_PRIMARY_ADJ 4
+#define ER_BATCH_PRIMARY_ADJ 8
 #endif
 #ifndef ER_BATCH_SHADOW_ADJ
-#define ER_BATCH_SHADOW_ADJ 4
+#define ER_BATCH_SHADOW_ADJ 8
 #endif
 
 struct AdjArgs {
