@@ -293,6 +293,8 @@ def main():
                     help="only the batched section (its own JSON line; for kernel profiles of the batch)")
     ap.add_argument("--large-only", action="store_true",
                     help="only the large section (its own JSON line; for kernel profiles of that workload)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint section (its own JSON line; for kernel profiles of the adjoint)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy a device buffer of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -309,6 +311,11 @@ def main():
         import exposure_render as er
         from exposure_render import scenes
         print(json.dumps({"large": bench_large(er, scenes, not args.no_tune)}))
+        return
+    if args.adjoint_only:
+        import exposure_render as er
+        from exposure_render import scenes
+        print(json.dumps({"adjoint": bench_adjoint(er, scenes)}))
         return
     if args.copy_probe:
         import torch

@@ -78,6 +78,10 @@ extern "C" hipError_t erd_launch_adjoint(const DevParams* p, int mode, const flo
                                          const float4* adj, float4* value, float* partials, uint32_t* s1, uint32_t* s2,
                                          int W, int H, size_t lds_bytes, hipStream_t st);
 extern "C" hipError_t erd_launch_adj_filter_t(const float* dldr, float4* out, int W, int H, float weight, hipStream_t st);
+/* workgroups of 256 lanes in each adjoint kernel (grid-stride over jobs, events and pixels) */
+#ifndef ER_ADJ_BLOCKS
+#define ER_ADJ_BLOCKS 1024
+#endif
 extern "C" hipError_t erd_launch_adj_reduce(const float* partials, int nblocks, double* out, hipStream_t st);
 
 namespace {
@@ -1782,7 +1786,7 @@ ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t*
     if ((st = adjoint_setup(*t, seeds1, seeds2, count, passes))) return st;
     if (!dL_drunning_xyz || !dL_dlut) return set_error(ER_ERROR, "adjoint pointer is NULL");
     const size_t n = (size_t)t->W * t->H;
-    const int blocks = 1024;
+    const int blocks = ER_ADJ_BLOCKS;
     const size_t per_block = erd_adj_partials_per_block();
     size_t rs = 0, re = 0, rj = 0;
     erd_pass_record_sizes(&rs, &re, &rj);
