@@ -16,20 +16,48 @@
  * to float); sin/cos/exp/pow/acos/atan2 in float are the project's er_fmath.h functions, whose
  * accuracy the tests pin against double libm; Real = double uses <cmath>.
  *
- * Parity status per function (pinning tests in tests/test_oracle.py):
- *   rng (rng.h:35-61)                    pinned: closed-form MWC recurrence, u = (r & 0x7fffff) 2^-23
- *   plf_eval (plf.h:68-91)               pinned: closed form, Q5 last-node zero, brute force
- *   get_intensity (buffer3d.h:222-242)   pinned: brute force vs. numpy restatement, corner bounds
- *   intersect_box (box.h:51-83)          pinned: closed-form slab distances
- *   sample_volume (raymarching.h:30-71)  pinned: homogeneous-slab survival law, step counts
- *   camera (camera.h:61-96, singlescattering.h:29-50) pinned: closed-form pinhole rays
- *   shading (shader.h)                   pinned: Lambert/phase closed forms, Fresnel at normal incidence
- *   post (filterframeestimate.cuh, estimate.cuh, tonemap.cuh) pinned: closed forms
- *   tf_lut_eval (node-table transfer function, no counterpart in the reference)
- *                                        pinned: node values, linear interpolation, clamps
- *   full-mode hybrid shading, MIS, clipping, bilateral, Woodcock: own design from the reference's
- *   commented code; pinned by invariants and statistics (the light-sampling and BSDF-sampling
- *   strategies must agree in expectation), reference parity unpinned.
+ * Parity status per function. Each line names the tests that pin it (tests/test_oracle.py,
+ * tests/test_oracle_pins.py); a function marked "unpinned" has no such test.
+ *   rng (rng.h:35-61)                     test_rng_mwc_closed_form, test_rng_zero_seed_is_fixed_point
+ *   -log(u) (raymarching.h:45,89)         test_neglog_correctly_rounded_exhaustive
+ *   float elementary functions            test_fmath_accuracy
+ *   plf_eval (plf.h:68-91)                test_plf_closed_form_and_last_node_zero, test_plf_brute_force_random
+ *   get_intensity (buffer3d.h:222-242)    test_trilinear_lattice_points_and_layout, test_trilinear_bounds_and_clamp
+ *   march (raymarching.h:30-113)          test_march_homogeneous_step_count_and_T_offset,
+ *                                         test_march_transmittance_density_scale_squared,
+ *                                         test_march_misses_and_shadow_tmax
+ *   intersect_box (box.h:51-83)           the march tests above (box entry, misses, MaxT)
+ *   camera (camera.h:61-96)               test_camera_frame
+ *   gradients FD/CD/Filtered, normalised  test_gradients_on_a_ramp_closed_form,
+ *     (volumes.h:31-91)                   test_gradient_delta_is_the_minimum_spacing_on_every_axis,
+ *                                         test_constant_volume_has_nan_normal
+ *   gradient_magnitude, gm max            test_gradients_on_a_ramp_closed_form,
+ *     (volumes.h:93-114, gradientmagnitude.cuh:28-71)  test_gradient_delta_is_the_minimum_spacing_on_every_axis
+ *   fresnel (shader.h:69-123)             test_fresnel_closed_forms
+ *   blinn_d / blinn_pdf (shader.h:166-185) test_blinn_d_is_normalised,
+ *                                         test_blinn_pdf_integrates_to_the_half_vector_mass
+ *   microfacet_f (shader.h:216-263)       test_microfacet_normal_incidence_and_reciprocity,
+ *                                         test_specular_floor_direct_light_quadrature
+ *   Lambert BRDF, MIS, estimate_direct    test_object_direct_light_quadrature, test_specular_floor_direct_light_quadrature
+ *     (transport.h:58-113)
+ *   phase scattering (shader.h:29-60,     test_phase_single_scatter_matches_the_discrete_march_integral
+ *     transport.h:137-139)
+ *   light choice, Count scaling           test_three_lights_expectation_is_the_sum_and_choice_is_uniform
+ *     (transport.h:115-156)
+ *   lights: plane, disk, ring, box,       test_object_direct_light_quadrature, test_round_light_direct_light_quadrature,
+ *     sphere (lights.h, plane.h, disk.h,  test_box_light_direct_light_quadrature, test_environment_sphere_light
+ *     ring.h, box.h, sphere.h)
+ *   textures, bitmaps (textures.h,        test_procedural_textures_closed_form, test_bitmap_texture_full_is_bilinear,
+ *     buffer2d.h:227-254)                 test_bitmap_texture_compat_reads_exact_texels_only
+ *   composition special cases             test_no_lights_gives_unscaled_emission, test_transparent_volume_is_invisible,
+ *     (singlescattering.h:52-102)         test_draw_schedule, test_fp32_mean_matches_fp64_expectation
+ *   post (filterframeestimate.cuh, estimate.cuh, tonemap.cuh)  test_post_filter_cma_tonemap_closed_forms
+ *   bilateral (filterrunningestimate.cuh) test_bilateral_constant_and_range_limits
+ *   node-table transfer function (no counterpart in the reference)  test_node_table_transfer_function
+ *   Woodcock and clipping (full mode)     test_woodcock_free_flight_and_clipping_closed_form
+ *   shader selection (full mode; design   test_shading_type_selection_limits at the limits; between them the
+ *     scatterevent.h:155-212)             hybrid and modulation probabilities are unpinned
+ *   draw order within Get2/Get3 (Q22)     unpinned: every order gives the same distributions
  */
 #include "../include/er_capi.h"
 #include "../exposure-render_amd/csrc/er_fmath.h"
@@ -1448,6 +1476,45 @@ float orc_intensity(void* h, float x, float y, float z)
     Rng r{1, 1};
     Est<float> e(*(Scene*)h, r, nullptr);
     return e.get_intensity(V3<float>(x, y, z));
+}
+/* GradientFD / CD / Filtered at P (volumes.h:31-74, mode = ER_GRADIENT_*), as computed
+ * (grad3) and normalised (norm3, volumes.h:76-91) */
+void orc_gradient(void* h, int mode, const float* p, float* grad3, float* norm3)
+{
+    Scene& S = *(Scene*)h;
+    const int keep = S.t.settings.gradient_computation;
+    S.t.settings.gradient_computation = mode;
+    Rng r{1, 1};
+    Est<float> e(S, r, nullptr);
+    const V3<float> P(p[0], p[1], p[2]);
+    const V3<float> g = mode == ER_GRADIENT_FORWARD ? e.grad_fd(P) : (mode == ER_GRADIENT_CENTRAL ? e.grad_cd(P) : e.grad_filtered(P));
+    const V3<float> n = e.normalized_gradient(P);
+    S.t.settings.gradient_computation = keep;
+    for (int i = 0; i < 3; ++i) { grad3[i] = g[i]; norm3[i] = n[i]; }
+}
+/* GradientMagnitude at P (volumes.h:93-114) */
+float orc_gradient_magnitude(void* h, const float* p)
+{
+    Rng r{1, 1};
+    Est<float> e(*(Scene*)h, r, nullptr);
+    return e.gradient_magnitude(V3<float>(p[0], p[1], p[2]));
+}
+/* shader pieces (shader.h:69-263), in the shading frame (z = normal) */
+float orc_fresnel(float cosi, float ior) { return Est<float>::fresnel(cosi, ior); }
+float orc_blinn_d(float exponent, float whz) { return Est<float>::blinn_d(exponent, V3<float>(0.0f, 0.0f, whz)); }
+float orc_blinn_pdf(float exponent, const float* wo, const float* wi)
+{
+    return Est<float>::blinn_pdf(exponent, V3<float>(wo[0], wo[1], wo[2]), V3<float>(wi[0], wi[1], wi[2]));
+}
+void orc_microfacet_f(const float* wo, const float* wi, const float* ks, float exponent, float ior, float* out3)
+{
+    typename Est<float>::Shader s{};
+    s.brdf = true;
+    s.Ks = V3<float>(ks[0], ks[1], ks[2]);
+    s.exponent = exponent;
+    s.ior = ior;
+    const V3<float> f = Est<float>::microfacet_f(s, V3<float>(wo[0], wo[1], wo[2]), V3<float>(wi[0], wi[1], wi[2]));
+    for (int i = 0; i < 3; ++i) out3[i] = f[i];
 }
 float orc_plf_eval(const er_plf* p, float x) { return plf_eval(*p, x); }
 double orc_plf_eval_f64(const er_plf* p, double x) { return plf_eval(*p, x); }

@@ -54,6 +54,13 @@ def load():
         "orc_camera": (C.c_int, [C.c_void_p, _P(C.c_float)]),
         "orc_march_ray": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, C.c_int,
                                     C.c_float, C.c_float, _P(C.c_float), _P(C.c_float), _P(C.c_uint64)]),
+        "orc_gradient": (None, [C.c_void_p, C.c_int, _P(C.c_float), _P(C.c_float), _P(C.c_float)]),
+        "orc_gradient_magnitude": (C.c_float, [C.c_void_p, _P(C.c_float)]),
+        "orc_fresnel": (C.c_float, [C.c_float, C.c_float]),
+        "orc_blinn_d": (C.c_float, [C.c_float, C.c_float]),
+        "orc_blinn_pdf": (C.c_float, [C.c_float, _P(C.c_float), _P(C.c_float)]),
+        "orc_microfacet_f": (None, [_P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float,
+                                    _P(C.c_float)]),
         "orc_abi_sizeof": (C.c_int, [_P(C.c_int), C.c_int]),
         "orc_fmath": (None, [C.c_int, _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_int]),
     }
@@ -200,6 +207,18 @@ class Oracle:
             display = self.bilateral(display)
         return display, running, frame, s1, s2, cnt
 
+    def gradient(self, mode, p):
+        """(gradient, normalised gradient) at p for gradient mode `mode` (ER_GRADIENT_*)."""
+        p = np.asarray(p, dtype=np.float32)
+        g = np.zeros(3, dtype=np.float32)
+        n = np.zeros(3, dtype=np.float32)
+        self.lib.orc_gradient(self.h, int(mode), _p(p, C.c_float), _p(g, C.c_float), _p(n, C.c_float))
+        return g, n
+
+    def gradient_magnitude(self, p):
+        p = np.asarray(p, dtype=np.float32)
+        return self.lib.orc_gradient_magnitude(self.h, _p(p, C.c_float))
+
     def intensity(self, p):
         return self.lib.orc_intensity(self.h, float(p[0]), float(p[1]), float(p[2]))
 
@@ -260,4 +279,28 @@ def fmath(fn, x, y=None):
     y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float32)
     out = np.zeros_like(x)
     lib.orc_fmath(fn, _p(x, C.c_float), _p(y, C.c_float), _p(out, C.c_float), x.size)
+    return out
+
+
+def fresnel(cosi, ior):
+    return load().orc_fresnel(float(cosi), float(ior))
+
+
+def blinn_d(exponent, whz):
+    return load().orc_blinn_d(float(exponent), float(whz))
+
+
+def blinn_pdf(exponent, wo, wi):
+    wo = np.asarray(wo, dtype=np.float32)
+    wi = np.asarray(wi, dtype=np.float32)
+    return load().orc_blinn_pdf(float(exponent), _p(wo, C.c_float), _p(wi, C.c_float))
+
+
+def microfacet_f(wo, wi, ks, exponent, ior=15.0):
+    wo = np.asarray(wo, dtype=np.float32)
+    wi = np.asarray(wi, dtype=np.float32)
+    ks = np.asarray(ks, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    load().orc_microfacet_f(_p(wo, C.c_float), _p(wi, C.c_float), _p(ks, C.c_float), float(exponent), float(ior),
+                            _p(out, C.c_float))
     return out
