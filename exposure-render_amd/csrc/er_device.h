@@ -96,7 +96,12 @@ struct DevParams {
     float step_primary, step_shadow, max_shadow, density_scale;
     float grad_threshold, grad_factor, gm_inv;
     int   shadows, estimator, shading_type, opacity_modulated, gradient_mode, sampler;
-    float woodcock_majorant;         /* DS^2 * max opacity (global majorant) */
+    /* Woodcock delta tracking: majorants on a fixed grid of 2^ER_WC_SHIFT-cell macrocells
+     * (independent of the tuned brick size, so the image does not depend on it); wc_sigma[m]
+     * is the largest sigma_lut value over the intensities a sample in macrocell m can take,
+     * and density_scale * wc_sigma[m] bounds the extinction DS^2 * opacity there */
+    const float* wc_sigma;
+    int   wc_dim[3];
     /* volume (Source/volume.h:100-138) */
     int   res[3];
     float minp[3], maxp[3], size[3], invsize[3], resf[3], spacing[3], minstep;

@@ -347,6 +347,61 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
     return 0;
 }
 
+/* Woodcock delta tracking (full estimator; design from the commented sampler of
+ * raymarching.h:115-273, its macrocells at :155 and majorant texture at :205) with a piecewise
+ * constant majorant: mu = DS * wc_sigma of the 2^ER_WC_SHIFT-cell macrocell the ray is in. Each
+ * tentative collision draws one optical depth tau = -log(u) and spends it across macrocells:
+ * in a segment [t, t_exit) the collision is at t + tau / mu if that is before t_exit, else tau
+ * loses mu (t_exit - t) and the walk moves to the next macrocell. A collision is real with
+ * probability sigma(P) / mu (one more draw), else tracking continues from it. The segment ends
+ * at the nearest macrocell face in lattice coordinates (a macrocell at the grid's edge extends
+ * to the end of the ray, as the clamped lookups do) and advances t by at least a millionth of
+ * it, so every segment makes progress. The oracle runs the same float operations. */
+#define ER_WC_SHIFT 3
+#define ER_WC_MAX_SEGMENTS 65536
+template<int FULL, int COUNT>
+__device__ __forceinline__ bool woodcock_track(const DevParams& p, const Tables& T, const Ray& r, float nearT, float farT,
+                                               bool primary, Rng& rng, float& Tout, f3& Pout, Cnt& cnt)
+{
+    float t = fmaxf(nearT, r.MinT);
+    const float MaxT = fminf(farT, r.MaxT);
+    /* lattice units per unit of t along each axis */
+    const float gx = (r.D.x * p.invsize[0]) * p.resf[0];
+    const float gy = (r.D.y * p.invsize[1]) * p.resf[1];
+    const float gz = (r.D.z * p.invsize[2]) * p.resf[2];
+    const int dx = p.wc_dim[0], dy = p.wc_dim[1], dz = p.wc_dim[2];
+    const float big = 3.402823466e+38f;
+    int segments = 0;
+    for (;;) {
+        float tau = gmem(p.neglog)[rng.next_bits()];
+        float mu = 0.0f;
+        for (;;) {
+            if (++segments > ER_WC_MAX_SEGMENTS || !(t < MaxT)) return false;
+            const Lattice l = to_lattice(p, add(r.O, scl(t, r.D)));
+            const int mx = mc_axis(l.Lx, ER_WC_SHIFT, dx), my = mc_axis(l.Ly, ER_WC_SHIFT, dy), mz = mc_axis(l.Lz, ER_WC_SHIFT, dz);
+            mu = p.density_scale * gmem(p.wc_sigma)[(mz * dy + my) * dx + mx];
+            const float ex = gx > 0.0f ? (mx == dx - 1 ? big : ((float)((mx + 1) << ER_WC_SHIFT) - l.Lx) / gx)
+                                       : (gx < 0.0f ? (mx == 0 ? big : ((float)(mx << ER_WC_SHIFT) - l.Lx) / gx) : big);
+            const float ey = gy > 0.0f ? (my == dy - 1 ? big : ((float)((my + 1) << ER_WC_SHIFT) - l.Ly) / gy)
+                                       : (gy < 0.0f ? (my == 0 ? big : ((float)(my << ER_WC_SHIFT) - l.Ly) / gy) : big);
+            const float ez = gz > 0.0f ? (mz == dz - 1 ? big : ((float)((mz + 1) << ER_WC_SHIFT) - l.Lz) / gz)
+                                       : (gz < 0.0f ? (mz == 0 ? big : ((float)(mz << ER_WC_SHIFT) - l.Lz) / gz) : big);
+            const float t_exit = fmaxf(fminf(t + fminf(ex, fminf(ey, ez)), MaxT), t + fmaxf(t * 1.0e-6f, 1.0e-7f));
+            if (mu > 0.0f) {
+                const float tc = t + tau / mu;
+                if (tc < t_exit) { t = tc; break; }
+                tau = fmaxf(tau - mu * (t_exit - t), 0.0f);
+            }
+            t = t_exit;
+        }
+        if (primary ? (t >= MaxT) : (t > MaxT)) return false;
+        const f3 Ps = add(r.O, scl(t, r.D));
+        if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; }
+        const float sigma = sample_sigma<FULL, COUNT>(p, T, Ps, primary, cnt) * p.density_scale;
+        if (rng.u() * mu < sigma) { Tout = t; Pout = Ps; return true; }
+    }
+}
+
 /* S_fixed >= 0: the threshold is still drawn (same draws) but S_fixed replaces it, so the march
  * reads every sample until that optical depth / DensityScale or the end of the ray (the
  * expected-transmittance form of a shadow test). */
@@ -356,21 +411,7 @@ __device__ __forceinline__ bool march_v(const DevParams& p, const Tables& T, con
 {
     float nearT, farT;
     if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return false;
-    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
-        /* delta tracking against the global majorant DS^2 * max opacity */
-        const float mu = p.woodcock_majorant;
-        if (!(mu > 0.0f)) return false;
-        float t = fmaxf(nearT, r.MinT);
-        const float MaxT = fminf(farT, r.MaxT);
-        for (;;) {
-            t += gmem(p.neglog)[rng.next_bits()] / mu;
-            if (primary ? (t >= MaxT) : (t > MaxT)) return false;
-            const f3 Ps = add(r.O, scl(t, r.D));
-            if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; }
-            const float sigma = sample_sigma<FULL, COUNT>(p, T, Ps, primary, cnt) * p.density_scale;
-            if (rng.u() * mu < sigma) { Tout = t; Pout = Ps; return true; }
-        }
-    }
+    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) return woodcock_track<FULL, COUNT>(p, T, r, nearT, farT, primary, rng, Tout, Pout, cnt);
     const float StepSize = (primary ? p.step_primary : p.step_shadow) * p.minstep;
     vis.begin(StepSize);
     MarchState m;

@@ -83,7 +83,6 @@ extern "C" hipError_t erd_launch_adj_filter_t(const float* dldr, float4* out, in
 #define ER_ADJ_BLOCKS 1024
 #endif
 extern "C" hipError_t erd_launch_adj_reduce(const float* partials, int nblocks, double* out, hipStream_t st);
-
 namespace {
 
 std::recursive_mutex g_mutex;
@@ -268,7 +267,7 @@ struct Tracer {
     DevBuf<float4> running;
     DevBuf<uchar4> display, disp_tmp, disp_filt;
     DevBuf<uint32_t> seeds1, seeds2;
-    DevBuf<float> lut, mc_sigma;
+    DevBuf<float> lut, mc_sigma, wc_sigma;
     DevBuf<uint32_t> mc_bits;
     size_t lds_bytes = 0;
     int lut_lds = 0;
@@ -298,7 +297,7 @@ struct Tracer {
     {
         set[0].release(); set[1].release();
         running.release(); display.release(); disp_tmp.release(); disp_filt.release();
-        seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); params.release();
+        seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); wc_sigma.release(); params.release();
         mc_bits.release(); counters.release(); gm_tmp.release(); bilateral.release();
         tile_cost.release(); tile_order.release();
         if (stream) (void)hipStreamDestroy(stream);
@@ -479,9 +478,10 @@ int upload_volume(Volume& v)
     const size_t n = (size_t)rx * ry * rz;
     if (rx <= 0 || ry <= 0 || rz <= 0) return set_error(ER_ERROR, "invalid volume resolution %d x %d x %d", rx, ry, rz);
     /* the cell records are indexed with 24-bit factors and a 32-bit index (er_device.h) */
-    if ((size_t)(rx + 1) * (ry + 1) >= ((size_t)1 << 24) || (size_t)(rx + 1) * (ry + 1) * (rz + 1) >= ((size_t)1 << 32))
-        return set_error(ER_ERROR, "volume %d x %d x %d too large: (Rx+1)(Ry+1) must stay below 2^24 and the cell "
-                                   "count below 2^32", rx, ry, rz);
+    if ((size_t)(rx + 1) * (ry + 1) >= ((size_t)1 << 24) || (size_t)(rz + 1) >= ((size_t)1 << 24) ||
+        (size_t)(rx + 1) * (ry + 1) * (rz + 1) >= ((size_t)1 << 32))
+        return set_error(ER_ERROR, "volume %d x %d x %d too large: (Rx+1)(Ry+1) and Rz+1 must stay below 2^24 and the "
+                                   "cell count below 2^32", rx, ry, rz);
     if (!d.voxels) return set_error(ER_ERROR, "volume voxels pointer is NULL");
     int st;
     if ((st = v.vox.alloc(n))) return st;
@@ -760,14 +760,11 @@ int build_params(Tracer& t)
     /* exact opacity LUT over the intensities the volume can produce */
     const int lut_n = (int)vol->vmax + 1;
     std::vector<float> lut(lut_n);
-    float omax = 0.0f;
     for (int i = 0; i < lut_n; ++i) {
         const float o = t.tf_lut.empty() ? plf_eval_host(d.opacity, (float)i)
                                          : erd::tf_lut_eval(t.tf_lut.data(), t.tf_scale, (float)i, 0);
         lut[i] = s.density_scale * o;
-        omax = std::max(omax, o);
     }
-    P.woodcock_majorant = s.density_scale * s.density_scale * omax;
     if ((st = t.lut.alloc(lut_n))) return st;
     HIP_TRY(hipMemcpy(t.lut.ptr, lut.data(), lut_n * sizeof(float), hipMemcpyHostToDevice));
     P.sigma_lut = t.lut.ptr;
@@ -777,6 +774,17 @@ int build_params(Tracer& t)
     HIP_TRY(erd_launch_macro_sigma(grid->mm.ptr, t.lut.ptr, P.lut_max, t.mc_sigma.ptr, nmc, t.stream));
     P.mc_sigma = t.mc_sigma.ptr;
     P.mc_shift = mc_shift;
+    /* Woodcock majorants on their own fixed grid of 2^ER_WC_SHIFT-cell macrocells */
+    P.wc_sigma = nullptr;
+    if (s.estimator == ER_ESTIMATOR_FULL && s.sampler == ER_SAMPLER_WOODCOCK) {
+        MacroGrid* wg = volume_grid(*vol, ER_WC_SHIFT, st);
+        if (!wg) return st;
+        const int nw = wg->mc[0] * wg->mc[1] * wg->mc[2];
+        if ((st = t.wc_sigma.alloc(nw))) return st;
+        HIP_TRY(erd_launch_macro_sigma(wg->mm.ptr, t.lut.ptr, P.lut_max, t.wc_sigma.ptr, nw, t.stream));
+        P.wc_sigma = t.wc_sigma.ptr;
+        for (int i = 0; i < 3; ++i) P.wc_dim[i] = wg->mc[i];
+    }
     /* occupancy bits: one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a side */
     const int mc_words = (nmc + 31) / 32;
     for (int i = 0; i < 3; ++i) P.mc2_dim[i] = (grid->mc[i] + (1 << ER_GROUP_SHIFT) - 1) >> ER_GROUP_SHIFT;
@@ -925,7 +933,10 @@ int bind_generic(R& reg, D* d, int bind, F make)
         d->id = id;
     }
     int st = make(obj, *d, fresh);
-    if (st) return st;
+    if (st) {
+        if (fresh) d->id = -1;   /* nothing was bound */
+        return st;
+    }
     if (fresh) { reg.map[d->id] = obj; reg.counter++; }
     g_scene_version++;
     return ER_OK;
@@ -993,21 +1004,24 @@ int apply_tune(Tracer& t, const er_tune& r)
 struct Snapshot {
     DevBuf<uint32_t> s1, s2;
     DevBuf<float4> run;
-    DevBuf<uchar4> disp;
+    DevBuf<uchar4> disp, filt;
     DevBuf<unsigned long long> cnt;
     int no_iter = 0;
-    ~Snapshot() { s1.release(); s2.release(); run.release(); disp.release(); cnt.release(); }
+    ~Snapshot() { s1.release(); s2.release(); run.release(); disp.release(); filt.release(); cnt.release(); }
     int save(Tracer& t)
     {
         int st;
         const size_t n = (size_t)t.W * t.H;
         if ((st = sync_tracer(t))) return st;
-        if ((st = s1.alloc(n)) || (st = s2.alloc(n)) || (st = run.alloc(n)) || (st = disp.alloc(n)) || (st = cnt.alloc(12)))
+        if ((st = s1.alloc(n)) || (st = s2.alloc(n)) || (st = run.alloc(n)) || (st = disp.alloc(n)) ||
+            (st = filt.alloc(n)) || (st = cnt.alloc(12)))
             return st;
         HIP_TRY(hipMemcpy(s1.ptr, t.seeds1.ptr, n * 4, hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(s2.ptr, t.seeds2.ptr, n * 4, hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(run.ptr, t.running.ptr, n * sizeof(float4), hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(disp.ptr, t.display.ptr, n * sizeof(uchar4), hipMemcpyDeviceToDevice));
+        /* the bilateral filter's output is what er_get_estimate returns when it is on */
+        HIP_TRY(hipMemcpy(filt.ptr, t.disp_filt.ptr, n * sizeof(uchar4), hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(cnt.ptr, t.counters.ptr, 12 * sizeof(unsigned long long), hipMemcpyDeviceToDevice));
         no_iter = t.desc.no_iterations;
         return ER_OK;
@@ -1029,6 +1043,7 @@ struct Snapshot {
         HIP_TRY(hipMemcpy(t.seeds2.ptr, s2.ptr, n * 4, hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(t.running.ptr, run.ptr, n * sizeof(float4), hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(t.display.ptr, disp.ptr, n * sizeof(uchar4), hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(t.disp_filt.ptr, filt.ptr, n * sizeof(uchar4), hipMemcpyDeviceToDevice));
         HIP_TRY(hipMemcpy(t.counters.ptr, cnt.ptr, 12 * sizeof(unsigned long long), hipMemcpyDeviceToDevice));
         t.desc.no_iterations = no_iter;
         return ER_OK;
@@ -1257,7 +1272,13 @@ ER_API int er_bind_volume(er_volume_desc* d, int bind)
     return bind_generic(g_volumes, d, bind, [](Volume*& v, er_volume_desc& desc, bool fresh) -> int {
         if (fresh) v = new Volume();
         const bool upload = fresh || desc.voxels_dirty;
+        /* without new voxels the device buffers keep their size: the volume keeps its own
+         * resolution and takes the rest (spacing, normalisation) from the descriptor, as
+         * Buffer3D::operator= skips a source that is not Dirty (buffer3d.h:52-65) */
+        int res[3] = {v->desc.resolution[0], v->desc.resolution[1], v->desc.resolution[2]};
         v->desc = desc;
+        if (!upload)
+            for (int i = 0; i < 3; ++i) v->desc.resolution[i] = res[i];
         if (upload) {
             int st = upload_volume(*v);
             if (st) { if (fresh) { delete v; v = nullptr; } return st; }
@@ -1866,6 +1887,7 @@ ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t*
     if ((st = sync_tracer(*t))) return st;
     return ER_OK;
 }
+
 
 ER_API int er_adjoint_value(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
                             const float* eval_lut, float* running_xyz)

@@ -79,8 +79,9 @@ extern "C" {
 #define ER_ESTIMATOR_FULL   1
 
 /* Free-flight samplers (new). RAYMARCH = Source/raymarching.h:30-113 (jittered fixed-step
- * tracking); WOODCOCK = delta tracking against the global majorant DS^2 * max opacity (full
- * estimator only). */
+ * tracking); WOODCOCK = delta tracking with a piecewise constant majorant, DS^2 times the
+ * largest opacity a sample can take in each 8-cell macrocell (full estimator only; design
+ * from the commented sampler at Source/raymarching.h:115-273). */
 #define ER_SAMPLER_RAYMARCH 0
 #define ER_SAMPLER_WOODCOCK 1
 
@@ -177,7 +178,8 @@ typedef struct er_shape {
 /* ErVolume (Source/ervolume.h:28-74). voxels: resolution[0]*resolution[1]*resolution[2]
  * uint16, x fastest (index z*Rx*Ry + y*Rx + x, Source/buffer3d.h:210-214). The voxels are
  * uploaded when voxels_dirty is set (Buffer3D dirty-gated copy, buffer3d.h:52-65); bind
- * clears it, so an unchanged rebind skips the upload. */
+ * clears it, so an unchanged rebind skips the upload, and a rebind without new voxels keeps
+ * the volume's resolution (spacing and normalisation come from the descriptor). */
 typedef struct er_volume_desc {
     int             id;
     int             enabled;

@@ -83,6 +83,8 @@ static const float INV_FOUR_PI_F = 1.0f / 4.0f * 3.141592654f;   /* Q6: expands 
 static const float RAY_EPS       = 0.0001f;
 static const float RAY_EPS_2     = 2.0f * 0.0001f;
 static const float TRUE_INV_FOUR_PI_F = 0.0795774715459476678844f; /* full mode phase pdf */
+static const int   WC_SHIFT = 3;              /* Woodcock majorant macrocells: 8 cells a side */
+static const int   WC_MAX_SEGMENTS = 65536;   /* segments per tracking walk (then: no event) */
 
 /* ---- elementary functions ------------------------------------------------------------- */
 template<class R> struct M;
@@ -210,7 +212,10 @@ struct Scene {
     struct Bitmap { int w = 0, h = 0; std::vector<uint8_t> rgba; };
     std::vector<Bitmap> bitmaps;           /* index = bitmap ID */
     float gm_max = 0.0f;
-    float maj_max = 0.0f;                  /* global majorant for Woodcock: DS^2 * max opacity */
+    /* Woodcock majorants: max sigma over the intensities a sample can take in each macrocell
+     * of 2^WC_SHIFT cells (the GPU's wc_sigma grid) */
+    std::vector<float> wc_sigma;
+    int wc_dim[3] = {0, 0, 0};
     std::vector<float> tf_lut;             /* node table (er_set_tf_lut), empty: the PLFs */
     float tf_scale = 0.0f;
 };
@@ -506,22 +511,58 @@ template<class R> struct Est {
         return true;
     }
 
-    /* Woodcock delta tracking against the global majorant DS^2 * max(opacity) (the effective
-     * extinction of the ray-marcher is DS^2 * opacity, Q3). Full mode only; design from the
-     * commented sampler at raymarching.h:115-273. */
+    /* Woodcock delta tracking with a piecewise constant majorant (full mode only; design from
+     * the commented sampler at raymarching.h:115-273: macrocells :155, majorant texture :205,
+     * acceptance sigma / sigma_max > u :261). The effective extinction of the ray-marcher is
+     * DS^2 * opacity (Q3); its majorant in a macrocell of 2^WC_SHIFT cells is DS * the largest
+     * sigma_lut value the macrocell's samples can take. One optical depth -log(u) per tentative
+     * collision is spent across macrocells: the collision is at t + tau / mu inside the current
+     * segment [t, t_exit), else tau loses mu (t_exit - t) and the walk moves on. Segments end
+     * at the nearest macrocell face in lattice coordinates (edge macrocells extend to the end
+     * of the ray, as the clamped lookups do) and advance t by at least a millionth of it. */
+    static int wc_axis(R L, int dim) { return std::min(std::max(((int)std::floor(L)) >> WC_SHIFT, 0), dim - 1); }
     bool woodcock(const Ray<R>& r, bool primary, R MinT, R MaxT, R& Tout, V3<R>& Pout)
     {
-        const R mu_max = R(S.maj_max);
-        if (!(mu_max > R(0))) return false;
         const R DS = R(S.t.settings.density_scale);
+        const R res[3] = {R((float)S.res[0]), R((float)S.res[1]), R((float)S.res[2])};
+        const R g[3] = {(r.D[0] * R(S.invsize[0])) * res[0], (r.D[1] * R(S.invsize[1])) * res[1],
+                        (r.D[2] * R(S.invsize[2])) * res[2]};
+        const int* dim = S.wc_dim;
+        const R big = R(3.402823466e+38f);
         R t = MinT;
+        int segments = 0;
         for (;;) {
-            t += M<R>::neglog(u()) / mu_max;
+            R tau = M<R>::neglog(u());
+            R mu = R(0);
+            for (;;) {
+                if (++segments > WC_MAX_SEGMENTS || !(t < MaxT)) return false;
+                const V3<R> P = r.O + scale(t, r.D);
+                const R L[3] = {((P[0] - R(S.minp[0])) * R(S.invsize[0])) * res[0],
+                                ((P[1] - R(S.minp[1])) * R(S.invsize[1])) * res[1],
+                                ((P[2] - R(S.minp[2])) * R(S.invsize[2])) * res[2]};
+                int m[3];
+                R e[3];
+                for (int a = 0; a < 3; ++a) {
+                    m[a] = wc_axis(L[a], dim[a]);
+                    if (g[a] > R(0)) e[a] = m[a] == dim[a] - 1 ? big : (R((float)((m[a] + 1) << WC_SHIFT)) - L[a]) / g[a];
+                    else if (g[a] < R(0)) e[a] = m[a] == 0 ? big : (R((float)(m[a] << WC_SHIFT)) - L[a]) / g[a];
+                    else e[a] = big;
+                }
+                mu = DS * R(S.wc_sigma[((size_t)m[2] * dim[1] + m[1]) * dim[0] + m[0]]);
+                const R t_exit = std::fmax(std::fmin(t + std::fmin(e[0], std::fmin(e[1], e[2])), MaxT),
+                                           t + std::fmax(t * R(1.0e-6f), R(1.0e-7f)));
+                if (mu > R(0)) {
+                    const R tc = t + tau / mu;
+                    if (tc < t_exit) { t = tc; break; }
+                    tau = std::fmax(tau - mu * (t_exit - t), R(0));
+                }
+                t = t_exit;
+            }
             if (primary ? (t >= MaxT) : (t > MaxT)) return false;
             const V3<R> Ps = r.O + scale(t, r.D);
             if (cnt) cnt[primary ? 1 : 2]++;
             const R sigma = sigma_at(Ps, primary) * DS;
-            if (u() * mu_max < sigma) { Tout = t; Pout = Ps; return true; }
+            if (u() * mu < sigma) { Tout = t; Pout = Ps; return true; }
         }
     }
 
@@ -1248,6 +1289,37 @@ static void bilateral_pass(const Scene& S, const uint8_t* in, uint8_t* tmp, uint
 using namespace orc;
 
 /* ---- C API (ctypes) -------------------------------------------------------------------- */
+/* Woodcock majorant grid: for each macrocell of 2^WC_SHIFT cells, the voxel range
+ * [b E, min(b E + E, R - 1)] per axis that its (clamped) trilinear samples read, the
+ * intensities [min - 3, max] a truncating sample can take (three truncating lerp levels), and
+ * the largest sigma = DS * opacity over them. */
+static void build_woodcock(Scene& S)
+{
+    const int E = 1 << WC_SHIFT;
+    for (int i = 0; i < 3; ++i) S.wc_dim[i] = (S.res[i] + E - 1) / E;
+    uint16_t vmax = 0;
+    for (uint16_t v : S.vox) vmax = std::max(vmax, v);
+    std::vector<float> sig((size_t)vmax + 1);
+    for (int i = 0; i <= (int)vmax; ++i) sig[i] = S.t.settings.density_scale * opacity_at(S, (float)i);
+    const int mx = S.wc_dim[0], my = S.wc_dim[1], mz = S.wc_dim[2];
+    S.wc_sigma.assign((size_t)mx * my * mz, 0.0f);
+    for (int bz = 0; bz < mz; ++bz)
+        for (int by = 0; by < my; ++by)
+            for (int bx = 0; bx < mx; ++bx) {
+                int lo = 0xffff, hi = 0;
+                for (int z = bz * E; z <= std::min(bz * E + E, S.res[2] - 1); ++z)
+                    for (int y = by * E; y <= std::min(by * E + E, S.res[1] - 1); ++y)
+                        for (int x = bx * E; x <= std::min(bx * E + E, S.res[0] - 1); ++x) {
+                            const int v = S.vox[((size_t)z * S.res[1] + y) * S.res[0] + x];
+                            lo = std::min(lo, v);
+                            hi = std::max(hi, v);
+                        }
+                float m = 0.0f;
+                for (int i = std::max(lo - 3, 0); i <= std::min(hi, (int)vmax); ++i) m = std::fmax(m, sig[i]);
+                S.wc_sigma[((size_t)bz * my + by) * mx + bx] = m;
+            }
+}
+
 extern "C" {
 
 void* orc_create(const er_tracer_desc* t, const er_volume_desc* v, const er_light_desc* lights, int nlights,
@@ -1262,12 +1334,7 @@ void* orc_create(const er_tracer_desc* t, const er_volume_desc* v, const er_ligh
     for (int i = 0; i < nlights; ++i) { er_light_desc L = lights[i]; L.shape.area = shape_area(L.shape); S->lights.push_back(L); }
     for (int i = 0; i < ntextures; ++i) S->textures.push_back(textures[i]);
     for (int i = 0; i < nclips; ++i) S->clips.push_back(clips[i]);
-    /* Woodcock global majorant: DS^2 * max opacity over intensities reachable by the volume */
-    uint16_t vmax = 0;
-    for (size_t i = 0; i < n; ++i) vmax = std::max(vmax, S->vox[i]);
-    float omax = 0.0f;
-    for (int i = 0; i <= (int)vmax; ++i) omax = std::max(omax, plf_eval(S->t.opacity, (float)i));
-    S->maj_max = S->t.settings.density_scale * S->t.settings.density_scale * omax;
+    build_woodcock(*S);
     return S;
 }
 
@@ -1349,11 +1416,7 @@ void orc_set_tf_lut(void* h, const float* lut, float range)
     Scene& S = *(Scene*)h;
     if (!lut) { S.tf_lut.clear(); S.tf_scale = 0.0f; }
     else { S.tf_lut.assign(lut, lut + ER_TF_LUT_NODES * 4); S.tf_scale = (float)(ER_TF_LUT_NODES - 1) / range; }
-    uint16_t vmax = 0;
-    for (uint16_t v : S.vox) vmax = std::max(vmax, v);
-    float omax = 0.0f;
-    for (int i = 0; i <= (int)vmax; ++i) omax = std::max(omax, opacity_at(S, (float)i));
-    S.maj_max = S.t.settings.density_scale * S.t.settings.density_scale * omax;
+    build_woodcock(S);
 }
 
 void orc_set_gm_max(void* h, float gm) { ((Scene*)h)->gm_max = gm; }
@@ -1468,6 +1531,16 @@ int orc_bilateral(void* h, const uint8_t* in, uint8_t* out)
     std::vector<uint8_t> tmp((size_t)S.t.camera.film_size[0] * S.t.camera.film_size[1] * 4);
     bilateral_pass(S, in, tmp.data(), out);
     return 0;
+}
+
+/* compiler and flags of this build, for the CPU baseline's record */
+const char* orc_build_info(void)
+{
+#ifdef __clang__
+    return "clang " __clang_version__ ", -O3 -march=x86-64-v2 -ffp-contract=off, OpenMP";
+#else
+    return "g++ " __VERSION__ ", -O3 -march=x86-64-v2 -ffp-contract=off, OpenMP";
+#endif
 }
 
 /* Single functions for pinning tests. */

@@ -61,6 +61,7 @@ def load():
         "orc_blinn_pdf": (C.c_float, [C.c_float, _P(C.c_float), _P(C.c_float)]),
         "orc_microfacet_f": (None, [_P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float,
                                     _P(C.c_float)]),
+        "orc_build_info": (C.c_char_p, []),
         "orc_abi_sizeof": (C.c_int, [_P(C.c_int), C.c_int]),
         "orc_fmath": (None, [C.c_int, _P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_int]),
     }
@@ -304,3 +305,8 @@ def microfacet_f(wo, wi, ks, exponent, ior=15.0):
     load().orc_microfacet_f(_p(wo, C.c_float), _p(wi, C.c_float), _p(ks, C.c_float), float(exponent), float(ior),
                             _p(out, C.c_float))
     return out
+
+
+def build_info():
+    """Compiler and flags the oracle library was built with."""
+    return load().orc_build_info().decode()

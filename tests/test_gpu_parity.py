@@ -195,10 +195,41 @@ def test_gradient_modes(fresh):
         er.shutdown()
 
 
-def test_woodcock(fresh):
-    sc = scenes.config_small(scale=0.5, estimator=capi.ESTIMATOR_FULL)
+def _woodcock_scene(case):
+    if case == "small_phase":
+        sc = scenes.config_small(scale=0.5, estimator=capi.ESTIMATOR_FULL)
+    elif case == "ct_hybrid_env_bilateral":
+        sc = scenes.config_ct(scale=0.125)
+    else:
+        sc = scenes.config_ct(scale=0.125, bilateral=False)
+        sc.add_clip(scenes.clipping_box((0.2, 0.0, 0.0), (0.5, 1.1, 1.1)))
     sc.tracer.settings.sampler = capi.SAMPLER_WOODCOCK
-    run_parity(sc, passes=2)
+    sc.name = f"woodcock_{case}"
+    return sc
+
+
+@pytest.mark.parametrize("case", ["small_phase", "ct_hybrid_env_bilateral", "ct_clip_box"])
+def test_woodcock(fresh, case):
+    """Delta tracking with per-macrocell majorants (full estimator): the same tentative
+    collisions, acceptances, RNG streams and radiance as the oracle's walk."""
+    run_parity(_woodcock_scene(case), passes=2)
+
+
+def test_woodcock_image_does_not_depend_on_the_brick_size(fresh):
+    """The majorants live on their own fixed grid of 8-cell macrocells, so the tuned brick
+    size (occupancy bits) changes no decision of the Woodcock walk."""
+    sc = _woodcock_scene("ct_hybrid_env_bilateral")
+    frames = []
+    for brick in (2, 8, 16):
+        r = er.Renderer(sc, seed=9)
+        r.set_tune(brick, 8, 7)
+        r.render_estimate()
+        r.render_estimate()
+        frames.append((r.frame().copy(), r.get_seeds()))
+        r.close()
+    for f, (g1, g2) in frames[1:]:
+        assert np.array_equal(f, frames[0][0])
+        assert np.array_equal(g1, frames[0][1][0]) and np.array_equal(g2, frames[0][1][1])
 
 
 def test_no_lights_emission_only(fresh):

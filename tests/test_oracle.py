@@ -858,3 +858,43 @@ def test_shading_type_selection_limits():
                       (expect(capi.SHADING_THRESHOLD, threshold=-1.0), brdf)):
         np.testing.assert_allclose(got[:, :3], want[:, :3], rtol=0.04, atol=0.02 * np.abs(want[:, :3]).max())
         assert np.array_equal(got[:, 3] > 0, want[:, 3] > 0)
+
+
+def test_woodcock_heterogeneous_volume_matches_the_optical_depth():
+    """Delta tracking with per-macrocell majorants on a blob volume, where the majorant exceeds
+    the extinction almost everywhere: a camera ray's scatter probability must still be
+    1 - exp(-tau), tau the integral of DS^2 * opacity(I(P(t))) along its chord, integrated here
+    by quadrature over the oracle's own trilinear samples. A majorant below the extinction
+    anywhere on the ray, or a tau spent wrongly across macrocells, biases it."""
+    vox = scenes.phantom_blobs((32, 32, 32), 4, seed=7)
+    sc = scenes.Scene(vox)
+    t = sc.tracer
+    t.opacity.add(0.0, 0.0).add(600.0, 0.0).add(2000.0, 0.5).add(4000.0, 1.0)
+    t.settings.density_scale = 3.0
+    t.settings.estimator = capi.ESTIMATOR_FULL
+    t.settings.shading_type = capi.SHADING_PHASE_ONLY
+    t.settings.sampler = capi.SAMPLER_WOODCOCK
+    scenes.set_camera(t, (24, 24), (1.2, 0.9, 1.6), (0.0, 0.0, 0.0), fov=30.0)
+    orc = O.Oracle(sc, nthreads=4)
+    W, H = sc.film
+    O_cam = np.array([t.camera.pos[i] for i in range(3)], dtype=np.float64)
+    px = np.array([5, 2, 11, 14, 17, 2, 5], dtype=np.int32)
+    py = np.array([2, 17, 14, 20, 17, 20, 20], dtype=np.int32)
+    e = orc.expect_f64(px, py, 20000, seed=31)
+    lo, hi = np.full(3, -0.5), np.full(3, 0.5)
+    n = 3000
+    for k in range(len(px)):
+        D = _camera_hits(orc, W, H, px[k], py[k])
+        t0, t1 = 0.0, np.inf
+        for i in range(3):
+            a, b = (lo[i] - O_cam[i]) / D[i], (hi[i] - O_cam[i]) / D[i]
+            t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+        ts = t0 + (np.arange(n) + 0.5) / n * (t1 - t0)
+        tau = 0.0
+        for tt in ts:
+            I = orc.intensity(O_cam + tt * D)
+            tau += 9.0 * O.load().orc_plf_eval(C.byref(t.opacity), I)
+        tau *= (t1 - t0) / n
+        want = 1.0 - math.exp(-tau)
+        assert 0.05 < want < 0.98, want
+        assert abs(e[k, 3] - want) < 0.012, (px[k], py[k], e[k, 3], want)
