@@ -85,3 +85,43 @@ def test_batch_with_objects_and_bitmaps_matches_sequential(fresh):
     for r in rs + seq:
         r.close()
     lib.er_set_list_capacity(32)
+
+
+def test_full_batch_first_pass_matches_the_oracle(fresh):
+    """The 64-job batching workload at full size (volumes of 32-256 cells, films of 128-1024
+    pixels, 1-3 lights, phase or hybrid shading): one batch round, then a seeded subset of
+    pixels of eight jobs against the CPU oracle's trace from the same seeds - the frame within
+    1e-4 relative, the alpha (scatter / no scatter) and the RNG state after the pass exactly."""
+    lib = capi.load()
+    lib.er_set_list_capacity(4096)
+    jobs = scenes.config_batch(n_jobs=64)
+    rs = [er.Renderer(sc) for sc in jobs]
+    seeds = [er.seeds_for(r.W, r.H, 40 + k) for k, r in enumerate(rs)]
+    for r, (s1, s2) in zip(rs, seeds):
+        r.set_seeds(s1, s2)
+    print(f"[batch] 64 jobs, films {sorted(r.W for r in rs)[::8]}..., one round", flush=True)
+    er.render_batch(rs, [1] * len(rs))
+    rng = np.random.default_rng(17)
+    exact = total = 0
+    for k in sorted(int(v) for v in rng.choice(len(rs), size=8, replace=False)):
+        r, sc = rs[k], jobs[k]
+        orc = O.Oracle(sc, nthreads=8)
+        if sc.tracer.settings.shading_type not in (capi.SHADING_BRDF_ONLY, capi.SHADING_PHASE_ONLY):
+            assert r.gm_max() == orc.compute_gm_max()
+        idx = rng.choice(r.W * r.H, size=256, replace=False)
+        px, py = (idx % r.W).astype(np.int32), (idx // r.W).astype(np.int32)
+        a1, a2 = seeds[k][0][idx].copy(), seeds[k][1][idx].copy()
+        want, _, _ = orc.trace_pixels(px, py, a1, a2)
+        got = r.frame()[py, px]
+        g1, g2 = r.get_seeds()
+        print(f"[batch] job {k}: film {r.W}x{r.H} vol {sc.voxels.shape} shading {sc.tracer.settings.shading_type} "
+              f"events {int(want[:, 3].sum())}/256", flush=True)
+        assert np.array_equal(g1[idx], a1) and np.array_equal(g2[idx], a2), f"job {k}: RNG state"
+        assert np.array_equal(got[:, 3], want[:, 3]), f"job {k}: alpha"
+        np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-6, err_msg=f"job {k}")
+        exact += int(np.sum(got == want))
+        total += got.size
+    print(f"[batch] exact-match fraction {exact / total:.6f}", flush=True)
+    for r in rs:
+        r.close()
+    lib.er_set_list_capacity(32)
