@@ -11,14 +11,22 @@ running average + tonemap + bilateral). Inputs are resident on the device before
 which it leaves where it was; `--no-tune` keeps the defaults); `autotune` reports the choice.
 
 Also reported, each on its own workload:
-  roofline      the primary-march kernel (the dominant one): algorithmic bytes = 16 B per
-                trilinear sample (8 uint16 corners) of the primary march + 32 B per path
-                (seeds in/out, frame estimate), unit counts from a counting replay of the same
-                passes with the same seeds, over its HIP-event-timed average duration;
-  rmse          tonemapped running estimate vs the FP64 oracle's expectation (fixture);
-  cpu_baseline  the CPU oracle (the reference estimator restated, FP32, OpenMP) on the host;
-  small         the 64^3 blob workload (256 x 256, 16 passes): Mpaths/s, RMSE, GPU / CPU;
-  large         the 512 x 512 x 460 clipped volume at 1920 x 1080, 256 passes: Mpaths/s;
+  roofline      the primary-march kernel (the dominant one) against the HBM roofline: the
+                bytes it must read, 16 B per trilinear sample it fetches (8 uint16 corners) +
+                32 B per path (seeds in/out, frame estimate), unit counts from a counting replay
+                of the same passes with the same seeds, over its HIP-event-timed average
+                duration. The bytes of every sample the reference would fetch (exact empty-space
+                skipping removes almost all of them) are reported apart as
+                reference_equivalent, never as the fraction;
+  rmse          tonemapped running estimate vs the FP64 oracle's converged image (fixtures
+                under tests/golden: every pixel for the small workload, 4096 for the others);
+  cpu_baseline  the CPU oracle (the reference estimator restated, FP32, OpenMP over rows) on
+                every core this process may use, after a warm-up pass; host and compiler
+                recorded;
+  small         the 64^3 blob workload (256 x 256, 16 passes, run on to the RMSE gate):
+                Mpaths/s, RMSE, GPU / CPU;
+  woodcock      the CT workload with Woodcock delta tracking (per-macrocell majorants);
+  large         the 512 x 512 x 460 clipped volume at 1920 x 1080, 256 passes: Mpaths/s, RMSE;
   adjoint       the differentiation workload (128^3, 512 x 512, 32 passes): forward and
                 path-replay adjoint wall times;
   batched       64 jobs of uneven size: one er_render_batch vs the same jobs one after another.
@@ -97,8 +105,33 @@ def _copy_rate_child(nbytes, reps):
     print(2.0 * nbytes / secs / 1e9, flush=True)
 
 
-def cpu_rate(scene_builder, passes, threads):
-    """The CPU oracle over `passes` full passes of the scene: Mpaths/s and seconds."""
+def cpu_threads():
+    """Every core this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_host():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return {"nproc": os.cpu_count(), "threads": cpu_threads(), "model": model, "compiler": O.build_info()}
+
+
+def cpu_rate(scene_builder, threads, target_s=15.0, max_passes=12):
+    """The CPU oracle on full passes of the scene after one untimed warm-up pass: as many
+    passes as fit about target_s (at least one, at most max_passes). Returns Mpaths/s, the
+    timed seconds and the pass count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     sc = scene_builder()
@@ -108,10 +141,14 @@ def cpu_rate(scene_builder, passes, threads):
     W, H = sc.film
     s1, s2 = O.seeds_for(W, H, 99)
     t0 = time.perf_counter()
+    orc.trace(s1, s2)                        # warm-up (caches, OpenMP pool, page faults)
+    one = time.perf_counter() - t0
+    passes = int(max(1, min(max_passes, target_s // max(one, 1e-6))))
+    t0 = time.perf_counter()
     for _ in range(passes):
         orc.trace(s1, s2)
     dt = time.perf_counter() - t0
-    return W * H * passes / dt / 1e6, dt
+    return W * H * passes / dt / 1e6, dt, passes
 
 
 def timed_passes(er, r, steps):
@@ -156,9 +193,10 @@ def bench_small(er, scenes, threads, no_cpu, tune=True):
     out = {"workload": "blobs_64c_256x256", "passes": passes, "mpaths_per_s": round(value, 2),
            "ms_per_frame": round(wall * 1e3 / passes, 4), "rmse": rm, "autotune": tn}
     if not no_cpu:
-        v, dt = cpu_rate(scenes.config_small, 4, threads)
+        v, dt, n = cpu_rate(scenes.config_small, threads)
         out["cpu_mpaths_per_s"] = round(v, 4)
         out["cpu_threads"] = threads
+        out["cpu_sample"] = f"{n} full passes of 256x256 paths after a warm-up pass, {dt:.1f} s"
         out["gpu_over_cpu"] = round(value / v, 1)
     r.close()
     return out
@@ -166,8 +204,8 @@ def bench_small(er, scenes, threads, no_cpu, tune=True):
 
 def bench_large(er, scenes, tune=True):
     """The large anisotropic workload (512 x 512 x 460, 0.7 x 0.7 x 1.0 spacing, clipping box,
-    1920 x 1080, 3 plane lights, hybrid shading): throughput over its 256 passes. No RMSE
-    gate here: an FP64 reference at this size is beyond a bounded CPU run."""
+    1920 x 1080, 3 plane lights, hybrid shading): throughput over its 256 passes and the RMSE
+    of the result against the FP64 fixture (4096 pixels at 16384 spp)."""
     sc = scenes.config_large()
     r = er.Renderer(sc)
     r.seed(99)
@@ -177,6 +215,29 @@ def bench_large(er, scenes, tune=True):
     dt, t = timed_passes(er, r, sc.passes)
     out = {"workload": "tubes_512x512x460_1920x1080_clip", "passes": sc.passes,
            "mpaths_per_s": round(r.W * r.H * sc.passes / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / sc.passes, 4),
+           "rmse": rmse_vs_golden(r.running(), os.path.join(ROOT, "tests", "golden", "fp64_large.npz")),
+           "autotune": tn}
+    r.close()
+    return out
+
+
+def bench_woodcock(er, scenes, passes, tune=True):
+    """The CT workload with Woodcock delta tracking (full estimator, per-macrocell majorants)
+    instead of the ray march: throughput over `passes` passes and the RMSE against the FP64
+    fixture of the same sampler (the two samplers converge to different images: the ray
+    march is a jittered fixed-step quadrature of the free-flight integral)."""
+    from exposure_render import capi
+    sc = scenes.config_ct()
+    sc.tracer.settings.sampler = capi.SAMPLER_WOODCOCK
+    r = er.Renderer(sc)
+    r.seed(77)
+    tn = tune_summary(r) if tune else None
+    r.render_passes(2)
+    r.rebind()
+    dt, t = timed_passes(er, r, passes)
+    out = {"workload": "ct_phantom_256c_1024x768_woodcock", "passes": passes,
+           "mpaths_per_s": round(r.W * r.H * passes / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / passes, 4),
+           "rmse": rmse_vs_golden(r.running(), os.path.join(ROOT, "tests", "golden", "fp64_ct_woodcock.npz")),
            "autotune": tn}
     r.close()
     return out
@@ -184,12 +245,10 @@ def bench_large(er, scenes, tune=True):
 
 def bench_adjoint(er, scenes):
     """Differentiation workload: forward passes vs the path-replay adjoint of the same passes."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     sc = scenes.config_adjoint()
     r = er.Renderer(sc)
     lib = er.load()
-    s1, s2 = O.seeds_for(r.W, r.H, 1)
+    s1, s2 = er.seeds_for(r.W, r.H, 1)
     out = {"workload": "blobs_128c_512x512_32spp_lut256x4", "passes": sc.passes}
     for name, lut in (("initial_lut", scenes.initial_tf_lut()), ("target_lut", scenes.random_tf_lut(4))):
         r.set_tf_lut(lut, sc.tf_range)
@@ -217,14 +276,12 @@ def bench_batched(er, scenes, n_jobs):
     size, tile shape and workgroups per CU er_autotune_batch picks; the sequential runs use the
     defaults. The results are bit-identical (same seeds, same paths, and no launch configuration
     changes the image), so the RMSE is equal."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     lib = er.load()
     lib.er_set_list_capacity(4096)
     jobs = scenes.config_batch(n_jobs=n_jobs)
     rs = [er.Renderer(sc) for sc in jobs]
     passes = [sc.passes for sc in jobs]
-    seeds = [O.seeds_for(r.W, r.H, 40 + k) for k, r in enumerate(rs)]
+    seeds = [er.seeds_for(r.W, r.H, 40 + k) for k, r in enumerate(rs)]
     for r in rs:
         r.render_passes(1)
 
@@ -286,8 +343,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--no-tune", action="store_true",
                     help="render with the default launch configuration instead of the autotuned one")
-    ap.add_argument("--cpu-passes", type=int, default=12)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (default: every core)")
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched section (its own JSON line; for kernel profiles of the batch)")
@@ -295,6 +351,10 @@ def main():
                     help="only the large section (its own JSON line; for kernel profiles of that workload)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint section (its own JSON line; for kernel profiles of the adjoint)")
+    ap.add_argument("--woodcock-only", action="store_true",
+                    help="only the Woodcock section (its own JSON line; for kernel profiles of that sampler)")
+    ap.add_argument("--small-only", action="store_true",
+                    help="only the small section, without its CPU leg (its own JSON line; for kernel profiles)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy a device buffer of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -316,6 +376,16 @@ def main():
         import exposure_render as er
         from exposure_render import scenes
         print(json.dumps({"adjoint": bench_adjoint(er, scenes)}))
+        return
+    if args.woodcock_only:
+        import exposure_render as er
+        from exposure_render import scenes
+        print(json.dumps({"woodcock": bench_woodcock(er, scenes, 64, not args.no_tune)}))
+        return
+    if args.small_only:
+        import exposure_render as er
+        from exposure_render import scenes
+        print(json.dumps({"small": bench_small(er, scenes, 1, True, not args.no_tune)}))
         return
     if args.copy_probe:
         import torch
@@ -344,7 +414,16 @@ def main():
     value = paths / wall / 1e6
     ms_per_step = wall * 1e3 / args.steps
     running = r.running()
-    rm = rmse_vs_golden(running, os.path.join(ROOT, "tests", "golden", "fp64_ct.npz"))
+    golden_ct = os.path.join(ROOT, "tests", "golden", "fp64_ct.npz")
+    rm = rmse_vs_golden(running, golden_ct)
+    if rm is not None:
+        # the passes this workload needs to reach the gate (rendered after the timed region)
+        total, g = args.steps, rm
+        while g["display"] > 0.01 and total < 2048:
+            r.render_passes(total)
+            total *= 2
+            g = rmse_vs_golden(r.running(), golden_ct)
+        rm["gate"] = {"passes": total, "display": g["display"], "met": g["display"] <= 0.01}
 
     roofline = None
     counts = None
@@ -356,42 +435,46 @@ def main():
         r.render_passes(args.steps)
         r.set_counting(False)
         counts = r.counters()
-        per_launch = (16.0 * counts["primary_samples"] + 32.0 * counts["paths"]) / args.steps
         avg_ms = timing.primary_ms / args.steps
+        # the bytes k_primary must read: 16 B per trilinear sample it fetches + 32 B per path
+        per_launch = (16.0 * counts["primary_fetches"] + 32.0 * counts["paths"]) / args.steps
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profile(os.path.join(ROOT, "profiles", "k_primary_ct_traffic.json")),
                     "kernel": "k_primary<1,0,1>", "avg_launch_ms": round(avg_ms, 4),
-                    "algorithmic_bytes_per_launch": int(per_launch)}
-        # what the march actually reads after exact empty-space skipping, and what HBM moved
-        fetched = (16.0 * counts["primary_fetches"] + 32.0 * counts["paths"]) / args.steps
-        roofline["fetched_bytes_per_launch"] = int(fetched)
-        roofline["fetched_GBs"] = round(fetched / (avg_ms * 1e-3) / 1e9, 1)
+                    "bytes_per_launch": int(per_launch),
+                    "units_per_launch": {"fetched_samples": round(counts["primary_fetches"] / args.steps),
+                                         "paths": W * H}}
         if roofline["traffic"]:
             roofline["hbm_GBs"] = round(roofline["traffic"] / (avg_ms * 1e-3) / 1e9, 1)
-        samples = counts["primary_samples"] + counts["shadow_samples"] + counts["shading_samples"]
-        whole = (16.0 * samples + 64.0 * counts["paths"]) / args.steps
         copy = copy_rate_gbs()
         roofline["measured_copy_GBs"] = round(copy, 1)
         roofline["frac_of_measured_copy"] = round(achieved / copy, 4)
-        roofline["whole_pass"] = {"algorithmic_bytes_per_pass": int(whole),
-                                  "achieved_GBs": round(whole / (ms_per_step * 1e-3) / 1e9, 1)}
+        # every sample of the reference's march, skipped or not (not a memory rate of this kernel)
+        ref_eq = (16.0 * counts["primary_samples"] + 32.0 * counts["paths"]) / args.steps
+        samples = counts["primary_samples"] + counts["shadow_samples"] + counts["shading_samples"]
+        whole = (16.0 * samples + 64.0 * counts["paths"]) / args.steps
+        roofline["reference_equivalent"] = {
+            "bytes_per_launch": int(ref_eq), "GBs": round(ref_eq / (avg_ms * 1e-3) / 1e9, 1),
+            "skipped_fraction": round(1.0 - counts["primary_fetches"] / max(counts["primary_samples"], 1), 5),
+            "whole_pass_bytes": int(whole), "whole_pass_GBs": round(whole / (ms_per_step * 1e-3) / 1e9, 1)}
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or cpu_threads()
     cpu = None
     if not args.no_cpu:
-        v, dt = cpu_rate(scenes.config_ct, args.cpu_passes, threads)
+        v, dt, n = cpu_rate(scenes.config_ct, threads)
         cpu = {"value": round(v, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_passes} full passes of the same scene ({W}x{H} paths each) by the FP32 "
-                         f"oracle restatement, {dt:.1f} s"}
+               "sample": f"{n} full passes of the same scene ({W}x{H} paths each) by the FP32 oracle restatement "
+                         f"after one warm-up pass, {dt:.1f} s",
+               "host": cpu_host()}
     r.close()
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "ct_phantom_256c_1024x768_64spp", "film": [W, H],
+        "config": {"workload": f"ct_phantom_256c_1024x768_{args.steps}spp", "film": [W, H],
                    "volume": list(sc.voxels.shape[::-1]), "spp": args.steps, "estimator": "full",
                    "shading": "hybrid", "lights": "2 plane + environment sphere", "bilateral": True},
         "ms_per_frame": {"total": round(timing.total_ms / args.steps, 4),
@@ -404,6 +487,7 @@ def main():
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     if not args.no_extra:
         out["small"] = bench_small(er, scenes, threads, args.no_cpu, not args.no_tune)
+        out["woodcock"] = bench_woodcock(er, scenes, 64, not args.no_tune)
         out["large"] = bench_large(er, scenes, not args.no_tune)
         out["adjoint"] = bench_adjoint(er, scenes)
         out["batched"] = bench_batched(er, scenes, args.batch_jobs)

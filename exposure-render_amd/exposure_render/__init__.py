@@ -303,3 +303,21 @@ def render_batch(renderers, passes):
 def shutdown():
     lib = load()
     _check(lib, lib.er_shutdown())
+
+
+def seeds_for(width, height, seed):
+    """The per-pixel RNG seeds er_seed_tracer writes, on the host: SplitMix64(seed ^ pixel)
+    split into its low and high 32-bit halves, a zero half replaced by 1 (zero is a fixed point
+    of the multiply-with-carry generators, Source/rng.h:35-51)."""
+    n = width * height
+    idx = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) ^ idx) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    s1 = (z & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    s2 = (z >> np.uint64(32)).astype(np.uint32)
+    s1[s1 == 0] = 1
+    s2[s2 == 0] = 1
+    return s1, s2

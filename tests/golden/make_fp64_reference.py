@@ -1,11 +1,20 @@
 """Writes the FP64 converged-image fixtures used by the RMSE gate (bench.py, tests).
 
-For a benchmark scene it picks seeded 3x3 pixel blocks, runs the FP64 oracle (the same
-estimator in double precision) at many samples per pixel on all nine pixels of each block,
-and stores the 3x3-Gaussian-filtered expectation at the block centres - the expectation of
-the running estimate (the per-pass filter of Source/filterframeestimate.cuh:28-64 is linear).
+The converged image is the expectation of the running estimate: the mean of the per-pass
+frames after the 3x3 Gaussian (Source/filterframeestimate.cuh:28-64), which is linear, so it
+is the 3x3-Gaussian-filtered per-pixel expectation. The FP64 oracle (the same estimator in
+double precision) estimates each needed pixel's expectation at many samples per pixel, and the
+filter is applied to those.
 
-usage: python tests/golden/make_fp64_reference.py {small|ct} [blocks] [spp] [threads]
+  small        every pixel of the 256 x 256 film (the filter renormalised at the borders);
+  ct, large,   4096 gated pixels: 16 seeded 16 x 16 tiles, one in each cell of a 4 x 4 grid
+  ct_woodcock  over the film and at least one pixel from its edge, each needing the
+               expectations of its 18 x 18 neighbourhood.
+
+Stored: cx, cy (gated pixels), expect (N x 4: filtered XYZ and the filtered alpha), spp,
+exposure, film, seconds.
+
+usage: python tests/golden/make_fp64_reference.py {small|ct|large|ct_woodcock} [spp] [threads]
 """
 import os
 import sys
@@ -18,41 +27,75 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "exposure-render_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-from exposure_render import scenes  # noqa: E402
+from exposure_render import capi, scenes  # noqa: E402
 import oracle as O  # noqa: E402
 
-G = np.exp(-np.array([[2, 1, 2], [1, 0, 1], [2, 1, 2]], dtype=np.float64) / 2.0)
+TILE = 16
+GRID = 4
 
 
-def block_centres(W, H, n, seed):
+def scene(name):
+    if name == "small":
+        return scenes.config_small()
+    if name == "ct":
+        return scenes.config_ct()
+    if name == "large":
+        return scenes.config_large()
+    if name == "ct_woodcock":
+        sc = scenes.config_ct()
+        sc.tracer.settings.sampler = capi.SAMPLER_WOODCOCK
+        return sc
+    raise ValueError(name)
+
+
+def tiles(W, H, seed=2024):
+    """Top-left corners of GRID x GRID seeded tiles, one per grid cell, 1 px inside the film."""
     rng = np.random.default_rng(seed)
-    xs = rng.integers(1, W - 1, size=n)
-    ys = rng.integers(1, H - 1, size=n)
-    return xs.astype(np.int32), ys.astype(np.int32)
+    out = []
+    for gy in range(GRID):
+        for gx in range(GRID):
+            x0, x1 = max(1, gx * W // GRID), min(W - 1 - TILE, (gx + 1) * W // GRID - TILE)
+            y0, y1 = max(1, gy * H // GRID), min(H - 1 - TILE, (gy + 1) * H // GRID - TILE)
+            out.append((int(rng.integers(x0, x1 + 1)), int(rng.integers(y0, y1 + 1))))
+    return out
 
 
-def make(name, blocks, spp, threads):
-    builder = {"small": scenes.config_small, "ct": scenes.config_ct}[name]
-    sc = builder()
+def make(name, spp, threads):
+    sc = scene(name)
     orc = O.Oracle(sc, nthreads=threads)
-    orc.compute_gm_max()
+    if sc.tracer.settings.shading_type not in (capi.SHADING_BRDF_ONLY, capi.SHADING_PHASE_ONLY):
+        orc.compute_gm_max()
     W, H = sc.film
-    cx, cy = block_centres(W, H, blocks, 2024)
-    px = (cx[:, None] + np.array([-1, 0, 1] * 3)[None, :]).reshape(-1)
-    py = (cy[:, None] + np.repeat([-1, 0, 1], 3)[None, :]).reshape(-1)
     t0 = time.time()
-    e = orc.expect_f64(px, py, spp, seed=777).reshape(blocks, 3, 3, 4)
-    filt = (e * G[None, :, :, None]).sum(axis=(1, 2)) / G.sum()
-    filt[:, 3] = e[:, 1, 1, 3] / G.sum()    # the filter divides the centre alpha by the total weight
+    if name == "small":
+        ys, xs = np.mgrid[0:H, 0:W]
+        e = orc.expect_f64(xs.ravel(), ys.ravel(), spp, seed=777).reshape(H, W, 4)
+        filt = O.gauss3x3_f64(e)                          # borders renormalise; alpha / total weight
+        cy, cx = ys.ravel().astype(np.int32), xs.ravel().astype(np.int32)
+        expect = filt.reshape(-1, 4).astype(np.float32)
+    else:
+        cxs, cys, exps = [], [], []
+        for k, (x0, y0) in enumerate(tiles(W, H)):
+            ys, xs = np.mgrid[y0 - 1:y0 + TILE + 1, x0 - 1:x0 + TILE + 1]
+            e = orc.expect_f64(xs.ravel(), ys.ravel(), spp, seed=777 + k).reshape(TILE + 2, TILE + 2, 4)
+            filt = O.gauss3x3_f64(e)[1:-1, 1:-1]          # interior pixels: all nine taps
+            gy, gx = np.mgrid[y0:y0 + TILE, x0:x0 + TILE]
+            cxs.append(gx.ravel())
+            cys.append(gy.ravel())
+            exps.append(filt.reshape(-1, 4))
+            print(f"{name}: tile {k + 1}/{GRID * GRID} at ({x0},{y0}) {time.time() - t0:.0f} s", flush=True)
+        cx = np.concatenate(cxs).astype(np.int32)
+        cy = np.concatenate(cys).astype(np.int32)
+        expect = np.concatenate(exps)
+    secs = time.time() - t0
     out = os.path.join(HERE, f"fp64_{name}.npz")
-    np.savez_compressed(out, cx=cx, cy=cy, expect=filt, spp=spp, exposure=sc.tracer.camera.exposure,
-                        film=np.array([W, H]))
-    print(f"{name}: {blocks} blocks x 9 px x {spp} spp in {time.time() - t0:.1f} s -> {out}")
+    np.savez_compressed(out, cx=cx, cy=cy, expect=expect, spp=spp, exposure=sc.tracer.camera.exposure,
+                        film=np.array([W, H]), seconds=secs)
+    print(f"{name}: {len(cx)} gated pixels at {spp} spp in {secs:.1f} s -> {out}", flush=True)
 
 
 if __name__ == "__main__":
     name = sys.argv[1] if len(sys.argv) > 1 else "small"
-    blocks = int(sys.argv[2]) if len(sys.argv) > 2 else 256
-    spp = int(sys.argv[3]) if len(sys.argv) > 3 else 4096
-    threads = int(sys.argv[4]) if len(sys.argv) > 4 else 8
-    make(name, blocks, spp, threads)
+    spp = int(sys.argv[2]) if len(sys.argv) > 2 else (4096 if name == "small" else 16384)
+    threads = int(sys.argv[3]) if len(sys.argv) > 3 else 8
+    make(name, spp, threads)
