@@ -552,3 +552,19 @@ extern "C" hipError_t erd_launch_adj_reset(unsigned int* counts, unsigned int* l
     hipLaunchKernelGGL(erd::k_adj_reset, dim3(1), dim3(64), 0, st, counts, lcount);
     return hipGetLastError();
 }
+
+namespace erd {
+/* p[0 .. words) = 0 */
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ p, size_t words)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+} /* namespace erd */
+
+extern "C" hipError_t erd_launch_zero_words(uint32_t* p, size_t words, hipStream_t st)
+{
+    if (words == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((words + 255) / 256, 1024);
+    hipLaunchKernelGGL(erd::k_zero_words, dim3((unsigned)blocks), dim3(256), 0, st, p, words);
+    return hipGetLastError();
+}

@@ -83,6 +83,48 @@ extern "C" hipError_t erd_launch_adj_filter_t(const float* dldr, float4* out, in
 #define ER_ADJ_BLOCKS 1024
 #endif
 extern "C" hipError_t erd_launch_adj_reduce(const float* partials, int nblocks, double* out, hipStream_t st);
+extern "C" hipError_t erd_launch_zero_words(uint32_t* p, size_t words, hipStream_t st);
+
+/* ---- device scratch of er_adjoint_lut_noalloc ----------------------------------------------
+ * The no-allocation adjoint keeps every buffer beyond its inputs and outputs in one file-scope
+ * device array: one pass's queues, the seeds, the adjoint image, the workgroup histograms and
+ * the FP64 gradient sum. Its layout for a film of n pixels is below; the array is sized for
+ * ER_NOALLOC_MAX_PIXELS (512 x 512), about 95 MiB, and each GPU the library runs on holds
+ * its own copy until the process ends. */
+#define ER_NOALLOC_MAX_PIXELS ((size_t)512 * 512)
+
+struct NoAllocLayout {
+    size_t d1, d2, s1s, s2s, dldr, adj, tvis, frame, queue, events, jobs, list, counts, lcount, partials, out, total;
+};
+
+static __host__ __device__ constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static __host__ __device__ constexpr NoAllocLayout noalloc_layout(size_t n)
+{
+    NoAllocLayout L{};
+    size_t o = 0;
+    L.d1 = o;       o += align256(n * sizeof(uint32_t));
+    L.d2 = o;       o += align256(n * sizeof(uint32_t));
+    L.s1s = o;      o += align256(n * sizeof(uint32_t));
+    L.s2s = o;      o += align256(n * sizeof(uint32_t));
+    L.dldr = o;     o += align256(n * 3 * sizeof(float));
+    L.adj = o;      o += align256(n * sizeof(float4));
+    L.tvis = o;     o += align256(2 * n * sizeof(float));
+    L.frame = o;    o += align256(n * sizeof(float4));
+    L.queue = o;    o += align256(n * sizeof(erd::ScatterRec));
+    L.events = o;   o += align256(n * sizeof(erd::EventRec));
+    L.jobs = o;     o += align256(2 * n * sizeof(erd::ShadowJob));
+    L.list = o;     o += align256(n * sizeof(uint2));
+    L.counts = o;   o += align256(4 * sizeof(uint32_t));
+    L.lcount = o;   o += align256(sizeof(uint32_t));
+    L.partials = o; o += align256((size_t)3 * ER_ADJ_BLOCKS * ER_TF_LUT_NODES * 4 * sizeof(float));
+    L.out = o;      o += align256((size_t)ER_TF_LUT_NODES * 4 * sizeof(double));
+    L.total = o;
+    return L;
+}
+
+static __device__ __align__(256) unsigned char scratch[noalloc_layout(ER_NOALLOC_MAX_PIXELS).total];
+
 namespace {
 
 std::recursive_mutex g_mutex;
@@ -117,12 +159,24 @@ namespace {
             return set_error(ER_ERROR, "%s (%s)", hipGetErrorString(e_), #expr);             \
     } while (0)
 
+/* Set while an entry point that must not allocate runs (er_adjoint_lut_noalloc): any device
+ * buffer that would need a hipMalloc or hipFree then fails the call instead. */
+bool g_no_alloc = false;
+struct NoAllocScope {
+    NoAllocScope() { g_no_alloc = true; }
+    ~NoAllocScope() { g_no_alloc = false; }
+};
+
 template<class T> struct DevBuf {
     T* ptr = nullptr;
     size_t n = 0;
     int alloc(size_t count)
     {
         if (count == n && ptr) return ER_OK;
+        if (g_no_alloc)
+            return set_error(ER_ERROR, "this entry point allocates no device memory, but the tracer's device tables "
+                                       "need (re)allocation: render one pass (or call er_adjoint_lut) after a change of "
+                                       "volume, film or launch configuration first");
         release();
         if (count == 0) return ER_OK;
         HIP_TRY(hipMalloc((void**)&ptr, count * sizeof(T)));
@@ -189,6 +243,9 @@ void build_neglog_host()
 int ensure_neglog_device()
 {
     if (g_neglog_dev) return ER_OK;
+    if (g_no_alloc)
+        return set_error(ER_ERROR, "this entry point allocates no device memory, but the library's -log table is not on "
+                                   "the device yet: call er_init or render one pass first");
     build_neglog_host();
     const size_t bytes = g_neglog_host.size() * sizeof(float);
     HIP_TRY(hipMalloc((void**)&g_neglog_dev, bytes));
@@ -354,6 +411,28 @@ int g_list_capacity = ER_LIST_CAPACITY;   /* list.cuh:27 MaxSize; er_set_list_ca
 int g_primary_blocks = 2048;   /* persistent primary workgroups: 7 per CU (LDS and VGPR bound) */
 int g_cus = 256;
 
+/* Device address of `scratch` per device. In host code the array's name is only a stand-in
+ * for the symbol; the address each device's copy lives at comes from hipGetSymbolAddress, with
+ * that device current, and only that address is passed to kernels and copies. */
+const int ER_MAX_DEVICES = 64;
+unsigned char* g_scratch_dev[ER_MAX_DEVICES] = {};
+
+int scratch_for_current_device(unsigned char*& out)
+{
+    out = nullptr;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= ER_MAX_DEVICES) return set_error(ER_ERROR, "device %d is beyond the scratch table", dev);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_error(ER_ERROR, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[dev] = (unsigned char*)p;
+    }
+    out = g_scratch_dev[dev];
+    return ER_OK;
+}
+
 int ensure_device()
 {
     if (g_device_ready) return ER_OK;
@@ -366,6 +445,10 @@ int ensure_device()
         g_cus = cus;
         g_primary_blocks = cus * 7;
     }
+    /* set-up of the no-allocation entry point's scratch on this device */
+    unsigned char* sp = nullptr;
+    int st = scratch_for_current_device(sp);
+    if (st) return st;
     g_device_ready = true;
     return ER_OK;
 }
@@ -1888,6 +1971,80 @@ ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t*
     return ER_OK;
 }
 
+/* er_adjoint_lut with every work buffer in the device scratch array: no hipMalloc or hipFree.
+ * The same kernels run for one pass at a time on the tracer's first stream (er_adjoint_lut
+ * overlaps consecutive passes on two buffer sets). */
+ER_API int er_adjoint_lut_noalloc(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
+                                  const float* dL_drunning_xyz, double* dL_dlut)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mutex);
+    int st;
+    Tracer* t = get_tracer(tracer_id, st);
+    if (!t) return st;
+    NoAllocScope no_alloc;
+    const size_t n = (size_t)t->W * t->H;
+    if (n > ER_NOALLOC_MAX_PIXELS)
+        return set_error(ER_ERROR, "er_adjoint_lut_noalloc: a film of %zu pixels exceeds the scratch limit of %zu pixels "
+                                   "(512 x 512); use er_adjoint_lut", n, (size_t)ER_NOALLOC_MAX_PIXELS);
+    if ((st = adjoint_setup(*t, seeds1, seeds2, count, passes))) return st;
+    if (!dL_drunning_xyz || !dL_dlut) return set_error(ER_ERROR, "adjoint pointer is NULL");
+    const size_t per_block = erd_adj_partials_per_block();
+    if (per_block != (size_t)ER_TF_LUT_NODES * 4) return set_error(ER_ERROR, "adjoint histogram size mismatch");
+    unsigned char* base = nullptr;
+    if ((st = scratch_for_current_device(base))) return st;
+    const NoAllocLayout L = noalloc_layout(n);
+    uint32_t* d1 = (uint32_t*)(base + L.d1);
+    uint32_t* d2 = (uint32_t*)(base + L.d2);
+    uint32_t* s1s = (uint32_t*)(base + L.s1s);
+    uint32_t* s2s = (uint32_t*)(base + L.s2s);
+    float* dldr = (float*)(base + L.dldr);
+    float4* adj = (float4*)(base + L.adj);
+    float* tvis = (float*)(base + L.tvis);
+    float4* frame = (float4*)(base + L.frame);
+    void* queue = base + L.queue;
+    void* events = base + L.events;
+    void* jobs = base + L.jobs;
+    uint2* list = (uint2*)(base + L.list);
+    unsigned int* counts = (unsigned int*)(base + L.counts);
+    unsigned int* lcount = (unsigned int*)(base + L.lcount);
+    float* partials = (float*)(base + L.partials);
+    double* out = (double*)(base + L.out);
+    const int blocks = ER_ADJ_BLOCKS;
+    hipStream_t s = t->stream;
+    if ((st = sync_tracer(*t))) return st;
+    HIP_TRY(hipMemcpy(d1, seeds1, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d2, seeds2, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dldr, dL_drunning_xyz, n * 12, hipMemcpyHostToDevice));
+    /* the array keeps the previous call's contents: the queue counters and the gradient sum are
+     * cleared by kernels (no memset nodes, should a caller capture the sequence) */
+    HIP_TRY(erd_launch_adj_reset(counts, lcount, s));
+    HIP_TRY(erd_launch_zero_words((uint32_t*)out, per_block * sizeof(double) / 4, s));
+    const int shade_blocks = std::max(1, std::min((int)((n + 255) / 256), 2048));
+    const size_t lds_bits = (size_t)t->host_params.mc_words * 4;
+    float cur_w = -1.0f;
+    for (int k = 1; k <= passes; ++k) {
+        const float w = passes == 1 ? 1.0f : (k >= 2 ? 1.0f / (float)(passes - 1) : 0.0f);
+        if (w != 0.0f) {
+            HIP_TRY(hipMemcpyAsync(s1s, d1, n * 4, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(hipMemcpyAsync(s2s, d2, n * 4, hipMemcpyDeviceToDevice, s));
+        }
+        HIP_TRY(erd_launch_primary_shade(t->params.ptr, frame, d1, d2, queue, counts, events, jobs, t->W, t->H, t->lut_lds,
+                                         t->lds_bytes, lds_bits, shade_blocks, primary_blocks(*t), t->tile_shift, s));
+        if (w != 0.0f) {
+            if (w != cur_w) {
+                HIP_TRY(erd_launch_adj_filter_t(dldr, adj, t->W, t->H, w, s));
+                cur_w = w;
+            }
+            HIP_TRY(erd_launch_adj_wavefront(t->params.ptr, counts, jobs, events, queue, adj, tvis, frame, list, lcount, s1s,
+                                             s2s, partials, blocks, (int)n, t->lds_bytes, s));
+            HIP_TRY(erd_launch_adj_reduce(partials, 3 * blocks, out, s));
+        }
+        HIP_TRY(erd_launch_adj_reset(counts, lcount, s));
+    }
+    HIP_TRY(hipMemcpyAsync(dL_dlut, out, per_block * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ER_OK;
+}
 
 ER_API int er_adjoint_value(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
                             const float* eval_lut, float* running_xyz)

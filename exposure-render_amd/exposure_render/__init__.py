@@ -213,6 +213,18 @@ class Renderer:
                                                  out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def adjoint_lut_noalloc(self, s1, s2, passes, dl_drunning):
+        """adjoint_lut through er_adjoint_lut_noalloc: no device allocation, films up to 512 x 512."""
+        s1 = np.ascontiguousarray(s1, dtype=np.uint32)
+        s2 = np.ascontiguousarray(s2, dtype=np.uint32)
+        g = np.ascontiguousarray(dl_drunning, dtype=np.float32).reshape(self.H, self.W, 3)
+        out = np.zeros((capi.TF_LUT_NODES, 4), dtype=np.float64)
+        _check(self.lib, self.lib.er_adjoint_lut_noalloc(self.id, s1.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                         s2.ctypes.data_as(C.POINTER(C.c_uint32)), s1.size,
+                                                         int(passes), g.ctypes.data_as(C.POINTER(C.c_float)),
+                                                         out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     def adjoint_value(self, s1, s2, passes, eval_lut):
         """The smooth estimator's running XYZ (H, W, 3) with values from eval_lut and the
         decisions of the current node table."""

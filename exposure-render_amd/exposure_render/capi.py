@@ -231,6 +231,8 @@ SIGNATURES = {
     "er_autofocus_seeds": (C.c_int, [C.c_int, C.c_int, C.c_int, _P(C.c_uint32), _P(C.c_uint32)]),
     "er_adjoint_lut": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
                                  _P(C.c_double)]),
+    "er_adjoint_lut_noalloc": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int,
+                                         _P(C.c_float), _P(C.c_double)]),
     "er_adjoint_value": (C.c_int, [C.c_int, _P(C.c_uint32), _P(C.c_uint32), C.c_size_t, C.c_int, _P(C.c_float),
                                    _P(C.c_float)]),
     "er_set_counting": (C.c_int, [C.c_int, C.c_int]),

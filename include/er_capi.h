@@ -437,6 +437,18 @@ ER_API int er_set_tf_lut(int tracer_id, const float* lut, float range);
 ER_API int er_adjoint_lut(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count, int passes,
                           const float* dL_drunning_xyz, double* dL_dlut);
 
+/* No counterpart in the reference. er_adjoint_lut for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight optimisation or serving loops, hosts that forbid
+ * hipMalloc): same arguments, same result up to FP32 summation order, but every work buffer
+ * lives in one static device array that the library holds per GPU until the process ends
+ * (about 95 MiB), so no call allocates or frees. Limits: films of at most 512 x 512 pixels
+ * (larger ones are rejected with ER_ERROR); the tracer's device tables must already exist for
+ * its current volume, film and launch configuration (a change of those needs one render pass or
+ * one er_adjoint_lut call first, else ER_ERROR). Passes run one at a time on the tracer's
+ * stream. Every call on a GPU shares that GPU's array: calls that overlap on one GPU race. */
+ER_API int er_adjoint_lut_noalloc(int tracer_id, const uint32_t* seeds1, const uint32_t* seeds2, size_t count,
+                                  int passes, const float* dL_drunning_xyz, double* dL_dlut);
+
 /* The smooth estimator the adjoint differentiates, evaluated with the node table eval_lut
  * while every decision (scatter sample, light choice, directions) is taken with the current
  * table: the running XYZ estimate (W*H*3 floats) after `passes` passes from the given seeds.
