@@ -57,22 +57,43 @@ struct AdjArgs {
     float* partials;            /* unused by the value kernel */
 };
 
-/* sums sigma over the visited samples; EVAL also sums the evaluation table's sigma */
+/* s += x with Kahan's compensation c: the error of a float sum over thousands of samples stays
+ * at a few ulp of the total instead of growing with the count (needs -ffp-contract=off, no
+ * reassociation) */
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x)
+{
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+/* sums sigma over the visited samples (sd); EVAL also sums the evaluation table's sigma minus
+ * the decision table's (dd), so that the difference of the two optical depths is accumulated
+ * directly instead of as the difference of two large sums */
 template<int EVAL>
 struct VisSum {
     const float* sig_eval;
     int lut_max;
-    double sd, se;
+    float sd, cd, dd, cdd;
     float last_d, last_e;
+    __device__ __forceinline__ void init(const float* sig, int lmax)
+    {
+        sig_eval = sig; lut_max = lmax;
+        sd = cd = dd = cdd = 0.0f;
+        last_d = last_e = 0.0f;
+    }
     __device__ __forceinline__ void begin(float) {}
     __device__ __forceinline__ void sample(int i, float sigma)
     {
         const float e = EVAL ? gmem(sig_eval)[min(i, lut_max)] : sigma;
-        sd += (double)sigma;
-        se += (double)e;
+        kahan_add(sd, cd, sigma);
+        if (EVAL) kahan_add(dd, cdd, e - sigma);
         last_d = sigma;
         last_e = e;
     }
+    /* sum of the evaluation table's sigma */
+    __device__ __forceinline__ float se() const { return EVAL ? sd + dd : sd; }
 };
 
 /* scatters w * dtau/dnode of each visited sample into the LDS node histogram; consecutive
@@ -123,16 +144,15 @@ __device__ __forceinline__ float transmittance(const DevParams& p, const Tables&
     if (p.object_count > 0 && intersects_any_object(p, r)) return 0.0f;
     rng_at = rng;
     VisSum<EVAL> vs;
-    vs.sig_eval = a.sigma_eval; vs.lut_max = p.lut_max;
-    vs.sd = vs.se = 0.0; vs.last_d = vs.last_e = 0.0f;
+    vs.init(a.sigma_eval, p.lut_max);
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     float Tt; f3 Pt;
     float nearT, farT;
     if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return 1.0f;
     marched = true;
     march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisSum<EVAL>>(p, T, r, false, rng, Tt, Pt, cnt, vs, ADJ_TAU_CAP / p.density_scale);
-    const double dt = (double)((p.step_shadow) * p.minstep);
-    return (float)exp(-(double)p.density_scale * dt * vs.se);
+    const float dt = p.step_shadow * p.minstep;
+    return erm::expf_(-((p.density_scale * dt) * vs.se()));
 }
 
 __device__ __forceinline__ f3 lut_diffuse(const float* lut, float scale, float I)
@@ -168,8 +188,7 @@ __global__ __launch_bounds__(256) void k_adj_value(const DevParams* __restrict__
         Sample smp;
         const Ray r = camera_ray(p, px, py, rng, smp);
         VisSum<1> vp;
-        vp.sig_eval = a.sigma_eval; vp.lut_max = p.lut_max;
-        vp.sd = vp.se = 0.0; vp.last_d = vp.last_e = 0.0f;
+        vp.init(a.sigma_eval, p.lut_max);
         Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         float Tv = 0.0f;
         f3 Pv = mk(0.0f, 0.0f, 0.0f);
@@ -246,19 +265,18 @@ __global__ __launch_bounds__(256) void k_adj_value(const DevParams* __restrict__
             value = Lel;
         }
 
-        const double DS = (double)p.density_scale;
-        const double dtp = (double)(p.step_primary * p.minstep);
-        /* P_k(theta) / P_k(theta0) of the sampled free flight */
-        double w = 1.0;
+        /* P_k(theta) / P_k(theta0) of the sampled free flight: exp(-DS dt (optical depth
+         * difference before k)) times the ratio of the two scatter probabilities at k */
+        const float kdt = p.density_scale * (p.step_primary * p.minstep);
+        float wf = 1.0f;
         if (vol) {
-            const double d_before = (vp.se - (double)vp.last_e) - (vp.sd - (double)vp.last_d);
-            const double num = -expm1(-DS * dtp * (double)vp.last_e);
-            const double den = -expm1(-DS * dtp * (double)vp.last_d);
-            w = exp(-DS * dtp * d_before) * (den > 0.0 ? num / den : 1.0);
+            const float d_before = vp.dd - (vp.last_e - vp.last_d);
+            const float num = -erm::expm1f_(-(kdt * vp.last_e));
+            const float den = -erm::expm1f_(-(kdt * vp.last_d));
+            wf = erm::expf_(-(kdt * d_before)) * (den > 0.0f ? num / den : 1.0f);
         } else {
-            w = exp(-DS * dtp * (vp.se - vp.sd));
+            wf = erm::expf_(-(kdt * vp.dd));
         }
-        const float wf = (float)w;
         a.value[i] = make_float4(wf * value.x, wf * value.y, wf * value.z, type >= 0 ? 1.0f : 0.0f);
         seeds1[i] = rng.s0;
         seeds2[i] = rng.s1;
@@ -311,8 +329,9 @@ __global__ __launch_bounds__(256) void k_adj_shadow(const DevParams* __restrict_
     for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) hist[k] = 0.0f;
     const Tables T = adj_stage(p, lds);   /* ends with a barrier */
     const unsigned int n = counts[1];
-    const double DS = (double)p.density_scale;
-    const double dts = (double)(p.step_shadow * p.minstep);
+    const float DS = p.density_scale;
+    const float dts = p.step_shadow * p.minstep;
+    const float k2 = (DS * DS) * dts;          /* d tau / d opacity per sample */
     const float cap = ADJ_TAU_CAP / p.density_scale;
     for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const ShadowJob job = jobq[j];
@@ -325,12 +344,11 @@ __global__ __launch_bounds__(256) void k_adj_shadow(const DevParams* __restrict_
         rng.s0 = job.s0;
         rng.s1 = job.s1;
         VisSum<0> vs;
-        vs.sig_eval = nullptr; vs.lut_max = p.lut_max;
-        vs.sd = vs.se = 0.0; vs.last_d = vs.last_e = 0.0f;
+        vs.init(nullptr, p.lut_max);
         Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         float Tt; f3 Pt;
         march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisSum<0>>(p, T, r, false, rng, Tt, Pt, cnt, vs, cap);
-        const float Tr = (float)exp(-DS * dts * vs.sd);
+        const float Tr = erm::expf_(-((DS * dts) * vs.sd));
         tvis[2 * job.event + job.slot] = Tr;
         const EventRec& ev = events[job.event];
         const f3 c = job.slot ? mk(ev.c2x, ev.c2y, ev.c2z) : mk(ev.c1x, ev.c1y, ev.c1z);
@@ -340,7 +358,7 @@ __global__ __launch_bounds__(256) void k_adj_shadow(const DevParams* __restrict_
             VisGrad vg;
             vg.hist = hist; vg.scale = p.tf_lut_scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f;
             vg.last_i = 0; vg.last_sigma = 0.0f;
-            vg.w = (float)(-(double)g * DS * DS * dts);
+            vg.w = -(g * k2);
             Rng rr;
             rr.s0 = job.s0;
             rr.s1 = job.s1;
@@ -424,8 +442,8 @@ __global__ __launch_bounds__(256) void k_adj_primary(const DevParams* __restrict
     for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) hist[k] = 0.0f;
     const Tables T = adj_stage(p, lds);
     const unsigned int n = *count;
-    const double DS = (double)p.density_scale;
-    const double dtp = (double)(p.step_primary * p.minstep);
+    const float kdt = p.density_scale * (p.step_primary * p.minstep);   /* d tau / d sigma per sample */
+    const float k2 = p.density_scale * kdt;                             /* d tau / d opacity per sample */
     for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint2 e = list[j];
         const int i = (int)e.x;
@@ -438,14 +456,14 @@ __global__ __launch_bounds__(256) void k_adj_primary(const DevParams* __restrict
         VisGrad vg;
         vg.hist = hist; vg.scale = p.tf_lut_scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f;
         vg.last_i = 0; vg.last_sigma = 0.0f;
-        vg.w = (float)(-(double)g * DS * DS * dtp);
+        vg.w = -(g * k2);
         Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         float Tq; f3 Pq;
         const bool vol = march_v<1, 0, ER_BATCH_PRIMARY_ADJ, VisGrad>(p, T, r, true, rng, Tq, Pq, cnt, vg, -1.0f);
         vg.flush();
         if (vol && vg.last_sigma > 0.0f) {
-            const double tau = DS * dtp * (double)vg.last_sigma;
-            vg.w = (float)(((double)g / expm1(tau) + (double)g) * DS * DS * dtp);
+            const float tau = kdt * vg.last_sigma;
+            vg.w = (g / erm::expm1f_(tau) + g) * k2;
             vg.sample(vg.last_i, vg.last_sigma);
             vg.flush();
         }

@@ -201,6 +201,23 @@ struct Rng {
     __device__ __forceinline__ float u() { return (float)next_bits() * 1.1920928955078125e-07f; }
 };
 
+/* The stream of a Woodcock shadow march (full estimator), whose draw count is not known before
+ * it runs: SplitMix64's finaliser of the path's state (s0 high, s1 low, xor a constant) split
+ * in two non-zero halves. The path's own stream advances by the two draws a ray-march shadow
+ * test takes, so shading can hand the march to a later kernel. */
+__device__ __forceinline__ Rng shadow_stream(const Rng& r)
+{
+    uint64_t z = (((uint64_t)r.s0 << 32) | (uint64_t)r.s1) ^ 0x5368616457636b21ull;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    Rng s;
+    s.s0 = (uint32_t)z ? (uint32_t)z : 1u;
+    s.s1 = (uint32_t)(z >> 32) ? (uint32_t)(z >> 32) : 1u;
+    return s;
+}
+
 /* ---- transfer functions (Source/plf.h:68-91) ----------------------------------------- */
 __device__ __forceinline__ float plf_eval(const er_plf& p, float x)
 {

@@ -70,6 +70,26 @@ ER_HD float expf_(float x)
     return (p * exp2i(k1)) * exp2i(k2);
 }
 
+/* exp(x) - 1 without the cancellation of expf_(x) - 1 near 0: for |x| < 1/4 the Taylor
+ * series x + x^2 (1/2! + x/3! + ... + x^7/9!) (the first omitted term is below 3e-13 |x|);
+ * beyond, expf_(x) - 1, whose cancellation costs at most a few ulp there. */
+ER_HD float expm1f_(float x)
+{
+    if (x != x) return x;
+    if (x > -0.25f && x < 0.25f) {
+        float p = 2.75573188446287530e-06f;      /* 1/9! */
+        p = p * x + 2.4801587642286904e-05f;     /* 1/8! */
+        p = p * x + 1.98412701138295233e-04f;    /* 1/7! */
+        p = p * x + 1.38888892251998186e-03f;    /* 1/6! */
+        p = p * x + 8.33333376795053482e-03f;    /* 1/5! */
+        p = p * x + 4.16666679084300995e-02f;    /* 1/4! */
+        p = p * x + 1.66666671633720398e-01f;    /* 1/3! */
+        p = p * x + 0.5f;
+        return x + (x * x) * p;
+    }
+    return expf_(x) - 1.0f;
+}
+
 ER_HD float logf_(float x)
 {
     if (x != x) return x;

@@ -355,8 +355,10 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
  * loses mu (t_exit - t) and the walk moves to the next macrocell. A collision is real with
  * probability sigma(P) / mu (one more draw), else tracking continues from it. The segment ends
  * at the nearest macrocell face in lattice coordinates (a macrocell at the grid's edge extends
- * to the end of the ray, as the clamped lookups do) and advances t by at least a millionth of
- * it, so every segment makes progress. The oracle runs the same float operations. */
+ * to the end of the ray, as the clamped lookups do; the distance to a face is its lattice
+ * distance times the reciprocal of the ray's lattice rate on that axis) and advances t by at
+ * least a millionth of it, so every segment makes progress. The oracle runs the same float
+ * operations. */
 #define ER_WC_SHIFT 3
 #define ER_WC_MAX_SEGMENTS 65536
 template<int FULL, int COUNT>
@@ -365,10 +367,12 @@ __device__ __forceinline__ bool woodcock_track(const DevParams& p, const Tables&
 {
     float t = fmaxf(nearT, r.MinT);
     const float MaxT = fminf(farT, r.MaxT);
-    /* lattice units per unit of t along each axis */
+    /* lattice units per unit of t along each axis, and their reciprocals */
     const float gx = (r.D.x * p.invsize[0]) * p.resf[0];
     const float gy = (r.D.y * p.invsize[1]) * p.resf[1];
     const float gz = (r.D.z * p.invsize[2]) * p.resf[2];
+    const float igx = gx != 0.0f ? 1.0f / gx : 0.0f, igy = gy != 0.0f ? 1.0f / gy : 0.0f;
+    const float igz = gz != 0.0f ? 1.0f / gz : 0.0f;
     const int dx = p.wc_dim[0], dy = p.wc_dim[1], dz = p.wc_dim[2];
     const float big = 3.402823466e+38f;
     int segments = 0;
@@ -380,12 +384,12 @@ __device__ __forceinline__ bool woodcock_track(const DevParams& p, const Tables&
             const Lattice l = to_lattice(p, add(r.O, scl(t, r.D)));
             const int mx = mc_axis(l.Lx, ER_WC_SHIFT, dx), my = mc_axis(l.Ly, ER_WC_SHIFT, dy), mz = mc_axis(l.Lz, ER_WC_SHIFT, dz);
             mu = p.density_scale * gmem(p.wc_sigma)[(mz * dy + my) * dx + mx];
-            const float ex = gx > 0.0f ? (mx == dx - 1 ? big : ((float)((mx + 1) << ER_WC_SHIFT) - l.Lx) / gx)
-                                       : (gx < 0.0f ? (mx == 0 ? big : ((float)(mx << ER_WC_SHIFT) - l.Lx) / gx) : big);
-            const float ey = gy > 0.0f ? (my == dy - 1 ? big : ((float)((my + 1) << ER_WC_SHIFT) - l.Ly) / gy)
-                                       : (gy < 0.0f ? (my == 0 ? big : ((float)(my << ER_WC_SHIFT) - l.Ly) / gy) : big);
-            const float ez = gz > 0.0f ? (mz == dz - 1 ? big : ((float)((mz + 1) << ER_WC_SHIFT) - l.Lz) / gz)
-                                       : (gz < 0.0f ? (mz == 0 ? big : ((float)(mz << ER_WC_SHIFT) - l.Lz) / gz) : big);
+            const float ex = gx > 0.0f ? (mx == dx - 1 ? big : ((float)((mx + 1) << ER_WC_SHIFT) - l.Lx) * igx)
+                                       : (gx < 0.0f ? (mx == 0 ? big : ((float)(mx << ER_WC_SHIFT) - l.Lx) * igx) : big);
+            const float ey = gy > 0.0f ? (my == dy - 1 ? big : ((float)((my + 1) << ER_WC_SHIFT) - l.Ly) * igy)
+                                       : (gy < 0.0f ? (my == 0 ? big : ((float)(my << ER_WC_SHIFT) - l.Ly) * igy) : big);
+            const float ez = gz > 0.0f ? (mz == dz - 1 ? big : ((float)((mz + 1) << ER_WC_SHIFT) - l.Lz) * igz)
+                                       : (gz < 0.0f ? (mz == 0 ? big : ((float)(mz << ER_WC_SHIFT) - l.Lz) * igz) : big);
             const float t_exit = fmaxf(fminf(t + fminf(ex, fminf(ey, ez)), MaxT), t + fmaxf(t * 1.0e-6f, 1.0e-7f));
             if (mu > 0.0f) {
                 const float tc = t + tau / mu;
@@ -904,8 +908,9 @@ struct ShadowJob {
 
 /* Visible (transport.h:46-56) up to the volume march. Returns 0 (occluded by a light shape),
  * 1 (visible without marching) or 2 (march needed: the ray and the RNG state the march starts
- * from are written to job, and rng is advanced past the march's two draws, raymarching.h:89,95). */
-template<int COUNT>
+ * from are written to job, and rng is advanced past the march's two draws, raymarching.h:89,95;
+ * a Woodcock march starts from shadow_stream of that state instead). */
+template<int FULL, int COUNT>
 __device__ int visible_defer(const DevParams& p, f3 P1, f3 P2, Rng& rng, Cnt& cnt, ShadowJob& job)
 {
     if (!p.shadows) return 1;
@@ -918,7 +923,12 @@ __device__ int visible_defer(const DevParams& p, f3 P1, f3 P2, Rng& rng, Cnt& cn
     job.Ox = r.O.x; job.Oy = r.O.y; job.Oz = r.O.z;
     job.Dx = r.D.x; job.Dy = r.D.y; job.Dz = r.D.z;
     job.MaxT = r.MaxT;
-    job.s0 = rng.s0; job.s1 = rng.s1;
+    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
+        const Rng rs = shadow_stream(rng);
+        job.s0 = rs.s0; job.s1 = rs.s1;
+    } else {
+        job.s0 = rng.s0; job.s1 = rng.s1;
+    }
     (void)rng.next_bits();
     (void)rng.next_bits();
     return 2;
@@ -935,6 +945,16 @@ __device__ bool visible(const DevParams& p, const Tables& T, f3 P1, f3 P2, Rng& 
     if (p.object_count > 0 && intersects_any_object(p, r)) return false;
     float Tt; f3 P;
     const Tables& Tb = T;
+    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
+        /* own stream for the march; the path's stream takes the two draws of a ray-march test
+         * when the ray meets the volume (Q18) */
+        float nearT, farT;
+        if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) return true;
+        Rng rs = shadow_stream(rng);
+        (void)rng.next_bits();
+        (void)rng.next_bits();
+        return !march<FULL, COUNT>(p, Tb, r, false, rs, Tt, P, cnt);
+    }
     if (march<FULL, COUNT>(p, Tb, r, false, rng, Tt, P, cnt)) return false;
     return true;
 }
@@ -1159,7 +1179,7 @@ __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SE
     f3 F = shader_f<FULL>(sh, Wo, Wi);
     float BsdfPdf = shader_pdf<FULL>(sh, Wo, Wi);
     if (!black(Li) && !black(F) && BsdfPdf > 0.0f) {
-        const int v = visible_defer<COUNT>(p, SEP, SP, rng, cnt, job0);
+        const int v = visible_defer<FULL, COUNT>(p, SEP, SP, rng, cnt, job0);
         if (v) {
             const f3 d = sub(SEP, SP);
             const float LightPdf = dot(d, d) / (absdot(SN, neg(Wi)) * L.area);
@@ -1185,7 +1205,7 @@ __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SE
     const int idx2 = intersect_lights(p, r2, false, T2, P2, N2, Le2);
     if (idx2 < 0 || idx2 != pos) return;
     if (!black(Le2)) {
-        const int v = visible_defer<COUNT>(p, SEP, P2, rng, cnt, job1);
+        const int v = visible_defer<FULL, COUNT>(p, SEP, P2, rng, cnt, job1);
         if (v) {
             const f3 d = sub(SEP, P2);
             const float LightPdf = dot(d, d) / (absdot(N2, neg(Wi)) * L.area);

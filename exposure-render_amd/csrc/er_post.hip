@@ -238,107 +238,19 @@ __global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits,
     bits[base + w] = v;
 }
 
-/* True when every sample the march can assign to macrocell m is clipped away (clipped() holds
- * for it), so the macrocell can be skipped like an empty one. A sample is assigned to m when its
- * lattice coordinates, computed in float, fall in m's lattice box; the box is widened by 0.01
- * lattice units against that rounding, and an edge macrocell (which also takes the samples the
- * clamp maps to it) extends one lattice unit plus edge_ext world units beyond the volume: samples
- * lie inside the volume's box up to the rounding of the ray arithmetic, which edge_ext bounds.
- * The test is done in double precision on the 8 corners of that world box: the clipped region
- * of one clipping object is convex (inside a convex shape) or contains the box when the box is
- * separated from the shape, and every comparison keeps a margin E = 1e-6 x (sum of the
- * magnitudes of the terms) over the float rounding of the local coordinates in clipped(), so
- * the float test of every sample in the box agrees. NaN anywhere fails the test. */
-__device__ bool mc_fully_clipped(const DevParams& p, int m, double edge_ext)
+/* occupancy bit of macrocell m: it can hold a sample with sigma > 0 that is not clipped away;
+ * clipped (may be NULL) has bit m set when every sample of macrocell m is clipped away (the
+ * host's conservative test; clipping objects apply in the full estimator only) */
+__global__ __launch_bounds__(256) void k_macro_bits(const float* __restrict__ mc_sigma, const uint32_t* __restrict__ clipped,
+                                                   uint32_t* __restrict__ bits, int n, int words)
 {
-    const int sh = p.mc_shift;
-    const int ix[3] = {m % p.mc_dim[0], (m / p.mc_dim[0]) % p.mc_dim[1], m / (p.mc_dim[0] * p.mc_dim[1])};
-    double lo[3], hi[3], amax[3];
-    for (int a = 0; a < 3; ++a) {
-        const double cw = 1.0 / ((double)p.invsize[a] * (double)p.resf[a]);   /* world size of one lattice unit */
-        const double l0 = ix[a] == 0 ? -1.0 : (double)(ix[a] << sh) - 0.01;
-        const double l1 = ix[a] == p.mc_dim[a] - 1 ? (double)p.resf[a] + 1.0 : (double)((ix[a] + 1) << sh) + 0.01;
-        lo[a] = (double)p.minp[a] + l0 * cw - (ix[a] == 0 ? edge_ext : 0.0);
-        hi[a] = (double)p.minp[a] + l1 * cw + (ix[a] == p.mc_dim[a] - 1 ? edge_ext : 0.0);
-        amax[a] = fmax(fabs(lo[a]), fabs(hi[a]));
-    }
-    for (int k = 0; k < p.clip_count; ++k) {
-        const DevClip& c = p.clips[k];
-        double pl[8][3], E[3], cmin[3], cmax[3];
-        for (int i = 0; i < 3; ++i) {
-            E[i] = 1.0e-6 * (fabs((double)c.inv_tm[i][0]) * amax[0] + fabs((double)c.inv_tm[i][1]) * amax[1] +
-                             fabs((double)c.inv_tm[i][2]) * amax[2] + fabs((double)c.inv_tm[i][3])) + 1.0e-30;
-            cmin[i] = 1.0e300; cmax[i] = -1.0e300;
-        }
-        for (int q = 0; q < 8; ++q) {
-            const double P[3] = {(q & 1) ? hi[0] : lo[0], (q & 2) ? hi[1] : lo[1], (q & 4) ? hi[2] : lo[2]};
-            for (int i = 0; i < 3; ++i) {
-                pl[q][i] = (double)c.inv_tm[i][0] * P[0] + (double)c.inv_tm[i][1] * P[1] + (double)c.inv_tm[i][2] * P[2] +
-                           (double)c.inv_tm[i][3];
-                cmin[i] = fmin(cmin[i], pl[q][i]);
-                cmax[i] = fmax(cmax[i], pl[q][i]);
-            }
-        }
-        bool all = true;
-        for (int i = 0; i < 3; ++i) all = all && cmin[i] == cmin[i] && cmax[i] == cmax[i] && E[i] < 1.0e300;
-        if (!all) continue;
-        const double h[3] = {0.5 * (double)c.size[0], 0.5 * (double)c.size[1], 0.5 * (double)c.size[2]};
-        const double r = (double)c.outer_radius;
-        bool full = false;
-        if (!c.invert) {
-            /* the box lies inside the (convex) shape: the corners are inside with the margin */
-            if (c.type == ER_SHAPE_BOX) {
-                full = true;
-                for (int i = 0; i < 3; ++i) full = full && cmax[i] < h[i] - E[i] && cmin[i] > -h[i] + E[i];
-            } else if (c.type == ER_SHAPE_SPHERE) {
-                const double lim = r - (sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) + 1.0e-6 * fabs(r));
-                full = true;
-                for (int q = 0; q < 8; ++q)
-                    full = full && sqrt(pl[q][0] * pl[q][0] + pl[q][1] * pl[q][1] + pl[q][2] * pl[q][2]) < lim;
-            } else if (c.type == ER_SHAPE_CYLINDER) {
-                const double lim = r - (sqrt(E[0] * E[0] + E[2] * E[2]) + 1.0e-6 * fabs(r));
-                full = cmax[1] < h[1] - E[1] && cmin[1] > -h[1] + E[1];
-                for (int q = 0; q < 8; ++q) full = full && sqrt(pl[q][0] * pl[q][0] + pl[q][2] * pl[q][2]) < lim;
-            } else {
-                full = cmin[2] > E[2];
-            }
-        } else {
-            /* clipped outside the shape: the box is separated from it */
-            if (c.type == ER_SHAPE_BOX) {
-                for (int i = 0; i < 3; ++i) full = full || cmin[i] > h[i] + E[i] || cmax[i] < -h[i] - E[i];
-            } else if (c.type == ER_SHAPE_SPHERE || c.type == ER_SHAPE_CYLINDER) {
-                /* distance from the axis (cylinder: in x-z) or the centre to the corners' bounding box */
-                double d2 = 0.0;
-                for (int i = 0; i < 3; ++i) {
-                    if (c.type == ER_SHAPE_CYLINDER && i == 1) continue;
-                    const double g = fmax(0.0, fmax(cmin[i], -cmax[i]));
-                    d2 += g * g;
-                }
-                const double e = c.type == ER_SHAPE_SPHERE ? sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2])
-                                                           : sqrt(E[0] * E[0] + E[2] * E[2]);
-                full = sqrt(d2) > r + e + 1.0e-6 * fabs(r);
-                if (c.type == ER_SHAPE_CYLINDER) full = full || cmin[1] > h[1] + E[1] || cmax[1] < -h[1] - E[1];
-            } else {
-                full = cmax[2] < -E[2];
-            }
-        }
-        if (full) return true;
-    }
-    return false;
-}
-
-/* occupancy bit of macrocell m: it can hold a sample with sigma > 0 that is not clipped away
- * (clipping objects apply in the full estimator only; the parameters carry none otherwise) */
-__global__ __launch_bounds__(256) void k_macro_bits(const DevParams* __restrict__ pp, const float* __restrict__ mc_sigma,
-                                                   uint32_t* __restrict__ bits, int n, int words, double edge_ext)
-{
-    const DevParams& p = *pp;
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= words) return;
+    const uint32_t gone = clipped ? clipped[w] : 0u;
     uint32_t v = 0;
     for (int b = 0; b < 32; ++b) {
         const int m = w * 32 + b;
-        if (m < n && mc_sigma[m] > 0.0f && !(p.clip_count > 0 && mc_fully_clipped(p, m, edge_ext))) v |= 1u << b;
+        if (m < n && mc_sigma[m] > 0.0f && !((gone >> b) & 1u)) v |= 1u << b;
     }
     bits[w] = v;
 }
@@ -398,10 +310,10 @@ extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm,
     return hipGetLastError();
 }
 
-extern "C" hipError_t erd_launch_macro_bits(const erd::DevParams* pp, const float* mc_sigma, uint32_t* bits, int n, int words,
-                                            double edge_ext, hipStream_t st)
+extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, const uint32_t* clipped, uint32_t* bits, int n, int words,
+                                            hipStream_t st)
 {
-    hipLaunchKernelGGL(erd::k_macro_bits, dim3((words + 255) / 256), dim3(256), 0, st, pp, mc_sigma, bits, n, words, edge_ext);
+    hipLaunchKernelGGL(erd::k_macro_bits, dim3((words + 255) / 256), dim3(256), 0, st, mc_sigma, clipped, bits, n, words);
     return hipGetLastError();
 }
 

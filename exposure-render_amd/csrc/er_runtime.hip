@@ -42,8 +42,8 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
-extern "C" hipError_t erd_launch_macro_bits(const erd::DevParams* pp, const float* mc_sigma, uint32_t* bits, int n, int words,
-                                            double edge_ext, hipStream_t st);
+extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, const uint32_t* clipped, uint32_t* bits, int n, int words,
+                                            hipStream_t st);
 extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, int mx, int my, int mz, int gx, int gy, int gz, int base, int words,
                                             hipStream_t st);
 extern "C" hipError_t erd_launch_bilateral(const void* bp, const uchar4* in, uchar4* tmp, uchar4* out, int W, int H,
@@ -257,6 +257,34 @@ int ensure_neglog_device()
 uint64_t g_scene_version = 1;
 hipStream_t g_batch_stream = nullptr, g_batch_stream2 = nullptr;
 
+/* Instantiated batch graphs, keyed by everything a graph bakes in: the jobs' buffers and
+ * parameter blocks (device pointers), film and tile layout, pass counts, the launch
+ * configuration and the estimator mode. A later batch of the same jobs reuses the graph after
+ * uploading its job tables (pass counters) and parameter blocks again. */
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf<erd::BatchJob> dJ[2];
+    void release()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        dJ[0].release();
+        dJ[1].release();
+    }
+};
+std::map<std::string, BatchGraph> g_batch_graphs;
+const size_t ER_BATCH_GRAPH_CACHE = 4;
+
+void clear_batch_graphs()
+{
+    for (auto& kv : g_batch_graphs) kv.second.release();
+    g_batch_graphs.clear();
+}
+
+
 /* Macrocells (bricks) of 2^shift cells a side: the min/max intensity of each, from which a
  * tracer derives its occupancy bits. A volume keeps one grid per brick size in use. */
 struct MacroGrid {
@@ -325,7 +353,7 @@ struct Tracer {
     DevBuf<uchar4> display, disp_tmp, disp_filt;
     DevBuf<uint32_t> seeds1, seeds2;
     DevBuf<float> lut, mc_sigma, wc_sigma;
-    DevBuf<uint32_t> mc_bits;
+    DevBuf<uint32_t> mc_bits, clip_bits;
     size_t lds_bytes = 0;
     int lut_lds = 0;
     DevBuf<DevParams> params;
@@ -355,7 +383,7 @@ struct Tracer {
         set[0].release(); set[1].release();
         running.release(); display.release(); disp_tmp.release(); disp_filt.release();
         seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); wc_sigma.release(); params.release();
-        mc_bits.release(); counters.release(); gm_tmp.release(); bilateral.release();
+        mc_bits.release(); clip_bits.release(); counters.release(); gm_tmp.release(); bilateral.release();
         tile_cost.release(); tile_order.release();
         if (stream) (void)hipStreamDestroy(stream);
         if (stream2) (void)hipStreamDestroy(stream2);
@@ -645,6 +673,117 @@ double ray_origin_bound(const erd::DevParams& P, const std::vector<const er_obje
     return r == r ? r : 1.0e300;
 }
 
+/* True when every sample the march can assign to macrocell m is clipped away (clipped() holds
+ * for it), so the macrocell can be skipped like an empty one. A sample is assigned to m when its
+ * lattice coordinates, computed in float, fall in m's lattice box; the box is widened by 0.01
+ * lattice units against that rounding, and an edge macrocell (which also takes the samples the
+ * clamp maps to it) extends one lattice unit plus edge_ext world units beyond the volume: samples
+ * lie inside the volume's box up to the rounding of the ray arithmetic, which edge_ext bounds.
+ * The test runs on the host, in double precision, on the 8 corners of that world box: the clipped region
+ * of one clipping object is convex (inside a convex shape) or contains the box when the box is
+ * separated from the shape, and every comparison keeps a margin E = 1e-6 x (sum of the
+ * magnitudes of the terms) over the float rounding of the local coordinates in clipped(), so
+ * the float test of every sample in the box agrees. NaN anywhere fails the test. */
+bool mc_fully_clipped(const DevParams& p, int m, double edge_ext)
+{
+    const int sh = p.mc_shift;
+    const int ix[3] = {m % p.mc_dim[0], (m / p.mc_dim[0]) % p.mc_dim[1], m / (p.mc_dim[0] * p.mc_dim[1])};
+    double lo[3], hi[3], amax[3];
+    for (int a = 0; a < 3; ++a) {
+        const double cw = 1.0 / ((double)p.invsize[a] * (double)p.resf[a]);   /* world size of one lattice unit */
+        const double l0 = ix[a] == 0 ? -1.0 : (double)(ix[a] << sh) - 0.01;
+        const double l1 = ix[a] == p.mc_dim[a] - 1 ? (double)p.resf[a] + 1.0 : (double)((ix[a] + 1) << sh) + 0.01;
+        lo[a] = (double)p.minp[a] + l0 * cw - (ix[a] == 0 ? edge_ext : 0.0);
+        hi[a] = (double)p.minp[a] + l1 * cw + (ix[a] == p.mc_dim[a] - 1 ? edge_ext : 0.0);
+        amax[a] = std::fmax(std::fabs(lo[a]), std::fabs(hi[a]));
+    }
+    for (int k = 0; k < p.clip_count; ++k) {
+        const erd::DevClip& c = p.clips[k];
+        double pl[8][3], E[3], cmin[3], cmax[3];
+        for (int i = 0; i < 3; ++i) {
+            E[i] = 1.0e-6 * (std::fabs((double)c.inv_tm[i][0]) * amax[0] + std::fabs((double)c.inv_tm[i][1]) * amax[1] +
+                             std::fabs((double)c.inv_tm[i][2]) * amax[2] + std::fabs((double)c.inv_tm[i][3])) + 1.0e-30;
+            cmin[i] = 1.0e300; cmax[i] = -1.0e300;
+        }
+        for (int q = 0; q < 8; ++q) {
+            const double P[3] = {(q & 1) ? hi[0] : lo[0], (q & 2) ? hi[1] : lo[1], (q & 4) ? hi[2] : lo[2]};
+            for (int i = 0; i < 3; ++i) {
+                pl[q][i] = (double)c.inv_tm[i][0] * P[0] + (double)c.inv_tm[i][1] * P[1] + (double)c.inv_tm[i][2] * P[2] +
+                           (double)c.inv_tm[i][3];
+                cmin[i] = std::fmin(cmin[i], pl[q][i]);
+                cmax[i] = std::fmax(cmax[i], pl[q][i]);
+            }
+        }
+        bool all = true;
+        for (int i = 0; i < 3; ++i) all = all && cmin[i] == cmin[i] && cmax[i] == cmax[i] && E[i] < 1.0e300;
+        if (!all) continue;
+        const double h[3] = {0.5 * (double)c.size[0], 0.5 * (double)c.size[1], 0.5 * (double)c.size[2]};
+        const double r = (double)c.outer_radius;
+        bool full = false;
+        if (!c.invert) {
+            /* the box lies inside the (convex) shape: the corners are inside with the margin */
+            if (c.type == ER_SHAPE_BOX) {
+                full = true;
+                for (int i = 0; i < 3; ++i) full = full && cmax[i] < h[i] - E[i] && cmin[i] > -h[i] + E[i];
+            } else if (c.type == ER_SHAPE_SPHERE) {
+                const double lim = r - (std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) + 1.0e-6 * std::fabs(r));
+                full = true;
+                for (int q = 0; q < 8; ++q)
+                    full = full && std::sqrt(pl[q][0] * pl[q][0] + pl[q][1] * pl[q][1] + pl[q][2] * pl[q][2]) < lim;
+            } else if (c.type == ER_SHAPE_CYLINDER) {
+                const double lim = r - (std::sqrt(E[0] * E[0] + E[2] * E[2]) + 1.0e-6 * std::fabs(r));
+                full = cmax[1] < h[1] - E[1] && cmin[1] > -h[1] + E[1];
+                for (int q = 0; q < 8; ++q) full = full && std::sqrt(pl[q][0] * pl[q][0] + pl[q][2] * pl[q][2]) < lim;
+            } else {
+                full = cmin[2] > E[2];
+            }
+        } else {
+            /* clipped outside the shape: the box is separated from it */
+            if (c.type == ER_SHAPE_BOX) {
+                for (int i = 0; i < 3; ++i) full = full || cmin[i] > h[i] + E[i] || cmax[i] < -h[i] - E[i];
+            } else if (c.type == ER_SHAPE_SPHERE || c.type == ER_SHAPE_CYLINDER) {
+                /* distance from the axis (cylinder: in x-z) or the centre to the corners' bounding box */
+                double d2 = 0.0;
+                for (int i = 0; i < 3; ++i) {
+                    if (c.type == ER_SHAPE_CYLINDER && i == 1) continue;
+                    const double g = std::fmax(0.0, std::fmax(cmin[i], -cmax[i]));
+                    d2 += g * g;
+                }
+                const double e = c.type == ER_SHAPE_SPHERE ? std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2])
+                                                           : std::sqrt(E[0] * E[0] + E[2] * E[2]);
+                full = std::sqrt(d2) > r + e + 1.0e-6 * std::fabs(r);
+                if (c.type == ER_SHAPE_CYLINDER) full = full || cmin[1] > h[1] + E[1] || cmax[1] < -h[1] - E[1];
+            } else {
+                full = cmax[2] < -E[2];
+            }
+        }
+        if (full) return true;
+    }
+    return false;
+}
+
+/* bit m set when macrocell m is wholly clipped away (mc_fully_clipped), on up to 16 host
+ * threads */
+void clipped_macrocells(const DevParams& P, int nmc, double edge_ext, std::vector<uint32_t>& bits)
+{
+    const int words = (nmc + 31) / 32;
+    bits.assign((size_t)words, 0u);
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t]() {
+            for (int w = t; w < words; w += nt) {
+                uint32_t v = 0;
+                for (int b = 0; b < 32; ++b) {
+                    const int m = w * 32 + b;
+                    if (m < nmc && mc_fully_clipped(P, m, edge_ext)) v |= 1u << b;
+                }
+                bits[w] = v;
+            }
+        });
+    for (auto& x : th) x.join();
+}
+
 /* Rebuild the tracer's DevParams (and the LUT / macrocell maxima / gm_max that depend on it). */
 int build_params(Tracer& t)
 {
@@ -886,8 +1025,16 @@ int build_params(Tracer& t)
     HIP_TRY(hipMemcpyAsync(t.params.ptr, &P, sizeof(P), hipMemcpyHostToDevice, t.stream));
     /* macrocells wholly clipped away count as empty (the bits kernel reads the clipping objects
      * from the parameters just uploaded) */
-    HIP_TRY(erd_launch_macro_bits(t.params.ptr, t.mc_sigma.ptr, t.mc_bits.ptr, nmc, mc_words,
-                                  ray_origin_bound(P, objs) * 1.0e-5, t.stream));
+    const uint32_t* clipped = nullptr;
+    if (P.clip_count > 0) {
+        std::vector<uint32_t> cb;
+        clipped_macrocells(P, nmc, ray_origin_bound(P, objs) * 1.0e-5, cb);
+        if ((st = t.clip_bits.alloc(cb.size()))) return st;
+        HIP_TRY(hipMemcpyAsync(t.clip_bits.ptr, cb.data(), cb.size() * sizeof(uint32_t), hipMemcpyHostToDevice, t.stream));
+        HIP_TRY(hipStreamSynchronize(t.stream));
+        clipped = t.clip_bits.ptr;
+    }
+    HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, clipped, t.mc_bits.ptr, nmc, mc_words, t.stream));
     HIP_TRY(erd_launch_group_bits(t.mc_bits.ptr, grid->mc[0], grid->mc[1], grid->mc[2], P.mc2_dim[0], P.mc2_dim[1],
                                   P.mc2_dim[2], mc_words, g_words, t.stream));
     if (needs_gm(d)) {
@@ -942,7 +1089,7 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
     /* the set was last used two passes ago: its post must have consumed the frame and queues */
     if (S.pending) HIP_TRY(hipStreamWaitEvent(t.stream, S.posted, 0));
     if (e) HIP_TRY(hipEventRecord(e[0], t.stream));
-    const int split = !(full && t.desc.settings.sampler == ER_SAMPLER_WOODCOCK);
+    const int split = 1;   /* every sampler defers its shadow marches (Woodcock's run on their own stream) */
     const size_t lds_bits = (size_t)t.host_params.mc_words * 4;
     const int ntiles = ((t.W + (1 << t.tile_shift) - 1) >> t.tile_shift) * ((t.H + (64 >> t.tile_shift) - 1) / (64 >> t.tile_shift));
     if (t.tile_cost.n < (size_t)ntiles) {
@@ -1312,6 +1459,7 @@ ER_API int er_shutdown(void)
     g_tracers.clear(); g_volumes.clear(); g_lights.clear(); g_textures.clear(); g_clips.clear();
     g_objects.clear(); g_bitmaps.clear();
     if (g_neglog_dev) { (void)hipFree(g_neglog_dev); g_neglog_dev = nullptr; }
+    clear_batch_graphs();
     if (g_batch_stream) { (void)hipStreamDestroy(g_batch_stream); g_batch_stream = nullptr; }
     if (g_batch_stream2) { (void)hipStreamDestroy(g_batch_stream2); g_batch_stream2 = nullptr; }
     g_scene_version++;
@@ -1697,18 +1845,6 @@ static int render_batch_chunk(std::vector<std::pair<int, Tracer*>>& jobs, int fu
         tile_base += table[0][k].ntiles;
         post_base += table[0][k].post_ntiles;
     }
-    DevBuf<erd::BatchJob> dJ[2];
-    for (int b = 0; b < 2; ++b) {
-        if ((st = dJ[b].alloc(nj))) return st;
-        HIP_TRY(hipMemcpy(dJ[b].ptr, table[b].data(), nj * sizeof(erd::BatchJob), hipMemcpyHostToDevice));
-        for (int k = 0; k < nj; ++k) HIP_TRY(hipMemset(table[b][k].counts, 0, 4 * sizeof(uint32_t)));
-    }
-    /* the jobs' parameter blocks, in table order, for the batch kernels */
-    {
-        std::vector<erd::DevParams> hp(nj);
-        for (int k = 0; k < nj; ++k) hp[k] = jobs[k].second->host_params;
-        HIP_TRY(erd_batch_params_upload(hp.data(), nj));
-    }
     if (!g_batch_stream) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream, hipStreamNonBlocking));
     if (!g_batch_stream2) HIP_TRY(hipStreamCreateWithFlags(&g_batch_stream2, hipStreamNonBlocking));
     hipStream_t bs = g_batch_stream, bs2 = g_batch_stream2;
@@ -1727,6 +1863,34 @@ static int render_batch_chunk(std::vector<std::pair<int, Tracer*>>& jobs, int fu
     }
     if (!all_lds) lds_words = 0;
     const int rounds = jobs[0].first;
+    /* the graph's key: the job tables without their first pass numbers, and the launch shape */
+    std::string key;
+    {
+        const int hdr[5] = {nj, full, batch_blocks, lds_words, shade_blocks};
+        key.append((const char*)hdr, sizeof(hdr));
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < nj; ++k) {
+                erd::BatchJob B = table[b][k];
+                B.no_iter0 = 0;
+                key.append((const char*)&B, sizeof(B));
+            }
+    }
+    auto cached = g_batch_graphs.find(key);
+    const bool hit = cached != g_batch_graphs.end();
+    if (!hit && g_batch_graphs.size() >= ER_BATCH_GRAPH_CACHE) clear_batch_graphs();
+    BatchGraph& G = hit ? cached->second : g_batch_graphs[key];
+    DevBuf<erd::BatchJob>* dJ = G.dJ;
+    for (int b = 0; b < 2; ++b) {
+        if ((st = dJ[b].alloc(nj))) { G.release(); g_batch_graphs.erase(key); return st; }
+        HIP_TRY(hipMemcpy(dJ[b].ptr, table[b].data(), nj * sizeof(erd::BatchJob), hipMemcpyHostToDevice));
+        for (int k = 0; k < nj; ++k) HIP_TRY(hipMemset(table[b][k].counts, 0, 4 * sizeof(uint32_t)));
+    }
+    /* the jobs' parameter blocks, in table order, for the batch kernels */
+    {
+        std::vector<erd::DevParams> hp(nj);
+        for (int k = 0; k < nj; ++k) hp[k] = jobs[k].second->host_params;
+        HIP_TRY(erd_batch_params_upload(hp.data(), nj));
+    }
     /* all rounds as one graph (no memset nodes): per round, primary + shading on the first
      * stream, then shadow + combine + post + queue reset on the second; a set is reused two
      * rounds later, after its post */
@@ -1736,38 +1900,42 @@ static int render_batch_chunk(std::vector<std::pair<int, Tracer*>>& jobs, int fu
         if (r == hipSuccess) evs.push_back(e);
         return r;
     };
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipError_t ec = hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal);
-    hipEvent_t fork = nullptr, posted[2] = {nullptr, nullptr};
-    if (ec == hipSuccess) ec = new_event(fork);
-    if (ec == hipSuccess) ec = hipEventRecord(fork, bs);
-    if (ec == hipSuccess) ec = hipStreamWaitEvent(bs2, fork, 0);
-    for (int r = 0; r < rounds && ec == hipSuccess; ++r) {
-        const int b = r & 1;
-        if (posted[b]) ec = hipStreamWaitEvent(bs, posted[b], 0);
-        hipEvent_t shaded = nullptr;
-        if (ec == hipSuccess) ec = new_event(shaded);
-        if (ec == hipSuccess)
-            ec = erd_launch_batch_trace(dJ[b].ptr, nj, full, batch_blocks, lds_words, shade_blocks, (unsigned int)r, bs,
-                                        bs2, shaded);
-        if (ec == hipSuccess) ec = erd_launch_batch_post(dJ[b].ptr, nj, (unsigned int)r, post_base, bs2);
-        if (ec == hipSuccess) ec = erd_launch_round_end(dJ[b].ptr, nj, bs2);
-        if (ec == hipSuccess) ec = new_event(posted[b]);
-        if (ec == hipSuccess) ec = hipEventRecord(posted[b], bs2);
-    }
-    hipEvent_t join = nullptr;
-    if (ec == hipSuccess) ec = new_event(join);
-    if (ec == hipSuccess) ec = hipEventRecord(join, bs2);
-    if (ec == hipSuccess) ec = hipStreamWaitEvent(bs, join, 0);
-    hipError_t ee = hipStreamEndCapture(bs, &graph);
-    if (ec == hipSuccess) ec = ee;
-    if (ec == hipSuccess) ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    for (auto e : evs) (void)hipEventDestroy(e);
-    if (ec != hipSuccess) {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        return set_error(ER_ERROR, "batch graph: %s", hipGetErrorString(ec));
+    hipGraph_t graph = G.graph;
+    hipGraphExec_t exec = G.exec;
+    hipError_t ec = hit ? hipSuccess : hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal);
+    if (!hit) {
+        hipEvent_t fork = nullptr, posted[2] = {nullptr, nullptr};
+        if (ec == hipSuccess) ec = new_event(fork);
+        if (ec == hipSuccess) ec = hipEventRecord(fork, bs);
+        if (ec == hipSuccess) ec = hipStreamWaitEvent(bs2, fork, 0);
+        for (int r = 0; r < rounds && ec == hipSuccess; ++r) {
+            const int b = r & 1;
+            if (posted[b]) ec = hipStreamWaitEvent(bs, posted[b], 0);
+            hipEvent_t shaded = nullptr;
+            if (ec == hipSuccess) ec = new_event(shaded);
+            if (ec == hipSuccess)
+                ec = erd_launch_batch_trace(dJ[b].ptr, nj, full, batch_blocks, lds_words, shade_blocks, (unsigned int)r, bs,
+                                            bs2, shaded);
+            if (ec == hipSuccess) ec = erd_launch_batch_post(dJ[b].ptr, nj, (unsigned int)r, post_base, bs2);
+            if (ec == hipSuccess) ec = erd_launch_round_end(dJ[b].ptr, nj, bs2);
+            if (ec == hipSuccess) ec = new_event(posted[b]);
+            if (ec == hipSuccess) ec = hipEventRecord(posted[b], bs2);
+        }
+        hipEvent_t join = nullptr;
+        if (ec == hipSuccess) ec = new_event(join);
+        if (ec == hipSuccess) ec = hipEventRecord(join, bs2);
+        if (ec == hipSuccess) ec = hipStreamWaitEvent(bs, join, 0);
+        hipError_t ee = hipStreamEndCapture(bs, &graph);
+        if (ec == hipSuccess) ec = ee;
+        if (ec == hipSuccess) ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        for (auto e : evs) (void)hipEventDestroy(e);
+        G.graph = graph;
+        G.exec = exec;
+        if (ec != hipSuccess) {
+            G.release();
+            g_batch_graphs.erase(key);
+            return set_error(ER_ERROR, "batch graph: %s", hipGetErrorString(ec));
+        }
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (timing) {
@@ -1778,8 +1946,6 @@ static int render_batch_chunk(std::vector<std::pair<int, Tracer*>>& jobs, int fu
     hipError_t el = hipGraphLaunch(exec, bs);
     if (timing && el == hipSuccess) el = hipEventRecord(t1, bs);
     const hipError_t es = hipStreamSynchronize(bs);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
     float ms = 0.0f;
     if (timing && el == hipSuccess && es == hipSuccess) (void)hipEventElapsedTime(&ms, t0, t1);
     if (t0) (void)hipEventDestroy(t0);

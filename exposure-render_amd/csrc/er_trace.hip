@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ p
     int paths = 0;
     /* claims [0, 64 hc): the rays of the hc heaviest tiles one by one (order[0 .. hc), 64 claims
      * each, every ray marched by a whole wave); then the other tiles in their order */
-    unsigned int hc = order ? min(__builtin_amdgcn_readfirstlane(order[ntiles]), (unsigned int)ntiles) : 0u;
+    unsigned int hc = order ? min((unsigned int)__builtin_amdgcn_readfirstlane(order[ntiles]), (unsigned int)ntiles) : 0u;
     if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) hc = 0u;
     const unsigned int nclaims = hc * 64u + ((unsigned int)ntiles - hc);
     for (;;) {
@@ -427,7 +427,8 @@ __global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp,
  * tissue takes hundreds of samples), so a wave does not wait for its longest march: the lane
  * starts job j (none if j >= n), and whenever at least ER_SHADOW_REFILL lanes are idle they
  * claim the next jobs from *claim (job claim_base + the counter's value) with one atomic and
- * start them, and the wave steps all its live marches one iteration at a time. `drained`:
+ * start them, and the wave steps all its live marches one iteration at a time (a Woodcock
+ * march runs whole when its job starts). `drained`:
  * nothing is left to claim. The job order does not matter: each march is independent and
  * writes only its own visibility byte. Every lane of the wave must call this. */
 template<int FULL, int COUNT>
@@ -453,8 +454,14 @@ __device__ __forceinline__ void shadow_marches(const DevParams& p, const Tables&
             rng.s0 = job.s0;
             rng.s1 = job.s1;
             float nearT, farT;
-            if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT) ||
-                !march_begin(p, r, nearT, farT, StepSize, rng, -1.0f, m)) {
+            if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) {
+                vis[slot] = 1;
+            } else if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
+                /* delta tracking runs to its end here: its length is not a count of steps */
+                float Tt;
+                f3 Pt;
+                vis[slot] = woodcock_track<FULL, COUNT>(p, T, r, nearT, farT, false, rng, Tt, Pt, cnt) ? 0 : 1;
+            } else if (!march_begin(p, r, nearT, farT, StepSize, rng, -1.0f, m)) {
                 vis[slot] = 1;
             } else if (!(m.Sum < m.S)) {
                 vis[slot] = 0;

@@ -181,6 +181,8 @@ template<class R> struct Isect {
 };
 
 /* ---- RNG (rng.h:26-66). Q22: Get2/Get3 draw left to right. ---------------------------- */
+struct Rng;
+static Rng shadow_stream(const Rng& r);
 struct Rng {
     uint32_t s0, s1;
     uint32_t draws = 0;
@@ -195,6 +197,20 @@ struct Rng {
         return (f - 2.f) / 2.f;
     }
 };
+
+/* The stream of a Woodcock shadow march (full mode; the march's draw count is not known before
+ * it runs): SplitMix64's finaliser of the path's state (s0 high, s1 low, xor a constant), in two
+ * non-zero halves. The path's own stream advances by the two draws of a ray-march shadow test. */
+static Rng shadow_stream(const Rng& r)
+{
+    uint64_t z = (((uint64_t)r.s0 << 32) | (uint64_t)r.s1) ^ 0x5368616457636b21ull;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    Rng s{(uint32_t)z ? (uint32_t)z : 1u, (uint32_t)(z >> 32) ? (uint32_t)(z >> 32) : 1u};
+    return s;
+}
 
 /* ---- scene --------------------------------------------------------------------------- */
 struct Scene {
@@ -519,7 +535,8 @@ template<class R> struct Est {
      * collision is spent across macrocells: the collision is at t + tau / mu inside the current
      * segment [t, t_exit), else tau loses mu (t_exit - t) and the walk moves on. Segments end
      * at the nearest macrocell face in lattice coordinates (edge macrocells extend to the end
-     * of the ray, as the clamped lookups do) and advance t by at least a millionth of it. */
+     * of the ray, as the clamped lookups do; lattice distance times the reciprocal of the ray's
+     * lattice rate) and advance t by at least a millionth of it. */
     static int wc_axis(R L, int dim) { return std::min(std::max(((int)std::floor(L)) >> WC_SHIFT, 0), dim - 1); }
     bool woodcock(const Ray<R>& r, bool primary, R MinT, R MaxT, R& Tout, V3<R>& Pout)
     {
@@ -527,6 +544,8 @@ template<class R> struct Est {
         const R res[3] = {R((float)S.res[0]), R((float)S.res[1]), R((float)S.res[2])};
         const R g[3] = {(r.D[0] * R(S.invsize[0])) * res[0], (r.D[1] * R(S.invsize[1])) * res[1],
                         (r.D[2] * R(S.invsize[2])) * res[2]};
+        const R ig[3] = {g[0] != R(0) ? R(1) / g[0] : R(0), g[1] != R(0) ? R(1) / g[1] : R(0),
+                         g[2] != R(0) ? R(1) / g[2] : R(0)};
         const int* dim = S.wc_dim;
         const R big = R(3.402823466e+38f);
         R t = MinT;
@@ -544,8 +563,8 @@ template<class R> struct Est {
                 R e[3];
                 for (int a = 0; a < 3; ++a) {
                     m[a] = wc_axis(L[a], dim[a]);
-                    if (g[a] > R(0)) e[a] = m[a] == dim[a] - 1 ? big : (R((float)((m[a] + 1) << WC_SHIFT)) - L[a]) / g[a];
-                    else if (g[a] < R(0)) e[a] = m[a] == 0 ? big : (R((float)(m[a] << WC_SHIFT)) - L[a]) / g[a];
+                    if (g[a] > R(0)) e[a] = m[a] == dim[a] - 1 ? big : (R((float)((m[a] + 1) << WC_SHIFT)) - L[a]) * ig[a];
+                    else if (g[a] < R(0)) e[a] = m[a] == 0 ? big : (R((float)(m[a] << WC_SHIFT)) - L[a]) * ig[a];
                     else e[a] = big;
                 }
                 mu = DS * R(S.wc_sigma[((size_t)m[2] * dim[1] + m[1]) * dim[0] + m[0]]);
@@ -1039,6 +1058,19 @@ template<class R> struct Est {
         if (intersects_any_light(r)) return false;
         if (intersects_any_object(r)) return false;
         R T; V3<R> P;
+        if (full && S.t.settings.sampler == ER_SAMPLER_WOODCOCK) {
+            /* own stream for the march; the path's stream takes the two draws of a ray-march
+             * test when the ray meets the volume (Q18) */
+            R nearT, farT;
+            if (!intersect_box(r, from<R>(S.minp), from<R>(S.maxp), nearT, farT)) return true;
+            const Rng path = rng;
+            rng = shadow_stream(path);
+            const bool hit = march(r, false, T, P);
+            rng = path;
+            (void)rng.get1();
+            (void)rng.get1();
+            return !hit;
+        }
         if (march(r, false, T, P)) return false;
         return true;
     }
@@ -1650,6 +1682,7 @@ void orc_fmath(int fn, const float* x, const float* y, float* out, int n)
             case 3: out[i] = erm::sinf_(x[i]); break;
             case 4: out[i] = erm::cosf_(x[i]); break;
             case 5: out[i] = erm::atan2f_(x[i], y[i]); break;
+            case 7: out[i] = erm::expm1f_(x[i]); break;
             default: out[i] = erm::acosf_(x[i]); break;
         }
     }

@@ -274,7 +274,7 @@ def tf_lut_eval(lut, range_, intensity, ch):
 
 def fmath(fn, x, y=None):
     """Evaluate the project's float elementary functions (0 exp, 1 log, 2 pow, 3 sin, 4 cos,
-    5 atan2, 6 acos) over float32 arrays."""
+    5 atan2, 6 acos, 7 expm1) over float32 arrays."""
     lib = load()
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float32)

@@ -106,6 +106,10 @@ def test_fmath_accuracy():
     assert rel.max() < 2.5e-7 * np.maximum(np.abs(e[keep] * np.log(b[keep])), 1.0).max() + 1e-6
     x = np.linspace(-1.0, 1.0, 200001, dtype=np.float32)
     assert np.abs(O.fmath(6, x) - np.arccos(x.astype(np.float64))).max() < 5e-7
+    # expm1 (the adjoint's free-flight probabilities): relative to the value, also near 0
+    x = np.concatenate([-np.geomspace(1e-30, 40.0, 200001), np.geomspace(1e-30, 20.0, 200001)]).astype(np.float32)
+    want = np.expm1(x.astype(np.float64))
+    assert _ulp_err(O.fmath(7, x), want).max() < 4.0
     yy, xx = np.meshgrid(np.linspace(-1, 1, 401), np.linspace(-1, 1, 401))
     yy = yy.astype(np.float32).ravel()
     xx = xx.astype(np.float32).ravel()

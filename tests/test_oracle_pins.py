@@ -423,3 +423,59 @@ def test_three_lights_expectation_is_the_sum_and_choice_is_uniform():
     assert n > 5000
     chi2 = float(np.sum((counts - n / 3) ** 2 / (n / 3)))
     assert chi2 < 13.8, (counts, chi2)       # 2 degrees of freedom, p = 0.001
+
+
+def test_woodcock_with_shadows_matches_the_transmittance_integral():
+    """Woodcock delta tracking for both the camera ray and the shadow ray (full estimator; the
+    shadow march on its own stream): in the homogeneous cube the scatter position has density
+    s e^{-s (t - t0)} on the camera ray's chord [t0, t1], s = DS^2 opacity, and the light
+    arrives through the shadow segment's chord c inside the cube with transmittance e^{-s c}:
+    the expectation is the integral of (Kd / pi) Le cos_l / d^2 e^{-s c} over the chord and the
+    light. A shadow stream that reused the path's draws, or a biased shadow tracker, would
+    miss it."""
+    L0 = LIGHTS[0]
+    sc = _phase_cube([L0[:4] + ((1.0, 1.0, 1.0),)])
+    t = sc.tracer
+    t.settings.sampler = capi.SAMPLER_WOODCOCK
+    t.settings.shadows = 1
+    t.settings.max_shadow_distance = 100.0
+    orc = O.Oracle(sc, nthreads=4)
+    W, H = sc.film
+    s = DS * DS * OPACITY
+    O_cam = np.array([t.camera.pos[i] for i in range(3)], dtype=np.float64)
+    px = np.array([8, 12, 15, 11], dtype=np.int32)
+    py = np.array([12, 12, 9, 15], dtype=np.int32)
+    got = orc.expect_f64(px, py, 40000, seed=25)
+    L = sc.lights[0]
+    tm = np.array([[L.shape.tm[i][j] for j in range(4)] for i in range(4)], dtype=np.float64)
+    n = 32
+    u = (np.arange(n) + 0.5) / n
+    uu, vv = np.meshgrid(u, u, indexing="ij")
+    local = np.stack([(uu - 0.5) * L.shape.size[0], (vv - 0.5) * L.shape.size[1], np.zeros_like(uu),
+                      np.ones_like(uu)], axis=-1)
+    X = (local @ tm.T)[..., :3].reshape(-1, 3)
+    Nl = tm[:3, :3] @ np.array([0.0, 0.0, 1.0])
+    lo, hi = np.full(3, -0.5), np.full(3, 0.5)
+    for i in range(len(px)):
+        D = _camera_hits(orc, W, H, px[i], py[i])
+        t0, t1 = 0.0, np.inf
+        for a in range(3):
+            p0, p1 = (lo[a] - O_cam[a]) / D[a], (hi[a] - O_cam[a]) / D[a]
+            t0, t1 = max(t0, min(p0, p1)), min(t1, max(p0, p1))
+        m = 400
+        ts = t0 + (np.arange(m) + 0.5) / m * (t1 - t0)
+        dens = s * np.exp(-s * (ts - t0))
+        total = 0.0
+        for tt, w in zip(ts, dens):
+            P = O_cam + tt * D
+            Dv = X - P
+            d2 = np.sum(Dv * Dv, axis=1)
+            dist = np.sqrt(d2)
+            cos_l = np.abs(Dv @ Nl) / dist
+            chord = np.array([_slab_chord(P, Dv[k] / dist[k], lo, hi) for k in range(len(X))])
+            chord = np.minimum(chord, dist)
+            irr = np.mean(cos_l / d2 * np.exp(-s * chord)) * L.shape.size[0] * L.shape.size[1] * L.multiplier
+            total += w * irr
+        total *= (t1 - t0) / m
+        want = total * np.asarray(KD_PHASE) / math.pi
+        np.testing.assert_allclose(got[i, :3], want, rtol=0.03, err_msg=f"pixel {px[i]},{py[i]}")
