@@ -292,7 +292,11 @@ __device__ __forceinline__ Lattice to_lattice(const DevParams& p, f3 P)
     return l;
 }
 
-__device__ __forceinline__ float trunc16(float v) { return (float)(unsigned short)v; }
+/* (float)(unsigned short)v, the reference's truncating store of a lerp into the uint16 voxel
+ * type (geometry.h:87-90). The lerped values are never negative and stay below 65536 (at most
+ * 65535 (1 + 2^-24)), where that truncation is the floor: one instruction instead of a
+ * float -> int -> float round trip. */
+__device__ __forceinline__ float trunc16(float v) { return floorf(v); }
 
 /* A trilinear sample split in two halves so that several cell loads can be in flight before
  * their values are used: the cell record and the fractional lattice offsets. */

@@ -350,7 +350,8 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
 /* Woodcock delta tracking (full estimator; design from the commented sampler of
  * raymarching.h:115-273, its macrocells at :155 and majorant texture at :205) with a piecewise
  * constant majorant: mu = DS * wc_sigma of the 2^ER_WC_SHIFT-cell macrocell the ray is in. Each
- * tentative collision draws one optical depth tau = -log(u) and spends it across macrocells:
+ * tentative collision draws one optical depth tau = -log(u) (the project's logf_, computed:
+ * a table lookup per collision would be a dependent random load) and spends it across macrocells:
  * in a segment [t, t_exit) the collision is at t + tau / mu if that is before t_exit, else tau
  * loses mu (t_exit - t) and the walk moves to the next macrocell. A collision is real with
  * probability sigma(P) / mu (one more draw), else tracking continues from it. The segment ends
@@ -377,7 +378,7 @@ __device__ __forceinline__ bool woodcock_track(const DevParams& p, const Tables&
     const float big = 3.402823466e+38f;
     int segments = 0;
     for (;;) {
-        float tau = gmem(p.neglog)[rng.next_bits()];
+        float tau = -erm::logf_(rng.u());   /* u = 0 gives +inf: no collision on this ray */
         float mu = 0.0f;
         for (;;) {
             if (++segments > ER_WC_MAX_SEGMENTS || !(t < MaxT)) return false;

@@ -90,6 +90,7 @@ static const int   WC_MAX_SEGMENTS = 65536;   /* segments per tracking walk (the
 template<class R> struct M;
 template<> struct M<float> {
     static float neglog(float u) { return (float)(-logl((long double)u)); }
+    static float log(float x) { return erm::logf_(x); }
     static float exp(float x) { return erm::expf_(x); }
     static float pow(float x, float y) { return erm::powf_(x, y); }
     static float sin(float x) { return erm::sinf_(x); }
@@ -100,6 +101,7 @@ template<> struct M<float> {
 };
 template<> struct M<double> {
     static double neglog(double u) { return -std::log(u); }
+    static double log(double x) { return std::log(x); }
     static double exp(double x) { return std::exp(x); }
     static double pow(double x, double y) { return std::pow(x, y); }
     static double sin(double x) { return std::sin(x); }
@@ -551,7 +553,7 @@ template<class R> struct Est {
         R t = MinT;
         int segments = 0;
         for (;;) {
-            R tau = M<R>::neglog(u());
+            R tau = -M<R>::log(u());   /* the project's logf_ in FP32, as the GPU walk */
             R mu = R(0);
             for (;;) {
                 if (++segments > WC_MAX_SEGMENTS || !(t < MaxT)) return false;
