@@ -38,7 +38,8 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
                                        int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
                                        hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary,
-                                       const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap);
+                                       const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap,
+                                       int lanes_per_ray);
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
@@ -371,7 +372,7 @@ struct Tracer {
     /* launch configuration (er_autotune / er_set_tune): brick edge 2^mc_shift (0: the
      * volume's default), primary tiles 2^tile_shift pixels wide, persistent primary
      * workgroups per CU (0: the library default) */
-    int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0, shadow_waves_per_cu = 4, heavy_tiles = 0;
+    int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0, shadow_waves_per_cu = 4, heavy_tiles = 0, rays_per_wave = 64;
     /* primary tiles: each pass records its tiles' march steps and orders the next pass's tiles
      * most expensive first; the order is valid for the tile shape and count it was built for */
     DevBuf<uint32_t> tile_cost, tile_order;
@@ -1090,6 +1091,8 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
     if (S.pending) HIP_TRY(hipStreamWaitEvent(t.stream, S.posted, 0));
     if (e) HIP_TRY(hipEventRecord(e[0], t.stream));
     const int split = 1;   /* every sampler defers its shadow marches (Woodcock's run on their own stream) */
+    /* Woodcock tracking is per lane: its marches take one lane per ray */
+    const bool woodcock = full && t.desc.settings.sampler == ER_SAMPLER_WOODCOCK;
     const size_t lds_bits = (size_t)t.host_params.mc_words * 4;
     const int ntiles = ((t.W + (1 << t.tile_shift) - 1) >> t.tile_shift) * ((t.H + (64 >> t.tile_shift) - 1) / (64 >> t.tile_shift));
     if (t.tile_cost.n < (size_t)ntiles) {
@@ -1109,7 +1112,8 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
                              S.events.ptr, S.jobs.ptr, S.vis.ptr, t.counters.ptr, t.W, t.H, full, t.counting, t.lut_lds,
                              split, t.lds_bytes, lds_bits, shade_blocks, primary_blocks(t), shadow_blocks(t), t.tile_shift, t.stream,
                              t.stream2, S.shaded, e ? e[1] : nullptr, order_valid ? t.tile_order.ptr : nullptr,
-                             t.tile_cost.ptr, reorder ? t.tile_order.ptr : nullptr, t.heavy_tiles));
+                             t.tile_cost.ptr, reorder ? t.tile_order.ptr : nullptr, t.heavy_tiles,
+                             woodcock ? 1 : 64 / t.rays_per_wave));
     t.order_shift = t.tile_shift;
     t.order_tiles = ntiles;
     if (e) HIP_TRY(hipEventRecord(e[2], t.stream2));
@@ -1202,6 +1206,7 @@ void fill_tune(const Tracer& t, er_tune& r)
     r.blocks_per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : g_primary_blocks / std::max(1, g_cus);
     r.shadow_waves_per_cu = t.shadow_waves_per_cu;
     r.heavy_tiles = t.heavy_tiles;
+    r.rays_per_wave = t.rays_per_wave;
 }
 
 int apply_tune(Tracer& t, const er_tune& r)
@@ -1216,6 +1221,8 @@ int apply_tune(Tracer& t, const er_tune& r)
     if (r.shadow_waves_per_cu < 1 || r.shadow_waves_per_cu > 64)
         return set_error(ER_ERROR, "shadow waves per CU must be in [1, 64]");
     if (r.heavy_tiles < 0 || r.heavy_tiles > 4096) return set_error(ER_ERROR, "heavy tiles must be in [0, 4096]");
+    if (r.rays_per_wave != 64 && r.rays_per_wave != 16 && r.rays_per_wave != 4 && r.rays_per_wave != 1)
+        return set_error(ER_ERROR, "rays per wave must be 64, 16, 4 or 1");
     const Volume* v = g_volumes.find(t.desc.volume_id);
     if (v && !brick_fits(v->desc, v->vmax, bs))
         return set_error(ER_ERROR, "a %d-cell brick's occupancy bits and opacity table exceed %zu bytes of LDS", r.brick,
@@ -1225,6 +1232,7 @@ int apply_tune(Tracer& t, const er_tune& r)
     t.blocks_per_cu = r.blocks_per_cu;
     t.shadow_waves_per_cu = r.shadow_waves_per_cu;
     t.heavy_tiles = r.heavy_tiles;
+    t.rays_per_wave = r.rays_per_wave;
     t.built_version = 0;
     return ER_OK;
 }
@@ -1302,7 +1310,8 @@ int time_passes(Tracer& t, Snapshot& snap, int passes, float& ms)
 }
 
 /* The candidates of one search axis around `cur`: 0 bricks that fit every volume, 1 tile
- * shapes, 2 workgroups per CU. */
+ * shapes, 2 workgroups per CU, 3 shadow waves per CU, 4 heavy tiles (with 64 rays per wave
+ * only), 5 rays per wave. */
 std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::vector<const Volume*>& vols)
 {
     std::vector<er_tune> c;
@@ -1321,9 +1330,13 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
     } else if (axis == 3) {
         for (int w = 2; w <= 8; w *= 2)
             if (w != cur.shadow_waves_per_cu) { er_tune x = cur; x.shadow_waves_per_cu = w; c.push_back(x); }
+    } else if (axis == 4) {
+        if (cur.rays_per_wave == 64)
+            for (int h : {0, 256})
+                if (h != cur.heavy_tiles) { er_tune x = cur; x.heavy_tiles = h; c.push_back(x); }
     } else {
-        for (int h : {0, 256})
-            if (h != cur.heavy_tiles) { er_tune x = cur; x.heavy_tiles = h; c.push_back(x); }
+        for (int r : {16, 4, 1})
+            if (r != cur.rays_per_wave) { er_tune x = cur; x.rays_per_wave = r; c.push_back(x); }
     }
     return c;
 }
@@ -1338,7 +1351,7 @@ const float ER_TUNE_TRIAL_MS = 30.0f;   /* target length of one timing trial */
 /* ============================== C ABI ================================================== */
 extern "C" {
 
-ER_API int er_abi_version(void) { return 1; }
+ER_API int er_abi_version(void) { return 2; }
 
 ER_API int er_abi_sizeof(int* sizes, int n)
 {
@@ -1651,7 +1664,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     Snapshot snap;
     if ((st = snap.save(t))) return st;
     const int mc0 = t.mc_shift, ts0 = t.tile_shift, bpc0 = t.blocks_per_cu, sw0 = t.shadow_waves_per_cu,
-              ht0 = t.heavy_tiles;
+              ht0 = t.heavy_tiles, rw0 = t.rays_per_wave;
     er_tune cur{};
     fill_tune(t, cur);
     int trials = 0;
@@ -1678,15 +1691,19 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     if (!st) st = measure(cur, cur_ms);
     const float default_ms = cur_ms;
     std::vector<const Volume*> vols{g_volumes.find(t.desc.volume_id)};
-    for (int axis = 0; axis < 5 && !st; ++axis)
+    /* rays per wave right after the brick: the later axes are tuned around it */
+    for (int axis : {0, 5, 1, 2, 3, 4}) {
+        if (st) break;
         for (const er_tune& c : tune_candidates(axis, cur, vols)) {
             float ms = 0.0f;
             if ((st = measure(c, ms))) break;
             if (ms < ER_TUNE_MARGIN * cur_ms) { cur = c; cur_ms = ms; }
         }
+    }
     if (!st) st = apply_tune(t, cur);
     if (st) {
         t.mc_shift = mc0; t.tile_shift = ts0; t.blocks_per_cu = bpc0; t.shadow_waves_per_cu = sw0; t.heavy_tiles = ht0;
+        t.rays_per_wave = rw0;
         t.built_version = 0;
     }
     const int rst = snap.restore(t);

@@ -74,17 +74,20 @@ __device__ __forceinline__ void flush_counters(const Cnt& cnt, bool active, int 
  * event is a light or nothing are finished here; pixels with a volume event are appended to
  * the scatter queue (wave-aggregated atomic on *qcount) with their RNG state and the draws
  * the shading still needs. */
-template<int FULL, int COUNT>
+template<int FULL, int COUNT, int G = 1>
 __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T, unsigned int tile, int tiles_x, int lane,
                                              float4* __restrict__ frame, uint32_t* __restrict__ seeds1,
                                              uint32_t* __restrict__ seeds2, ScatterRec* __restrict__ queue,
                                              unsigned int* __restrict__ qcount, Cnt& cnt, int& paths, int& steps, int ray)
 {
     const int ts = p.tile_shift;
-    /* ray >= 0: the whole wave computes that one pixel of the tile (every lane the same) and
-     * marches it with coop_march; lane 0 writes the results */
-    const int pl = ray >= 0 ? ray : lane;
-    const bool owner = ray < 0 || lane == 0;
+    /* G == 1, ray >= 0: the whole wave computes that one pixel of the tile (every lane the same)
+     * and marches it with coop_march; lane 0 writes the results. G == 1, ray < 0: lane k takes
+     * pixel k. G = 4 or 16: ray is a sub-tile of 64 / G pixels, the aligned group of G lanes
+     * lane / G computes its pixel lane / G (every lane of the group the same) and marches it
+     * with group_march; the group's first lane writes the results. */
+    const int pl = G > 1 ? ray * (64 / G) + lane / G : (ray >= 0 ? ray : lane);
+    const bool owner = G > 1 ? (lane & (G - 1)) == 0 : (ray < 0 || lane == 0);
     const int px = (int)(tile % (unsigned int)tiles_x) * (1 << ts) + (pl & ((1 << ts) - 1));
     const int py = (int)(tile / (unsigned int)tiles_x) * (64 >> ts) + (pl >> ts);
     const bool active = px < p.film_w && py < p.film_h;
@@ -124,6 +127,14 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
      * it: the lane's march iterations (coop_march counts its equivalent), for the tile cost. */
     NoVisit nv;
     int it = 0;
+    if constexpr (G > 1) {
+        if (marching) {
+            f3 P = mk(0.0f, 0.0f, 0.0f);
+            const int st = group_march<FULL, COUNT, G>(p, T, m, StepSize, lane, cnt, P, it);
+            marching = false; vol = st > 0; Tv = m.MinT;
+            if (st > 0) Pv = P;
+        }
+    } else {
     for (;;) {
         const int nm = __popcll(__ballot(marching));
         if (nm == 0 || nm <= ER_COOP_LANES || ray >= 0) break;
@@ -156,6 +167,7 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
             if (st > 0) Pv = P;
             it += csteps;
         }
+    }
     }
     /* the tile's cost: its longest march */
     for (int o = 32; o > 0; o >>= 1) it = max(it, __shfl_xor(it, o));
@@ -292,9 +304,16 @@ __device__ __forceinline__ void finish_item(const EventRec& ev, unsigned int q, 
  * from an atomic counter (counts[2]), so the cost of a tile - from a few samples for background
  * pixels to thousands behind low-density tissue - is balanced over the whole device. With an
  * order (k_tile_order, from the previous pass) the tiles are claimed most expensive first, and
- * each tile's cost is recorded for the next order. */
-template<int FULL, int COUNT, int LUT_LDS>
-__global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
+ * each tile's cost is recorded for the next order. LPR, the lanes that march one ray (64 / LPR
+ * rays per wave): 1, one lane per ray (the heavy-tile claims aside); 4 or 16, each claim is a
+ * sub-tile of 64 / LPR pixels marched by groups of LPR lanes (group_march); 64, each claim is
+ * one ray marched by the whole wave (coop_march). With LPR > 1 the heavy-tile claims are not
+ * used and a tile's cost is its first claim's. */
+#ifndef ER_PRIMARY_WAVES_PER_EU
+#define ER_PRIMARY_WAVES_PER_EU 1
+#endif
+template<int FULL, int COUNT, int LUT_LDS, int LPR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ER_PRIMARY_WAVES_PER_EU))) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
                                                 ScatterRec* __restrict__ queue, unsigned int* __restrict__ counts,
                                                 unsigned long long* __restrict__ counters, int tiles_x, int ntiles,
@@ -309,13 +328,22 @@ __global__ __launch_bounds__(256) void k_primary(const DevParams* __restrict__ p
     /* claims [0, 64 hc): the rays of the hc heaviest tiles one by one (order[0 .. hc), 64 claims
      * each, every ray marched by a whole wave); then the other tiles in their order */
     unsigned int hc = order ? min((unsigned int)__builtin_amdgcn_readfirstlane(order[ntiles]), (unsigned int)ntiles) : 0u;
-    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) hc = 0u;
-    const unsigned int nclaims = hc * 64u + ((unsigned int)ntiles - hc);
+    if ((FULL && p.sampler == ER_SAMPLER_WOODCOCK) || LPR > 1) hc = 0u;
+    const unsigned int nclaims = LPR > 1 ? (unsigned int)ntiles * (unsigned int)LPR : hc * 64u + ((unsigned int)ntiles - hc);
     for (;;) {
         unsigned int claim = 0;
         if (lane == 0) claim = atomicAdd(&counts[2], 1u);
         claim = __builtin_amdgcn_readfirstlane(claim);
         if (claim >= nclaims) break;
+        if constexpr (LPR > 1) {
+            const unsigned int idx = claim / (unsigned int)LPR, sub = claim % (unsigned int)LPR;
+            const unsigned int tile = order ? __builtin_amdgcn_readfirstlane(order[idx]) : idx;
+            int steps = 0;
+            primary_tile<FULL, COUNT, (LPR < 64 ? LPR : 1)>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue,
+                                                           &counts[0], cnt, paths, steps, (int)sub);
+            if (cost && lane == 0 && sub == 0u) cost[tile] = (uint32_t)steps;
+            continue;
+        }
         const bool heavy = claim < hc * 64u;
         const unsigned int idx = heavy ? (claim >> 6) : hc + (claim - hc * 64u);
         const unsigned int tile = order ? __builtin_amdgcn_readfirstlane(order[idx]) : idx;
@@ -782,10 +810,15 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
                        unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks,
                        int shadow_blocks, int split, size_t lds, size_t lds_bits, hipStream_t st, hipStream_t st2,
                        hipEvent_t shaded, hipEvent_t after_primary, const uint32_t* order_in, uint32_t* cost,
-                       uint32_t* order_out, int heavy_cap)
+                       uint32_t* order_out, int heavy_cap, int lpr)
 {
-    hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS>), dim3(nblocks), dim3(256), lds, st, p, frame, s1, s2, queue,
-                       counts, counters, tiles_x, ntiles, order_in, cost);
+#define ER_PRIMARY(G) hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS, G>), dim3(nblocks), dim3(256), lds, st, p, \
+                                         frame, s1, s2, queue, counts, counters, tiles_x, ntiles, order_in, cost)
+    if (lpr == 4) ER_PRIMARY(4);
+    else if (lpr == 16) ER_PRIMARY(16);
+    else if (lpr == 64) ER_PRIMARY(64);
+    else ER_PRIMARY(1);
+#undef ER_PRIMARY
     if (after_primary) (void)hipEventRecord(after_primary, st);
     if (split) {
         hipLaunchKernelGGL((erd::k_shade<FULL, COUNT, LUT_LDS, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1,
@@ -816,19 +849,22 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
                                        int split, size_t lds_bytes, size_t lds_bits, int shade_blocks,
                                        int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
                                        hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary,
-                                       const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap)
+                                       const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap,
+                                       int lanes_per_ray)
 {
     const int tw = 1 << tile_shift, th = 64 >> tile_shift;
     const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
     const int ntiles = tiles_x * tiles_y;
-    /* persistent workgroups: enough to fill every CU, never more than there are tiles */
-    const int nblocks = std::max(1, std::min((ntiles + 3) / 4, primary_blocks));
+    const int lpr = (lanes_per_ray == 4 || lanes_per_ray == 16 || lanes_per_ray == 64) ? lanes_per_ray : 1;
+    /* persistent workgroups: enough to fill every CU, never more than there are claims */
+    const long long nclaims = (long long)ntiles * lpr;
+    const int nblocks = (int)std::max(1LL, std::min((nclaims + 3) / 4, (long long)primary_blocks));
     erd::ScatterRec* q = (erd::ScatterRec*)queue;
     erd::EventRec* ev = (erd::EventRec*)events;
     erd::ShadowJob* jb = (erd::ShadowJob*)jobs;
 #define ER_ALL(F, C, L) launch_all<F, C, L>(p, frame, s1, s2, q, counts, ev, jb, vis, counters, tiles_x, ntiles, nblocks, \
                                             shade_blocks, std::max(1, shadow_blocks), split, lds_bytes, lds_bits, st, st2, \
-                                            shaded, after_primary, order_in, cost, order_out, heavy_cap)
+                                            shaded, after_primary, order_in, cost, order_out, heavy_cap, lpr)
     if (full) {
         if (count) { if (lut_lds) ER_ALL(1, 1, 1); else ER_ALL(1, 1, 0); }
         else { if (lut_lds) ER_ALL(1, 0, 1); else ER_ALL(1, 0, 0); }
@@ -897,10 +933,10 @@ extern "C" hipError_t erd_launch_primary_shade(const erd::DevParams* p, float4* 
     const int nblocks = std::max(1, std::min((ntiles + 3) / 4, primary_blocks));
     erd::ScatterRec* q = (erd::ScatterRec*)queue;
     if (lut_lds)
-        hipLaunchKernelGGL((erd::k_primary<1, 0, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
+        hipLaunchKernelGGL((erd::k_primary<1, 0, 1, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
                            (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     else
-        hipLaunchKernelGGL((erd::k_primary<1, 0, 0>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
+        hipLaunchKernelGGL((erd::k_primary<1, 0, 0, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
                            (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr);
     hipLaunchKernelGGL((erd::k_shade<1, 0, 1, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1, s2,
                        (const erd::ScatterRec*)q, counts, (erd::EventRec*)events, (erd::ShadowJob*)jobs,
