@@ -8,14 +8,16 @@ filter is applied to those.
 
   small        every pixel of the 256 x 256 film (the filter renormalised at the borders);
   ct, large,   4096 gated pixels: 16 seeded 16 x 16 tiles, one in each cell of a 4 x 4 grid
-  ct_woodcock  over the film and at least one pixel from its edge, each needing the
-               expectations of its 18 x 18 neighbourhood.
+  ct_woodcock  over the volume's footprint on the film, each centred on a ray that passes
+               through non-zero opacity and at least one pixel from the film's edge; each
+               needs the expectations of its 18 x 18 neighbourhood.
 
 Stored: cx, cy (gated pixels), expect (N x 4: filtered XYZ and the filtered alpha), spp,
 exposure, film, seconds.
 
 usage: python tests/golden/make_fp64_reference.py {small|ct|large|ct_woodcock} [spp] [threads]
 """
+import ctypes as C
 import os
 import sys
 import time
@@ -48,15 +50,74 @@ def scene(name):
     raise ValueError(name)
 
 
-def tiles(W, H, seed=2024):
-    """Top-left corners of GRID x GRID seeded tiles, one per grid cell, 1 px inside the film."""
+def _rays(orc, sc, xs, ys):
+    """Camera rays through pixel centres (the film jitter's mean, Q12)."""
+    W, H = sc.film
+    cam = orc.camera().astype(np.float64)
+    N, U, V = cam[0:3], cam[3:6], cam[6:9]
+    sx = cam[9] + cam[11] * (xs + 0.5 / W)
+    sy = cam[10] + cam[12] * (ys + 0.5 / H)
+    D = N + sx[..., None] * U - sy[..., None] * V
+    return D / np.linalg.norm(D, axis=-1)[..., None]
+
+
+def _box(sc):
+    res = np.array([sc.volume.resolution[i] for i in range(3)], dtype=np.float64)
+    sp = np.array([sc.volume.spacing[i] for i in range(3)], dtype=np.float64)
+    ext = res * sp / np.max(res * sp)
+    return -0.5 * ext, 0.5 * ext
+
+
+def _chord(O, D, lo, hi):
+    t0 = np.zeros(D.shape[:-1])
+    t1 = np.full(D.shape[:-1], np.inf)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i in range(3):
+            a, b = (lo[i] - O[i]) / D[..., i], (hi[i] - O[i]) / D[..., i]
+            t0, t1 = np.maximum(t0, np.minimum(a, b)), np.minimum(t1, np.maximum(a, b))
+    return t0, t1
+
+
+def tiles(orc, sc, seed=2024):
+    """Top-left corners of GRID x GRID seeded tiles over the volume's footprint on the film
+    (the bounding rectangle of the pixels whose centre ray meets the volume's box), one per
+    cell, 1 px inside the film; a candidate is taken when its centre ray passes through
+    non-zero opacity (up to 200 candidates per cell)."""
+    W, H = sc.film
+    org = np.array([sc.tracer.camera.pos[i] for i in range(3)], dtype=np.float64)
+    ys, xs = np.mgrid[0:H, 0:W]
+    lo, hi = _box(sc)
+    t0, t1 = _chord(org, _rays(orc, sc, xs, ys), lo, hi)
+    hit = t1 > t0
+    hx, hy = xs[hit], ys[hit]
+    fx0, fx1 = max(1, int(hx.min())), min(W - 1 - TILE, int(hx.max()) - TILE)
+    fy0, fy1 = max(1, int(hy.min())), min(H - 1 - TILE, int(hy.max()) - TILE)
+    lib = O.load()
+
+    def opaque(x, y):
+        D = _rays(orc, sc, np.array(float(x)), np.array(float(y)))
+        a, b = _chord(org, D, lo, hi)
+        if not b > a:
+            return False
+        for t in np.linspace(a, b, 256):
+            I = orc.intensity(org + t * D)
+            if lib.orc_plf_eval(C.byref(sc.tracer.opacity), I) > 0.0:
+                return True
+        return False
+
     rng = np.random.default_rng(seed)
     out = []
     for gy in range(GRID):
         for gx in range(GRID):
-            x0, x1 = max(1, gx * W // GRID), min(W - 1 - TILE, (gx + 1) * W // GRID - TILE)
-            y0, y1 = max(1, gy * H // GRID), min(H - 1 - TILE, (gy + 1) * H // GRID - TILE)
-            out.append((int(rng.integers(x0, x1 + 1)), int(rng.integers(y0, y1 + 1))))
+            x0 = fx0 + gx * (fx1 - fx0) // GRID
+            x1 = max(x0, fx0 + (gx + 1) * (fx1 - fx0) // GRID)
+            y0 = fy0 + gy * (fy1 - fy0) // GRID
+            y1 = max(y0, fy0 + (gy + 1) * (fy1 - fy0) // GRID)
+            for _ in range(200):
+                x, y = int(rng.integers(x0, x1 + 1)), int(rng.integers(y0, y1 + 1))
+                if opaque(x + TILE // 2, y + TILE // 2):
+                    break
+            out.append((x, y))
     return out
 
 
@@ -75,7 +136,7 @@ def make(name, spp, threads):
         expect = filt.reshape(-1, 4).astype(np.float32)
     else:
         cxs, cys, exps = [], [], []
-        for k, (x0, y0) in enumerate(tiles(W, H)):
+        for k, (x0, y0) in enumerate(tiles(orc, sc)):
             ys, xs = np.mgrid[y0 - 1:y0 + TILE + 1, x0 - 1:x0 + TILE + 1]
             e = orc.expect_f64(xs.ravel(), ys.ravel(), spp, seed=777 + k).reshape(TILE + 2, TILE + 2, 4)
             filt = O.gauss3x3_f64(e)[1:-1, 1:-1]          # interior pixels: all nine taps
