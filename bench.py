@@ -189,11 +189,12 @@ def timed_passes(er, r, steps):
 
 def tune_summary(r):
     """er_autotune on the scene (outside the timed region; the render state is restored):
-    brick, tile, primary workgroups and shadow waves per CU, heavy tiles."""
+    brick, tile, primary workgroups and shadow waves per CU, heavy tiles, rays per wave."""
     t0 = time.perf_counter()
     t = r.autotune(passes=0)
     return {"brick": t["brick"], "tile": [t["tile_w"], t["tile_h"]], "blocks_per_cu": t["blocks_per_cu"],
-            "shadow_waves_per_cu": t["shadow_waves_per_cu"], "heavy_tiles": t["heavy_tiles"], "trials": t["trials"],
+            "shadow_waves_per_cu": t["shadow_waves_per_cu"], "heavy_tiles": t["heavy_tiles"],
+            "rays_per_wave": t["rays_per_wave"], "shadow_rays_per_wave": t["shadow_rays_per_wave"], "trials": t["trials"],
             "ms_per_pass": round(t["ms_per_pass"], 4), "default_ms_per_pass": round(t["default_ms_per_pass"], 4),
             "tune_s": round(time.perf_counter() - t0, 3)}
 

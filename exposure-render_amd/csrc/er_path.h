@@ -547,113 +547,131 @@ __device__ __forceinline__ void group_accumulate(const float (&c)[KB], float& Su
     }
 }
 
-/* One primary march (SampleVolume's loop, raymarching.h:55-68) run by an aligned group of G
- * lanes (G = 4 or 16; 64 / G rays per wave): m is the same in every lane of the group. Each
- * round trip reads B = GroupBatch<G> x G consecutive samples, lane k of the group taking
- * samples k, k + G, ..., k + (B/G - 1) G, so a ray advances B samples per memory round trip
- * instead of ER_BATCH_PRIMARY. As in coop_march every position is the sequential loop's (the
- * same float additions of StepSize), the optical depth is accumulated in sample order (one
- * addition per sample, its addend fetched from the lane that loaded it) with the loop's end
- * test before each sample and threshold test after it, and empty macrocells at the start of a
- * block are skipped by march_step's exact skip; so the event, its position and the sums are
- * the sequential march's. Control flow is uniform within a group (it depends only on the
- * group's ray), so the cross-lane reads stay inside active lanes. Returns 1 on an event (Pout =
- * the sample, m.MinT one step beyond it), -1 when the ray leaves the volume first; counts go to
- * the group's first lane; steps counts round trips (a skip, or B samples). */
-template<int FULL, int COUNT, int G>
-__device__ int group_march(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane, Cnt& cnt, f3& Pout,
-                           int& steps)
+/* One round of a march (SampleVolume's loop, raymarching.h:55-68, primary; the same loop with
+ * the shadow end test, raymarching.h:95-113, otherwise) run by an aligned group of G lanes
+ * (G = 4 or 16; 64 / G rays per wave): m is the same in every lane of the group. A round either
+ * skips the empty macrocell at MinT (march_step's exact skip) or reads B = GroupBatch<G> x G
+ * consecutive samples, lane k of the group taking samples k, k + G, ..., k + (B/G - 1) G, so a
+ * ray advances B samples per memory round trip instead of ER_BATCH_PRIMARY / ER_BATCH_SHADOW.
+ * As in coop_march every position is the sequential loop's (the same float additions of
+ * StepSize), the optical depth is accumulated in sample order (one addition per sample, its
+ * addend fetched from the lane that loaded it) with the loop's end test before each sample and
+ * threshold test after it; so the event, its position and the sums are the sequential
+ * march's. Control flow is uniform within a group (it depends only on the group's ray), so the
+ * cross-lane reads stay inside active lanes. Returns 0 to go on, 1 on an event (Pout = the
+ * sample, m.MinT one step beyond it), -1 when the ray leaves the volume first; counts go to
+ * the group's first lane. */
+template<int FULL, int COUNT, int G, bool PRIMARY>
+__device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane,
+                                          Cnt& cnt, f3& Pout)
 {
     constexpr int KB = GroupBatch<G>::value;
     constexpr int NS = KB * G;
     const int k = lane & (G - 1);
     const int gb = lane & ~(G - 1);
     const unsigned long long gmask = (1ull << G) - 1ull;
-    NoVisit nv;
-    for (;;) {
-        if (m.MinT >= m.MaxT) return -1;
-        const int sh = p.mc_shift;
-        {
-            const Lattice l0 = to_lattice(p, add(m.O, scl(m.MinT, m.D)));
-            if (!mc_occupied(T, mc_index(p, l0, sh))) {
-                /* the sequential march would skip: the same skip in every lane of the group */
-                Cnt c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-                f3 P;
-                (void)march_step<FULL, COUNT, 1, NoVisit>(p, T, m, true, StepSize, c, nv, P);
-                if (COUNT && k == 0) { cnt.primary += c.primary; cnt.iters += c.iters; cnt.skipped += c.skipped; cnt.undone += c.undone; }
-                ++steps;
-                continue;
+    if (PRIMARY ? (m.MinT >= m.MaxT) : (m.MinT > m.MaxT)) return -1;
+    const int sh = p.mc_shift;
+    {
+        const Lattice l0 = to_lattice(p, add(m.O, scl(m.MinT, m.D)));
+        if (!mc_occupied(T, mc_index(p, l0, sh))) {
+            /* the sequential march would skip: the same skip in every lane of the group */
+            Cnt c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            NoVisit nv;
+            f3 P;
+            (void)march_step<FULL, COUNT, 1, NoVisit>(p, T, m, PRIMARY, StepSize, c, nv, P);
+            if (COUNT && k == 0) {
+                cnt.primary += c.primary; cnt.shadow += c.shadow; cnt.iters += c.iters; cnt.skipped += c.skipped;
+                cnt.undone += c.undone;
             }
+            return 0;
         }
-        /* sample j G + k: t after j G + k additions of StepSize to MinT */
-        float t = m.MinT;
-        for (int s = 0; s < k; ++s) t += StepSize;
-        float tk[KB], c[KB];
-        bool ins[KB], live[KB];
-        CellFetch f[KB];
+    }
+    /* sample j G + k: t after j G + k additions of StepSize to MinT */
+    float t = m.MinT;
+    for (int s = 0; s < k; ++s) t += StepSize;
+    float tk[KB], c[KB];
+    bool ins[KB], live[KB];
+    CellFetch f[KB];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+        if (j > 0) {
+#pragma unroll
+            for (int s = 0; s < G; ++s) t += StepSize;
+        }
+        tk[j] = t;
+        ins[j] = PRIMARY ? (t < m.MaxT) : (t <= m.MaxT);
+        const f3 Pk = add(m.O, scl(t, m.D));
+        const Lattice lk = to_lattice(p, Pk);
+        live[j] = ins[j] && mc_occupied(T, mc_index(p, lk, sh)) && !(FULL && p.clip_count > 0 && clipped(p, Pk));
+        f[j] = cell_fetch(p, lk);
+    }
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+        const float sigma = live[j] ? T.lut(min((int)cell_value(f[j]), p.lut_max)) : 0.0f;
+        c[j] = sigma * StepSize;
+    }
+    /* samples 0 .. nin-1 lie before the end of the segment (t grows with the sample index) */
+    int nin = 0;
+#pragma unroll
+    for (int j = 0; j < KB; ++j) nin += __popcll((__ballot(ins[j]) >> gb) & gmask);
+    float Sum = m.Sum;
+    int send = NS, term = 0;
+    if constexpr (G == 4) {
+        group_accumulate<G, KB>(c, Sum, m.S, nin, send, term);
+    } else {
+        /* ds_bpermute has LDS latency: every addend is fetched before the chain starts */
+        float v[NS];
+        group_gather<G, KB>(c, v);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (s >= nin) { term = -1; send = s; break; }
+            Sum = Sum + v[s];
+            if (!(Sum < m.S)) { term = 1; send = s; break; }
+        }
+    }
+    const int ns = term == 1 ? send + 1 : send;
+    if (COUNT) {
+        uint32_t fetched = 0;
 #pragma unroll
         for (int j = 0; j < KB; ++j) {
-            if (j > 0) {
-#pragma unroll
-                for (int s = 0; s < G; ++s) t += StepSize;
-            }
-            tk[j] = t;
-            ins[j] = t < m.MaxT;
-            const f3 Pk = add(m.O, scl(t, m.D));
-            const Lattice lk = to_lattice(p, Pk);
-            live[j] = ins[j] && mc_occupied(T, mc_index(p, lk, sh)) && !(FULL && p.clip_count > 0 && clipped(p, Pk));
-            f[j] = cell_fetch(p, lk);
+            const int lim = min(max(ns - j * G, 0), G);
+            const unsigned long long within = (1ull << lim) - 1ull;
+            fetched += (uint32_t)__popcll(((__ballot(live[j]) >> gb) & gmask) & within);
         }
-#pragma unroll
-        for (int j = 0; j < KB; ++j) {
-            const float sigma = live[j] ? T.lut(min((int)cell_value(f[j]), p.lut_max)) : 0.0f;
-            c[j] = sigma * StepSize;
+        if (k == 0) {
+            if (PRIMARY) { cnt.primary += (uint32_t)ns; cnt.pfetch += fetched; }
+            else { cnt.shadow += (uint32_t)ns; cnt.sfetch += fetched; }
+            cnt.iters += (uint32_t)ns;
         }
-        /* samples 0 .. nin-1 lie before the end of the segment (t grows with the sample index) */
-        int nin = 0;
+    }
+    if (term == 1) {
+        const int je = send / G;
+        float te = tk[0];
 #pragma unroll
-        for (int j = 0; j < KB; ++j) nin += __popcll((__ballot(ins[j]) >> gb) & gmask);
-        float Sum = m.Sum;
-        int send = NS, term = 0;
-        if constexpr (G == 4) {
-            group_accumulate<G, KB>(c, Sum, m.S, nin, send, term);
-        } else {
-            /* ds_bpermute has LDS latency: every addend is fetched before the chain starts */
-            float v[NS];
-            group_gather<G, KB>(c, v);
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                if (s >= nin) { term = -1; send = s; break; }
-                Sum = Sum + v[s];
-                if (!(Sum < m.S)) { term = 1; send = s; break; }
-            }
-        }
-        const int ns = term == 1 ? send + 1 : send;
-        ++steps;
-        if (COUNT) {
-            uint32_t fetched = 0;
-#pragma unroll
-            for (int j = 0; j < KB; ++j) {
-                const int lim = min(max(ns - j * G, 0), G);
-                const unsigned long long within = (1ull << lim) - 1ull;
-                fetched += (uint32_t)__popcll(((__ballot(live[j]) >> gb) & gmask) & within);
-            }
-            if (k == 0) { cnt.primary += (uint32_t)ns; cnt.iters += (uint32_t)ns; cnt.pfetch += fetched; }
-        }
-        if (term == 1) {
-            const int je = send / G;
-            float te = tk[0];
-#pragma unroll
-            for (int j = 1; j < KB; ++j) if (j == je) te = tk[j];
-            te = __shfl(te, send - je * G, G);
-            Pout = add(m.O, scl(te, m.D));
-            m.MinT = te + StepSize;
-            m.Sum = Sum;
-            return 1;
-        }
-        if (term == -1) return -1;
+        for (int j = 1; j < KB; ++j) if (j == je) te = tk[j];
+        te = __shfl(te, send - je * G, G);
+        Pout = add(m.O, scl(te, m.D));
+        m.MinT = te + StepSize;
         m.Sum = Sum;
-        m.MinT = __shfl(tk[KB - 1], G - 1, G) + StepSize;
+        return 1;
+    }
+    if (term == -1) return -1;
+    m.Sum = Sum;
+    m.MinT = __shfl(tk[KB - 1], G - 1, G) + StepSize;
+    return 0;
+}
+
+/* A whole primary march by an aligned group of G lanes, round by round (group_step); steps
+ * counts the rounds. */
+template<int FULL, int COUNT, int G>
+__device__ int group_march(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane, Cnt& cnt, f3& Pout,
+                           int& steps)
+{
+    for (;;) {
+        ++steps;
+        const int st = group_step<FULL, COUNT, G, true>(p, T, m, StepSize, lane, cnt, Pout);
+        if (st != 0) return st;
     }
 }
 

@@ -521,11 +521,76 @@ __device__ __forceinline__ void shadow_marches(const DevParams& p, const Tables&
     }
 }
 
+/* shadow_marches with an aligned group of G lanes (G = 4 or 16) per march, stepped one
+ * group_step round at a time: the group starts job j (the same in every lane of the group),
+ * and whenever at least ER_SHADOW_REFILL lanes belong to idle groups those groups claim the next
+ * jobs together. Woodcock tracking is not marched this way (its shadow jobs take one lane each).
+ * The group's first lane writes the visibility byte. Every lane of the wave must call this. */
+template<int FULL, int COUNT, int G>
+__device__ __forceinline__ void shadow_marches_g(const DevParams& p, const Tables& T, const ShadowJob* __restrict__ jobq,
+                                                 unsigned char* __restrict__ vis, unsigned int* claim, unsigned int n,
+                                                 unsigned int j, unsigned int claim_base, bool drained, int lane, Cnt& cnt)
+{
+    const float StepSize = p.step_shadow * p.minstep;
+    const bool first = (lane & (G - 1)) == 0;
+    /* the first lane of every group */
+    const unsigned long long leaders = G == 4 ? 0x1111111111111111ull : 0x0001000100010001ull;
+    MarchState m;
+    unsigned int slot = 0;
+    bool busy = false;
+    for (;;) {
+        if (j < n) {
+            const ShadowJob job = jobq[j];
+            slot = 2u * job.event + job.slot;
+            Ray r;
+            r.O = mk(job.Ox, job.Oy, job.Oz);
+            r.D = mk(job.Dx, job.Dy, job.Dz);
+            r.MinT = 0.0f;
+            r.MaxT = job.MaxT;
+            Rng rng;
+            rng.s0 = job.s0;
+            rng.s1 = job.s1;
+            float nearT, farT;
+            if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) {
+                if (first) vis[slot] = 1;
+            } else if (!march_begin(p, r, nearT, farT, StepSize, rng, -1.0f, m)) {
+                if (first) vis[slot] = 1;
+            } else if (!(m.Sum < m.S)) {
+                if (first) vis[slot] = 0;
+            } else {
+                busy = true;
+            }
+            j = n;
+        }
+        if (busy) {
+            f3 P;
+            const int st = group_step<FULL, COUNT, G, false>(p, T, m, StepSize, lane, cnt, P);
+            if (st != 0) {
+                if (first) vis[slot] = st > 0 ? 0 : 1;
+                busy = false;
+            }
+        }
+        const unsigned long long idle = __ballot(!busy) & leaders;
+        const int nidle = __popcll(idle);
+        if (drained) {
+            if (nidle == 64 / G) break;
+        } else if (nidle * G >= ER_SHADOW_REFILL) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(claim, (unsigned int)nidle);
+            base = claim_base + __shfl(base, leader);
+            if (base + (unsigned int)nidle >= n) drained = true;
+            if (!busy) j = base + (unsigned int)__popcll(idle & ((1ull << (lane & ~(G - 1))) - 1ull));
+        }
+    }
+}
+
 /* Shadow-march kernel over the shadow-job queue (counts[1] entries). The grid is persistent
  * (its size, in waves per CU, is a launch parameter: fewer waves leave the CUs to the primary
- * kernel of the next pass, which runs beside it); wave w first takes jobs [64 w, 64 w + 64),
- * and idle lanes then claim the jobs beyond the static chunks from counts[3]. */
-template<int FULL, int COUNT, int LUT_LDS>
+ * kernel of the next pass, which runs beside it); with LPR lanes per march (1, 4 or 16) wave w
+ * first takes jobs [R w, R w + R), R = 64 / LPR, and idle lanes or groups then claim the jobs
+ * beyond the static chunks from counts[3]. */
+template<int FULL, int COUNT, int LUT_LDS, int LPR>
 __global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp, unsigned int* __restrict__ counts,
                                                const ShadowJob* __restrict__ jobq, unsigned char* __restrict__ vis,
                                                unsigned long long* __restrict__ counters)
@@ -533,14 +598,19 @@ __global__ __launch_bounds__(256) void k_shadow(const DevParams* __restrict__ pp
     extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
     const unsigned int n = counts[1];
+    constexpr unsigned int R = 64u / LPR;
     const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const unsigned int static_end = ((gridDim.x * blockDim.x) >> 6) * 64u;
-    if (blockIdx.x * blockDim.x >= n) return;   /* no static chunk in the workgroup (workgroup-uniform) */
+    const unsigned int static_end = ((gridDim.x * blockDim.x) >> 6) * R;
+    if ((blockIdx.x * blockDim.x >> 6) * R >= n) return;   /* no static chunk in the workgroup (workgroup-uniform) */
     const Tables T = stage_tables<LUT_LDS>(p, lds);
     const int lane = threadIdx.x & 63;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    shadow_marches<FULL, COUNT>(p, T, jobq, vis, &counts[3], n, wave * 64u + (unsigned int)lane, static_end,
-                                static_end >= n, lane, cnt);
+    if constexpr (LPR > 1)
+        shadow_marches_g<FULL, COUNT, LPR>(p, T, jobq, vis, &counts[3], n, wave * R + (unsigned int)lane / LPR, static_end,
+                                           static_end >= n, lane, cnt);
+    else
+        shadow_marches<FULL, COUNT>(p, T, jobq, vis, &counts[3], n, wave * 64u + (unsigned int)lane, static_end,
+                                    static_end >= n, lane, cnt);
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
@@ -810,7 +880,7 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
                        unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks,
                        int shadow_blocks, int split, size_t lds, size_t lds_bits, hipStream_t st, hipStream_t st2,
                        hipEvent_t shaded, hipEvent_t after_primary, const uint32_t* order_in, uint32_t* cost,
-                       uint32_t* order_out, int heavy_cap, int lpr)
+                       uint32_t* order_out, int heavy_cap, int lpr, int shadow_lpr)
 {
 #define ER_PRIMARY(G) hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS, G>), dim3(nblocks), dim3(256), lds, st, p, \
                                          frame, s1, s2, queue, counts, counters, tiles_x, ntiles, order_in, cost)
@@ -825,8 +895,12 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
                            s2, (const erd::ScatterRec*)queue, counts, events, jobs, counters);
         (void)hipEventRecord(shaded, st);
         (void)hipStreamWaitEvent(st2, shaded, 0);
-        hipLaunchKernelGGL((erd::k_shadow<FULL, COUNT, LUT_LDS>), dim3(shadow_blocks), dim3(256), lds, st2, p, counts,
-                           (const erd::ShadowJob*)jobs, vis, counters);
+#define ER_SHADOW(G) hipLaunchKernelGGL((erd::k_shadow<FULL, COUNT, LUT_LDS, G>), dim3(shadow_blocks), dim3(256), lds, st2, \
+                                        p, counts, (const erd::ShadowJob*)jobs, vis, counters)
+        if (shadow_lpr == 4) ER_SHADOW(4);
+        else if (shadow_lpr == 16) ER_SHADOW(16);
+        else ER_SHADOW(1);
+#undef ER_SHADOW
         hipLaunchKernelGGL(erd::k_finish, dim3(shade_blocks), dim3(256), 0, st2, (const unsigned int*)counts,
                            (const erd::EventRec*)events, (const unsigned char*)vis, frame);
     } else {
@@ -850,7 +924,7 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
                                        int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
                                        hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary,
                                        const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap,
-                                       int lanes_per_ray)
+                                       int lanes_per_ray, int shadow_lanes_per_ray)
 {
     const int tw = 1 << tile_shift, th = 64 >> tile_shift;
     const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
@@ -864,7 +938,8 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
     erd::ShadowJob* jb = (erd::ShadowJob*)jobs;
 #define ER_ALL(F, C, L) launch_all<F, C, L>(p, frame, s1, s2, q, counts, ev, jb, vis, counters, tiles_x, ntiles, nblocks, \
                                             shade_blocks, std::max(1, shadow_blocks), split, lds_bytes, lds_bits, st, st2, \
-                                            shaded, after_primary, order_in, cost, order_out, heavy_cap, lpr)
+                                            shaded, after_primary, order_in, cost, order_out, heavy_cap, lpr, \
+                                            (shadow_lanes_per_ray == 4 || shadow_lanes_per_ray == 16) ? shadow_lanes_per_ray : 1)
     if (full) {
         if (count) { if (lut_lds) ER_ALL(1, 1, 1); else ER_ALL(1, 1, 0); }
         else { if (lut_lds) ER_ALL(1, 0, 1); else ER_ALL(1, 0, 0); }

@@ -132,16 +132,18 @@ class Renderer:
         return t
 
     def autotune(self, passes=6, force=False):
-        """Pick the fastest brick size, rays per wave, tile shape, primary workgroups per CU,
-        shadow waves per CU and heavy-tile count for this scene (er_autotune); the render state
-        is left as it was. Returns the chosen er_tune."""
+        """Pick the fastest brick size, rays per wave (primary and shadow), tile shape, primary
+        workgroups per CU, shadow waves per CU and heavy-tile count for this scene (er_autotune);
+        the render state is left as it was. Returns the chosen er_tune."""
         t = capi.Tune()
         _check(self.lib, self.lib.er_autotune(self.id, int(passes), capi.TUNE_FORCE if force else 0, C.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, brick, tile_w, blocks_per_cu, shadow_waves_per_cu=4, heavy_tiles=0, rays_per_wave=64):
+    def set_tune(self, brick, tile_w, blocks_per_cu, shadow_waves_per_cu=4, heavy_tiles=0, rays_per_wave=64,
+                 shadow_rays_per_wave=64):
         t = capi.Tune(brick=brick, tile_w=tile_w, tile_h=64 // tile_w, blocks_per_cu=blocks_per_cu,
-                      shadow_waves_per_cu=shadow_waves_per_cu, heavy_tiles=heavy_tiles, rays_per_wave=rays_per_wave)
+                      shadow_waves_per_cu=shadow_waves_per_cu, heavy_tiles=heavy_tiles, rays_per_wave=rays_per_wave,
+                      shadow_rays_per_wave=shadow_rays_per_wave)
         _check(self.lib, self.lib.er_set_tune(self.id, C.byref(t)))
 
     def get_tune(self):

@@ -386,6 +386,9 @@ typedef struct er_tune {
                                      (the whole wave per ray); Woodcock tracking always takes 64.
                                      The reference runs one thread per pixel
                                      (singlescattering.cuh:34-38). */
+    int   shadow_rays_per_wave;   /* shadow rays one wave marches at once: 64 (one lane per ray,
+                                     default), 16 or 4 (groups of 4 or 16 lanes per ray); Woodcock
+                                     tracking always takes 64 */
     float ms_per_pass;            /* er_autotune: the chosen configuration's time per pass */
     float default_ms_per_pass;    /* er_autotune: the starting configuration's time per pass */
     int   trials;                 /* er_autotune: configurations timed (0: taken from the cache) */
@@ -394,8 +397,8 @@ typedef struct er_tune {
 #define ER_TUNE_FORCE 1           /* er_autotune flag: measure even if the scene is cached */
 
 /* Times `passes` passes (default 6) of each candidate from the tracer's current seeds and
- * keeps the fastest: bricks, then rays per wave, then tile shapes, then workgroups per CU, then
- * shadow waves per CU, then heavy tiles. The tracer's seeds,
+ * keeps the fastest: bricks, then rays per wave, then shadow rays per wave, then tile shapes,
+ * then workgroups per CU, then shadow waves per CU, then heavy tiles. The tracer's seeds,
  * running estimate, display, counters and pass count are restored afterwards. Results are
  * cached per scene (descriptor, node table and volume contents). */
 ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out);

@@ -116,7 +116,8 @@ def test_set_tune_rejects_invalid_configurations(fresh):
     sc = scenes.config_small(scale=0.25)
     r = er.Renderer(sc, seed=1)
     for bad in [(3, 8, 7), (8, 6, 7), (8, 64, 7), (8, 8, 0), (8, 8, 17), (128, 8, 7), (8, 8, 7, 0), (8, 8, 7, 65),
-                (8, 8, 7, 4, -1), (8, 8, 7, 4, 4097), (8, 8, 7, 4, 0, 0), (8, 8, 7, 4, 0, 8), (8, 8, 7, 4, 0, 128)]:
+                (8, 8, 7, 4, -1), (8, 8, 7, 4, 4097), (8, 8, 7, 4, 0, 0), (8, 8, 7, 4, 0, 8), (8, 8, 7, 4, 0, 128), (8, 8, 7, 4, 0, 64, 1),
+                (8, 8, 7, 4, 0, 64, 32)]:
         with pytest.raises(er.ExposureRenderError):
             r.set_tune(*bad)
     r.close()
@@ -173,11 +174,12 @@ def test_tile_order_and_heavy_rays_do_not_change_the_image(fresh, scale):
 
 @pytest.mark.parametrize("case", ["small_compat", "small_full", "ct", "large_clipped"])
 def test_rays_per_wave_do_not_change_the_image(fresh, case):
-    """64 rays per wave (a lane per ray), 16 and 4 (groups of 4 and 16 lanes per ray, each
-    round trip reading 24 and 32 consecutive samples of the ray in group_march) and 1 (the
-    whole wave per ray) render the same frames, running estimates, RNG states and reference
-    sample counts bit for bit: every position is the sequential march's chain of additions and
-    the optical depth is summed in sample order. The large case has a clipping box."""
+    """Primary and shadow rays per wave: 64 (a lane per ray), 16 and 4 (groups of 4 and 16
+    lanes per ray, each round trip reading 24 and 32 consecutive samples of the ray in
+    group_step) and, for primary rays, 1 (the whole wave per ray) render the same frames,
+    running estimates, RNG states and reference sample counts bit for bit: every position is
+    the sequential march's chain of additions and the optical depth is summed in sample order.
+    The large case has a clipping box."""
     if case == "small_compat":
         sc = scenes.config_small(scale=0.5, estimator=capi.ESTIMATOR_COMPAT)
     elif case == "small_full":
@@ -189,20 +191,20 @@ def test_rays_per_wave_do_not_change_the_image(fresh, case):
         sc.tracer.camera.film_size[0], sc.tracer.camera.film_size[1] = 480, 270
     print(f"[tune] rays per wave {case}", flush=True)
     base = _render(sc, 3)
-    assert base[5]["rays_per_wave"] == 64
-    for rpw in (16, 4, 1):
-        print(f"[tune] rays per wave {case} {rpw}", flush=True)
+    assert base[5]["rays_per_wave"] == 64 and base[5]["shadow_rays_per_wave"] == 64
+    for rpw, srpw in ((16, 64), (4, 64), (1, 64), (64, 16), (64, 4), (16, 16)):
+        print(f"[tune] rays per wave {case} {rpw} shadow {srpw}", flush=True)
         r = er.Renderer(sc, seed=9)
         t = r.get_tune()
-        r.set_tune(t["brick"], t["tile_w"], t["blocks_per_cu"], t["shadow_waves_per_cu"], 0, rpw)
-        assert r.get_tune()["rays_per_wave"] == rpw
+        r.set_tune(t["brick"], t["tile_w"], t["blocks_per_cu"], t["shadow_waves_per_cu"], 0, rpw, srpw)
+        assert r.get_tune()["rays_per_wave"] == rpw and r.get_tune()["shadow_rays_per_wave"] == srpw
         r.set_counting(True)
         r.reset_counters()
         for _ in range(3):
             r.render_estimate()
         got = (r.frame(), r.running(), r.get_seeds(), r.get_estimate(), r.counters(), r.get_tune())
         r.close()
-        _same(base, got, f"{rpw} rays per wave")
+        _same(base, got, f"{rpw} / {srpw} rays per wave")
 
 
 def test_batch_with_tuned_jobs_matches_sequential(fresh):
