@@ -106,6 +106,18 @@ __device__ __forceinline__ int mc_index(const DevParams& p, const Lattice& l, in
                  (unsigned)mc_axis(l.Lx, sh, p.mc_dim[0]));
 }
 
+/* whether the clipping objects can cut the sample at lattice point l: false when the group of
+ * macrocells holding it has its clip-free bit (every occupied macrocell of the group wholly
+ * unclipped), so clipped() need not run for it */
+__device__ __forceinline__ bool clip_possible(const DevParams& p, const Tables& T, const Lattice& l)
+{
+    const int sh = p.mc_shift + ER_GROUP_SHIFT;
+    const int g = (int)(__umul24(__umul24((unsigned)mc_axis(l.Lz, sh, p.mc2_dim[2]), (unsigned)p.mc2_dim[1]) +
+                                 (unsigned)mc_axis(l.Ly, sh, p.mc2_dim[1]), (unsigned)p.mc2_dim[0]) +
+                        (unsigned)mc_axis(l.Lx, sh, p.mc2_dim[0]));
+    return !mc_occupied(T, (p.mc3_base << 5) + g);
+}
+
 /* sigma of one sample (Woodcock path) */
 template<int FULL, int COUNT>
 __device__ __forceinline__ float sample_sigma(const DevParams& p, const Tables& T, f3 Ps, bool primary, Cnt& cnt)
@@ -113,7 +125,7 @@ __device__ __forceinline__ float sample_sigma(const DevParams& p, const Tables& 
     const Lattice l = to_lattice(p, Ps);
     const int m = mc_index(p, l, p.mc_shift);
     if (!mc_occupied(T, m)) return 0.0f;
-    if (FULL && p.clip_count > 0 && clipped(p, Ps)) return 0.0f;
+    if (FULL && p.clip_count > 0 && clip_possible(p, T, l) && clipped(p, Ps)) return 0.0f;
     if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
     return T.lut(min((int)trilinear_cell(p, l), p.lut_max));
 }
@@ -236,7 +248,7 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
                          (unsigned)mxi);
     if (K == 1 && mc_occupied(T, mc)) {
         float SigmaT = 0.0f;
-        if (!(FULL && p.clip_count > 0 && clipped(p, Ps))) {
+        if (!(FULL && p.clip_count > 0 && clip_possible(p, T, l) && clipped(p, Ps))) {
             if (COUNT) { if (primary) cnt.pfetch++; else cnt.sfetch++; }
             const int iv = (int)trilinear_cell(p, l);
             SigmaT = T.lut(min(iv, p.lut_max));
@@ -271,7 +283,7 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
                 lk = to_lattice(p, Pk);
                 occ = mc_occupied(T, mc_index(p, lk, sh));
             }
-            live[k] = occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
+            live[k] = occ && !(FULL && p.clip_count > 0 && clip_possible(p, T, lk) && clipped(p, Pk));
             f[k] = cell_fetch(p, lk);
         }
         /* every sample's intensity and sigma first, so that the K table reads are in flight
@@ -467,7 +479,7 @@ __device__ int coop_march(const DevParams& p, const Tables& T, MarchState& m, fl
         const f3 Pk = add(m.O, scl(tk, m.D));
         const Lattice lk = to_lattice(p, Pk);
         const bool occ = mc_occupied(T, mc_index(p, lk, sh));
-        const bool live = inseg && occ && !(FULL && p.clip_count > 0 && clipped(p, Pk));
+        const bool live = inseg && occ && !(FULL && p.clip_count > 0 && clip_possible(p, T, lk) && clipped(p, Pk));
         float sigma = 0.0f;
         if (live) sigma = T.lut(min((int)trilinear_cell(p, lk), p.lut_max));
         const float c = sigma * StepSize;
@@ -603,7 +615,8 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
         ins[j] = PRIMARY ? (t < m.MaxT) : (t <= m.MaxT);
         const f3 Pk = add(m.O, scl(t, m.D));
         const Lattice lk = to_lattice(p, Pk);
-        live[j] = ins[j] && mc_occupied(T, mc_index(p, lk, sh)) && !(FULL && p.clip_count > 0 && clipped(p, Pk));
+        live[j] = ins[j] && mc_occupied(T, mc_index(p, lk, sh)) &&
+                  !(FULL && p.clip_count > 0 && clip_possible(p, T, lk) && clipped(p, Pk));
         f[j] = cell_fetch(p, lk);
     }
 #pragma unroll

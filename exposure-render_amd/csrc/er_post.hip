@@ -214,28 +214,36 @@ __global__ __launch_bounds__(256) void k_macro_sigma(const uint32_t* __restrict_
 }
 
 /* group bits: bit g set iff any macrocell of group g (2^ER_GROUP_SHIFT macrocells a side, clipped to the
- * grid) has its bit set; the group bits start at word `base` of the same array */
-__global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits, int mx, int my, int mz, int gx, int gy,
-                                                   int gz, int base, int words)
+ * grid) has its bit set; the group bits start at word `base` of the same array. Then the
+ * groups' clip-free bits, from word base + words: bit g set iff every occupied macrocell of
+ * group g is wholly unclipped (bit m of `unclipped`, the host's conservative test; NULL: no
+ * clipping objects, every bit clear), so the march need not test its samples against them. */
+__global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits, const uint32_t* __restrict__ unclipped,
+                                                   int mx, int my, int mz, int gx, int gy, int gz, int base, int words)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= words) return;
-    uint32_t v = 0;
+    uint32_t v = 0, f = 0;
     for (int b = 0; b < 32; ++b) {
         const int g = w * 32 + b;
         if (g >= gx * gy * gz) break;
         const int x = g % gx, y = (g / gx) % gy, z = g / (gx * gy);
-        bool any = false;
+        bool any = false, free = unclipped != nullptr;
         const int E = 1 << ER_GROUP_SHIFT;
-        for (int k = E * z; k < min(E * z + E, mz) && !any; ++k)
-            for (int j = E * y; j < min(E * y + E, my) && !any; ++j)
-                for (int i = E * x; i < min(E * x + E, mx) && !any; ++i) {
+        for (int k = E * z; k < min(E * z + E, mz); ++k)
+            for (int j = E * y; j < min(E * y + E, my); ++j)
+                for (int i = E * x; i < min(E * x + E, mx); ++i) {
                     const int m = (k * my + j) * mx + i;
-                    any = (bits[m >> 5] >> (m & 31)) & 1u;
+                    if ((bits[m >> 5] >> (m & 31)) & 1u) {
+                        any = true;
+                        free = free && ((unclipped[m >> 5] >> (m & 31)) & 1u);
+                    }
                 }
         if (any) v |= 1u << b;
+        if (free) f |= 1u << b;
     }
     bits[base + w] = v;
+    bits[base + words + w] = f;
 }
 
 /* occupancy bit of macrocell m: it can hold a sample with sigma > 0 that is not clipped away;
@@ -317,11 +325,11 @@ extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, const uint32_
     return hipGetLastError();
 }
 
-extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, int mx, int my, int mz, int gx, int gy, int gz, int base, int words,
-                                            hipStream_t st)
+extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, const uint32_t* unclipped, int mx, int my, int mz, int gx, int gy,
+                                            int gz, int base, int words, hipStream_t st)
 {
-    hipLaunchKernelGGL(erd::k_group_bits, dim3((words + 255) / 256), dim3(256), 0, st, bits, mx, my, mz, gx, gy, gz, base,
-                       words);
+    hipLaunchKernelGGL(erd::k_group_bits, dim3((words + 255) / 256), dim3(256), 0, st, bits, unclipped, mx, my, mz, gx, gy,
+                       gz, base, words);
     return hipGetLastError();
 }
 

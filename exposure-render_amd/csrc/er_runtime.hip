@@ -45,8 +45,8 @@ extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, f
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
 extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, const uint32_t* clipped, uint32_t* bits, int n, int words,
                                             hipStream_t st);
-extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, int mx, int my, int mz, int gx, int gy, int gz, int base, int words,
-                                            hipStream_t st);
+extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, const uint32_t* unclipped, int mx, int my, int mz, int gx, int gy,
+                                            int gz, int base, int words, hipStream_t st);
 extern "C" hipError_t erd_launch_bilateral(const void* bp, const uchar4* in, uchar4* tmp, uchar4* out, int W, int H,
                                            hipStream_t st);
 extern "C" size_t erd_bilateral_params_size(void);
@@ -675,18 +675,26 @@ double ray_origin_bound(const erd::DevParams& P, const std::vector<const er_obje
     return r == r ? r : 1.0e300;
 }
 
-/* True when every sample the march can assign to macrocell m is clipped away (clipped() holds
- * for it), so the macrocell can be skipped like an empty one. A sample is assigned to m when its
- * lattice coordinates, computed in float, fall in m's lattice box; the box is widened by 0.01
- * lattice units against that rounding, and an edge macrocell (which also takes the samples the
- * clamp maps to it) extends one lattice unit plus edge_ext world units beyond the volume: samples
- * lie inside the volume's box up to the rounding of the ray arithmetic, which edge_ext bounds.
- * The test runs on the host, in double precision, on the 8 corners of that world box: the clipped region
- * of one clipping object is convex (inside a convex shape) or contains the box when the box is
- * separated from the shape, and every comparison keeps a margin E = 1e-6 x (sum of the
- * magnitudes of the terms) over the float rounding of the local coordinates in clipped(), so
- * the float test of every sample in the box agrees. NaN anywhere fails the test. */
-bool mc_fully_clipped(const DevParams& p, int m, double edge_ext)
+/* The clipping class of macrocell m: bit 0 (ER_MC_CLIPPED) when every sample the march can
+ * assign to m is clipped away (clipped() holds for it), so the macrocell can be skipped like an
+ * empty one; bit 1 (ER_MC_UNCLIPPED) when no such sample is clipped away, so the march need not
+ * test it. A sample is assigned to m when its lattice coordinates, computed in float, fall in
+ * m's lattice box; the box is widened by 0.01 lattice units against that rounding, and an edge
+ * macrocell (which also takes the samples the clamp maps to it) extends one lattice unit plus
+ * edge_ext world units beyond the volume: samples lie inside the volume's box up to the
+ * rounding of the ray arithmetic, which edge_ext bounds. The test runs on the host, in double
+ * precision, on the 8 corners of that world box. Per clipping object, the box lies inside the
+ * (convex) shape when all 8 corners do, and outside it when it is separated from the shape (a
+ * face plane of the box or cylinder, or the corners' bounding box beyond the sphere's or
+ * cylinder's radius); every comparison keeps a margin E = 1e-6 x (sum of the magnitudes of the
+ * terms) over the float rounding of the local coordinates in clipped(), so the float test of
+ * every sample in the box agrees. A sample is clipped when it is inside a plain object or
+ * outside an inverted one (clipped()); so m is wholly clipped when some object clips the whole
+ * box and wholly unclipped when every object clips none of it. NaN anywhere leaves both bits
+ * clear. */
+#define ER_MC_CLIPPED 1
+#define ER_MC_UNCLIPPED 2
+int mc_clip_class(const DevParams& p, int m, double edge_ext)
 {
     const int sh = p.mc_shift;
     const int ix[3] = {m % p.mc_dim[0], (m / p.mc_dim[0]) % p.mc_dim[1], m / (p.mc_dim[0] * p.mc_dim[1])};
@@ -699,6 +707,7 @@ bool mc_fully_clipped(const DevParams& p, int m, double edge_ext)
         hi[a] = (double)p.minp[a] + l1 * cw + (ix[a] == p.mc_dim[a] - 1 ? edge_ext : 0.0);
         amax[a] = std::fmax(std::fabs(lo[a]), std::fabs(hi[a]));
     }
+    bool unclipped = true;
     for (int k = 0; k < p.clip_count; ++k) {
         const erd::DevClip& c = p.clips[k];
         double pl[8][3], E[3], cmin[3], cmax[3];
@@ -718,69 +727,73 @@ bool mc_fully_clipped(const DevParams& p, int m, double edge_ext)
         }
         bool all = true;
         for (int i = 0; i < 3; ++i) all = all && cmin[i] == cmin[i] && cmax[i] == cmax[i] && E[i] < 1.0e300;
-        if (!all) continue;
+        if (!all) { unclipped = false; continue; }
         const double h[3] = {0.5 * (double)c.size[0], 0.5 * (double)c.size[1], 0.5 * (double)c.size[2]};
         const double r = (double)c.outer_radius;
-        bool full = false;
-        if (!c.invert) {
-            /* the box lies inside the (convex) shape: the corners are inside with the margin */
-            if (c.type == ER_SHAPE_BOX) {
-                full = true;
-                for (int i = 0; i < 3; ++i) full = full && cmax[i] < h[i] - E[i] && cmin[i] > -h[i] + E[i];
-            } else if (c.type == ER_SHAPE_SPHERE) {
-                const double lim = r - (std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) + 1.0e-6 * std::fabs(r));
-                full = true;
-                for (int q = 0; q < 8; ++q)
-                    full = full && std::sqrt(pl[q][0] * pl[q][0] + pl[q][1] * pl[q][1] + pl[q][2] * pl[q][2]) < lim;
-            } else if (c.type == ER_SHAPE_CYLINDER) {
-                const double lim = r - (std::sqrt(E[0] * E[0] + E[2] * E[2]) + 1.0e-6 * std::fabs(r));
-                full = cmax[1] < h[1] - E[1] && cmin[1] > -h[1] + E[1];
-                for (int q = 0; q < 8; ++q) full = full && std::sqrt(pl[q][0] * pl[q][0] + pl[q][2] * pl[q][2]) < lim;
-            } else {
-                full = cmin[2] > E[2];
-            }
+        /* the box inside the shape: the corners are inside with the margin */
+        bool inside = false;
+        if (c.type == ER_SHAPE_BOX) {
+            inside = true;
+            for (int i = 0; i < 3; ++i) inside = inside && cmax[i] < h[i] - E[i] && cmin[i] > -h[i] + E[i];
+        } else if (c.type == ER_SHAPE_SPHERE) {
+            const double lim = r - (std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) + 1.0e-6 * std::fabs(r));
+            inside = true;
+            for (int q = 0; q < 8; ++q)
+                inside = inside && std::sqrt(pl[q][0] * pl[q][0] + pl[q][1] * pl[q][1] + pl[q][2] * pl[q][2]) < lim;
+        } else if (c.type == ER_SHAPE_CYLINDER) {
+            const double lim = r - (std::sqrt(E[0] * E[0] + E[2] * E[2]) + 1.0e-6 * std::fabs(r));
+            inside = cmax[1] < h[1] - E[1] && cmin[1] > -h[1] + E[1];
+            for (int q = 0; q < 8; ++q) inside = inside && std::sqrt(pl[q][0] * pl[q][0] + pl[q][2] * pl[q][2]) < lim;
         } else {
-            /* clipped outside the shape: the box is separated from it */
-            if (c.type == ER_SHAPE_BOX) {
-                for (int i = 0; i < 3; ++i) full = full || cmin[i] > h[i] + E[i] || cmax[i] < -h[i] - E[i];
-            } else if (c.type == ER_SHAPE_SPHERE || c.type == ER_SHAPE_CYLINDER) {
-                /* distance from the axis (cylinder: in x-z) or the centre to the corners' bounding box */
-                double d2 = 0.0;
-                for (int i = 0; i < 3; ++i) {
-                    if (c.type == ER_SHAPE_CYLINDER && i == 1) continue;
-                    const double g = std::fmax(0.0, std::fmax(cmin[i], -cmax[i]));
-                    d2 += g * g;
-                }
-                const double e = c.type == ER_SHAPE_SPHERE ? std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2])
-                                                           : std::sqrt(E[0] * E[0] + E[2] * E[2]);
-                full = std::sqrt(d2) > r + e + 1.0e-6 * std::fabs(r);
-                if (c.type == ER_SHAPE_CYLINDER) full = full || cmin[1] > h[1] + E[1] || cmax[1] < -h[1] - E[1];
-            } else {
-                full = cmax[2] < -E[2];
-            }
+            inside = cmin[2] > E[2];
         }
-        if (full) return true;
+        /* the box separated from the shape */
+        bool outside = false;
+        if (c.type == ER_SHAPE_BOX) {
+            for (int i = 0; i < 3; ++i) outside = outside || cmin[i] > h[i] + E[i] || cmax[i] < -h[i] - E[i];
+        } else if (c.type == ER_SHAPE_SPHERE || c.type == ER_SHAPE_CYLINDER) {
+            /* distance from the axis (cylinder: in x-z) or the centre to the corners' bounding box */
+            double d2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                if (c.type == ER_SHAPE_CYLINDER && i == 1) continue;
+                const double g = std::fmax(0.0, std::fmax(cmin[i], -cmax[i]));
+                d2 += g * g;
+            }
+            const double e = c.type == ER_SHAPE_SPHERE ? std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2])
+                                                       : std::sqrt(E[0] * E[0] + E[2] * E[2]);
+            outside = std::sqrt(d2) > r + e + 1.0e-6 * std::fabs(r);
+            if (c.type == ER_SHAPE_CYLINDER) outside = outside || cmin[1] > h[1] + E[1] || cmax[1] < -h[1] - E[1];
+        } else {
+            outside = cmax[2] < -E[2];
+        }
+        if (c.invert ? outside : inside) return ER_MC_CLIPPED;
+        if (!(c.invert ? inside : outside)) unclipped = false;
     }
-    return false;
+    return unclipped ? ER_MC_UNCLIPPED : 0;
 }
 
-/* bit m set when macrocell m is wholly clipped away (mc_fully_clipped), on up to 16 host
- * threads */
-void clipped_macrocells(const DevParams& P, int nmc, double edge_ext, std::vector<uint32_t>& bits)
+/* the clipping classes of every macrocell (mc_clip_class), on up to 16 host threads: gone bit m
+ * set when macrocell m is wholly clipped away, free bit m when it is wholly unclipped */
+void clip_classes(const DevParams& P, int nmc, double edge_ext, std::vector<uint32_t>& gone, std::vector<uint32_t>& free)
 {
     const int words = (nmc + 31) / 32;
-    bits.assign((size_t)words, 0u);
+    gone.assign((size_t)words, 0u);
+    free.assign((size_t)words, 0u);
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t)
         th.emplace_back([&, t]() {
             for (int w = t; w < words; w += nt) {
-                uint32_t v = 0;
+                uint32_t g = 0, f = 0;
                 for (int b = 0; b < 32; ++b) {
                     const int m = w * 32 + b;
-                    if (m < nmc && mc_fully_clipped(P, m, edge_ext)) v |= 1u << b;
+                    if (m >= nmc) break;
+                    const int c = mc_clip_class(P, m, edge_ext);
+                    if (c & ER_MC_CLIPPED) g |= 1u << b;
+                    if (c & ER_MC_UNCLIPPED) f |= 1u << b;
                 }
-                bits[w] = v;
+                gone[w] = g;
+                free[w] = f;
             }
         });
     for (auto& x : th) x.join();
@@ -1009,13 +1022,15 @@ int build_params(Tracer& t)
         P.wc_sigma = t.wc_sigma.ptr;
         for (int i = 0; i < 3; ++i) P.wc_dim[i] = wg->mc[i];
     }
-    /* occupancy bits: one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a side */
+    /* occupancy bits: one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a side;
+     * then the groups' clip-free bits */
     const int mc_words = (nmc + 31) / 32;
     for (int i = 0; i < 3; ++i) P.mc2_dim[i] = (grid->mc[i] + (1 << ER_GROUP_SHIFT) - 1) >> ER_GROUP_SHIFT;
     const int ng = P.mc2_dim[0] * P.mc2_dim[1] * P.mc2_dim[2];
     const int g_words = (ng + 31) / 32;
     P.mc2_base = mc_words;
-    P.mc_words = mc_words + g_words;
+    P.mc3_base = mc_words + g_words;
+    P.mc_words = mc_words + 2 * g_words;
     if ((st = t.mc_bits.alloc(P.mc_words))) return st;
     P.mc_bits = t.mc_bits.ptr;
     t.lut_lds = (size_t)lut_n * sizeof(float) <= 32768u;
@@ -1028,17 +1043,20 @@ int build_params(Tracer& t)
     /* macrocells wholly clipped away count as empty (the bits kernel reads the clipping objects
      * from the parameters just uploaded) */
     const uint32_t* clipped = nullptr;
+    const uint32_t* unclipped = nullptr;
     if (P.clip_count > 0) {
-        std::vector<uint32_t> cb;
-        clipped_macrocells(P, nmc, ray_origin_bound(P, objs) * 1.0e-5, cb);
-        if ((st = t.clip_bits.alloc(cb.size()))) return st;
-        HIP_TRY(hipMemcpyAsync(t.clip_bits.ptr, cb.data(), cb.size() * sizeof(uint32_t), hipMemcpyHostToDevice, t.stream));
+        std::vector<uint32_t> gone, free;
+        clip_classes(P, nmc, ray_origin_bound(P, objs) * 1.0e-5, gone, free);
+        gone.insert(gone.end(), free.begin(), free.end());
+        if ((st = t.clip_bits.alloc(gone.size()))) return st;
+        HIP_TRY(hipMemcpyAsync(t.clip_bits.ptr, gone.data(), gone.size() * sizeof(uint32_t), hipMemcpyHostToDevice, t.stream));
         HIP_TRY(hipStreamSynchronize(t.stream));
         clipped = t.clip_bits.ptr;
+        unclipped = t.clip_bits.ptr + mc_words;
     }
     HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, clipped, t.mc_bits.ptr, nmc, mc_words, t.stream));
-    HIP_TRY(erd_launch_group_bits(t.mc_bits.ptr, grid->mc[0], grid->mc[1], grid->mc[2], P.mc2_dim[0], P.mc2_dim[1],
-                                  P.mc2_dim[2], mc_words, g_words, t.stream));
+    HIP_TRY(erd_launch_group_bits(t.mc_bits.ptr, unclipped, grid->mc[0], grid->mc[1], grid->mc[2], P.mc2_dim[0],
+                                  P.mc2_dim[1], P.mc2_dim[2], mc_words, g_words, t.stream));
     if (needs_gm(d)) {
         if ((st = t.gm_tmp.alloc(1))) return st;
         HIP_TRY(hipMemsetAsync(t.gm_tmp.ptr, 0, sizeof(unsigned int), t.stream));
