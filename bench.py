@@ -362,7 +362,8 @@ def bench_batched(er, scenes, n_jobs):
             "batch_over_sequential_estimate": round(bat / seq_est, 4),
             "batch_mpaths_per_s": round(paths / bat / 1e6, 2), "identical_frames": bool(same),
             "autotune": {"brick": tune["brick"], "tile": [tune["tile_w"], tune["tile_h"]],
-                         "blocks_per_cu": tune["blocks_per_cu"], "trials": tune["trials"],
+                         "blocks_per_cu": tune["blocks_per_cu"], "rays_per_wave": tune["rays_per_wave"],
+                         "shadow_rays_per_wave": tune["shadow_rays_per_wave"], "trials": tune["trials"],
                          "round_ms": round(tune["ms_per_pass"], 3),
                          "default_round_ms": round(tune["default_ms_per_pass"], 3), "tune_s": round(tune_s, 3)}}
 

@@ -1466,15 +1466,6 @@ struct BatchJob {
     int passes, no_iter0;
 };
 
-
-/* number of jobs still rendering in this round (a prefix of the table) */
-__device__ __forceinline__ int batch_active(const BatchJob* __restrict__ J, int n, unsigned int round)
-{
-    int a = 0;
-    while (a < n && (unsigned int)J[a].passes > round) ++a;
-    return a;
-}
-
 } /* namespace erd */
 
 #endif /* ER_PATH_H */

@@ -94,15 +94,20 @@ __global__ __launch_bounds__(256) void k_post(const DevParams* __restrict__ pp, 
     post_tile(*pp, frame, running, display, no_iterations, blockIdx.x * POST_TILE, blockIdx.y * POST_TILE, tile);
 }
 
-/* post-processing of a batch round: one workgroup per 16 x 16 tile of every active job */
-__global__ __launch_bounds__(256) void k_post_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
+/* post-processing of a batch round: one workgroup per 16 x 16 tile of each of the na jobs
+ * taking part (the first na of the table, whose tiles come first); the job by binary search
+ * over the tiles' ascending first indices */
+__global__ __launch_bounds__(256) void k_post_b(const BatchJob* __restrict__ J, int na, unsigned int round)
 {
-    const int na = batch_active(J, njobs, round);
     const int b = blockIdx.x;
-    int job = -1;
-    for (int k = 0; k < na; ++k)
-        if (b >= J[k].post_base && b < J[k].post_base + J[k].post_ntiles) { job = k; break; }
-    if (job < 0) return;
+    if (na == 0) return;
+    int lo = 0, hi = na - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (J[mid].post_base <= b) lo = mid; else hi = mid - 1;
+    }
+    const int job = lo;
+    if (b < J[job].post_base || b >= J[job].post_base + J[job].post_ntiles) return;
     const BatchJob& B = J[job];
     const int t = b - B.post_base;
     __shared__ float4 tile[POST_TILE + 2][POST_TILE + 2];
@@ -345,9 +350,10 @@ extern "C" hipError_t erd_launch_gm_max(const erd::DevParams* p, unsigned int* o
     return hipGetLastError();
 }
 
-extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, unsigned int round, int total_tiles,
+extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int na, unsigned int round, int active_tiles,
                                             hipStream_t stream)
 {
-    hipLaunchKernelGGL(erd::k_post_b, dim3(total_tiles), dim3(256), 0, stream, J, njobs, round);
+    if (na == 0 || active_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(erd::k_post_b, dim3(active_tiles), dim3(256), 0, stream, J, na, round);
     return hipGetLastError();
 }

@@ -56,10 +56,10 @@ extern "C" hipError_t erd_launch_macro_minmax(const uint16_t* vox, uint32_t* mm,
 extern "C" hipError_t erd_launch_macro_sigma(const uint32_t* mm, const float* lut, int lut_max, float* out, int n,
                                              hipStream_t st);
 extern "C" hipError_t erd_launch_gm_max(const DevParams* p, unsigned int* out, hipStream_t st);
-extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, int full, int primary_blocks,
-                                             int lds_words, int shade_blocks, unsigned int round, hipStream_t stream,
-                                             hipStream_t stream2, hipEvent_t shaded);
-extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int njobs, unsigned int round, int total_tiles,
+extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int na, int full, int primary_blocks,
+                                             int lds_words, int shade_blocks, hipStream_t stream,
+                                             hipStream_t stream2, hipEvent_t shaded, int lpr, int shadow_lpr);
+extern "C" hipError_t erd_launch_batch_post(const erd::BatchJob* J, int na, unsigned int round, int active_tiles,
                                             hipStream_t stream);
 extern "C" hipError_t erd_launch_round_end(const erd::BatchJob* J, int njobs, hipStream_t stream);
 extern "C" hipError_t erd_batch_params_upload(const erd::DevParams* host, int n);
@@ -1801,12 +1801,14 @@ ER_API int er_autotune_batch(const er_batch_item* items, int n, int rounds, er_t
     float cur_ms = 0.0f;
     st = measure(cur, cur_ms);
     const float default_ms = cur_ms;
-    for (int axis = 0; axis < 3 && !st; ++axis)
+    for (int axis : {0, 5, 6, 1, 2}) {
+        if (st) break;
         for (const er_tune& c : tune_candidates(axis, cur, vols)) {
             float ms = 0.0f;
             if ((st = measure(c, ms))) break;
             if (ms < ER_TUNE_MARGIN * cur_ms) { cur = c; cur_ms = ms; }
         }
+    }
     if (!st) st = apply_all(cur);
     if (st) for (int k = 0; k < n; ++k) (void)apply_tune(*ts[k], orig[k]);
     for (int k = 0; k < n; ++k) {
@@ -1905,11 +1907,20 @@ static int render_batch_chunk(std::vector<std::pair<int, Tracer*>>& jobs, int fu
         all_lds = all_lds && t.lut_lds;
     }
     if (!all_lds) lds_words = 0;
+    /* lanes per primary and shadow march, for the whole batch: the jobs' common rays per wave
+     * (one lane per ray when they differ, or when a job uses Woodcock tracking) */
+    int lpr = 64 / jobs[0].second->rays_per_wave, slpr = 64 / jobs[0].second->shadow_rays_per_wave;
+    for (auto& j : jobs) {
+        const Tracer& t = *j.second;
+        const bool wc = full && t.desc.settings.sampler == ER_SAMPLER_WOODCOCK;
+        if (wc || 64 / t.rays_per_wave != lpr) lpr = 1;
+        if (wc || 64 / t.shadow_rays_per_wave != slpr) slpr = 1;
+    }
     const int rounds = jobs[0].first;
     /* the graph's key: the job tables without their first pass numbers, and the launch shape */
     std::string key;
     {
-        const int hdr[5] = {nj, full, batch_blocks, lds_words, shade_blocks};
+        const int hdr[7] = {nj, full, batch_blocks, lds_words, shade_blocks, lpr, slpr};
         key.append((const char*)hdr, sizeof(hdr));
         for (int b = 0; b < 2; ++b)
             for (int k = 0; k < nj; ++k) {
@@ -1953,13 +1964,17 @@ static int render_batch_chunk(std::vector<std::pair<int, Tracer*>>& jobs, int fu
         if (ec == hipSuccess) ec = hipStreamWaitEvent(bs2, fork, 0);
         for (int r = 0; r < rounds && ec == hipSuccess; ++r) {
             const int b = r & 1;
+            /* the jobs taking part in round r: those with more than r passes, a prefix of the
+             * table (ordered by pass count, largest first), and their post tiles */
+            int na = 0, active_tiles = 0;
+            while (na < nj && jobs[na].first > r) active_tiles += table[0][na++].post_ntiles;
             if (posted[b]) ec = hipStreamWaitEvent(bs, posted[b], 0);
             hipEvent_t shaded = nullptr;
             if (ec == hipSuccess) ec = new_event(shaded);
             if (ec == hipSuccess)
-                ec = erd_launch_batch_trace(dJ[b].ptr, nj, full, batch_blocks, lds_words, shade_blocks, (unsigned int)r, bs,
-                                            bs2, shaded);
-            if (ec == hipSuccess) ec = erd_launch_batch_post(dJ[b].ptr, nj, (unsigned int)r, post_base, bs2);
+                ec = erd_launch_batch_trace(dJ[b].ptr, na, full, batch_blocks, lds_words, shade_blocks, bs, bs2, shaded, lpr,
+                                            slpr);
+            if (ec == hipSuccess) ec = erd_launch_batch_post(dJ[b].ptr, na, (unsigned int)r, active_tiles, bs2);
             if (ec == hipSuccess) ec = erd_launch_round_end(dJ[b].ptr, nj, bs2);
             if (ec == hipSuccess) ec = new_event(posted[b]);
             if (ec == hipSuccess) ec = hipEventRecord(posted[b], bs2);

@@ -713,14 +713,16 @@ __device__ __forceinline__ Tables stage_job_tables(const DevParams& p, uint32_t*
 }
 
 /* The next job after j (cyclically) whose work counter counts[which] is still below its item
- * count (ntiles for tiles, counts[1] for shadow jobs); -1 when every item is claimed. Counters
- * only grow, so a stale read can only send a workgroup to a job that has just run dry. */
-__device__ __forceinline__ int next_job_with_work(const BatchJob* __restrict__ J, int na, int j, int which)
+ * count (ntiles x claims_per_tile for tiles, counts[1] for shadow jobs); -1 when every item is
+ * claimed. Counters only grow, so a stale read can only send a workgroup to a job that has just
+ * run dry. */
+__device__ __forceinline__ int next_job_with_work(const BatchJob* __restrict__ J, int na, int j, int which,
+                                                  unsigned int claims_per_tile = 1u)
 {
     for (int k = 1; k <= na; ++k) {
         const int c = (j + k) % na;
         const unsigned int claimed = __hip_atomic_load(&J[c].counts[which], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned int items = which == 2 ? (unsigned int)J[c].ntiles
+        const unsigned int items = which == 2 ? (unsigned int)J[c].ntiles * claims_per_tile
                                               : __hip_atomic_load(&J[c].counts[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (claimed < items) return c;
     }
@@ -731,14 +733,14 @@ __device__ __forceinline__ int next_job_with_work(const BatchJob* __restrict__ J
  * in LDS as in a single pass: a workgroup starts at the job that holds its proportional share
  * of the round's tiles, its waves take that job's 64-pixel tiles from the job's own counter
  * (counts[2]), and when the job has no tiles left the workgroup moves on to the next job that
- * has (a counter only grows, so a workgroup leaves only when every tile is claimed). */
-template<int FULL, int LDS>
-__global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
+ * has (a counter only grows, so a workgroup leaves only when every tile is claimed). LPR: the
+ * lanes that march one ray, as in k_primary (with LPR > 1 a claim is a sub-tile). */
+template<int FULL, int LDS, int LPR>
+__global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ J, int na)
 {
     extern __shared__ uint32_t lds[];
     __shared__ int next_job;
     const int lane = threadIdx.x & 63;
-    const int na = batch_active(J, njobs, round);
     if (na == 0) return;
     const unsigned int total = (unsigned int)(J[na - 1].tile_base + J[na - 1].ntiles);
     const unsigned int pos = (unsigned int)(((unsigned long long)blockIdx.x * total) / gridDim.x);
@@ -751,47 +753,79 @@ __global__ __launch_bounds__(256) void k_primary_b(const BatchJob* __restrict__ 
         const DevParams& p = c_batch_params[j];
         const Tables T = stage_job_tables<LDS>(p, lds);
         for (;;) {
-            unsigned int tile = 0;
-            if (lane == 0) tile = atomicAdd(&B.counts[2], 1u);
-            tile = __builtin_amdgcn_readfirstlane(tile);
-            if (tile >= (unsigned int)B.ntiles) break;
+            unsigned int claim = 0;
+            if (lane == 0) claim = atomicAdd(&B.counts[2], 1u);
+            claim = __builtin_amdgcn_readfirstlane(claim);
+            if (claim >= (unsigned int)B.ntiles * (unsigned int)LPR) break;
             int steps = 0;
-            primary_tile<FULL, 0>(p, T, tile, B.tiles_x, lane, B.frame, B.seeds1, B.seeds2, B.queue, &B.counts[0], cnt,
-                                  paths, steps, -1);
+            if constexpr (LPR > 1)
+                primary_tile<FULL, 0, (LPR < 64 ? LPR : 1)>(p, T, claim / (unsigned int)LPR, B.tiles_x, lane, B.frame,
+                                                           B.seeds1, B.seeds2, B.queue, &B.counts[0], cnt, paths, steps,
+                                                           (int)(claim % (unsigned int)LPR));
+            else
+                primary_tile<FULL, 0>(p, T, claim, B.tiles_x, lane, B.frame, B.seeds1, B.seeds2, B.queue, &B.counts[0], cnt,
+                                      paths, steps, -1);
         }
         __syncthreads();   /* every wave is done with this job's tables */
-        if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 2);
+        if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 2, (unsigned int)LPR);
         __syncthreads();
         j = __builtin_amdgcn_readfirstlane(next_job);   /* uniform: the job's fields load as scalars */
         if (j < 0) break;
     }
 }
 
-/* wave slot w of a per-job work list of count[which] items per job: job index and the
- * slot's first item; false when w is past the last job */
-__device__ __forceinline__ bool batch_slot(const BatchJob* __restrict__ J, int na, int which, unsigned int w, int& job,
-                                           unsigned int& first, unsigned int& n)
+/* The wave slots of a per-job work list of counts[which] items per job (64 items a slot), for
+ * the na active jobs: wb[k] = the first slot of job k, wb[na] = the total. Built in LDS by the
+ * first wave of the workgroup (a wave-wide scan per 64 jobs); the whole workgroup calls this. */
+__device__ __forceinline__ void batch_wave_table(const BatchJob* __restrict__ J, int na, int which, unsigned int* wb)
 {
-    unsigned int base = 0;
-    for (int k = 0; k < na; ++k) {
-        const unsigned int c = J[k].counts[which];
-        const unsigned int waves = (c + 63u) >> 6;
-        if (w < base + waves) { job = k; first = (w - base) << 6; n = c; return true; }
-        base += waves;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        unsigned int carry = 0;
+        for (int c0 = 0; c0 < na; c0 += 64) {
+            const int k = c0 + lane;
+            unsigned int v = k < na ? (J[k].counts[which] + 63u) >> 6 : 0u;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned int y = __shfl_up(v, o);
+                if (lane >= o) v += y;
+            }
+            if (k < na) wb[k + 1] = carry + v;
+            carry += __shfl(v, 63);
+        }
+        if (lane == 0) wb[0] = 0u;
     }
-    return false;
+    __syncthreads();
+}
+
+/* wave slot w of that work list: job index (the last job whose first slot is at most w, by
+ * binary search in wb), the slot's first item and the job's item count; false when w is past
+ * the last job */
+__device__ __forceinline__ bool batch_slot(const BatchJob* __restrict__ J, const unsigned int* wb, int na, int which,
+                                           unsigned int w, int& job, unsigned int& first, unsigned int& n)
+{
+    if (na == 0 || w >= wb[na]) return false;
+    int lo = 0, hi = na - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wb[mid] <= w) lo = mid; else hi = mid - 1;
+    }
+    job = lo;
+    first = (w - wb[lo]) << 6;
+    n = J[lo].counts[which];
+    return true;
 }
 
 template<int FULL>
-__global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
+__global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J, int na)
 {
-    const int na = batch_active(J, njobs, round);
+    __shared__ unsigned int wb[ER_BATCH_CONST_JOBS + 1];
+    batch_wave_table(J, na, 0, wb);
     const int lane = threadIdx.x & 63;
     const unsigned int waves = gridDim.x * (blockDim.x >> 6);
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);; w += waves) {
         int job; unsigned int q0, n;
-        if (!batch_slot(J, na, 0, w, job, q0, n)) break;
+        if (!batch_slot(J, wb, na, 0, w, job, q0, n)) break;
         job = __builtin_amdgcn_readfirstlane(job);   /* uniform (the counts are vector loads) */
         q0 = __builtin_amdgcn_readfirstlane(q0);
         n = __builtin_amdgcn_readfirstlane(n);
@@ -804,14 +838,13 @@ __global__ __launch_bounds__(256) void k_shade_b(const BatchJob* __restrict__ J,
 
 /* Shadow marches of a batch round, job by job like k_primary_b: a workgroup starts at the job
  * holding its proportional share of the round's shadow jobs, its waves' idle lanes claim that
- * job's shadow jobs (counts[3], shadow_marches), then it moves on to the next job with shadow
- * jobs left. */
-template<int FULL, int LDS>
-__global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
+ * job's shadow jobs (counts[3], shadow_marches, or shadow_marches_g with LPR = 4 or 16 lanes per
+ * march), then it moves on to the next job with shadow jobs left. */
+template<int FULL, int LDS, int LPR>
+__global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J, int na)
 {
     extern __shared__ uint32_t lds[];
     __shared__ int next_job;
-    const int na = batch_active(J, njobs, round);
     if (na == 0) return;
     const int lane = threadIdx.x & 63;
     unsigned int total = 0;
@@ -830,7 +863,10 @@ __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J
         const DevParams& p = c_batch_params[j];
         const Tables T = stage_job_tables<LDS>(p, lds);
         const unsigned int n = __builtin_amdgcn_readfirstlane(B.counts[1]);
-        shadow_marches<FULL, 0>(p, T, B.jobs, B.vis, &B.counts[3], n, n, 0u, false, lane, cnt);
+        if constexpr (LPR > 1)
+            shadow_marches_g<FULL, 0, LPR>(p, T, B.jobs, B.vis, &B.counts[3], n, n, 0u, false, lane, cnt);
+        else
+            shadow_marches<FULL, 0>(p, T, B.jobs, B.vis, &B.counts[3], n, n, 0u, false, lane, cnt);
         __syncthreads();
         if (threadIdx.x == 0) next_job = next_job_with_work(J, na, j, 3);
         __syncthreads();
@@ -839,14 +875,15 @@ __global__ __launch_bounds__(256) void k_shadow_b(const BatchJob* __restrict__ J
     }
 }
 
-__global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J, int njobs, unsigned int round)
+__global__ __launch_bounds__(256) void k_finish_b(const BatchJob* __restrict__ J, int na)
 {
-    const int na = batch_active(J, njobs, round);
+    __shared__ unsigned int wb[ER_BATCH_CONST_JOBS + 1];
+    batch_wave_table(J, na, 0, wb);
     const int lane = threadIdx.x & 63;
     const unsigned int waves = gridDim.x * (blockDim.x >> 6);
     for (unsigned int w = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);; w += waves) {
         int job; unsigned int q0, n;
-        if (!batch_slot(J, na, 0, w, job, q0, n)) break;
+        if (!batch_slot(J, wb, na, 0, w, job, q0, n)) break;
         const BatchJob& B = J[job];
         const unsigned int q = q0 + lane;
         if (q < n) finish_item(B.events[q], q, B.vis, B.frame);
@@ -952,32 +989,50 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
 }
 
 /* One batch round, split like a single pass: primary and shading on st; shadow marches and
- * combine on st2 after the event `shaded`. */
-extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int njobs, int full, int primary_blocks,
-                                             int lds_words, int shade_blocks, unsigned int round, hipStream_t stream,
-                                             hipStream_t stream2, hipEvent_t shaded)
+ * combine on st2 after the event `shaded`. lpr / shadow_lpr: lanes per primary / shadow march
+ * (1, 4, 16 or 64 / 1, 4 or 16), for every job of the batch. */
+template<int F, int L>
+static void launch_primary_b(const erd::BatchJob* J, int na, int lpr, int blocks, size_t lds, hipStream_t stream)
+{
+    if (lpr == 4) hipLaunchKernelGGL((erd::k_primary_b<F, L, 4>), dim3(blocks), dim3(256), lds, stream, J, na);
+    else if (lpr == 16) hipLaunchKernelGGL((erd::k_primary_b<F, L, 16>), dim3(blocks), dim3(256), lds, stream, J, na);
+    else if (lpr == 64) hipLaunchKernelGGL((erd::k_primary_b<F, L, 64>), dim3(blocks), dim3(256), lds, stream, J, na);
+    else hipLaunchKernelGGL((erd::k_primary_b<F, L, 1>), dim3(blocks), dim3(256), lds, stream, J, na);
+}
+
+template<int F, int L>
+static void launch_shadow_b(const erd::BatchJob* J, int na, int lpr, int blocks, size_t lds, hipStream_t stream)
+{
+    if (lpr == 4) hipLaunchKernelGGL((erd::k_shadow_b<F, L, 4>), dim3(blocks), dim3(256), lds, stream, J, na);
+    else if (lpr == 16) hipLaunchKernelGGL((erd::k_shadow_b<F, L, 16>), dim3(blocks), dim3(256), lds, stream, J, na);
+    else hipLaunchKernelGGL((erd::k_shadow_b<F, L, 1>), dim3(blocks), dim3(256), lds, stream, J, na);
+}
+
+/* na: the jobs taking part in this round, the first na of the job table (the table is ordered
+ * by pass count, largest first) */
+extern "C" hipError_t erd_launch_batch_trace(const erd::BatchJob* J, int na, int full, int primary_blocks,
+                                             int lds_words, int shade_blocks, hipStream_t stream,
+                                             hipStream_t stream2, hipEvent_t shaded, int lpr, int shadow_lpr)
 {
     /* lds_words > 0: every job's tables fit in that many words of LDS */
     const size_t lds = (size_t)lds_words * 4;
-#define ER_PRIMARY_B(F, L) \
-    hipLaunchKernelGGL((erd::k_primary_b<F, L>), dim3(primary_blocks), dim3(256), lds, stream, J, njobs, round)
+#define ER_PRIMARY_B(F, L) launch_primary_b<F, L>(J, na, lpr, primary_blocks, lds, stream)
     if (full) {
         if (lds_words > 0) ER_PRIMARY_B(1, 1); else ER_PRIMARY_B(1, 0);
-        hipLaunchKernelGGL(erd::k_shade_b<1>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, round);
+        hipLaunchKernelGGL(erd::k_shade_b<1>, dim3(shade_blocks), dim3(256), 0, stream, J, na);
     } else {
         if (lds_words > 0) ER_PRIMARY_B(0, 1); else ER_PRIMARY_B(0, 0);
-        hipLaunchKernelGGL(erd::k_shade_b<0>, dim3(shade_blocks), dim3(256), 0, stream, J, njobs, round);
+        hipLaunchKernelGGL(erd::k_shade_b<0>, dim3(shade_blocks), dim3(256), 0, stream, J, na);
     }
 #undef ER_PRIMARY_B
     hipError_t e = hipEventRecord(shaded, stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(stream2, shaded, 0);
     if (e != hipSuccess) return e;
-#define ER_SHADOW_B(F, L) \
-    hipLaunchKernelGGL((erd::k_shadow_b<F, L>), dim3(primary_blocks), dim3(256), lds, stream2, J, njobs, round)
+#define ER_SHADOW_B(F, L) launch_shadow_b<F, L>(J, na, shadow_lpr, primary_blocks, lds, stream2)
     if (full) { if (lds_words > 0) ER_SHADOW_B(1, 1); else ER_SHADOW_B(1, 0); }
     else { if (lds_words > 0) ER_SHADOW_B(0, 1); else ER_SHADOW_B(0, 0); }
 #undef ER_SHADOW_B
-    hipLaunchKernelGGL(erd::k_finish_b, dim3(shade_blocks), dim3(256), 0, stream2, J, njobs, round);
+    hipLaunchKernelGGL(erd::k_finish_b, dim3(shade_blocks), dim3(256), 0, stream2, J, na);
     return hipGetLastError();
 }
 

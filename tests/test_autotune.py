@@ -207,6 +207,37 @@ def test_rays_per_wave_do_not_change_the_image(fresh, case):
         _same(base, got, f"{rpw} / {srpw} rays per wave")
 
 
+@pytest.mark.parametrize("rpw,srpw", [(16, 16), (4, 4), (1, 64)])
+def test_batch_rays_per_wave_matches_sequential(fresh, rpw, srpw):
+    """A batch whose jobs share a rays-per-wave setting runs its primary and shadow marches with
+    that many lanes per ray (k_primary_b / k_shadow_b); it renders what the same jobs render
+    one after another with one lane per ray."""
+    lib = capi.load()
+    lib.er_set_list_capacity(64)
+    scs = [scenes.config_small(scale=0.25 + 0.125 * k, seed=k + 1) for k in range(3)]
+    seq = []
+    for sc in scs:
+        r = er.Renderer(sc, seed=5)
+        r.render_passes(3)
+        seq.append((r.running().copy(), r.get_estimate().copy(), r.get_seeds()))
+        r.close()
+    rs = []
+    for sc in scs:
+        r = er.Renderer(sc, seed=5)
+        t = r.get_tune()
+        r.set_tune(t["brick"], t["tile_w"], t["blocks_per_cu"], t["shadow_waves_per_cu"], 0, rpw, srpw)
+        rs.append(r)
+    print(f"[tune] batch rays per wave {rpw} shadow {srpw}", flush=True)
+    er.render_batch(rs, [3, 3, 3])
+    for r, (run, disp, (s1, s2)) in zip(rs, seq):
+        assert np.array_equal(r.running(), run)
+        assert np.array_equal(r.get_estimate(), disp)
+        g1, g2 = r.get_seeds()
+        assert np.array_equal(g1, s1) and np.array_equal(g2, s2)
+        r.close()
+    lib.er_set_list_capacity(32)
+
+
 def test_batch_with_tuned_jobs_matches_sequential(fresh):
     """Jobs of one batch with different tiles and bricks (the batch kernels read each job's
     configuration) render what the same jobs render one after another."""
