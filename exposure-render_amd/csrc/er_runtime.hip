@@ -2043,7 +2043,6 @@ ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing)
         const er_render_settings& s = t->desc.settings;
         const int f = s.estimator == ER_ESTIMATOR_FULL;
         if (full >= 0 && f != full) return set_error(ER_ERROR, "all tracers of a batch must use the same estimator mode");
-        if (f && s.sampler == ER_SAMPLER_WOODCOCK) return set_error(ER_ERROR, "the Woodcock sampler is not batched");
         full = f;
         if ((st = build_params(*t))) return st;
         if ((st = sync_tracer(*t))) return st;

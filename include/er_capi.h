@@ -418,9 +418,11 @@ typedef struct er_batch_item {
 /* Renders items[i].passes progressive passes of each tracer - the same result, bit for bit,
  * as er_render_passes on each tracer in turn - with one pass round of every tracer per launch
  * sequence: a persistent primary kernel over the tiles of all tracers still rendering, then
- * shading, shadow-march, combine and post kernels over all of their queues. The round is
- * captured once in a HIP graph and replayed max(passes) times. Each tracer may appear once;
- * all must use the same estimator mode, and none the Woodcock sampler. timing may be NULL
+ * shading, shadow-march, combine and post kernels over all of their queues. All rounds are
+ * captured in one HIP graph, cached for the next call with the same jobs and launch shape.
+ * Each tracer may appear once, and all must use the same estimator mode; samplers may differ
+ * (with a Woodcock job in the batch every march takes one lane per ray). The batch marches
+ * with the jobs' common rays per wave (one lane per ray when they differ). timing may be NULL
  * (trace_ms then covers whole rounds). */
 ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing);
 

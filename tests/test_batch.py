@@ -44,6 +44,37 @@ def test_batch_matches_sequential(fresh, n_jobs):
         r.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_woodcock_batch_matches_sequential(fresh, mixed):
+    """Jobs tracked with Woodcock delta tracking (every job, or every other job beside ray
+    marching ones) render in a batch what they render one after another: the batch kernels
+    defer the Woodcock shadow marches like the single-tracer ones (on their own RNG streams)
+    and then march one lane per ray."""
+    jobs = scenes.config_batch(n_jobs=5, scale=0.1, seed=6)
+    for k, sc in enumerate(jobs):
+        if not mixed or k % 2 == 0:
+            sc.tracer.settings.sampler = capi.SAMPLER_WOODCOCK
+    passes = [min(sc.passes, 6) for sc in jobs]
+    print(f"[batch] woodcock mixed={mixed} films={[sc.film for sc in jobs]} passes={passes}", flush=True)
+    seq = [er.Renderer(sc) for sc in jobs]
+    rs = [er.Renderer(sc) for sc in jobs]
+    seeds = [O.seeds_for(r.W, r.H, 60 + k) for k, r in enumerate(rs)]
+    for r, (s1, s2), n in zip(seq, seeds, passes):
+        r.set_seeds(s1, s2)
+        r.render_passes(n)
+    ref = [state(r) for r in seq]
+    for r, (s1, s2) in zip(rs, seeds):
+        r.set_seeds(s1, s2)
+    er.render_batch(rs, passes)
+    for k, r in enumerate(rs):
+        got = state(r)
+        for name, a, b in zip(("frame", "running", "seeds1", "seeds2", "display"), ref[k][:5], got[:5]):
+            assert np.array_equal(a, b), f"job {k}: {name} differs"
+        assert got[5] == ref[k][5] == passes[k]
+    for r in rs + seq:
+        r.close()
+
+
 def test_batch_rejects_duplicates(fresh):
     sc = scenes.config_batch(n_jobs=1, scale=0.1, seed=4)[0]
     r = er.Renderer(sc)
