@@ -194,7 +194,8 @@ def tune_summary(r):
     t = r.autotune(passes=0)
     return {"brick": t["brick"], "tile": [t["tile_w"], t["tile_h"]], "blocks_per_cu": t["blocks_per_cu"],
             "shadow_waves_per_cu": t["shadow_waves_per_cu"], "heavy_tiles": t["heavy_tiles"],
-            "rays_per_wave": t["rays_per_wave"], "shadow_rays_per_wave": t["shadow_rays_per_wave"], "trials": t["trials"],
+            "heavy_rays_per_wave": t["heavy_rays_per_wave"], "rays_per_wave": t["rays_per_wave"],
+            "shadow_rays_per_wave": t["shadow_rays_per_wave"], "trials": t["trials"],
             "ms_per_pass": round(t["ms_per_pass"], 4), "default_ms_per_pass": round(t["default_ms_per_pass"], 4),
             "tune_s": round(time.perf_counter() - t0, 3)}
 

@@ -572,10 +572,11 @@ __device__ __forceinline__ void group_accumulate(const float (&c)[KB], float& Su
  * march's. Control flow is uniform within a group (it depends only on the group's ray), so the
  * cross-lane reads stay inside active lanes. Returns 0 to go on, 1 on an event (Pout = the
  * sample, m.MinT one step beyond it), -1 when the ray leaves the volume first; counts go to
- * the group's first lane. */
+ * the group's first lane. iters: the iterations the one-lane-per-ray loop would have taken for
+ * this round (1 for a skip, else the samples consumed over ER_BATCH_PRIMARY, rounded up). */
 template<int FULL, int COUNT, int G, bool PRIMARY>
 __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane,
-                                          Cnt& cnt, f3& Pout)
+                                          Cnt& cnt, f3& Pout, int& iters)
 {
     constexpr int KB = GroupBatch<G>::value;
     constexpr int NS = KB * G;
@@ -596,6 +597,7 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
                 cnt.primary += c.primary; cnt.shadow += c.shadow; cnt.iters += c.iters; cnt.skipped += c.skipped;
                 cnt.undone += c.undone;
             }
+            iters = 1;
             return 0;
         }
     }
@@ -644,6 +646,7 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
         }
     }
     const int ns = term == 1 ? send + 1 : send;
+    iters = (ns + ER_BATCH_PRIMARY - 1) / ER_BATCH_PRIMARY;
     if (COUNT) {
         uint32_t fetched = 0;
 #pragma unroll
@@ -676,14 +679,16 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
 }
 
 /* A whole primary march by an aligned group of G lanes, round by round (group_step); steps
- * counts the rounds. */
+ * counts the iterations the one-lane-per-ray loop would have taken, so that tile costs compare
+ * across lane counts. */
 template<int FULL, int COUNT, int G>
 __device__ int group_march(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane, Cnt& cnt, f3& Pout,
                            int& steps)
 {
     for (;;) {
-        ++steps;
-        const int st = group_step<FULL, COUNT, G, true>(p, T, m, StepSize, lane, cnt, Pout);
+        int iters = 0;
+        const int st = group_step<FULL, COUNT, G, true>(p, T, m, StepSize, lane, cnt, Pout, iters);
+        steps += iters;
         if (st != 0) return st;
     }
 }

@@ -39,7 +39,7 @@ extern "C" hipError_t erd_launch_trace(const DevParams* p, float4* frame, uint32
                                        int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
                                        hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary,
                                        const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap,
-                                       int lanes_per_ray, int shadow_lanes_per_ray);
+                                       int lanes_per_ray, int shadow_lanes_per_ray, int heavy_lanes_per_ray);
 extern "C" void erd_pass_record_sizes(size_t* scatter, size_t* event, size_t* job);
 extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, float4* running, uchar4* display,
                                       int no_iterations, unsigned int* qcount, int W, int H, hipStream_t st);
@@ -373,7 +373,7 @@ struct Tracer {
      * volume's default), primary tiles 2^tile_shift pixels wide, persistent primary
      * workgroups per CU (0: the library default) */
     int mc_shift = 0, tile_shift = 3, blocks_per_cu = 0, shadow_waves_per_cu = 4, heavy_tiles = 0, rays_per_wave = 64,
-        shadow_rays_per_wave = 64;
+        shadow_rays_per_wave = 64, heavy_rays_per_wave = 4;
     /* primary tiles: each pass records its tiles' march steps and orders the next pass's tiles
      * most expensive first; the order is valid for the tile shape and count it was built for */
     DevBuf<uint32_t> tile_cost, tile_order;
@@ -1132,7 +1132,8 @@ int enqueue_pass(Tracer& t, hipEvent_t* e)
                              split, t.lds_bytes, lds_bits, shade_blocks, primary_blocks(t), shadow_blocks(t), t.tile_shift, t.stream,
                              t.stream2, S.shaded, e ? e[1] : nullptr, order_valid ? t.tile_order.ptr : nullptr,
                              t.tile_cost.ptr, reorder ? t.tile_order.ptr : nullptr, t.heavy_tiles,
-                             woodcock ? 1 : 64 / t.rays_per_wave, woodcock ? 1 : 64 / t.shadow_rays_per_wave));
+                             woodcock ? 1 : 64 / t.rays_per_wave, woodcock ? 1 : 64 / t.shadow_rays_per_wave,
+                             64 / t.heavy_rays_per_wave));
     t.order_shift = t.tile_shift;
     t.order_tiles = ntiles;
     if (e) HIP_TRY(hipEventRecord(e[2], t.stream2));
@@ -1227,6 +1228,7 @@ void fill_tune(const Tracer& t, er_tune& r)
     r.heavy_tiles = t.heavy_tiles;
     r.rays_per_wave = t.rays_per_wave;
     r.shadow_rays_per_wave = t.shadow_rays_per_wave;
+    r.heavy_rays_per_wave = t.heavy_rays_per_wave;
 }
 
 int apply_tune(Tracer& t, const er_tune& r)
@@ -1245,6 +1247,8 @@ int apply_tune(Tracer& t, const er_tune& r)
         return set_error(ER_ERROR, "rays per wave must be 64, 16, 4 or 1");
     if (r.shadow_rays_per_wave != 64 && r.shadow_rays_per_wave != 16 && r.shadow_rays_per_wave != 4)
         return set_error(ER_ERROR, "shadow rays per wave must be 64, 16 or 4");
+    if (r.heavy_rays_per_wave != 16 && r.heavy_rays_per_wave != 4 && r.heavy_rays_per_wave != 1)
+        return set_error(ER_ERROR, "heavy-tile rays per wave must be 16, 4 or 1");
     const Volume* v = g_volumes.find(t.desc.volume_id);
     if (v && !brick_fits(v->desc, v->vmax, bs))
         return set_error(ER_ERROR, "a %d-cell brick's occupancy bits and opacity table exceed %zu bytes of LDS", r.brick,
@@ -1256,6 +1260,7 @@ int apply_tune(Tracer& t, const er_tune& r)
     t.heavy_tiles = r.heavy_tiles;
     t.rays_per_wave = r.rays_per_wave;
     t.shadow_rays_per_wave = r.shadow_rays_per_wave;
+    t.heavy_rays_per_wave = r.heavy_rays_per_wave;
     t.built_version = 0;
     return ER_OK;
 }
@@ -1334,7 +1339,8 @@ int time_passes(Tracer& t, Snapshot& snap, int passes, float& ms)
 
 /* The candidates of one search axis around `cur`: 0 bricks that fit every volume, 1 tile
  * shapes, 2 workgroups per CU, 3 shadow waves per CU, 4 heavy tiles (with 64 rays per wave
- * only), 5 rays per wave, 6 shadow rays per wave. */
+ * only), 5 rays per wave, 6 shadow rays per wave, 7 heavy-tile rays per wave (with heavy tiles
+ * only). */
 std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::vector<const Volume*>& vols)
 {
     std::vector<er_tune> c;
@@ -1355,8 +1361,12 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
             if (w != cur.shadow_waves_per_cu) { er_tune x = cur; x.shadow_waves_per_cu = w; c.push_back(x); }
     } else if (axis == 4) {
         if (cur.rays_per_wave == 64)
-            for (int h : {0, 256})
+            for (int h : {0, 256, 768})
                 if (h != cur.heavy_tiles) { er_tune x = cur; x.heavy_tiles = h; c.push_back(x); }
+    } else if (axis == 7) {
+        if (cur.rays_per_wave == 64 && cur.heavy_tiles > 0)
+            for (int r : {16, 4, 1})
+                if (r != cur.heavy_rays_per_wave) { er_tune x = cur; x.heavy_rays_per_wave = r; c.push_back(x); }
     } else if (axis == 5) {
         for (int r : {16, 4, 1})
             if (r != cur.rays_per_wave) { er_tune x = cur; x.rays_per_wave = r; c.push_back(x); }
@@ -1690,7 +1700,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     Snapshot snap;
     if ((st = snap.save(t))) return st;
     const int mc0 = t.mc_shift, ts0 = t.tile_shift, bpc0 = t.blocks_per_cu, sw0 = t.shadow_waves_per_cu,
-              ht0 = t.heavy_tiles, rw0 = t.rays_per_wave, srw0 = t.shadow_rays_per_wave;
+              ht0 = t.heavy_tiles, rw0 = t.rays_per_wave, srw0 = t.shadow_rays_per_wave, hrw0 = t.heavy_rays_per_wave;
     er_tune cur{};
     fill_tune(t, cur);
     int trials = 0;
@@ -1718,7 +1728,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     const float default_ms = cur_ms;
     std::vector<const Volume*> vols{g_volumes.find(t.desc.volume_id)};
     /* rays per wave right after the brick: the later axes are tuned around them */
-    for (int axis : {0, 5, 6, 1, 2, 3, 4}) {
+    for (int axis : {0, 5, 6, 1, 2, 3, 4, 7}) {
         if (st) break;
         for (const er_tune& c : tune_candidates(axis, cur, vols)) {
             float ms = 0.0f;
@@ -1729,7 +1739,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     if (!st) st = apply_tune(t, cur);
     if (st) {
         t.mc_shift = mc0; t.tile_shift = ts0; t.blocks_per_cu = bpc0; t.shadow_waves_per_cu = sw0; t.heavy_tiles = ht0;
-        t.rays_per_wave = rw0; t.shadow_rays_per_wave = srw0;
+        t.rays_per_wave = rw0; t.shadow_rays_per_wave = srw0; t.heavy_rays_per_wave = hrw0;
         t.built_version = 0;
     }
     const int rst = snap.restore(t);

@@ -317,7 +317,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ER_PRIMARY_
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
                                                 ScatterRec* __restrict__ queue, unsigned int* __restrict__ counts,
                                                 unsigned long long* __restrict__ counters, int tiles_x, int ntiles,
-                                                const uint32_t* __restrict__ order, uint32_t* __restrict__ cost)
+                                                const uint32_t* __restrict__ order, uint32_t* __restrict__ cost,
+                                                int heavy_lanes)
 {
     extern __shared__ uint32_t lds[];
     const DevParams& p = *pp;
@@ -325,11 +326,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ER_PRIMARY_
     const int lane = threadIdx.x & 63;
     Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int paths = 0;
-    /* claims [0, 64 hc): the rays of the hc heaviest tiles one by one (order[0 .. hc), 64 claims
-     * each, every ray marched by a whole wave); then the other tiles in their order */
+    /* claims [0, HL hc): the hc heaviest tiles (order[0 .. hc)) in HL claims each, HL =
+     * heavy_lanes lanes per ray (64: every ray of the tile marched by a whole wave; 4 or 16:
+     * sub-tiles of 64 / HL rays marched by groups of HL lanes); then the other tiles in their
+     * order */
+    const unsigned int HL = heavy_lanes == 4 || heavy_lanes == 16 ? (unsigned int)heavy_lanes : 64u;
     unsigned int hc = order ? min((unsigned int)__builtin_amdgcn_readfirstlane(order[ntiles]), (unsigned int)ntiles) : 0u;
     if ((FULL && p.sampler == ER_SAMPLER_WOODCOCK) || LPR > 1) hc = 0u;
-    const unsigned int nclaims = LPR > 1 ? (unsigned int)ntiles * (unsigned int)LPR : hc * 64u + ((unsigned int)ntiles - hc);
+    const unsigned int nclaims = LPR > 1 ? (unsigned int)ntiles * (unsigned int)LPR : hc * HL + ((unsigned int)ntiles - hc);
     for (;;) {
         unsigned int claim = 0;
         if (lane == 0) claim = atomicAdd(&counts[2], 1u);
@@ -344,12 +348,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ER_PRIMARY_
             if (cost && lane == 0 && sub == 0u) cost[tile] = (uint32_t)steps;
             continue;
         }
-        const bool heavy = claim < hc * 64u;
-        const unsigned int idx = heavy ? (claim >> 6) : hc + (claim - hc * 64u);
+        const bool heavy = claim < hc * HL;
+        const unsigned int idx = heavy ? claim / HL : hc + (claim - hc * HL);
         const unsigned int tile = order ? __builtin_amdgcn_readfirstlane(order[idx]) : idx;
         int steps = 0;
-        primary_tile<FULL, COUNT>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths, steps,
-                                  heavy ? (int)(claim & 63u) : -1);
+        if (heavy && HL == 4u)
+            primary_tile<FULL, COUNT, 4>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths,
+                                         steps, (int)(claim % HL));
+        else if (heavy && HL == 16u)
+            primary_tile<FULL, COUNT, 16>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths,
+                                          steps, (int)(claim % HL));
+        else
+            primary_tile<FULL, COUNT>(p, T, tile, tiles_x, lane, frame, seeds1, seeds2, queue, &counts[0], cnt, paths, steps,
+                                      heavy ? (int)(claim % HL) : -1);
         if (cost && lane == 0) {
             if (heavy) atomicMax(&cost[tile], (uint32_t)steps);
             else cost[tile] = (uint32_t)steps;
@@ -564,7 +575,8 @@ __device__ __forceinline__ void shadow_marches_g(const DevParams& p, const Table
         }
         if (busy) {
             f3 P;
-            const int st = group_step<FULL, COUNT, G, false>(p, T, m, StepSize, lane, cnt, P);
+            int iters = 0;
+            const int st = group_step<FULL, COUNT, G, false>(p, T, m, StepSize, lane, cnt, P, iters);
             if (st != 0) {
                 if (first) vis[slot] = st > 0 ? 0 : 1;
                 busy = false;
@@ -917,10 +929,10 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
                        unsigned long long* counters, int tiles_x, int ntiles, int nblocks, int shade_blocks,
                        int shadow_blocks, int split, size_t lds, size_t lds_bits, hipStream_t st, hipStream_t st2,
                        hipEvent_t shaded, hipEvent_t after_primary, const uint32_t* order_in, uint32_t* cost,
-                       uint32_t* order_out, int heavy_cap, int lpr, int shadow_lpr)
+                       uint32_t* order_out, int heavy_cap, int lpr, int shadow_lpr, int heavy_lanes)
 {
 #define ER_PRIMARY(G) hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS, G>), dim3(nblocks), dim3(256), lds, st, p, \
-                                         frame, s1, s2, queue, counts, counters, tiles_x, ntiles, order_in, cost)
+                                         frame, s1, s2, queue, counts, counters, tiles_x, ntiles, order_in, cost, heavy_lanes)
     if (lpr == 4) ER_PRIMARY(4);
     else if (lpr == 16) ER_PRIMARY(16);
     else if (lpr == 64) ER_PRIMARY(64);
@@ -961,7 +973,7 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
                                        int primary_blocks, int shadow_blocks, int tile_shift, hipStream_t st,
                                        hipStream_t st2, hipEvent_t shaded, hipEvent_t after_primary,
                                        const uint32_t* order_in, uint32_t* cost, uint32_t* order_out, int heavy_cap,
-                                       int lanes_per_ray, int shadow_lanes_per_ray)
+                                       int lanes_per_ray, int shadow_lanes_per_ray, int heavy_lanes_per_ray)
 {
     const int tw = 1 << tile_shift, th = 64 >> tile_shift;
     const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
@@ -976,7 +988,8 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
 #define ER_ALL(F, C, L) launch_all<F, C, L>(p, frame, s1, s2, q, counts, ev, jb, vis, counters, tiles_x, ntiles, nblocks, \
                                             shade_blocks, std::max(1, shadow_blocks), split, lds_bytes, lds_bits, st, st2, \
                                             shaded, after_primary, order_in, cost, order_out, heavy_cap, lpr, \
-                                            (shadow_lanes_per_ray == 4 || shadow_lanes_per_ray == 16) ? shadow_lanes_per_ray : 1)
+                                            (shadow_lanes_per_ray == 4 || shadow_lanes_per_ray == 16) ? shadow_lanes_per_ray : 1, \
+                                            heavy_lanes_per_ray)
     if (full) {
         if (count) { if (lut_lds) ER_ALL(1, 1, 1); else ER_ALL(1, 1, 0); }
         else { if (lut_lds) ER_ALL(1, 0, 1); else ER_ALL(1, 0, 0); }
@@ -1064,10 +1077,10 @@ extern "C" hipError_t erd_launch_primary_shade(const erd::DevParams* p, float4* 
     erd::ScatterRec* q = (erd::ScatterRec*)queue;
     if (lut_lds)
         hipLaunchKernelGGL((erd::k_primary<1, 0, 1, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
-                           (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr);
+                           (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr, 64);
     else
         hipLaunchKernelGGL((erd::k_primary<1, 0, 0, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
-                           (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr);
+                           (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr, 64);
     hipLaunchKernelGGL((erd::k_shade<1, 0, 1, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1, s2,
                        (const erd::ScatterRec*)q, counts, (erd::EventRec*)events, (erd::ShadowJob*)jobs,
                        (unsigned long long*)nullptr);

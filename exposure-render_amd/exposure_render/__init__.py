@@ -140,10 +140,10 @@ class Renderer:
         return t.as_dict()
 
     def set_tune(self, brick, tile_w, blocks_per_cu, shadow_waves_per_cu=4, heavy_tiles=0, rays_per_wave=64,
-                 shadow_rays_per_wave=64):
+                 shadow_rays_per_wave=64, heavy_rays_per_wave=4):
         t = capi.Tune(brick=brick, tile_w=tile_w, tile_h=64 // tile_w, blocks_per_cu=blocks_per_cu,
                       shadow_waves_per_cu=shadow_waves_per_cu, heavy_tiles=heavy_tiles, rays_per_wave=rays_per_wave,
-                      shadow_rays_per_wave=shadow_rays_per_wave)
+                      shadow_rays_per_wave=shadow_rays_per_wave, heavy_rays_per_wave=heavy_rays_per_wave)
         _check(self.lib, self.lib.er_set_tune(self.id, C.byref(t)))
 
     def get_tune(self):

@@ -169,7 +169,7 @@ class BatchItem(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("brick", C.c_int), ("tile_w", C.c_int), ("tile_h", C.c_int), ("blocks_per_cu", C.c_int),
-                ("shadow_waves_per_cu", C.c_int), ("heavy_tiles", C.c_int), ("rays_per_wave", C.c_int), ("shadow_rays_per_wave", C.c_int), ("ms_per_pass", C.c_float), ("default_ms_per_pass", C.c_float), ("trials", C.c_int)]
+                ("shadow_waves_per_cu", C.c_int), ("heavy_tiles", C.c_int), ("rays_per_wave", C.c_int), ("shadow_rays_per_wave", C.c_int), ("heavy_rays_per_wave", C.c_int), ("ms_per_pass", C.c_float), ("default_ms_per_pass", C.c_float), ("trials", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -379,7 +379,8 @@ typedef struct er_tune {
     int   blocks_per_cu;          /* persistent 256-lane primary workgroups per CU */
     int   shadow_waves_per_cu;    /* persistent 64-lane shadow-march waves per CU (default 4) */
     int   heavy_tiles;            /* most expensive tiles of a pass whose rays the next pass marches
-                                     one whole wave per ray (default 0; with 64 rays per wave only) */
+                                     with several lanes per ray (heavy_rays_per_wave; default 0,
+                                     with 64 rays per wave only) */
     int   rays_per_wave;          /* primary rays one 64-lane wave marches at once: 64 (one lane per
                                      ray, default), 16 or 4 (groups of 4 or 16 lanes per ray, each
                                      round trip reading 24 or 32 consecutive samples of the ray), 1
@@ -389,6 +390,9 @@ typedef struct er_tune {
     int   shadow_rays_per_wave;   /* shadow rays one wave marches at once: 64 (one lane per ray,
                                      default), 16 or 4 (groups of 4 or 16 lanes per ray); Woodcock
                                      tracking always takes 64 */
+    int   heavy_rays_per_wave;    /* rays of a heavy tile one wave marches at once: 4 (groups of
+                                     16 lanes per ray, default), 16 (groups of 4 lanes) or 1 (the
+                                     whole wave per ray) */
     float ms_per_pass;            /* er_autotune: the chosen configuration's time per pass */
     float default_ms_per_pass;    /* er_autotune: the starting configuration's time per pass */
     int   trials;                 /* er_autotune: configurations timed (0: taken from the cache) */
@@ -398,7 +402,7 @@ typedef struct er_tune {
 
 /* Times `passes` passes (default 6) of each candidate from the tracer's current seeds and
  * keeps the fastest: bricks, then rays per wave, then shadow rays per wave, then tile shapes,
- * then workgroups per CU, then shadow waves per CU, then heavy tiles. The tracer's seeds,
+ * then workgroups per CU, then shadow waves per CU, then heavy tiles and their rays per wave. The tracer's seeds,
  * running estimate, display, counters and pass count are restored afterwards. Results are
  * cached per scene (descriptor, node table and volume contents). */
 ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out);

@@ -117,7 +117,7 @@ def test_set_tune_rejects_invalid_configurations(fresh):
     r = er.Renderer(sc, seed=1)
     for bad in [(3, 8, 7), (8, 6, 7), (8, 64, 7), (8, 8, 0), (8, 8, 17), (128, 8, 7), (8, 8, 7, 0), (8, 8, 7, 65),
                 (8, 8, 7, 4, -1), (8, 8, 7, 4, 4097), (8, 8, 7, 4, 0, 0), (8, 8, 7, 4, 0, 8), (8, 8, 7, 4, 0, 128), (8, 8, 7, 4, 0, 64, 1),
-                (8, 8, 7, 4, 0, 64, 32)]:
+                (8, 8, 7, 4, 0, 64, 32), (8, 8, 7, 4, 16, 64, 64, 64), (8, 8, 7, 4, 16, 64, 64, 2)]:
         with pytest.raises(er.ExposureRenderError):
             r.set_tune(*bad)
     r.close()
@@ -149,15 +149,16 @@ def test_shadow_grid_does_not_change_the_image(fresh):
 @pytest.mark.parametrize("scale", [0.5, 1.0])
 def test_tile_order_and_heavy_rays_do_not_change_the_image(fresh, scale):
     """From its second pass on, a pass claims its tiles most expensive first (by the previous
-    pass's costs), and with heavy_tiles > 0 marches the rays of the most expensive tiles one
-    whole wave per ray (coop_march). Neither changes any pixel, RNG state or sample count."""
+    pass's costs), and with heavy_tiles > 0 marches the rays of the most expensive tiles with
+    16 or 4 lanes per ray (group_march) or one whole wave per ray (coop_march). None of it
+    changes any pixel, RNG state or sample count."""
     sc = scenes.config_ct(scale=scale)
     out = []
-    for heavy in (0, 16, 256, 4096):
-        print(f"[tune] heavy tiles {heavy} scale {scale}", flush=True)
+    for heavy, hrpw in ((0, 4), (16, 4), (256, 16), (256, 1), (4096, 4)):
+        print(f"[tune] heavy tiles {heavy} rays per wave {hrpw} scale {scale}", flush=True)
         r = er.Renderer(sc, seed=9)
-        r.set_tune(8, 8, 7, 4, heavy)
-        assert r.get_tune()["heavy_tiles"] == heavy
+        r.set_tune(8, 8, 7, 4, heavy, 64, 64, hrpw)
+        assert r.get_tune()["heavy_tiles"] == heavy and r.get_tune()["heavy_rays_per_wave"] == hrpw
         r.set_counting(True)
         r.reset_counters()
         r.render_passes(4)
