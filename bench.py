@@ -505,7 +505,8 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profile(os.path.join(ROOT, "profiles", "k_primary_ct_traffic.json")),
-                    "kernel": "k_primary<1,0,1>", "avg_launch_ms": round(avg_ms, 4),
+                    "kernel": "k_primary<1,0,1,%d>" % (64 // tune["rays_per_wave"] if tune else 1),
+                    "avg_launch_ms": round(avg_ms, 4),
                     "bytes_per_launch": int(per_launch),
                     "units_per_launch": {"fetched_samples": round(counts["primary_fetches"] / args.steps),
                                          "paths": W * H}}

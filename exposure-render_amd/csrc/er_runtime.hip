@@ -1727,14 +1727,22 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     if (!st) st = measure(cur, cur_ms);
     const float default_ms = cur_ms;
     std::vector<const Volume*> vols{g_volumes.find(t.desc.volume_id)};
-    /* rays per wave right after the brick: the later axes are tuned around them */
-    for (int axis : {0, 5, 6, 1, 2, 3, 4, 7}) {
-        if (st) break;
-        for (const er_tune& c : tune_candidates(axis, cur, vols)) {
-            float ms = 0.0f;
-            if ((st = measure(c, ms))) break;
-            if (ms < ER_TUNE_MARGIN * cur_ms) { cur = c; cur_ms = ms; }
+    /* rays per wave right after the brick: the later axes are tuned around them. A second sweep,
+     * from a fresh measurement of the incumbent, catches axes that interact (the primary
+     * workgroups and the shadow waves share the CUs); it is skipped when the first moved
+     * nothing. */
+    for (int sweep = 0; sweep < 2 && !st; ++sweep) {
+        bool moved = false;
+        if (sweep > 0 && (st = measure(cur, cur_ms))) break;
+        for (int axis : {0, 5, 6, 1, 2, 3, 4, 7}) {
+            if (st) break;
+            for (const er_tune& c : tune_candidates(axis, cur, vols)) {
+                float ms = 0.0f;
+                if ((st = measure(c, ms))) break;
+                if (ms < ER_TUNE_MARGIN * cur_ms) { cur = c; cur_ms = ms; moved = true; }
+            }
         }
+        if (!moved) break;
     }
     if (!st) st = apply_tune(t, cur);
     if (st) {

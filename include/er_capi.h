@@ -400,15 +400,14 @@ typedef struct er_tune {
 
 #define ER_TUNE_FORCE 1           /* er_autotune flag: measure even if the scene is cached */
 
-/* Times `passes` passes (default 6) of each candidate from the tracer's current seeds and
- * keeps the fastest: bricks, then rays per wave, then shadow rays per wave, then tile shapes,
- * then workgroups per CU, then shadow waves per CU, then heavy tiles and their rays per wave. The tracer's seeds,
- * running estimate, display, counters and pass count are restored afterwards. Results are
- * cached per scene (descriptor, node table and volume contents). */
+/* Times `passes` passes of each candidate (passes <= 0: as many as make a trial of about
+ * 30 ms, from one probe) from the tracer's current seeds, best of three, and keeps a candidate
+ * that is at least 3% faster: bricks, then rays per wave, then shadow rays per wave, then tile
+ * shapes, then workgroups per CU, then shadow waves per CU, then heavy tiles and their rays
+ * per wave; a second sweep over the same axes follows when the first changed anything. The
+ * tracer's seeds, running estimate, display, counters and pass count are restored afterwards.
+ * Results are cached per scene (descriptor, node table and volume contents). */
 ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out);
-/* One configuration for a batch (er_render_batch): the same search with each candidate applied
- * to every job and timed as `rounds` batch rounds (default 4); every job's render state is
- * restored and the jobs keep the chosen configuration. */
 ER_API int er_set_tune(int tracer_id, const er_tune* tune);
 ER_API int er_get_tune(int tracer_id, er_tune* tune);
 
@@ -429,6 +428,11 @@ typedef struct er_batch_item {
  * with the jobs' common rays per wave (one lane per ray when they differ). timing may be NULL
  * (trace_ms then covers whole rounds). */
 ER_API int er_render_batch(const er_batch_item* items, int n, er_timing* timing);
+/* One configuration for a batch: the er_autotune search over bricks, rays per wave, shadow
+ * rays per wave, tile shapes and workgroups per CU, each candidate applied to every job (a
+ * brick only where it fits the job's volume) and timed as `rounds` batch rounds (default 4);
+ * every job's render state is restored and the jobs keep the chosen configuration. */
+ER_API int er_autotune_batch(const er_batch_item* items, int n, int rounds, er_tune* out);
 
 /* ---- differentiable transfer function (new; no counterpart in the reference) ----------- */
 
