@@ -347,14 +347,21 @@ __device__ __forceinline__ float trilinear_cell(const DevParams& p, const Lattic
 __device__ __forceinline__ float intensity(const DevParams& p, f3 P) { return trilinear_cell(p, to_lattice(p, P)); }
 
 /* GradientCD / FD / Filtered (volumes.h:31-74); Normalize (Q20: zero gradient -> NaN) */
-__device__ __forceinline__ f3 grad_cd(const DevParams& p, f3 P)
+/* the six central-difference samples of GradientCD (volumes.h:31-41): I(P + dX), I(P - dX),
+ * I(P + dY), I(P - dY), I(P + dZ), I(P - dZ), d = MinStep on every axis (Q23) */
+__device__ __forceinline__ void cd_samples(const DevParams& p, f3 P, float g[6])
 {
     const float d = p.minstep;
     const f3 dX = mk(d, 0.0f, 0.0f), dY = mk(0.0f, d, 0.0f), dZ = mk(0.0f, 0.0f, d);
-    const float i00 = intensity(p, add(P, dX)), i01 = intensity(p, sub(P, dX));
-    const float i10 = intensity(p, add(P, dY)), i11 = intensity(p, sub(P, dY));
-    const float i20 = intensity(p, add(P, dZ)), i21 = intensity(p, sub(P, dZ));
-    return mk(i01 - i00, i11 - i10, i21 - i20);
+    g[0] = intensity(p, add(P, dX)); g[1] = intensity(p, sub(P, dX));
+    g[2] = intensity(p, add(P, dY)); g[3] = intensity(p, sub(P, dY));
+    g[4] = intensity(p, add(P, dZ)); g[5] = intensity(p, sub(P, dZ));
+}
+__device__ __forceinline__ f3 grad_cd(const DevParams& p, f3 P)
+{
+    float g[6];
+    cd_samples(p, P, g);
+    return mk(g[1] - g[0], g[3] - g[2], g[5] - g[4]);
 }
 __device__ __forceinline__ f3 grad_fd(const DevParams& p, f3 P)
 {
@@ -391,6 +398,35 @@ __device__ __forceinline__ f3 normalized_gradient(const DevParams& p, f3 P)
     if (p.gradient_mode == ER_GRADIENT_CENTRAL) return nrm(grad_cd(p, P));
     if (p.gradient_mode == ER_GRADIENT_FILTERED) return nrm(grad_filtered(p, P));
     return nrm(grad_fd(p, P));
+}
+/* the six central-difference samples of an event, carried by value into the shading (valid:
+ * taken at the event's point) */
+struct CD6 {
+    float g[6];
+    bool valid;
+};
+
+/* GradientMagnitude (volumes.h:93-114) from the six central-difference samples at its point
+ * (cd_samples: the same sample positions as GradientCD's) */
+__device__ __forceinline__ float gradient_magnitude_cd(const DevParams& p, const float g[6])
+{
+    float Sum = 0.0f;
+    {
+        float D = g[1] - g[0];
+        D *= 0.5f / p.spacing[0];
+        Sum += D * D;
+    }
+    {
+        float D = g[3] - g[2];
+        D *= 0.5f / p.spacing[1];
+        Sum += D * D;
+    }
+    {
+        float D = g[5] - g[4];
+        D *= 0.5f / p.spacing[2];
+        Sum += D * D;
+    }
+    return sqrtf(Sum);
 }
 /* GradientMagnitude (volumes.h:93-114) */
 __device__ __forceinline__ float gradient_magnitude(const DevParams& p, f3 P)

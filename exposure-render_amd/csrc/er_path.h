@@ -1259,7 +1259,7 @@ __device__ f3 estimate_direct(const DevParams& p, const Tables& T, const DevLigh
  * glossiness (its Y) from the object's textures at (U, V). */
 template<int FULL, int COUNT>
 __device__ __forceinline__ void event_shader(const DevParams& p, float I, int obj, float U, float V, f3 SEP, f3 SEN, f3 Wo,
-                                             Rng& rng, Cnt& cnt, Shader& sh)
+                                             Rng& rng, Cnt& cnt, Shader& sh, const CD6& cd = CD6{{}, false})
 {
     bool brdf = true;
     if (obj >= 0) {
@@ -1273,7 +1273,9 @@ __device__ __forceinline__ void event_shader(const DevParams& p, float I, int ob
             if (type == ER_SHADING_PHASE_ONLY) {
                 brdf = false;
             } else if (type != ER_SHADING_BRDF_ONLY) {
-                const float gm = gradient_magnitude(p, SEP);
+                /* cd: the event's central-difference samples when the caller has them (the same
+                 * positions as GradientMagnitude's), so no sample is fetched twice */
+                const float gm = cd.valid ? gradient_magnitude_cd(p, cd.g) : gradient_magnitude(p, SEP);
                 if (COUNT) cnt.shading += 6;
                 const float NGM = gm * p.gm_inv;
                 if (type == ER_SHADING_THRESHOLD) {
@@ -1337,7 +1339,7 @@ struct EventRec {
 template<int FULL, int COUNT>
 __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt,
                             EventRec& ev, ShadowJob& job0, ShadowJob& job1, bool& has0, bool& has1, int obj = -1,
-                            float U = 0.0f, float V = 0.0f)
+                            float U = 0.0f, float V = 0.0f, CD6 cd = CD6{{}, false})
 {
     f3 Ld = mk(0.0f, 0.0f, 0.0f);
     const float I = intensity(p, SEP);
@@ -1357,7 +1359,7 @@ __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SE
     const DevLight& L = *Lp;
     ev.count = (float)count;
     Shader sh;
-    event_shader<FULL, COUNT>(p, I, obj, U, V, SEP, SEN, Wo, rng, cnt, sh);
+    event_shader<FULL, COUNT>(p, I, obj, U, V, SEP, SEN, Wo, rng, cnt, sh, cd);
 
     /* light-sampling half */
     f3 SP, SN; float su, sv;

@@ -247,13 +247,26 @@ __device__ __forceinline__ void shade_chunk(const DevParams& p, const Tables& T,
         const f3 Wo = mk(rec.Wox, rec.Woy, rec.Woz);
         /* SE.N: the normalised gradient of a volume event (raymarching.h:70), the object's
          * transformed shape normal of an object event (objects.h:36) */
-        const f3 Nv = rec.obj >= 0 ? mk(rec.Nx, rec.Ny, rec.Nz) : normalized_gradient(p, Pv);
+        /* a volume event with central differences keeps its six gradient samples for the
+         * gradient magnitude of the shading-type choice (the same positions) */
+        CD6 cd;
+        cd.valid = rec.obj < 0 && p.gradient_mode == ER_GRADIENT_CENTRAL;
+        f3 Nv;
+        if (rec.obj >= 0) {
+            Nv = mk(rec.Nx, rec.Ny, rec.Nz);
+        } else if (cd.valid) {
+            cd_samples(p, Pv, cd.g);
+            Nv = nrm(mk(cd.g[1] - cd.g[0], cd.g[3] - cd.g[2], cd.g[5] - cd.g[4]));
+        } else {
+            Nv = normalized_gradient(p, Pv);
+        }
         if (SPLIT) {
             EventRec ev;
             ev.pixel = rec.pixel;
             ev.c1x = ev.c1y = ev.c1z = ev.c2x = ev.c2y = ev.c2z = 0.0f;
             ev.pad = ev.pad2 = 0.0f;
-            shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1, rec.obj, rec.U, rec.V);
+            shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1, rec.obj, rec.U, rec.V,
+                                     cd);
             events[q] = ev;
         } else {
             f3 Lv = mk(0.0f, 0.0f, 0.0f);
