@@ -1338,11 +1338,12 @@ struct EventRec {
 
 template<int FULL, int COUNT>
 __device__ void shade_split(const DevParams& p, const Sample& smp, f3 SEP, f3 SEN, f3 Wo, Rng& rng, Cnt& cnt,
-                            EventRec& ev, ShadowJob& job0, ShadowJob& job1, bool& has0, bool& has1, int obj = -1,
-                            float U = 0.0f, float V = 0.0f, CD6 cd = CD6{{}, false})
+                            EventRec& ev, ShadowJob& job0, ShadowJob& job1, bool& has0, bool& has1, int obj,
+                            float U, float V, float I, CD6 cd)
 {
+    /* I: the intensity at SEP (GetIntensity, transport.h:119), fetched by the caller together
+     * with the gradient samples */
     f3 Ld = mk(0.0f, 0.0f, 0.0f);
-    const float I = intensity(p, SEP);
     if (COUNT) cnt.shading += 1 + gradient_samples(p);
     Ld = add(Ld, ctf_eval(p.emission, I));
     ev.Ld0x = Ld.x; ev.Ld0y = Ld.y; ev.Ld0z = Ld.z;

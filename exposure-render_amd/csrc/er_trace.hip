@@ -251,6 +251,7 @@ __device__ __forceinline__ void shade_chunk(const DevParams& p, const Tables& T,
          * gradient magnitude of the shading-type choice (the same positions) */
         CD6 cd;
         cd.valid = rec.obj < 0 && p.gradient_mode == ER_GRADIENT_CENTRAL;
+        const float Iv = intensity(p, Pv);   /* the emission's and shader's intensity, loaded with the gradient */
         f3 Nv;
         if (rec.obj >= 0) {
             Nv = mk(rec.Nx, rec.Ny, rec.Nz);
@@ -266,7 +267,7 @@ __device__ __forceinline__ void shade_chunk(const DevParams& p, const Tables& T,
             ev.c1x = ev.c1y = ev.c1z = ev.c2x = ev.c2y = ev.c2z = 0.0f;
             ev.pad = ev.pad2 = 0.0f;
             shade_split<FULL, COUNT>(p, smp, Pv, Nv, Wo, rng, cnt, ev, job0, job1, has0, has1, rec.obj, rec.U, rec.V,
-                                     cd);
+                                     Iv, cd);
             events[q] = ev;
         } else {
             f3 Lv = mk(0.0f, 0.0f, 0.0f);
