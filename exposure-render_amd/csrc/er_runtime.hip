@@ -322,7 +322,8 @@ bool brick_fits(const er_volume_desc& d, uint16_t vmax, int shift)
     const size_t n = mc[0] * mc[1] * mc[2];
     const size_t g = ((mc[0] + G - 1) / G) * ((mc[1] + G - 1) / G) * ((mc[2] + G - 1) / G);
     const size_t lut = ((size_t)vmax + 1) * sizeof(float);
-    return 4 * ((n + 31) / 32 + (g + 31) / 32) + (lut <= 32768 ? lut : 0) <= ER_MAX_TABLES_BYTES;
+    /* macrocell bits, group occupancy bits and group clip-free bits, as build_params lays them out */
+    return 4 * ((n + 31) / 32 + 2 * ((g + 31) / 32)) + (lut <= 32768 ? lut : 0) <= ER_MAX_TABLES_BYTES;
 }
 
 /* The state of one pass in flight: its frame estimate and its work queues. A tracer keeps
