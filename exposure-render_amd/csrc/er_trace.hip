@@ -326,8 +326,12 @@ __device__ __forceinline__ void finish_item(const EventRec& ev, unsigned int q, 
 #ifndef ER_PRIMARY_WAVES_PER_EU
 #define ER_PRIMARY_WAVES_PER_EU 1
 #endif
+/* lanes per primary workgroup (the waves of a workgroup share one LDS copy of the tables) */
+#ifndef ER_PRIMARY_WG
+#define ER_PRIMARY_WG 256
+#endif
 template<int FULL, int COUNT, int LUT_LDS, int LPR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ER_PRIMARY_WAVES_PER_EU))) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
+__global__ __launch_bounds__(ER_PRIMARY_WG) __attribute__((amdgpu_waves_per_eu(ER_PRIMARY_WAVES_PER_EU))) void k_primary(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                                 uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
                                                 ScatterRec* __restrict__ queue, unsigned int* __restrict__ counts,
                                                 unsigned long long* __restrict__ counters, int tiles_x, int ntiles,
@@ -945,7 +949,7 @@ static void launch_all(const erd::DevParams* p, float4* frame, uint32_t* s1, uin
                        hipEvent_t shaded, hipEvent_t after_primary, const uint32_t* order_in, uint32_t* cost,
                        uint32_t* order_out, int heavy_cap, int lpr, int shadow_lpr, int heavy_lanes)
 {
-#define ER_PRIMARY(G) hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS, G>), dim3(nblocks), dim3(256), lds, st, p, \
+#define ER_PRIMARY(G) hipLaunchKernelGGL((erd::k_primary<FULL, COUNT, LUT_LDS, G>), dim3(nblocks), dim3(ER_PRIMARY_WG), lds, st, p, \
                                          frame, s1, s2, queue, counts, counters, tiles_x, ntiles, order_in, cost, heavy_lanes)
     if (lpr == 4) ER_PRIMARY(4);
     else if (lpr == 16) ER_PRIMARY(16);
@@ -995,7 +999,8 @@ extern "C" hipError_t erd_launch_trace(const erd::DevParams* p, float4* frame, u
     const int lpr = (lanes_per_ray == 4 || lanes_per_ray == 16 || lanes_per_ray == 64) ? lanes_per_ray : 1;
     /* persistent workgroups: enough to fill every CU, never more than there are claims */
     const long long nclaims = (long long)ntiles * lpr;
-    const int nblocks = (int)std::max(1LL, std::min((nclaims + 3) / 4, (long long)primary_blocks));
+    constexpr long long wpb = ER_PRIMARY_WG / 64;   /* waves per workgroup */
+    const int nblocks = (int)std::max(1LL, std::min((nclaims + wpb - 1) / wpb, (long long)primary_blocks));
     erd::ScatterRec* q = (erd::ScatterRec*)queue;
     erd::EventRec* ev = (erd::EventRec*)events;
     erd::ShadowJob* jb = (erd::ShadowJob*)jobs;
@@ -1087,13 +1092,14 @@ extern "C" hipError_t erd_launch_primary_shade(const erd::DevParams* p, float4* 
     const int tw = 1 << tile_shift, th = 64 >> tile_shift;
     const int tiles_x = (W + tw - 1) / tw, tiles_y = (H + th - 1) / th;
     const int ntiles = tiles_x * tiles_y;
-    const int nblocks = std::max(1, std::min((ntiles + 3) / 4, primary_blocks));
+    constexpr int wpb = ER_PRIMARY_WG / 64;   /* waves per workgroup */
+    const int nblocks = std::max(1, std::min((ntiles + wpb - 1) / wpb, primary_blocks));
     erd::ScatterRec* q = (erd::ScatterRec*)queue;
     if (lut_lds)
-        hipLaunchKernelGGL((erd::k_primary<1, 0, 1, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
+        hipLaunchKernelGGL((erd::k_primary<1, 0, 1, 1>), dim3(nblocks), dim3(ER_PRIMARY_WG), lds_bytes, st, p, frame, s1, s2, q, counts,
                            (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr, 64);
     else
-        hipLaunchKernelGGL((erd::k_primary<1, 0, 0, 1>), dim3(nblocks), dim3(256), lds_bytes, st, p, frame, s1, s2, q, counts,
+        hipLaunchKernelGGL((erd::k_primary<1, 0, 0, 1>), dim3(nblocks), dim3(ER_PRIMARY_WG), lds_bytes, st, p, frame, s1, s2, q, counts,
                            (unsigned long long*)nullptr, tiles_x, ntiles, (const uint32_t*)nullptr, (uint32_t*)nullptr, 64);
     hipLaunchKernelGGL((erd::k_shade<1, 0, 1, 1>), dim3(shade_blocks), dim3(256), lds_bits, st, p, frame, s1, s2,
                        (const erd::ScatterRec*)q, counts, (erd::EventRec*)events, (erd::ShadowJob*)jobs,
