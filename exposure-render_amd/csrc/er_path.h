@@ -394,6 +394,7 @@ __device__ __forceinline__ bool woodcock_track(const DevParams& p, const Tables&
         float mu = 0.0f;
         for (;;) {
             if (++segments > ER_WC_MAX_SEGMENTS || !(t < MaxT)) return false;
+            if (COUNT) cnt.iters++;
             const Lattice l = to_lattice(p, add(r.O, scl(t, r.D)));
             const int mx = mc_axis(l.Lx, ER_WC_SHIFT, dx), my = mc_axis(l.Ly, ER_WC_SHIFT, dy), mz = mc_axis(l.Lz, ER_WC_SHIFT, dz);
             mu = p.density_scale * gmem(p.wc_sigma)[(mz * dy + my) * dx + mx];

@@ -271,7 +271,8 @@ typedef struct er_counters {
     uint64_t shadow_fetches;
     uint64_t scatter_events;
     uint64_t shadow_rays;
-    uint64_t march_iterations;  /* iterations of the march loops that computed a sample position */
+    uint64_t march_iterations;  /* iterations of the march loops that computed a sample position
+                                   (Woodcock: majorant segments walked) */
     uint64_t skipped_samples;   /* samples advanced over inside empty macrocells without a position */
     uint64_t skips_undone;      /* macrocell skips rejected by the exactness check */
     uint64_t reserved;
