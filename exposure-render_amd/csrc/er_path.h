@@ -283,7 +283,11 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
                 lk = to_lattice(p, Pk);
                 occ = mc_occupied(T, mc_index(p, lk, sh));
             }
-            live[k] = occ && !(FULL && p.clip_count > 0 && clip_possible(p, T, lk) && clipped(p, Pk));
+            /* no branch on the occupancy bit (the clip test branches on the uniform clip count
+             * only): its LDS read completes while the following positions and loads issue */
+            bool cut = false;
+            if (FULL && p.clip_count > 0) cut = clip_possible(p, T, lk) && clipped(p, Pk);
+            live[k] = occ & !cut;
             f[k] = cell_fetch(p, lk);
         }
         /* every sample's intensity and sigma first, so that the K table reads are in flight
@@ -618,8 +622,9 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
         ins[j] = PRIMARY ? (t < m.MaxT) : (t <= m.MaxT);
         const f3 Pk = add(m.O, scl(t, m.D));
         const Lattice lk = to_lattice(p, Pk);
-        live[j] = ins[j] && mc_occupied(T, mc_index(p, lk, sh)) &&
-                  !(FULL && p.clip_count > 0 && clip_possible(p, T, lk) && clipped(p, Pk));
+        bool cut = false;   /* as in march_step: no branch on the occupancy bit */
+        if (FULL && p.clip_count > 0) cut = clip_possible(p, T, lk) && clipped(p, Pk);
+        live[j] = ins[j] & mc_occupied(T, mc_index(p, lk, sh)) & !cut;
         f[j] = cell_fetch(p, lk);
     }
 #pragma unroll
