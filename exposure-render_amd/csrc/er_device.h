@@ -373,18 +373,22 @@ __device__ __forceinline__ f3 grad_fd(const DevParams& p, f3 P)
     return mk(i0 - i1, i0 - i2, i0 - i3);
 }
 __device__ __forceinline__ f3 vlerp(f3 a, f3 b, float c) { return add(a, scl(c, sub(b, a))); }
+/* grad_cd as a call of its own: grad_filtered's nine gradients then hold six cell records in
+ * registers at a time instead of 54, and the kernels that can call grad_filtered keep the
+ * register budget (and so the occupancy) of their other paths */
+__device__ __noinline__ f3 grad_cd_call(const DevParams& p, f3 P) { return grad_cd(p, P); }
 __device__ __noinline__ f3 grad_filtered(const DevParams& p, f3 P)
 {
     const float o = p.minstep;
-    const f3 G0 = grad_cd(p, P);
-    const f3 G1 = grad_cd(p, add(P, mk(-o, -o, -o)));
-    const f3 G2 = grad_cd(p, add(P, mk(o, o, o)));
-    const f3 G3 = grad_cd(p, add(P, mk(-o, o, -o)));
-    const f3 G4 = grad_cd(p, add(P, mk(o, -o, o)));
-    const f3 G5 = grad_cd(p, add(P, mk(-o, -o, o)));
-    const f3 G6 = grad_cd(p, add(P, mk(o, o, -o)));
-    const f3 G7 = grad_cd(p, add(P, mk(-o, o, o)));
-    const f3 G8 = grad_cd(p, add(P, mk(o, -o, -o)));
+    const f3 G0 = grad_cd_call(p, P);
+    const f3 G1 = grad_cd_call(p, add(P, mk(-o, -o, -o)));
+    const f3 G2 = grad_cd_call(p, add(P, mk(o, o, o)));
+    const f3 G3 = grad_cd_call(p, add(P, mk(-o, o, -o)));
+    const f3 G4 = grad_cd_call(p, add(P, mk(o, -o, o)));
+    const f3 G5 = grad_cd_call(p, add(P, mk(-o, -o, o)));
+    const f3 G6 = grad_cd_call(p, add(P, mk(o, o, -o)));
+    const f3 G7 = grad_cd_call(p, add(P, mk(-o, o, o)));
+    const f3 G8 = grad_cd_call(p, add(P, mk(o, -o, -o)));
     const f3 L0 = vlerp(vlerp(G1, G2, 0.5f), vlerp(G3, G4, 0.5f), 0.5f);
     const f3 L1 = vlerp(vlerp(G5, G6, 0.5f), vlerp(G7, G8, 0.5f), 0.5f);
     return vlerp(G0, vlerp(L0, L1, 0.5f), 0.75f);

@@ -458,9 +458,13 @@ __global__ __launch_bounds__(1024) void k_tile_order(uint32_t* __restrict__ cost
     if (tid == 0) order[n] = heavy;
 }
 
+#ifndef ER_SHADE_WAVES_PER_EU
+#define ER_SHADE_WAVES_PER_EU 1
+#endif
+
 /* Shading kernel over the scatter queue (counts[0] entries), wave-aligned grid stride. */
 template<int FULL, int COUNT, int LUT_LDS, int SPLIT>
-__global__ __launch_bounds__(256) void k_shade(const DevParams* __restrict__ pp, float4* __restrict__ frame,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ER_SHADE_WAVES_PER_EU))) void k_shade(const DevParams* __restrict__ pp, float4* __restrict__ frame,
                                               uint32_t* __restrict__ seeds1, uint32_t* __restrict__ seeds2,
                                               const ScatterRec* __restrict__ queue, unsigned int* __restrict__ counts,
                                               EventRec* __restrict__ events, ShadowJob* __restrict__ jobq,
