@@ -115,7 +115,7 @@ struct DevParams {
     int   mc_shift;                  /* macrocell edge = 2^mc_shift cells */
     int   mc_words;                  /* 32-bit words of mc_bits: macrocell bits, group bits, group clip-free bits */
     int   mc2_dim[3];                /* groups of 4 x 4 x 4 macrocells */
-    int   mc2_base;                  /* first word of the group bits in mc_bits */
+    int   mc2_base;                  /* first word of the group radii (4 bits a group) in mc_bits */
     int   mc3_base;                  /* first word of the group clip-free bits in mc_bits */
     const float* neglog;             /* -log(m 2^-23), m in [0, 2^23) */
     /* the tracer's lights and textures by slot (light_ids maps list position -> slot), clips */

@@ -92,6 +92,15 @@ __device__ __forceinline__ bool mc_occupied(const Tables& T, int m)
     return (w >> (m & 31)) & 1u;
 }
 
+/* the empty-space radius of group g (4 bits, eight groups to a word from word `base`): 0 for
+ * an occupied group, else R >= 1 with every group within Chebyshev distance R - 1 of g empty */
+__device__ __forceinline__ int group_radius(const Tables& T, int base, int g)
+{
+    const int wi = base + (g >> 3);
+    const uint32_t w = T.bits_glb ? T.bits_glb[wi] : T.bits[wi];
+    return (int)((w >> ((g & 7) << 2)) & 15u);
+}
+
 __device__ __forceinline__ int mc_axis(float L, int shift, int dim)
 {
     return min(max(((int)floorf(L)) >> shift, 0), dim - 1);
@@ -330,19 +339,26 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
     const float iLdx = __builtin_amdgcn_rcpf(Ldx), iLdy = __builtin_amdgcn_rcpf(Ldy), iLdz = __builtin_amdgcn_rcpf(Ldz);
     const float margin = 1.0e-3f * __builtin_amdgcn_rcpf(fmaxf(fabsf(Ldx), fmaxf(fabsf(Ldy), fabsf(Ldz))));
     m.MinT += StepSize;
-    /* the box to skip: the macrocell, or the whole group of macrocells around it when the group
-     * is empty (the same argument holds for any lattice-aligned box) */
-    int bsh = sh, bx = mxi, by = myi, bz = mzi, nx = p.mc_dim[0], ny = p.mc_dim[1], nz = p.mc_dim[2];
+    /* the box to skip: the macrocell, or, when its group of macrocells is empty, the cube of
+     * groups [g - (R - 1), g + (R - 1)] on each axis that the group's radius R marks empty (the
+     * same argument holds for any lattice-aligned box; a box reaching the grid's edge extends
+     * beyond it, where the clamped lookups land in its edge groups) */
+    int bsh = sh, nx = p.mc_dim[0], ny = p.mc_dim[1], nz = p.mc_dim[2];
+    int x0 = mxi, x1 = mxi, y0 = myi, y1 = myi, z0 = mzi, z1 = mzi;
     {
         const int gx = mxi >> ER_GROUP_SHIFT, gy = myi >> ER_GROUP_SHIFT, gz = mzi >> ER_GROUP_SHIFT;
-        if (!mc_occupied(T, (p.mc2_base << 5) + (gz * p.mc2_dim[1] + gy) * p.mc2_dim[0] + gx)) {
-            bsh = sh + ER_GROUP_SHIFT; bx = gx; by = gy; bz = gz; nx = p.mc2_dim[0]; ny = p.mc2_dim[1]; nz = p.mc2_dim[2];
+        const int R = group_radius(T, p.mc2_base, (gz * p.mc2_dim[1] + gy) * p.mc2_dim[0] + gx);
+        if (R > 0) {
+            bsh = sh + ER_GROUP_SHIFT; nx = p.mc2_dim[0]; ny = p.mc2_dim[1]; nz = p.mc2_dim[2];
+            x0 = gx - (R - 1); x1 = gx + (R - 1);
+            y0 = gy - (R - 1); y1 = gy + (R - 1);
+            z0 = gz - (R - 1); z1 = gz + (R - 1);
         }
     }
     const float big = 3.0e38f;
-    const float lo_x = bx == 0 ? -big : (float)(bx << bsh), hi_x = bx == nx - 1 ? big : (float)((bx + 1) << bsh);
-    const float lo_y = by == 0 ? -big : (float)(by << bsh), hi_y = by == ny - 1 ? big : (float)((by + 1) << bsh);
-    const float lo_z = bz == 0 ? -big : (float)(bz << bsh), hi_z = bz == nz - 1 ? big : (float)((bz + 1) << bsh);
+    const float lo_x = x0 <= 0 ? -big : (float)(x0 << bsh), hi_x = x1 >= nx - 1 ? big : (float)((x1 + 1) << bsh);
+    const float lo_y = y0 <= 0 ? -big : (float)(y0 << bsh), hi_y = y1 >= ny - 1 ? big : (float)((y1 + 1) << bsh);
+    const float lo_z = z0 <= 0 ? -big : (float)(z0 << bsh), hi_z = z1 >= nz - 1 ? big : (float)((z1 + 1) << bsh);
     const float tx = Ldx > 0.0f ? (hi_x - l.Lx) * iLdx : (Ldx < 0.0f ? (lo_x - l.Lx) * iLdx : big);
     const float ty = Ldy > 0.0f ? (hi_y - l.Ly) * iLdy : (Ldy < 0.0f ? (lo_y - l.Ly) * iLdy : big);
     const float tz = Ldz > 0.0f ? (hi_z - l.Lz) * iLdz : (Ldz < 0.0f ? (lo_z - l.Lz) * iLdz : big);
@@ -353,7 +369,8 @@ __device__ __forceinline__ int march_step(const DevParams& p, const Tables& T, M
     uint32_t n = skip_additions(m.MinT, StepSize, t_lim, t_prev);
     if (n > 0) {
         const Lattice lp = to_lattice(p, add(m.O, scl(t_prev, m.D)));
-        if (mc_axis(lp.Lx, bsh, nx) != bx || mc_axis(lp.Ly, bsh, ny) != by || mc_axis(lp.Lz, bsh, nz) != bz) {
+        const int ax = mc_axis(lp.Lx, bsh, nx), ay = mc_axis(lp.Ly, bsh, ny), az = mc_axis(lp.Lz, bsh, nz);
+        if (ax < x0 || ax > x1 || ay < y0 || ay > y1 || az < z0 || az > z1) {
             m.MinT = t_first;
             n = 0;
             if (COUNT) cnt.undone++;

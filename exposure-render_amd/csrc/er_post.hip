@@ -219,12 +219,13 @@ __global__ __launch_bounds__(256) void k_macro_sigma(const uint32_t* __restrict_
 }
 
 /* group bits: bit g set iff any macrocell of group g (2^ER_GROUP_SHIFT macrocells a side, clipped to the
- * grid) has its bit set; the group bits start at word `base` of the same array. Then the
- * groups' clip-free bits, from word base + words: bit g set iff every occupied macrocell of
- * group g is wholly unclipped (bit m of `unclipped`, the host's conservative test; NULL: no
- * clipping objects, every bit clear), so the march need not test its samples against them. */
+ * grid) has its bit set, from word `occ_base` of the same array; the groups' clip-free bits
+ * from word `free_base`: bit g set iff every occupied macrocell of group g is wholly unclipped
+ * (bit m of `unclipped`, the host's conservative test; NULL: no clipping objects, every bit
+ * clear), so the march need not test its samples against them. */
 __global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits, const uint32_t* __restrict__ unclipped,
-                                                   int mx, int my, int mz, int gx, int gy, int gz, int base, int words)
+                                                   int mx, int my, int mz, int gx, int gy, int gz, int occ_base,
+                                                   int free_base, int words)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= words) return;
@@ -247,8 +248,8 @@ __global__ __launch_bounds__(256) void k_group_bits(uint32_t* __restrict__ bits,
         if (any) v |= 1u << b;
         if (free) f |= 1u << b;
     }
-    bits[base + w] = v;
-    bits[base + words + w] = f;
+    bits[occ_base + w] = v;
+    bits[free_base + w] = f;
 }
 
 /* occupancy bit of macrocell m: it can hold a sample with sigma > 0 that is not clipped away;
@@ -331,10 +332,10 @@ extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, const uint32_
 }
 
 extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, const uint32_t* unclipped, int mx, int my, int mz, int gx, int gy,
-                                            int gz, int base, int words, hipStream_t st)
+                                            int gz, int occ_base, int free_base, int words, hipStream_t st)
 {
     hipLaunchKernelGGL(erd::k_group_bits, dim3((words + 255) / 256), dim3(256), 0, st, bits, unclipped, mx, my, mz, gx, gy,
-                       gz, base, words);
+                       gz, occ_base, free_base, words);
     return hipGetLastError();
 }
 

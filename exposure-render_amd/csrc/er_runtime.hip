@@ -46,7 +46,7 @@ extern "C" hipError_t erd_launch_post(const DevParams* p, const float4* frame, f
 extern "C" hipError_t erd_launch_macro_bits(const float* mc_sigma, const uint32_t* clipped, uint32_t* bits, int n, int words,
                                             hipStream_t st);
 extern "C" hipError_t erd_launch_group_bits(uint32_t* bits, const uint32_t* unclipped, int mx, int my, int mz, int gx, int gy,
-                                            int gz, int base, int words, hipStream_t st);
+                                            int gz, int occ_base, int free_base, int words, hipStream_t st);
 extern "C" hipError_t erd_launch_bilateral(const void* bp, const uchar4* in, uchar4* tmp, uchar4* out, int W, int H,
                                            hipStream_t st);
 extern "C" size_t erd_bilateral_params_size(void);
@@ -308,11 +308,11 @@ struct Volume {
     }
 };
 
-/* The occupancy bits (one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a
- * side) and, when it takes at most 32 KB, the opacity table (one float per intensity up to the
- * volume's maximum) are staged in LDS by the march kernels; together they must fit 52 KB, so
- * that three march workgroups (the primary kernel's register-bound occupancy) share a CU's
- * 160 KB. */
+/* The occupancy bits (one per macrocell), the empty-space radii (4 bits per group of
+ * 2^ER_GROUP_SHIFT macrocells a side), the groups' clip-free bits and, when it takes at most
+ * 32 KB, the opacity table (one float per intensity up to the volume's maximum) are staged in
+ * LDS by the march kernels; together they must fit 52 KB, so that three march workgroups (the
+ * primary kernel's register-bound occupancy) share a CU's 160 KB. */
 const size_t ER_MAX_TABLES_BYTES = 53248;
 bool brick_fits(const er_volume_desc& d, uint16_t vmax, int shift)
 {
@@ -322,8 +322,8 @@ bool brick_fits(const er_volume_desc& d, uint16_t vmax, int shift)
     const size_t n = mc[0] * mc[1] * mc[2];
     const size_t g = ((mc[0] + G - 1) / G) * ((mc[1] + G - 1) / G) * ((mc[2] + G - 1) / G);
     const size_t lut = ((size_t)vmax + 1) * sizeof(float);
-    /* macrocell bits, group occupancy bits and group clip-free bits, as build_params lays them out */
-    return 4 * ((n + 31) / 32 + 2 * ((g + 31) / 32)) + (lut <= 32768 ? lut : 0) <= ER_MAX_TABLES_BYTES;
+    /* macrocell bits, group radii and group clip-free bits, as build_params lays them out */
+    return 4 * ((n + 31) / 32 + (g + 7) / 8 + (g + 31) / 32) + (lut <= 32768 ? lut : 0) <= ER_MAX_TABLES_BYTES;
 }
 
 /* The state of one pass in flight: its frame estimate and its work queues. A tracer keeps
@@ -800,6 +800,48 @@ void clip_classes(const DevParams& P, int nmc, double edge_ext, std::vector<uint
     for (auto& x : th) x.join();
 }
 
+/* The empty-space radius of every group of macrocells from the groups' occupancy bits (dims:
+ * the group grid), 4 bits each, eight to a word: 0 for an occupied group, else the smallest
+ * r >= 1 such that the groups within Chebyshev distance r of it include an occupied one, at
+ * most 15. Every group within distance r - 1 is then empty, so a march in that group may skip
+ * the whole cube at once. The set of groups within distance r of an occupied group grows by
+ * one group along each axis per round (a cube dilates as three axis segments). */
+void group_radii(const std::vector<uint32_t>& occ, const int dims[3], std::vector<uint32_t>& out)
+{
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    const size_t ng = (size_t)nx * ny * nz, sy = (size_t)nx, sz = (size_t)nx * ny;
+    std::vector<uint8_t> cur(ng), tmp(ng), rad(ng);
+    for (size_t g = 0; g < ng; ++g) {
+        cur[g] = (uint8_t)((occ[g >> 5] >> (g & 31)) & 1u);
+        rad[g] = cur[g] ? 0 : 15;
+    }
+    for (int r = 1; r < 15; ++r) {
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const size_t i = z * sz + y * sy + x;
+                    tmp[i] = cur[i] | (x > 0 ? cur[i - 1] : 0) | (x < nx - 1 ? cur[i + 1] : 0);
+                }
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const size_t i = z * sz + y * sy + x;
+                    cur[i] = tmp[i] | (y > 0 ? tmp[i - sy] : 0) | (y < ny - 1 ? tmp[i + sy] : 0);
+                }
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const size_t i = z * sz + y * sy + x;
+                    tmp[i] = cur[i] | (z > 0 ? cur[i - sz] : 0) | (z < nz - 1 ? cur[i + sz] : 0);
+                }
+        cur.swap(tmp);
+        for (size_t g = 0; g < ng; ++g)
+            if (rad[g] == 15 && cur[g]) rad[g] = (uint8_t)r;
+    }
+    out.assign((ng + 7) / 8, 0u);
+    for (size_t g = 0; g < ng; ++g) out[g >> 3] |= (uint32_t)rad[g] << ((g & 7) * 4);
+}
+
 /* Rebuild the tracer's DevParams (and the LUT / macrocell maxima / gm_max that depend on it). */
 int build_params(Tracer& t)
 {
@@ -1023,16 +1065,18 @@ int build_params(Tracer& t)
         P.wc_sigma = t.wc_sigma.ptr;
         for (int i = 0; i < 3; ++i) P.wc_dim[i] = wg->mc[i];
     }
-    /* occupancy bits: one per macrocell, then one per group of 2^ER_GROUP_SHIFT macrocells a side;
-     * then the groups' clip-free bits */
+    /* occupancy bits: one per macrocell; then the radii of the groups of 2^ER_GROUP_SHIFT
+     * macrocells a side (4 bits each) and the groups' clip-free bits, the part the march kernels
+     * stage in LDS; then the groups' occupancy bits the radii are computed from */
     const int mc_words = (nmc + 31) / 32;
     for (int i = 0; i < 3; ++i) P.mc2_dim[i] = (grid->mc[i] + (1 << ER_GROUP_SHIFT) - 1) >> ER_GROUP_SHIFT;
     const int ng = P.mc2_dim[0] * P.mc2_dim[1] * P.mc2_dim[2];
-    const int g_words = (ng + 31) / 32;
+    const int g_words = (ng + 31) / 32, r_words = (ng + 7) / 8;
     P.mc2_base = mc_words;
-    P.mc3_base = mc_words + g_words;
-    P.mc_words = mc_words + 2 * g_words;
-    if ((st = t.mc_bits.alloc(P.mc_words))) return st;
+    P.mc3_base = mc_words + r_words;
+    P.mc_words = mc_words + r_words + g_words;
+    const int occ_base = P.mc_words;
+    if ((st = t.mc_bits.alloc(P.mc_words + g_words))) return st;
     P.mc_bits = t.mc_bits.ptr;
     t.lut_lds = (size_t)lut_n * sizeof(float) <= 32768u;
     t.lds_bytes = (size_t)P.mc_words * 4 + (t.lut_lds ? (size_t)lut_n * sizeof(float) : 0);
@@ -1057,7 +1101,17 @@ int build_params(Tracer& t)
     }
     HIP_TRY(erd_launch_macro_bits(t.mc_sigma.ptr, clipped, t.mc_bits.ptr, nmc, mc_words, t.stream));
     HIP_TRY(erd_launch_group_bits(t.mc_bits.ptr, unclipped, grid->mc[0], grid->mc[1], grid->mc[2], P.mc2_dim[0],
-                                  P.mc2_dim[1], P.mc2_dim[2], mc_words, g_words, t.stream));
+                                  P.mc2_dim[1], P.mc2_dim[2], occ_base, P.mc3_base, g_words, t.stream));
+    {
+        std::vector<uint32_t> occ((size_t)g_words), radii;
+        HIP_TRY(hipMemcpyAsync(occ.data(), t.mc_bits.ptr + occ_base, occ.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               t.stream));
+        HIP_TRY(hipStreamSynchronize(t.stream));
+        group_radii(occ, P.mc2_dim, radii);
+        HIP_TRY(hipMemcpyAsync(t.mc_bits.ptr + P.mc2_base, radii.data(), radii.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, t.stream));
+        HIP_TRY(hipStreamSynchronize(t.stream));
+    }
     if (needs_gm(d)) {
         if ((st = t.gm_tmp.alloc(1))) return st;
         HIP_TRY(hipMemsetAsync(t.gm_tmp.ptr, 0, sizeof(unsigned int), t.stream));
