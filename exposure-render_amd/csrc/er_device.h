@@ -8,7 +8,7 @@
  *             2-byte gathers; the clamp-to-edge of the reference is folded into the records.
  *             (Rx+1)(Ry+1)(Rz+1) x 16 B (8x the voxel bytes; 1.9 GB for 512x512x460).
  *   mc_*    : macrocells of 8^3 cells (16^3 or 32^3 for volumes whose occupancy bits and
- *             opacity table would not fit 52 KB of LDS). For each, the
+ *             opacity table would not fit 52.5 KB of LDS). For each, the
  *             min/max voxel over the voxels its samples can
  *             read, and (per tracer) the max of DensityScale * Opacity over the intensities a
  *             truncating trilinear sample can produce from that range ([min-3, max]). A sample

@@ -311,9 +311,10 @@ struct Volume {
 /* The occupancy bits (one per macrocell), the empty-space radii (4 bits per group of
  * 2^ER_GROUP_SHIFT macrocells a side), the groups' clip-free bits and, when it takes at most
  * 32 KB, the opacity table (one float per intensity up to the volume's maximum) are staged in
- * LDS by the march kernels; together they must fit 52 KB, so that three march workgroups (the
- * primary kernel's register-bound occupancy) share a CU's 160 KB. */
-const size_t ER_MAX_TABLES_BYTES = 53248;
+ * LDS by the march kernels; together they must fit 52.5 KB (105 LDS allocation blocks of 512
+ * bytes, with one more for a kernel's own LDS), so that three march workgroups (the primary
+ * kernel's register-bound occupancy) share a CU's 160 KB. */
+const size_t ER_MAX_TABLES_BYTES = 53760;
 bool brick_fits(const er_volume_desc& d, uint16_t vmax, int shift)
 {
     const size_t E = (size_t)1 << shift, G = (size_t)1 << ER_GROUP_SHIFT;
@@ -607,7 +608,7 @@ int upload_volume(Volume& v)
     if ((st = v.cells.alloc(nc))) return st;
     HIP_TRY(erd_launch_build_cells(v.vox.ptr, v.cells.ptr, rx, ry, rz, 0));
     /* default macrocell edge: 8 cells, coarser when the occupancy bits and the opacity table
-     * would not fit their 52 KB of LDS */
+     * would not fit their 52.5 KB of LDS */
     v.mc_shift = 3;
     while (!brick_fits(d, v.vmax, v.mc_shift) && v.mc_shift < 8) v.mc_shift++;
     static uint64_t uploads = 0;

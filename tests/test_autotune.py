@@ -69,7 +69,7 @@ def test_large_volume_bricks_render_the_same_image(fresh):
     assert base[5]["brick"] == 8
     for tune in [(16, 8, 7), (32, 16, 5)]:
         _same(base, _render(sc, 2, tune), "tune")
-    # 4-cell bricks would need 251 KB of occupancy bits: beyond the 52 KB LDS budget
+    # 4-cell bricks would need 251 KB of occupancy bits: beyond the 52.5 KB LDS budget
     r = er.Renderer(sc, seed=9)
     with pytest.raises(er.ExposureRenderError):
         r.set_tune(4, 8, 7)
