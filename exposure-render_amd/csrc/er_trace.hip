@@ -483,6 +483,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ER_SHADE_WA
     flush_counters<COUNT>(cnt, true, 0, counters);
 }
 
+/* Once a wave's job queue is drained and at most 64 / G of its lanes still march, those
+ * marches are handed to aligned groups of G lanes (group i takes the march of the i-th busy
+ * lane: its state is the same in every lane of the group) and stepped with group_step, which
+ * makes the same decisions as the one-lane march, until every group is done; the group's first
+ * lane writes the visibility byte. The last, longest marches of a wave then advance G / K
+ * times as many samples per round instead of leaving the wave's other lanes idle. Every lane of
+ * the wave must call this. */
+#ifndef ER_SHADOW_DRAIN_G
+#define ER_SHADOW_DRAIN_G 4   /* 0: no hand-off (the lanes finish their marches alone) */
+#endif
+template<int FULL, int COUNT, int G>
+__device__ __forceinline__ void shadow_drain(const DevParams& p, const Tables& T, const MarchState& m, unsigned int slot,
+                                             bool busy, unsigned char* __restrict__ vis, float StepSize, int lane, Cnt& cnt)
+{
+    const unsigned long long bm = __ballot(busy);
+    /* the group's march: the (lane / G)-th busy lane's, if there is one */
+    unsigned long long x = bm;
+    for (int k = 0; k < lane / G; ++k) x &= x - 1ull;
+    const bool has = x != 0ull;
+    const int src = has ? __ffsll((long long)x) - 1 : 0;
+    MarchState g;
+    g.O = mk(__shfl(m.O.x, src), __shfl(m.O.y, src), __shfl(m.O.z, src));
+    g.D = mk(__shfl(m.D.x, src), __shfl(m.D.y, src), __shfl(m.D.z, src));
+    g.MinT = __shfl(m.MinT, src);
+    g.MaxT = __shfl(m.MaxT, src);
+    g.Sum = __shfl(m.Sum, src);
+    g.S = __shfl(m.S, src);
+    const unsigned int gslot = (unsigned int)__shfl((int)slot, src);
+    const bool first = (lane & (G - 1)) == 0;
+    bool gbusy = has;
+    while (__ballot(gbusy)) {
+        if (gbusy) {
+            f3 P;
+            int iters = 0;
+            const int st = group_step<FULL, COUNT, G, false>(p, T, g, StepSize, lane, cnt, P, iters);
+            if (st != 0) {
+                if (first) vis[gslot] = st > 0 ? 0 : 1;
+                gbusy = false;
+            }
+        }
+    }
+}
+
 /* The shadow marches of one queue of n jobs (jobq), one march per lane. The marches differ in
  * length by orders of magnitude (a ray through empty space ends after a few skips, one through
  * tissue takes hundreds of samples), so a wave does not wait for its longest march: the lane
@@ -543,6 +586,11 @@ __device__ __forceinline__ void shadow_marches(const DevParams& p, const Tables&
         const int nidle = __popcll(idle);
         if (drained) {
             if (nidle == 64) break;
+            if (ER_SHADOW_DRAIN_G > 0 && 64 - nidle <= 64 / (ER_SHADOW_DRAIN_G > 0 ? ER_SHADOW_DRAIN_G : 1)) {
+                shadow_drain<FULL, COUNT, (ER_SHADOW_DRAIN_G > 0 ? ER_SHADOW_DRAIN_G : 4)>(p, T, m, slot, busy, vis, StepSize,
+                                                                                           lane, cnt);
+                break;
+            }
         } else if (nidle >= ER_SHADOW_REFILL) {
             const int leader = __ffsll((long long)idle) - 1;
             unsigned int base = 0;
