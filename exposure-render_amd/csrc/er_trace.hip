@@ -69,6 +69,13 @@ __device__ __forceinline__ void flush_counters(const Cnt& cnt, bool active, int 
     if (threadIdx.x < 12) atomicAdd(&counters[threadIdx.x], acc[threadIdx.x]);
 }
 
+/* the primary march's last marches of a wave go to groups of this many lanes once at most
+ * 64 / ER_PRIMARY_DRAIN_G are left (0: to coop_march, one march at a time with the whole
+ * wave, once at most ER_COOP_LANES are left; a wave marching a single ray still does) */
+#ifndef ER_PRIMARY_DRAIN_G
+#define ER_PRIMARY_DRAIN_G 4
+#endif
+
 /* One wave renders one 8 x 8 pixel tile: the 11 sample draws (sample.h:258-262), the camera
  * ray (singlescattering.h:29-50), SampleRay (singlescattering.h:52-74). Pixels whose nearest
  * event is a light or nothing are finished here; pixels with a volume event are appended to
@@ -135,13 +142,58 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
             if (st > 0) Pv = P;
         }
     } else {
+    constexpr int GD = ER_PRIMARY_DRAIN_G > 0 ? ER_PRIMARY_DRAIN_G : 4;
     for (;;) {
         const int nm = __popcll(__ballot(marching));
-        if (nm == 0 || nm <= ER_COOP_LANES || ray >= 0) break;
+        if (nm == 0 || nm <= (ER_PRIMARY_DRAIN_G > 0 ? 64 / GD : ER_COOP_LANES) || ray >= 0) break;
         if (marching) {
             ++it;
             const int st = march_step<FULL, COUNT, ER_BATCH_PRIMARY, NoVisit>(p, T, m, true, StepSize, cnt, nv, Pv);
             if (st != 0) { marching = false; vol = st > 0; Tv = m.MinT; }
+        }
+    }
+    /* At most 64 / GD marches left: aligned groups of GD lanes take them over (group i the
+     * i-th marching lane's, its state the same in every lane of the group) and step them with
+     * group_step, which makes the same decisions as the one-lane march; then each marching lane
+     * reads its march's end from its group's first lane. */
+    if (ER_PRIMARY_DRAIN_G > 0 && ray < 0) {
+        const unsigned long long bm = __ballot(marching);
+        if (bm) {
+            unsigned long long x = bm;
+            for (int k = 0; k < lane / GD; ++k) x &= x - 1ull;
+            const bool has = x != 0ull;
+            const int src = has ? __ffsll((long long)x) - 1 : 0;
+            MarchState g;
+            g.O = mk(__shfl(m.O.x, src), __shfl(m.O.y, src), __shfl(m.O.z, src));
+            g.D = mk(__shfl(m.D.x, src), __shfl(m.D.y, src), __shfl(m.D.z, src));
+            g.MinT = __shfl(m.MinT, src);
+            g.MaxT = __shfl(m.MaxT, src);
+            g.Sum = __shfl(m.Sum, src);
+            g.S = __shfl(m.S, src);
+            bool gb = has;
+            int gst = 0, git = 0;
+            f3 gP = mk(0.0f, 0.0f, 0.0f);
+            while (__ballot(gb)) {
+                if (gb) {
+                    f3 P = mk(0.0f, 0.0f, 0.0f);
+                    int iters = 0;
+                    const int st = group_step<FULL, COUNT, GD, true>(p, T, g, StepSize, lane, cnt, P, iters);
+                    git += iters;
+                    if (st != 0) { gst = st; gP = P; gb = false; }
+                }
+            }
+            const int from = marching ? __popcll(bm & ((1ull << lane) - 1ull)) * GD : 0;
+            const int rst = __shfl(gst, from);
+            const float rT = __shfl(g.MinT, from);
+            const f3 rP = mk(__shfl(gP.x, from), __shfl(gP.y, from), __shfl(gP.z, from));
+            const int rit = __shfl(git, from);
+            if (marching) {
+                marching = false;
+                vol = rst > 0;
+                Tv = rT;
+                if (rst > 0) Pv = rP;
+                it += rit;
+            }
         }
     }
     unsigned long long rest = __ballot(marching);
