@@ -92,6 +92,11 @@ struct VisSum {
         last_d = sigma;
         last_e = e;
     }
+    __device__ __forceinline__ void event(int i, float sigma)
+    {
+        last_d = sigma;
+        last_e = EVAL ? gmem(sig_eval)[min(i, lut_max)] : sigma;
+    }
     /* sum of the evaluation table's sigma */
     __device__ __forceinline__ float se() const { return EVAL ? sd + dd : sd; }
 };
@@ -127,7 +132,162 @@ struct VisGrad {
         a0 += w * (1.0f - f);
         a1 += w * f;
     }
+    __device__ __forceinline__ void event(int i, float sigma)
+    {
+        last_i = i;
+        last_sigma = sigma;
+    }
 };
+
+/* ---- marches stepped a wave at a time, their last ones finished by groups of lanes ----- */
+
+/* The adjoint kernels' marches, stepped together by the lanes of a wave (march_step with an
+ * observer, the same samples in the same order as march_v) until at most 64 / ADJ_DRAIN_G of
+ * them are left; those are then handed to aligned groups of ADJ_DRAIN_G lanes (group i the
+ * i-th marching lane's march, its state the same in every lane of the group) and stepped with
+ * group_step_v, which makes the same decisions. Each lane of a group observes its own samples;
+ * the group's observations are merged back into the marching lane's observer. The longest
+ * marches of a wave then advance ADJ_DRAIN_G / K times as many samples per round instead of
+ * leaving the wave's other lanes idle. Every lane of the wave must call these. */
+#ifndef ADJ_DRAIN_G
+#define ADJ_DRAIN_G 4
+#endif
+
+/* the group's march: the (lane / G)-th marching lane's; returns its lane (0 if none) */
+template<int G>
+__device__ __forceinline__ int adj_group_source(unsigned long long busy, int lane, bool& has)
+{
+    unsigned long long x = busy;
+    for (int k = 0; k < lane / G; ++k) x &= x - 1ull;
+    has = x != 0ull;
+    return has ? __ffsll((long long)x) - 1 : 0;
+}
+
+__device__ __forceinline__ MarchState adj_march_from(const MarchState& m, int src)
+{
+    MarchState g;
+    g.O = mk(__shfl(m.O.x, src), __shfl(m.O.y, src), __shfl(m.O.z, src));
+    g.D = mk(__shfl(m.D.x, src), __shfl(m.D.y, src), __shfl(m.D.z, src));
+    g.MinT = __shfl(m.MinT, src);
+    g.MaxT = __shfl(m.MaxT, src);
+    g.Sum = __shfl(m.Sum, src);
+    g.S = __shfl(m.S, src);
+    return g;
+}
+
+/* the rest of the marching lanes' transmittance marches (sum of sigma, VisSum<0>): on return
+ * busy is false everywhere and each marching lane's vs.sd holds its whole march's sum */
+template<bool PRIMARY>
+__device__ __forceinline__ void adj_drain_sum(const DevParams& p, const Tables& T, const MarchState& m, bool busy, VisSum<0>& vs,
+                                           float StepSize, int lane);
+template<bool PRIMARY>
+__device__ __forceinline__ void adj_march_sum(const DevParams& p, const Tables& T, MarchState& m, bool busy, VisSum<0>& vs,
+                                              float StepSize, int lane)
+{
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (;;) {
+        const int nb = __popcll(__ballot(busy));
+        if (nb == 0) return;
+        if (nb <= 64 / ADJ_DRAIN_G) break;
+        if (busy) {
+            f3 P;
+            if (march_step<1, 0, ER_BATCH_SHADOW_ADJ, VisSum<0>>(p, T, m, PRIMARY, StepSize, cnt, vs, P) != 0) busy = false;
+        }
+    }
+    adj_drain_sum<PRIMARY>(p, T, m, busy, vs, StepSize, lane);
+}
+template<bool PRIMARY>
+__device__ __forceinline__ void adj_drain_sum(const DevParams& p, const Tables& T, const MarchState& m, bool busy, VisSum<0>& vs,
+                                           float StepSize, int lane)
+{
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int G = ADJ_DRAIN_G;
+    const unsigned long long bm = __ballot(busy);
+    bool has;
+    const int src = adj_group_source<G>(bm, lane, has);
+    MarchState g = adj_march_from(m, src);
+    /* the group's first lane continues the marching lane's compensated sum, the others start at 0 */
+    const bool first = (lane & (G - 1)) == 0;
+    VisSum<0> gv;
+    gv.init(nullptr, vs.lut_max);
+    const float sd0 = __shfl(vs.sd, src), cd0 = __shfl(vs.cd, src);
+    if (first) { gv.sd = sd0; gv.cd = cd0; }
+    bool gb = has;
+    while (__ballot(gb)) {
+        if (gb) {
+            f3 P;
+            int iters = 0;
+            if (group_step_v<1, 0, G, PRIMARY, VisSum<0>>(p, T, g, StepSize, lane, cnt, P, iters, gv) != 0) gb = false;
+        }
+    }
+    /* the group's sum: the lanes' compensated sums added across the group */
+    float v = gv.sd - gv.cd;
+    float tot = v;
+    for (int o = 1; o < G; o <<= 1) tot += __shfl_xor(tot, o, G);
+    const int from = busy ? __popcll(bm & ((1ull << lane) - 1ull)) * G : 0;
+    const float total = __shfl(tot, from);
+    if (busy) { vs.sd = total; vs.cd = 0.0f; }
+}
+
+/* the rest of the marching lanes' gradient marches (VisGrad into the LDS histogram): on return
+ * each marching lane's vol is whether its march ended on an event, and vg.last_i / last_sigma
+ * are the event sample's */
+template<bool PRIMARY>
+__device__ __forceinline__ void adj_drain_grad(const DevParams& p, const Tables& T, const MarchState& m, bool busy, bool& vol,
+                                            VisGrad& vg, float StepSize, int lane);
+template<bool PRIMARY, int K>
+__device__ __forceinline__ void adj_march_grad(const DevParams& p, const Tables& T, MarchState& m, bool busy, bool& vol,
+                                               VisGrad& vg, float StepSize, int lane)
+{
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (;;) {
+        const int nb = __popcll(__ballot(busy));
+        if (nb == 0) return;
+        if (nb <= 64 / ADJ_DRAIN_G) break;
+        if (busy) {
+            f3 P;
+            const int st = march_step<1, 0, K, VisGrad>(p, T, m, PRIMARY, StepSize, cnt, vg, P);
+            if (st != 0) { busy = false; vol = st > 0; }
+        }
+    }
+    adj_drain_grad<PRIMARY>(p, T, m, busy, vol, vg, StepSize, lane);
+}
+template<bool PRIMARY>
+__device__ __forceinline__ void adj_drain_grad(const DevParams& p, const Tables& T, const MarchState& m, bool busy, bool& vol,
+                                            VisGrad& vg, float StepSize, int lane)
+{
+    Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int G = ADJ_DRAIN_G;
+    const unsigned long long bm = __ballot(busy);
+    bool has;
+    const int src = adj_group_source<G>(bm, lane, has);
+    MarchState g = adj_march_from(m, src);
+    if (busy) vg.flush();
+    VisGrad gv;
+    gv.hist = vg.hist; gv.scale = vg.scale; gv.cur = -1; gv.a0 = gv.a1 = 0.0f;
+    gv.last_i = 0; gv.last_sigma = 0.0f;
+    gv.w = __shfl(vg.w, src);
+    bool gb = has;
+    int gst = 0;
+    while (__ballot(gb)) {
+        if (gb) {
+            f3 P;
+            int iters = 0;
+            const int st = group_step_v<1, 0, G, PRIMARY, VisGrad>(p, T, g, StepSize, lane, cnt, P, iters, gv);
+            if (st != 0) { gst = st; gb = false; }
+        }
+    }
+    gv.flush();
+    const int from = busy ? __popcll(bm & ((1ull << lane) - 1ull)) * G : 0;
+    const int rst = __shfl(gst, from);
+    const int li = __shfl(gv.last_i, from);
+    const float ls = __shfl(gv.last_sigma, from);
+    if (busy) {
+        vol = rst > 0;
+        vg.last_i = li;
+        vg.last_sigma = ls;
+    }
+}
 
 /* Visible (transport.h:46-56) in expected-value form: the transmittance of the shadow segment,
  * with the forward's draw consumption (none when shadows are off, a light shape blocks, or the
@@ -318,7 +478,10 @@ __device__ __forceinline__ void adj_flush_hist(const float* hist, float* __restr
     for (int k = threadIdx.x; k < ADJ_NODES; k += blockDim.x) out[k] = hist[k];
 }
 
-__global__ __launch_bounds__(256) void k_adj_shadow(const DevParams* __restrict__ pp, const unsigned int* __restrict__ counts,
+#ifndef ADJ_SHADOW_WAVES_PER_EU
+#define ADJ_SHADOW_WAVES_PER_EU 3
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ_SHADOW_WAVES_PER_EU))) void k_adj_shadow(const DevParams* __restrict__ pp, const unsigned int* __restrict__ counts,
                                                    const ShadowJob* __restrict__ jobq, const EventRec* __restrict__ events,
                                                    const float4* __restrict__ adj, float* __restrict__ tvis,
                                                    float* __restrict__ partials)
@@ -333,38 +496,55 @@ __global__ __launch_bounds__(256) void k_adj_shadow(const DevParams* __restrict_
     const float dts = p.step_shadow * p.minstep;
     const float k2 = (DS * DS) * dts;          /* d tau / d opacity per sample */
     const float cap = ADJ_TAU_CAP / p.density_scale;
-    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const ShadowJob job = jobq[j];
+    const int lane = threadIdx.x & 63;
+    /* wave-aligned grid stride: the marches of a wave step together (adj_march_sum / _grad) */
+    for (unsigned int j0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); j0 < n; j0 += gridDim.x * blockDim.x) {
+        const unsigned int j = j0 + (unsigned int)lane;
+        const bool has = j < n;
+        ShadowJob job;
         Ray r;
-        r.O = mk(job.Ox, job.Oy, job.Oz);
-        r.D = mk(job.Dx, job.Dy, job.Dz);
-        r.MinT = 0.0f;
-        r.MaxT = job.MaxT;
-        Rng rng;
-        rng.s0 = job.s0;
-        rng.s1 = job.s1;
+        MarchState m;
         VisSum<0> vs;
         vs.init(nullptr, p.lut_max);
-        Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        float Tt; f3 Pt;
-        march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisSum<0>>(p, T, r, false, rng, Tt, Pt, cnt, vs, cap);
-        const float Tr = erm::expf_(-((DS * dts) * vs.sd));
-        tvis[2 * job.event + job.slot] = Tr;
-        const EventRec& ev = events[job.event];
-        const f3 c = job.slot ? mk(ev.c2x, ev.c2y, ev.c2z) : mk(ev.c1x, ev.c1y, ev.c1z);
-        const float4 A4 = adj[ev.pixel];
-        const float g = ev.count * Tr * dot(mk(A4.x, A4.y, A4.z), c);
-        if (g != 0.0f) {
-            VisGrad vg;
-            vg.hist = hist; vg.scale = p.tf_lut_scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f;
-            vg.last_i = 0; vg.last_sigma = 0.0f;
-            vg.w = -(g * k2);
+        bool busy = false;
+        float nearT = 0.0f, farT = 0.0f;
+        bool inbox = false;
+        if (has) {
+            job = jobq[j];
+            r.O = mk(job.Ox, job.Oy, job.Oz);
+            r.D = mk(job.Dx, job.Dy, job.Dz);
+            r.MinT = 0.0f;
+            r.MaxT = job.MaxT;
+            Rng rng;
+            rng.s0 = job.s0;
+            rng.s1 = job.s1;
+            inbox = intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT);
+            if (inbox && march_begin(p, r, nearT, farT, dts, rng, cap, m)) busy = m.Sum < m.S;
+        }
+        adj_march_sum<false>(p, T, m, busy, vs, dts, lane);
+        float g = 0.0f;
+        if (has) {
+            const float Tr = erm::expf_(-((DS * dts) * vs.sd));
+            tvis[2 * job.event + job.slot] = Tr;
+            const EventRec& ev = events[job.event];
+            const f3 c = job.slot ? mk(ev.c2x, ev.c2y, ev.c2z) : mk(ev.c1x, ev.c1y, ev.c1z);
+            const float4 A4 = adj[ev.pixel];
+            g = ev.count * Tr * dot(mk(A4.x, A4.y, A4.z), c);
+        }
+        VisGrad vg;
+        vg.hist = hist; vg.scale = p.tf_lut_scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f;
+        vg.last_i = 0; vg.last_sigma = 0.0f;
+        vg.w = -(g * k2);
+        bool busy2 = false, vol = false;
+        if (has && g != 0.0f && inbox) {
+            /* the same samples again: the march restarts from the job's RNG state */
             Rng rr;
             rr.s0 = job.s0;
             rr.s1 = job.s1;
-            march_v<1, 0, ER_BATCH_SHADOW_ADJ, VisGrad>(p, T, r, false, rr, Tt, Pt, cnt, vg, cap);
-            vg.flush();
+            if (march_begin(p, r, nearT, farT, dts, rr, cap, m)) busy2 = m.Sum < m.S;
         }
+        adj_march_grad<false, ER_BATCH_SHADOW_ADJ>(p, T, m, busy2, vol, vg, dts, lane);
+        vg.flush();
     }
     adj_flush_hist(hist, partials);
 }
@@ -444,24 +624,39 @@ __global__ __launch_bounds__(256) void k_adj_primary(const DevParams* __restrict
     const unsigned int n = *count;
     const float kdt = p.density_scale * (p.step_primary * p.minstep);   /* d tau / d sigma per sample */
     const float k2 = p.density_scale * kdt;                             /* d tau / d opacity per sample */
-    for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint2 e = list[j];
-        const int i = (int)e.x;
-        const float g = __uint_as_float(e.y);
-        Rng rng;
-        rng.s0 = seeds1[i];
-        rng.s1 = seeds2[i];
-        Sample smp;
-        const Ray r = camera_ray(p, i % p.film_w, i / p.film_w, rng, smp);
+    const float StepSize = p.step_primary * p.minstep;
+    const int lane = threadIdx.x & 63;
+    /* wave-aligned grid stride: the marches of a wave step together (adj_march_grad) */
+    for (unsigned int j0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); j0 < n; j0 += gridDim.x * blockDim.x) {
+        const unsigned int j = j0 + (unsigned int)lane;
+        const bool has = j < n;
+        float g = 0.0f;
         VisGrad vg;
         vg.hist = hist; vg.scale = p.tf_lut_scale; vg.cur = -1; vg.a0 = vg.a1 = 0.0f;
         vg.last_i = 0; vg.last_sigma = 0.0f;
-        vg.w = -(g * k2);
-        Cnt cnt = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        float Tq; f3 Pq;
-        const bool vol = march_v<1, 0, ER_BATCH_PRIMARY_ADJ, VisGrad>(p, T, r, true, rng, Tq, Pq, cnt, vg, -1.0f);
+        vg.w = 0.0f;
+        MarchState m;
+        bool busy = false, vol = false;
+        if (has) {
+            const uint2 e = list[j];
+            const int i = (int)e.x;
+            g = __uint_as_float(e.y);
+            Rng rng;
+            rng.s0 = seeds1[i];
+            rng.s1 = seeds2[i];
+            Sample smp;
+            const Ray r = camera_ray(p, i % p.film_w, i / p.film_w, rng, smp);
+            vg.w = -(g * k2);
+            float nearT, farT;
+            if (intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT) &&
+                march_begin(p, r, nearT, farT, StepSize, rng, -1.0f, m)) {
+                if (!(m.Sum < m.S)) vol = true;   /* an event at the first sample, no sample visited */
+                else busy = true;
+            }
+        }
+        adj_march_grad<true, ER_BATCH_PRIMARY_ADJ>(p, T, m, busy, vol, vg, StepSize, lane);
         vg.flush();
-        if (vol && vg.last_sigma > 0.0f) {
+        if (has && vol && vg.last_sigma > 0.0f) {
             const float tau = kdt * vg.last_sigma;
             vg.w = (g / erm::expm1f_(tau) + g) * k2;
             vg.sample(vg.last_i, vg.last_sigma);

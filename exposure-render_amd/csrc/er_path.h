@@ -196,7 +196,11 @@ __device__ __forceinline__ uint32_t skip_additions(float& t, float b, float t_li
 struct NoVisit {
     __device__ __forceinline__ void begin(float) {}
     __device__ __forceinline__ void sample(int, float) {}
+    /* group marches (group_step_v): the event sample, reported to every lane of the group */
+    __device__ __forceinline__ void event(int, float) {}
 };
+template<class V> struct IsNoVisit { static constexpr bool value = false; };
+template<> struct IsNoVisit<NoVisit> { static constexpr bool value = true; };
 
 /* march_v with no observer: the forward march */
 template<int FULL, int COUNT, int K = 1, class Visit = NoVisit>
@@ -595,10 +599,14 @@ __device__ __forceinline__ void group_accumulate(const float (&c)[KB], float& Su
  * cross-lane reads stay inside active lanes. Returns 0 to go on, 1 on an event (Pout = the
  * sample, m.MinT one step beyond it), -1 when the ray leaves the volume first; counts go to
  * the group's first lane. iters: the iterations the one-lane-per-ray loop would have taken for
- * this round (1 for a skip, else the samples consumed over ER_BATCH_PRIMARY, rounded up). */
-template<int FULL, int COUNT, int G, bool PRIMARY>
-__device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane,
-                                          Cnt& cnt, f3& Pout, int& iters)
+ * this round (1 for a skip, else the samples consumed over ER_BATCH_PRIMARY, rounded up).
+ * group_step_v also reports the round's fetched samples to an observer: each lane passes its
+ * own samples that the round consumed (vis.sample, in sample order within the lane, so each
+ * lane's observer sees a subsequence of the march), and on an event every lane of the group
+ * passes the event sample (vis.event). */
+template<int FULL, int COUNT, int G, bool PRIMARY, class Visit>
+__device__ __forceinline__ int group_step_v(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane,
+                                            Cnt& cnt, f3& Pout, int& iters, Visit& vis)
 {
     constexpr int KB = GroupBatch<G>::value;
     constexpr int NS = KB * G;
@@ -644,10 +652,13 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
         live[j] = ins[j] & mc_occupied(T, mc_index(p, lk, sh)) & !cut;
         f[j] = cell_fetch(p, lk);
     }
+    int ivs[KB];
+    float sg[KB];
 #pragma unroll
     for (int j = 0; j < KB; ++j) {
-        const float sigma = live[j] ? T.lut(min((int)cell_value(f[j]), p.lut_max)) : 0.0f;
-        c[j] = sigma * StepSize;
+        ivs[j] = (int)cell_value(f[j]);
+        sg[j] = live[j] ? T.lut(min(ivs[j], p.lut_max)) : 0.0f;
+        c[j] = sg[j] * StepSize;
     }
     /* samples 0 .. nin-1 lie before the end of the segment (t grows with the sample index) */
     int nin = 0;
@@ -670,6 +681,19 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
     }
     const int ns = term == 1 ? send + 1 : send;
     iters = (ns + ER_BATCH_PRIMARY - 1) / ER_BATCH_PRIMARY;
+    if constexpr (!IsNoVisit<Visit>::value) {
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+            if (live[j] && j * G + k < ns) vis.sample(ivs[j], sg[j]);
+        if (term == 1) {
+            const int je = send / G;
+            int ei = ivs[0];
+            float es = sg[0];
+#pragma unroll
+            for (int j = 1; j < KB; ++j) if (j == je) { ei = ivs[j]; es = sg[j]; }
+            vis.event(__shfl(ei, send - je * G, G), __shfl(es, send - je * G, G));
+        }
+    }
     if (COUNT) {
         uint32_t fetched = 0;
 #pragma unroll
@@ -699,6 +723,14 @@ __device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, M
     m.Sum = Sum;
     m.MinT = __shfl(tk[KB - 1], G - 1, G) + StepSize;
     return 0;
+}
+
+template<int FULL, int COUNT, int G, bool PRIMARY>
+__device__ __forceinline__ int group_step(const DevParams& p, const Tables& T, MarchState& m, float StepSize, int lane,
+                                          Cnt& cnt, f3& Pout, int& iters)
+{
+    NoVisit nv;
+    return group_step_v<FULL, COUNT, G, PRIMARY, NoVisit>(p, T, m, StepSize, lane, cnt, Pout, iters, nv);
 }
 
 /* A whole primary march by an aligned group of G lanes, round by round (group_step); steps
