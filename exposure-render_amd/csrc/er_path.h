@@ -139,15 +139,31 @@ __device__ __forceinline__ float sample_sigma(const DevParams& p, const Tables& 
     return T.lut(min((int)trilinear_cell(p, l), p.lut_max));
 }
 
+/* floor(x / y) for integers x < 2^24, 1 <= y < 2^24: the hardware reciprocal (1 ulp) makes the
+ * float quotient at most 2 too small or 1 too large (exact for y a power of 2), which the
+ * remainder then corrects without a loop */
+__device__ __forceinline__ uint32_t udiv24(uint32_t x, uint32_t y)
+{
+    uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)y));
+    int r = (int)x - (int)(q * y);
+    if (r < 0) { q -= 1u; r += (int)y; }
+    if (r >= (int)y) { q += 1u; r -= (int)y; }
+    if (r >= (int)y) { q += 1u; }
+    return q;
+}
+
 /* Advance t by repeated float additions of b while t < t_lim, without performing them one by
  * one: returns the number of additions n and sets t to the value after the last one and t_prev
  * to the value before it - bit-identical to `while (t < t_lim) { t_prev = t; t += b; ++n; }`.
  * Within one binade [2^e, 2^(e+1)) of t every value is an integer multiple A of the spacing
  * u = 2^(e-23), so t + b rounds to (A + D) u with the constant D = round(b / u) unless b / u
  * has fractional part exactly 1/2 (then the rounding depends on the parity of A and the
- * addition is done plainly). The additions inside a binade are counted with integer arithmetic;
- * the one that crosses into the next binade is done as a float addition. Requires b > 0 and
- * t_lim + b > t_lim (the march checks it), so D >= 1 wherever t < t_lim. */
+ * addition is done plainly). The additions are counted with integer arithmetic: reaching
+ * t_lim takes J = ceil((ceil(t_lim / u) - A) / D) of them (t_lim / u is exact, a power-of-2
+ * scaling, and A + j D is an integer); if A + J D stays below 2^24 the binade holds them all,
+ * else the binade's last addition, which crosses into the next binade, is done as a float
+ * addition and the count goes on from there. Requires b > 0 and t_lim + b > t_lim (the march
+ * checks it), so D >= 1 wherever t < t_lim. */
 __device__ __forceinline__ uint32_t skip_additions(float& t, float b, float t_lim, float& t_prev)
 {
     uint32_t n = 0;
@@ -163,23 +179,20 @@ __device__ __forceinline__ uint32_t skip_additions(float& t, float b, float t_li
         if (fr == 0.5f || q >= 8388608.0f) { t_prev = t; t += b; ++n; continue; }
         const uint32_t D = (uint32_t)fq + (fr > 0.5f ? 1u : 0u);
         const uint32_t A = (tb & 0x007fffffu) | 0x00800000u;                      /* t / u */
-        const uint32_t room = 0x00ffffffu - A;                                     /* additions stay below 2^24 u */
-        uint32_t kbin = (uint32_t)((float)room * __builtin_amdgcn_rcpf((float)D));
-        while ((uint64_t)kbin * D > room) --kbin;
-        while ((uint64_t)(kbin + 1) * D <= room) ++kbin;
-        /* J = number of j >= 0 with (A + j D) u < t_lim, found from an estimate and exact checks */
-        uint32_t J = (uint32_t)fminf((t_lim - t) * __builtin_amdgcn_rcpf((float)D * u), 16777216.0f);
-        if (J > kbin + 1) J = kbin + 1;
-        while (J > 0 && (float)(A + (J - 1) * D) * u >= t_lim) --J;
-        while (J <= kbin && (float)(A + J * D) * u < t_lim) ++J;
-        if (J <= kbin) {
-            n += J;
-            t_prev = (float)(A + (J - 1) * D) * u;
-            t = (float)(A + J * D) * u;
-            break;
+        const float T = t_lim * inv_u;                                             /* exact */
+        if (T < 16777216.0f) {
+            /* T > A (t < t_lim), so J >= 1; A + j D >= ceil(T) <=> (A + j D) u >= t_lim */
+            const uint32_t J = udiv24((uint32_t)ceilf(T) - A + D - 1u, D);
+            if (A + J * D < 16777216u) {
+                n += J;
+                t_prev = (float)(A + (J - 1u) * D) * u;
+                t = (float)(A + J * D) * u;
+                break;
+            }
         }
         /* every addition inside the binade is taken; the next one crosses into the next binade */
-        n += kbin + 1;
+        const uint32_t kbin = udiv24(0x00ffffffu - A, D);
+        n += kbin + 1u;
         t_prev = (float)(A + kbin * D) * u;
         t = t_prev + b;
     }
