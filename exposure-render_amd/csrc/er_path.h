@@ -458,6 +458,76 @@ __device__ __forceinline__ bool woodcock_track(const DevParams& p, const Tables&
     }
 }
 
+/* woodcock_track one segment at a time, for loops that step many walks together (the shadow
+ * marches with lane refill): the same operations and draws in the same order. */
+struct WcWalk {
+    f3 O, D;
+    float t, MaxT, tau;
+    float gx, gy, gz, igx, igy, igz;
+    int segments;
+    bool need_tau;
+    Rng rng;
+};
+
+__device__ __forceinline__ void wc_begin(const DevParams& p, const Ray& r, float nearT, float farT, const Rng& rng, WcWalk& w)
+{
+    w.O = r.O;
+    w.D = r.D;
+    w.t = fmaxf(nearT, r.MinT);
+    w.MaxT = fminf(farT, r.MaxT);
+    w.gx = (r.D.x * p.invsize[0]) * p.resf[0];
+    w.gy = (r.D.y * p.invsize[1]) * p.resf[1];
+    w.gz = (r.D.z * p.invsize[2]) * p.resf[2];
+    w.igx = w.gx != 0.0f ? 1.0f / w.gx : 0.0f;
+    w.igy = w.gy != 0.0f ? 1.0f / w.gy : 0.0f;
+    w.igz = w.gz != 0.0f ? 1.0f / w.gz : 0.0f;
+    w.segments = 0;
+    w.need_tau = true;
+    w.rng = rng;
+}
+
+/* one step: draws the walk's next optical depth if it needs one, then walks one majorant
+ * segment and, if the tentative collision falls in it, tests it. 1: a real collision (w.t its
+ * distance), -1: the walk left the segment [t, MaxT] first, 0: go on. */
+template<int FULL, int COUNT>
+__device__ __forceinline__ int wc_step(const DevParams& p, const Tables& T, WcWalk& w, bool primary, Cnt& cnt)
+{
+    if (w.need_tau) {
+        w.tau = -erm::logf_(w.rng.u());
+        w.need_tau = false;
+    }
+    if (++w.segments > ER_WC_MAX_SEGMENTS || !(w.t < w.MaxT)) return -1;
+    if (COUNT) cnt.iters++;
+    const int dx = p.wc_dim[0], dy = p.wc_dim[1], dz = p.wc_dim[2];
+    const float big = 3.402823466e+38f;
+    const Lattice l = to_lattice(p, add(w.O, scl(w.t, w.D)));
+    const int mx = mc_axis(l.Lx, ER_WC_SHIFT, dx), my = mc_axis(l.Ly, ER_WC_SHIFT, dy), mz = mc_axis(l.Lz, ER_WC_SHIFT, dz);
+    const float mu = p.density_scale * gmem(p.wc_sigma)[(mz * dy + my) * dx + mx];
+    const float ex = w.gx > 0.0f ? (mx == dx - 1 ? big : ((float)((mx + 1) << ER_WC_SHIFT) - l.Lx) * w.igx)
+                                 : (w.gx < 0.0f ? (mx == 0 ? big : ((float)(mx << ER_WC_SHIFT) - l.Lx) * w.igx) : big);
+    const float ey = w.gy > 0.0f ? (my == dy - 1 ? big : ((float)((my + 1) << ER_WC_SHIFT) - l.Ly) * w.igy)
+                                 : (w.gy < 0.0f ? (my == 0 ? big : ((float)(my << ER_WC_SHIFT) - l.Ly) * w.igy) : big);
+    const float ez = w.gz > 0.0f ? (mz == dz - 1 ? big : ((float)((mz + 1) << ER_WC_SHIFT) - l.Lz) * w.igz)
+                                 : (w.gz < 0.0f ? (mz == 0 ? big : ((float)(mz << ER_WC_SHIFT) - l.Lz) * w.igz) : big);
+    const float t_exit = fmaxf(fminf(w.t + fminf(ex, fminf(ey, ez)), w.MaxT), w.t + fmaxf(w.t * 1.0e-6f, 1.0e-7f));
+    if (mu > 0.0f) {
+        const float tc = w.t + w.tau / mu;
+        if (tc < t_exit) {
+            w.t = tc;
+            if (primary ? (w.t >= w.MaxT) : (w.t > w.MaxT)) return -1;
+            const f3 Ps = add(w.O, scl(w.t, w.D));
+            if (COUNT) { if (primary) cnt.primary++; else cnt.shadow++; }
+            const float sigma = sample_sigma<FULL, COUNT>(p, T, Ps, primary, cnt) * p.density_scale;
+            if (w.rng.u() * mu < sigma) return 1;
+            w.need_tau = true;
+            return 0;
+        }
+        w.tau = fmaxf(w.tau - mu * (t_exit - w.t), 0.0f);
+    }
+    w.t = t_exit;
+    return 0;
+}
+
 /* S_fixed >= 0: the threshold is still drawn (same draws) but S_fixed replaces it, so the march
  * reads every sample until that optical depth / DensityScale or the end of the ray (the
  * expected-transmittance form of a shadow test). */

@@ -578,13 +578,67 @@ __device__ __forceinline__ void shadow_drain(const DevParams& p, const Tables& T
     }
 }
 
+/* shadow_marches for Woodcock tracking: each lane's walk advances one majorant segment per
+ * loop iteration (wc_step) instead of running whole when its job starts, so that idle lanes
+ * refill while the others walk. */
+template<int FULL, int COUNT>
+__device__ __forceinline__ void shadow_marches_wc(const DevParams& p, const Tables& T, const ShadowJob* __restrict__ jobq,
+                                                  unsigned char* __restrict__ vis, unsigned int* claim, unsigned int n,
+                                                  unsigned int j, unsigned int claim_base, bool drained, int lane, Cnt& cnt)
+{
+    WcWalk w;
+    unsigned int slot = 0;
+    bool busy = false;
+    for (;;) {
+        if (j < n) {
+            const ShadowJob job = jobq[j];
+            slot = 2u * job.event + job.slot;
+            Ray r;
+            r.O = mk(job.Ox, job.Oy, job.Oz);
+            r.D = mk(job.Dx, job.Dy, job.Dz);
+            r.MinT = 0.0f;
+            r.MaxT = job.MaxT;
+            Rng rng;
+            rng.s0 = job.s0;
+            rng.s1 = job.s1;
+            float nearT, farT;
+            if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) {
+                vis[slot] = 1;
+            } else {
+                wc_begin(p, r, nearT, farT, rng, w);
+                busy = true;
+            }
+            j = n;
+        }
+        if (busy) {
+            const int st = wc_step<FULL, COUNT>(p, T, w, false, cnt);
+            if (st != 0) {
+                vis[slot] = st > 0 ? 0 : 1;
+                busy = false;
+            }
+        }
+        const unsigned long long idle = __ballot(!busy);
+        const int nidle = __popcll(idle);
+        if (drained) {
+            if (nidle == 64) break;
+        } else if (nidle >= ER_SHADOW_REFILL) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(claim, (unsigned int)nidle);
+            base = claim_base + __shfl(base, leader);
+            if (base + (unsigned int)nidle >= n) drained = true;
+            if (!busy) j = base + (unsigned int)__popcll(idle & ((1ull << lane) - 1ull));
+        }
+    }
+}
+
 /* The shadow marches of one queue of n jobs (jobq), one march per lane. The marches differ in
  * length by orders of magnitude (a ray through empty space ends after a few skips, one through
  * tissue takes hundreds of samples), so a wave does not wait for its longest march: the lane
  * starts job j (none if j >= n), and whenever at least ER_SHADOW_REFILL lanes are idle they
  * claim the next jobs from *claim (job claim_base + the counter's value) with one atomic and
- * start them, and the wave steps all its live marches one iteration at a time (a Woodcock
- * march runs whole when its job starts). `drained`:
+ * start them, and the wave steps all its live marches one iteration at a time (Woodcock
+ * walks one majorant segment at a time, in shadow_marches_wc). `drained`:
  * nothing is left to claim. The job order does not matter: each march is independent and
  * writes only its own visibility byte. Every lane of the wave must call this. */
 template<int FULL, int COUNT>
@@ -592,6 +646,10 @@ __device__ __forceinline__ void shadow_marches(const DevParams& p, const Tables&
                                                unsigned char* __restrict__ vis, unsigned int* claim, unsigned int n,
                                                unsigned int j, unsigned int claim_base, bool drained, int lane, Cnt& cnt)
 {
+    if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
+        shadow_marches_wc<FULL, COUNT>(p, T, jobq, vis, claim, n, j, claim_base, drained, lane, cnt);
+        return;
+    }
     const float StepSize = p.step_shadow * p.minstep;
     NoVisit nv;
     MarchState m;
@@ -612,11 +670,6 @@ __device__ __forceinline__ void shadow_marches(const DevParams& p, const Tables&
             float nearT, farT;
             if (!intersect_box(r, ld3(p.minp), ld3(p.maxp), nearT, farT)) {
                 vis[slot] = 1;
-            } else if (FULL && p.sampler == ER_SAMPLER_WOODCOCK) {
-                /* delta tracking runs to its end here: its length is not a count of steps */
-                float Tt;
-                f3 Pt;
-                vis[slot] = woodcock_track<FULL, COUNT>(p, T, r, nearT, farT, false, rng, Tt, Pt, cnt) ? 0 : 1;
             } else if (!march_begin(p, r, nearT, farT, StepSize, rng, -1.0f, m)) {
                 vis[slot] = 1;
             } else if (!(m.Sum < m.S)) {
