@@ -1416,9 +1416,18 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
         for (int w = 2; w <= 8; w *= 2)
             if (w != cur.shadow_waves_per_cu) { er_tune x = cur; x.shadow_waves_per_cu = w; c.push_back(x); }
     } else if (axis == 4) {
+        /* heavy tiles together with their lanes per ray: a count of heavy tiles pays off only
+         * with the right lane count, which the later axis alone would not find from the default */
         if (cur.rays_per_wave == 64)
             for (int h : {0, 256, 768})
-                if (h != cur.heavy_tiles) { er_tune x = cur; x.heavy_tiles = h; c.push_back(x); }
+                for (int r : {16, 4, 1}) {
+                    if (h == 0 && r != cur.heavy_rays_per_wave) continue;
+                    if (h == cur.heavy_tiles && (h == 0 || r == cur.heavy_rays_per_wave)) continue;
+                    er_tune x = cur;
+                    x.heavy_tiles = h;
+                    x.heavy_rays_per_wave = r;
+                    c.push_back(x);
+                }
     } else if (axis == 7) {
         if (cur.rays_per_wave == 64 && cur.heavy_tiles > 0)
             for (int r : {16, 4, 1})
