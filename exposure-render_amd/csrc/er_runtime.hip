@@ -1455,6 +1455,7 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
 /* a candidate replaces the incumbent only when it is clearly faster: timing noise must not
  * walk the search away from a good configuration */
 const float ER_TUNE_MARGIN = 0.97f;
+const float ER_TUNE_FINE_MARGIN = 0.985f;   /* the closer look's margin (best of six trials) */
 const float ER_TUNE_TRIAL_MS = 30.0f;   /* target length of one timing trial */
 
 } /* namespace */
@@ -1779,11 +1780,12 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     er_tune cur{};
     fill_tune(t, cur);
     int trials = 0;
+    int reps = 3;
     auto measure = [&](const er_tune& c, float& best_ms) -> int {
         int s;
         if ((s = apply_tune(t, c))) return s;
         best_ms = FLT_MAX;
-        for (int rep = 0; rep < 3; ++rep) {
+        for (int rep = 0; rep < reps; ++rep) {
             float ms = 0.0f;
             if ((s = time_passes(t, snap, passes, ms))) return s;
             best_ms = std::min(best_ms, ms);
@@ -1818,6 +1820,21 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
             }
         }
         if (!moved) break;
+    }
+    /* a closer look at the axes whose values lie within a few percent of each other (tile
+     * shape, primary workgroups, shadow waves per CU): best of six trials instead of three,
+     * and a candidate need only be 1.5% faster */
+    if (!st) {
+        reps = 6;
+        st = measure(cur, cur_ms);
+        for (int axis : {1, 2, 3}) {
+            if (st) break;
+            for (const er_tune& c : tune_candidates(axis, cur, vols)) {
+                float ms = 0.0f;
+                if ((st = measure(c, ms))) break;
+                if (ms < ER_TUNE_FINE_MARGIN * cur_ms) { cur = c; cur_ms = ms; }
+            }
+        }
     }
     if (!st) st = apply_tune(t, cur);
     if (st) {
