@@ -128,10 +128,14 @@ __device__ __forceinline__ void primary_tile(const DevParams& p, const Tables& T
             }
         }
     }
-    /* The march loop. Once at most ER_COOP_LANES lanes of the wave are still marching, their
-     * marches are finished one at a time by the whole wave (coop_march): a lone ray through
-     * kilo-samples of tissue would otherwise keep its wave for hundreds of microseconds.
-     * it: the lane's march iterations (coop_march counts its equivalent), for the tile cost. */
+    /* The march loop, in three stages: every lane steps its own march until at most 64 / GD
+     * lanes (ER_COOP_LANES without group tails) are still marching; groups of GD lanes then
+     * take those marches over; a march left after that (a single-ray wave's) is finished by the
+     * whole wave (coop_march). A lone ray through kilo-samples of tissue would otherwise keep its
+     * wave for hundreds of microseconds. With G > 1 lanes to a ray, group_march steps every
+     * march in its group from the start instead.
+     * it: the lane's march iterations (the group and wave stages count their equivalents), for
+     * the tile cost. */
     NoVisit nv;
     int it = 0;
     if constexpr (G > 1) {
