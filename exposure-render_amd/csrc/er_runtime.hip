@@ -1455,7 +1455,7 @@ std::vector<er_tune> tune_candidates(int axis, const er_tune& cur, const std::ve
 /* a candidate replaces the incumbent only when it is clearly faster: timing noise must not
  * walk the search away from a good configuration */
 const float ER_TUNE_MARGIN = 0.97f;
-const float ER_TUNE_FINE_MARGIN = 0.985f;   /* the closer look's margin (best of six trials) */
+const float ER_TUNE_FINE_MARGIN = 0.985f;   /* the closer look's margin (median of six trials) */
 const float ER_TUNE_TRIAL_MS = 30.0f;   /* target length of one timing trial */
 
 } /* namespace */
@@ -1751,7 +1751,7 @@ ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
 }
 
 /* Coordinate search over brick size, tile shape and persistent workgroups per CU. Each
- * candidate renders the same passes from the same seeds (best of three trials of about
+ * candidate renders the same passes from the same seeds (median of three trials of about
  * 30 ms, or `passes` passes when given); a candidate must be 3% faster to be taken. The
  * tracer's render state is restored afterwards. Results are cached by scene. */
 ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
@@ -1781,15 +1781,17 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
     fill_tune(t, cur);
     int trials = 0;
     int reps = 3;
-    auto measure = [&](const er_tune& c, float& best_ms) -> int {
+    /* a candidate's time is the median of its trials: some configurations (many shadow waves
+     * per CU) vary by 20% from trial to trial, and the best of their trials ranked them ahead
+     * of steadier ones that render long runs faster */
+    auto measure = [&](const er_tune& c, float& med_ms) -> int {
         int s;
         if ((s = apply_tune(t, c))) return s;
-        best_ms = FLT_MAX;
-        for (int rep = 0; rep < reps; ++rep) {
-            float ms = 0.0f;
-            if ((s = time_passes(t, snap, passes, ms))) return s;
-            best_ms = std::min(best_ms, ms);
-        }
+        float ms[8];
+        for (int rep = 0; rep < reps; ++rep)
+            if ((s = time_passes(t, snap, passes, ms[rep]))) return s;
+        std::sort(ms, ms + reps);
+        med_ms = (reps & 1) ? ms[reps / 2] : 0.5f * (ms[reps / 2 - 1] + ms[reps / 2]);
         ++trials;
         return ER_OK;
     };
@@ -1822,7 +1824,7 @@ ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out)
         if (!moved) break;
     }
     /* a closer look at the axes whose values lie within a few percent of each other (tile
-     * shape, primary workgroups, shadow waves per CU): best of six trials instead of three,
+     * shape, primary workgroups, shadow waves per CU): median of six trials instead of three,
      * and a candidate need only be 1.5% faster */
     if (!st) {
         reps = 6;

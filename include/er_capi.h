@@ -402,12 +402,12 @@ typedef struct er_tune {
 #define ER_TUNE_FORCE 1           /* er_autotune flag: measure even if the scene is cached */
 
 /* Times `passes` passes of each candidate (passes <= 0: as many as make a trial of about
- * 30 ms, from one probe) from the tracer's current seeds, best of three, and keeps a candidate
+ * 30 ms, from one probe) from the tracer's current seeds, median of three, and keeps a candidate
  * that is at least 3% faster: bricks, then rays per wave, then shadow rays per wave, then tile
  * shapes, then workgroups per CU, then shadow waves per CU, then heavy tiles (0, 256 or 768,
  * each with 16, 4 or 1 rays per wave), then their rays per wave alone; a second sweep over the
  * same axes follows when the first changed anything, then a closer look at the tile shapes,
- * workgroups per CU and shadow waves per CU (best of six, 1.5% faster to replace). The
+ * workgroups per CU and shadow waves per CU (median of six, 1.5% faster to replace). The
  * tracer's seeds, running estimate, display, counters and pass count are restored afterwards.
  * Results are cached per scene (descriptor, node table and volume contents). */
 ER_API int er_autotune(int tracer_id, int passes, int flags, er_tune* out);
