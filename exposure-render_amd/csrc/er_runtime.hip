@@ -389,8 +389,12 @@ struct Tracer {
     int order_shift = -1, order_tiles = 0;
     hipStream_t stream = nullptr;     /* primary + shading, uploads, readbacks */
     hipStream_t stream2 = nullptr;    /* shadow marches, combine, post */
+    /* the timing events of er_render_passes and of the tuner's trials (four a pass), kept from
+     * call to call so that creating them is not part of a timed render */
+    std::vector<hipEvent_t> timing_events;
     ~Tracer()
     {
+        for (hipEvent_t e : timing_events) (void)hipEventDestroy(e);
         set[0].release(); set[1].release();
         running.release(); display.release(); disp_tmp.release(); disp_filt.release();
         seeds1.release(); seeds2.release(); lut.release(); mc_sigma.release(); wc_sigma.release(); params.release();
@@ -400,6 +404,17 @@ struct Tracer {
         if (stream2) (void)hipStreamDestroy(stream2);
     }
 };
+
+/* at least n timing events in t.timing_events */
+int timing_events(Tracer& t, size_t n)
+{
+    while (t.timing_events.size() < n) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e));
+        t.timing_events.push_back(e);
+    }
+    return ER_OK;
+}
 
 /* both streams of a tracer idle */
 int sync_tracer(Tracer& t)
@@ -1390,8 +1405,8 @@ int time_passes(Tracer& t, Snapshot& snap, int passes, float& ms)
     if ((st = build_params(t))) return st;
     if ((st = snap.rewind(t)) || (st = enqueue_pass(t, nullptr)) || (st = sync_tracer(t)) || (st = snap.rewind(t)))
         return st;
-    std::vector<hipEvent_t> ev((size_t)passes * 4, nullptr);
-    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    if ((st = timing_events(t, (size_t)passes * 4))) return st;
+    hipEvent_t* ev = t.timing_events.data();
     for (int k = 0; k < passes && !st; ++k) st = enqueue_pass(t, &ev[4 * k]);
     const int sst = sync_tracer(t);
     if (!st) st = sst;
@@ -1399,7 +1414,6 @@ int time_passes(Tracer& t, Snapshot& snap, int passes, float& ms)
         hipError_t e = hipEventElapsedTime(&ms, ev[0], ev[4 * (passes - 1) + 3]);
         if (e != hipSuccess) st = set_error(ER_ERROR, "%s (hipEventElapsedTime)", hipGetErrorString(e));
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     return st;
 }
 
@@ -1724,11 +1738,8 @@ ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
     if (!t) return st;
     if (passes < 0) return set_error(ER_ERROR, "negative pass count");
     if ((st = build_params(*t))) return st;
-    std::vector<hipEvent_t> ev;
-    if (timing) {
-        ev.resize((size_t)passes * 4, nullptr);
-        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    }
+    if (timing && (st = timing_events(*t, (size_t)passes * 4))) return st;
+    hipEvent_t* ev = t->timing_events.data();
     for (int k = 0; k < passes && !st; ++k) st = enqueue_pass(*t, timing ? &ev[4 * k] : nullptr);
     hipError_t se = hipStreamSynchronize(t->stream);
     hipError_t se2 = hipStreamSynchronize(t->stream2);
@@ -1744,7 +1755,6 @@ ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing)
         timing->total_ms = total; timing->trace_ms = tr; timing->post_ms = po; timing->passes = passes;
         timing->primary_ms = pr;
     }
-    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (st) return st;
     if (se != hipSuccess) return set_error(ER_ERROR, "%s (hipStreamSynchronize)", hipGetErrorString(se));
     return ER_OK;
