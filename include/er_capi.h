@@ -349,7 +349,8 @@ ER_API int er_get_frame_xyza(int tracer_id, float* xyza, size_t size);
 ER_API int er_get_running_xyza(int tracer_id, float* xyza, size_t size);
 
 /* Runs `passes` progressive passes back-to-back with one host synchronisation at the end
- * (same result as `passes` calls of er_render_estimate). timing may be NULL. */
+ * (same result as `passes` calls of er_render_estimate). timing may be NULL; with timing the
+ * tracer keeps four events per pass of its longest timed call until it is released. */
 ER_API int er_render_passes(int tracer_id, int passes, er_timing* timing);
 
 /* Sample counting: while on, each pass accumulates counters (a slower kernel variant). */
