@@ -292,3 +292,26 @@ def test_autotune_batch_restores_and_matches_sequential(fresh):
         assert np.array_equal(g1, s1) and np.array_equal(g2, s2)
         r.close()
     lib.er_set_list_capacity(32)
+
+
+def test_timed_calls_of_different_lengths_match_single_passes(fresh):
+    """er_render_passes with timing over calls of 2, 7 and 3 passes (the tracer keeps its
+    timing events from call to call, growing them for the longer call): the same frame,
+    running estimate, RNG state, display and sample counts as 12 single passes, and every
+    call's timing covers its own passes."""
+    sc = scenes.config_small(scale=0.5)
+    print("[timing] small, calls of 2, 7 and 3 passes", flush=True)
+    base = _render(sc, 12)
+    r = er.Renderer(sc, seed=9)
+    r.set_counting(True)
+    r.reset_counters()
+    for n in (2, 7, 3):
+        t = r.render_passes(n)
+        assert t.passes == n
+        assert t.total_ms > 0.0 and t.trace_ms > 0.0 and t.post_ms > 0.0 and t.primary_ms > 0.0
+        # per pass the primary kernel lies within the pass's trace (events of microsecond resolution)
+        assert t.primary_ms <= t.trace_ms + 0.01 * n
+    out = (r.frame(), r.running(), r.get_seeds(), r.get_estimate(), r.counters(), r.get_tune())
+    assert r.no_iterations() == 12
+    r.close()
+    _same(base, out, "timed calls")
